@@ -1,0 +1,154 @@
+"""Benchmark: Mpaths/s of the gfx950 path integrator on the 4M-triangle
+sphere field (1920x1080, 1 spp, max depth 8, PCG seed 0).
+
+A step is one render_to_target pass over the whole frame with the scene,
+its BVH and the path buffers already resident in HBM.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WIDTH, HEIGHT, DEPTH = 1920, 1080, 8
+# Algorithmic bytes of one closest-hit query besides the BVH: the 32-B ray
+# record read (origin + path id, direction) and the 16-B hit record written.
+RAY_RECORD_BYTES = 32
+HIT_RECORD_BYTES = 16
+HBM_PEAK_GBS = 8000.0
+
+
+def main() -> int:
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--cpu-fraction", type=float, default=0.25,
+                   help="fraction of the frame's pixels the CPU baseline renders")
+    args = p.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    import numpy as np
+
+    from tinytrace_amd import Config, LightSampling, Renderer, RenderSettings, _abi, scenes
+
+    scene, camera = scenes.sphere_field(seed=0)
+    settings = RenderSettings(rng=_abi.TT_RNG_PCG, seed=0)
+    renderer = Renderer((WIDTH, HEIGHT), settings=settings)
+    t0 = time.perf_counter()
+    renderer.set_scene(scene)
+    build_s = time.perf_counter() - t0
+    info = renderer.scene.info()
+    renderer.camera = camera
+    renderer.set_config(Config(bounce_count=DEPTH, sample_count=1, light_sampling=LightSampling.NEXT_EVENT_ESTIMATION))
+
+    import ctypes as C
+
+    lib = _abi.lib()
+
+    def step():
+        k = renderer.constants()
+        _abi.check(lib.tt_render_ex(renderer.context.handle, renderer.scene.handle, C.byref(k),
+                                    C.byref(renderer.settings._desc()), renderer.target.handle),
+                   renderer.context.handle, "render")
+        renderer.accumulated_frame_count += 1
+        renderer.frame_index += 1
+
+    def sync():
+        # Waits on the render's end event.
+        v = C.c_double()
+        lib.tt_timing_ms(renderer.context.handle, b"integrate", C.byref(v))
+
+    for _ in range(args.warmup):
+        step()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    sync()
+    elapsed = time.perf_counter() - t0
+    paths = WIDTH * HEIGHT
+    ms_per_step = elapsed / args.steps * 1e3
+    mpaths = paths * args.steps / elapsed / 1e6
+
+    # Ray counts and per-kernel durations of a few more passes (HIP events on
+    # the context's stream around every closest-hit launch).
+    renderer.settings = RenderSettings(rng=_abi.TT_RNG_PCG, seed=0, collect_stats=True)
+    ext_rays = shadow_rays = 0
+    ext_ms = shadow_ms = 0.0
+    ext_launches = 0
+    passes = 5
+    for _ in range(passes):
+        step()
+        s = renderer.stats()
+        ext_rays += s["extension_rays"]
+        shadow_rays += s["shadow_rays"]
+        ext_ms += s["extend_ms"]
+        shadow_ms += s["shadow_ms"]
+        ext_launches += s["extend_launches"]
+    rays_per_path = (ext_rays + shadow_rays) / (passes * paths)
+    mrays = mpaths * rays_per_path
+    bytes_per_ray = RAY_RECORD_BYTES + HIT_RECORD_BYTES + info["expected_bytes_per_ray"]
+    achieved_gbs = ext_rays * bytes_per_ray / (ext_ms * 1e-3) / 1e9
+
+    result = {
+        "metric": "Mpaths/s & Mrays/s at fixed RMSE vs FP64 oracle; p50/p99 request latency",
+        "value": round(mpaths, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (procedural 4M-triangle sphere field, seed 0)",
+        "config": {"workload": "sphere field, 800 icospheres (4 096 258 triangles, 256 emissive), 1920x1080, 1 spp, "
+                               "max depth 8, next-event estimation, PCG seed 0",
+                   "width": WIDTH, "height": HEIGHT, "spp": 1, "max_depth": DEPTH, "light_sampling": "nee",
+                   "triangles": info["triangle_count"], "bvh_nodes": info["node_count"],
+                   "bvh_expected_bytes_per_ray": round(info["expected_bytes_per_ray"], 1),
+                   "scene_build_s": round(build_s, 3)},
+        "mrays_per_s": round(mrays, 2),
+        "rays_per_path": round(rays_per_path, 4),
+        "roofline": {"kernel": "extend_kernel", "bound": "hbm", "achieved": round(achieved_gbs, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
+                     "traffic": None, "bytes_per_ray": round(bytes_per_ray, 1),
+                     "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4),
+                     "rays_per_launch": round(ext_rays / max(ext_launches, 1), 1)},
+    }
+
+    if not args.no_cpu:
+        from oracle.binding import OracleScene
+
+        threads = os.cpu_count() or 1
+        oracle = OracleScene(scene, 32)
+        k = renderer.constants()
+        rng = np.random.default_rng(0)
+        pixels = np.sort(rng.choice(paths, size=int(paths * args.cpu_fraction), replace=False)).astype(np.uint32)
+        t0 = time.perf_counter()
+        _, rays = oracle.render_pixels(k, pixels, rng=_abi.TT_RNG_PCG, seed=0, threads=threads)
+        cpu_s = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": round(pixels.size / cpu_s / 1e6, 4), "unit": "Mpaths/s",
+                                  "cores": threads, "kind": "port",
+                                  "sample": f"{pixels.size} random pixels of the same frame (1 spp, depth 8), "
+                                            f"{cpu_s:.1f} s",
+                                  "mrays_per_s": round(float(rays.sum()) / cpu_s / 1e6, 3)}
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

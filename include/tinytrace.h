@@ -1,0 +1,436 @@
+/*
+ * tinytrace_amd C ABI: the drop-in boundary for tinytrace's path-integration
+ * hot path on MI355X (gfx950).
+ *
+ * The reference renders through `Renderer::render_to_target`
+ * (src/lib.rs:124-161), which uploads `Constants` and dispatches the
+ * integrate megakernel through `Integrator::integrate`
+ * (src/integrate/mod.rs:39-61) over a scene that `scene::Scene::new`
+ * (src/scene.rs:28-186) uploaded and whose BLAS/TLAS the Vulkan driver built
+ * (crates/tinytrace_backend/src/resource.rs:804-878, :1014-1083).  Results are
+ * read back with the blocking `Context::download`
+ * (crates/tinytrace_backend/src/copy.rs:183-298).  The functions below carry
+ * exactly those three capabilities (scene upload + BVH build, integrate +
+ * accumulate, download) plus the host-side helpers the Rust library keeps
+ * above them (camera constants, asset processing).
+ *
+ * Conventions
+ *  - Every function returns tt_status; no C++ exception crosses the ABI.
+ *  - Host arrays passed in are copied before the call returns.
+ *  - Handles are owned by the context that created them; destroy scenes and
+ *    targets before their context.  One context per host thread.
+ *  - All device work of a context is ordered on one HIP stream;
+ *    tt_target_download is the synchronisation point.
+ *  - Structs marked "reference layout" are byte-identical to the reference's
+ *    #[repr(C)] / std430 types and are checked with static asserts.
+ */
+#ifndef TINYTRACE_H
+#define TINYTRACE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes.  Mirrors tinytrace::Error (src/error.rs:3-9) and
+ * tinytrace_backend::Error (crates/tinytrace_backend/src/error.rs:7-28);
+ * misuse that panics in the reference (stale handles, missing bindings)
+ * returns TT_INVALID_ARGUMENT / TT_STALE_HANDLE instead. */
+typedef enum tt_status {
+    TT_OK = 0,
+    TT_NO_DEVICE = 1,        /* Error::NoSuitableDevice */
+    TT_HIP_ERROR = 2,        /* Error::VulkanResult */
+    TT_KERNEL_LOAD = 3,      /* Error::Shader */
+    TT_INVALID_ARGUMENT = 4, /* reference panics */
+    TT_STALE_HANDLE = 5,     /* reference Handle epoch check panics */
+    TT_OUT_OF_MEMORY = 6,
+    TT_OVER_BUDGET = 7,      /* serving admission */
+    TT_UNSUPPORTED = 8       /* feature outside the implemented path */
+} tt_status;
+
+/* LightSampling (src/config.rs:4-12).  TT_LIGHT_SAMPLING_RESTIR is new: ReSTIR
+ * DI replaces uniform next-event estimation. */
+enum {
+    TT_LIGHT_SAMPLING_ON_HIT = 1,
+    TT_LIGHT_SAMPLING_NEXT_EVENT_ESTIMATION = 2,
+    TT_LIGHT_SAMPLING_RESTIR = 3
+};
+
+/* Sentinel for "no texture" (src/includes/scene.glsl:76) and "no hit". */
+#define TT_INVALID_INDEX 0xffffffffu
+
+/* ---------------------------------------------------------------------- */
+/* Reference byte layouts                                                  */
+/* ---------------------------------------------------------------------- */
+
+/* Constants (src/lib.rs:234-250, src/includes/constants.glsl:8-15), 368 B.
+ * Matrices are column-major like glam::Mat4. */
+typedef struct tt_constants {
+    float view[16];
+    float proj[16];
+    float proj_view[16];
+    float inverse_view[16];
+    float inverse_proj[16];
+    float camera_position[4];
+    uint32_t frame_index;
+    uint32_t accumulated_frame_count;
+    uint32_t sample_count;
+    uint32_t bounce_count;
+    uint32_t screen_size[2];
+    uint32_t tonemap;
+    uint32_t light_sampling;
+} tt_constants;
+
+/* ProcessedVertex (src/asset/mod.rs:42-47): f16x2 texture coordinates and the
+ * 32-bit tangent frame of src/asset/normal.rs:7-16. 8 B. */
+typedef struct tt_vertex {
+    uint16_t texture_coordinates[2];
+    uint32_t tangent_frame;
+} tt_vertex;
+
+/* ProcessedMesh (src/asset/mod.rs:49-64), 32 B. */
+typedef struct tt_mesh_record {
+    float center[3];
+    float radius;
+    uint32_t vertex_offset;
+    uint32_t vertex_count;
+    uint32_t index_offset;
+    uint32_t index_count;
+} tt_mesh_record;
+
+/* MaterialConstants (src/asset/material.rs:203-231), 192 B. */
+typedef struct tt_material_constants {
+    float base_color[4];
+    float specular_color[4];
+    float transmission_color[4];
+    float coat_color[4];
+    float fuzz_color[4];
+    float emission_color[4];
+    float geometry_normal[4];
+    float base_weight;
+    float base_metalness;
+    float base_diffuse_roughness;
+    float specular_weight;
+    float specular_roughness;
+    float specular_roughness_anisotropy;
+    float specular_ior;
+    float specular_rotation;
+    float transmission_weight;
+    float transmission_depth;
+    float coat_weight;
+    float coat_roughness;
+    float coat_roughness_anisotropy;
+    float coat_ior;
+    float coat_darkening;
+    float coat_rotation;
+    float fuzz_weight;
+    float fuzz_roughness;
+    float emission_luminance;
+    float geometry_opacity;
+} tt_material_constants;
+
+/* ProcessedMaterial (src/asset/material.rs:268-274): constants, 27 texture
+ * indices (TT_INVALID_INDEX = none) and padding, 304 B. */
+typedef struct tt_material {
+    tt_material_constants constants;
+    uint32_t textures[27];
+    uint32_t padding;
+} tt_material;
+
+/* Instance (src/scene.rs:236-246, src/includes/scene.glsl:155-158), 208 B. */
+typedef struct tt_instance {
+    float transform[16];
+    float inverse_transform[16];
+    float normal_transform[16];
+    uint32_t vertex_offset;
+    uint32_t index_offset;
+    uint32_t material_index;
+    uint32_t mesh_index;
+} tt_instance;
+
+/* EmissiveTriangle (src/asset/mod.rs:66-73, src/includes/scene.glsl:11-16),
+ * 36 B: snorm16 object-space positions, u16 visibility hash, f16 UVs. */
+typedef struct tt_emissive_triangle {
+    int16_t positions[3][3];
+    uint16_t hash;
+    uint16_t texture_coordinates[3][2];
+    uint32_t instance_index;
+} tt_emissive_triangle;
+
+/* Everything `scene::Scene::new` uploads (src/scene.rs:28-186), already
+ * flattened into instances and the emissive-triangle list. */
+typedef struct tt_scene_desc {
+    const int16_t* positions; /* 3 x snorm16 per vertex (BLAS stride 6 B) */
+    uint32_t position_count;
+    const tt_vertex* vertices;
+    uint32_t vertex_count;
+    const uint32_t* indices;
+    uint32_t index_count;
+    const tt_mesh_record* meshes;
+    uint32_t mesh_count;
+    const tt_material* materials;
+    uint32_t material_count;
+    const tt_instance* instances;
+    uint32_t instance_count;
+    const tt_emissive_triangle* emissive_triangles;
+    uint32_t emissive_triangle_count;
+} tt_scene_desc;
+
+/* ---------------------------------------------------------------------- */
+/* Context                                                                 */
+/* ---------------------------------------------------------------------- */
+
+typedef struct tt_context tt_context;
+typedef struct tt_scene tt_scene;
+typedef struct tt_target tt_target;
+
+typedef struct tt_context_desc {
+    int32_t device; /* HIP device ordinal */
+    uint32_t flags; /* reserved, 0 */
+} tt_context_desc;
+
+/* Replaces Context::new (crates/tinytrace_backend/src/lib.rs:171-213). */
+tt_status tt_context_create(const tt_context_desc* desc, tt_context** out);
+void tt_context_destroy(tt_context* context);
+/* Message of the last failed call on this context ("" if none). */
+const char* tt_last_error(const tt_context* context);
+
+/* ---------------------------------------------------------------------- */
+/* Scene upload and BVH build                                              */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces scene::Scene::new (src/scene.rs:28-186) with the driver's
+ * BLAS/TLAS build (resource.rs:804-878, :1014-1083): uploads the arrays and
+ * builds one BLAS per mesh plus a TLAS over the instances. */
+tt_status tt_scene_create(tt_context* context, const tt_scene_desc* desc, tt_scene** out);
+void tt_scene_destroy(tt_scene* scene);
+
+typedef struct tt_scene_info {
+    uint32_t triangle_count;       /* sum over instances */
+    uint32_t unique_triangle_count;/* sum over meshes */
+    uint32_t node_count;           /* BLAS + TLAS nodes */
+    uint32_t max_depth;            /* TLAS depth + deepest BLAS depth */
+    double sah_cost;               /* expected traversal cost, SAH model */
+    double expected_bytes_per_ray; /* SAH-expected node + triangle bytes */
+    double build_ms;               /* host build + upload */
+    uint64_t device_bytes;         /* scene bytes resident in HBM */
+} tt_scene_info;
+tt_status tt_scene_info_get(const tt_scene* scene, tt_scene_info* out);
+
+/* ---------------------------------------------------------------------- */
+/* Render target, render, download                                         */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces create_render_target (src/lib.rs:213-226): RGBA32F. */
+tt_status tt_target_create(tt_context* context, uint32_t width, uint32_t height, tt_target** out);
+void tt_target_destroy(tt_target* target);
+
+enum { TT_RNG_REFERENCE = 0, /* jenkins_hash + xorshift32, random.glsl:4-42 */
+       TT_RNG_PCG = 1 /* PCG-RXS-M-XS 32 (O'Neill 2014) */ };
+
+enum { TT_RENDER_COLLECT_STATS = 1u /* count rays, time stages with events */ };
+
+/* Settings the reference does not have (new with this library). */
+typedef struct tt_render_settings {
+    uint32_t rng;                /* TT_RNG_* */
+    uint32_t seed;               /* mixed into the PCG pixel seed */
+    uint32_t restir_candidates;  /* RIS candidates (light_sampling == RESTIR) */
+    uint32_t restir_temporal;    /* 0 or 1 temporal reuse pass */
+    uint32_t restir_spatial;     /* 0 or 1 spatial reuse pass */
+    uint32_t flags;              /* TT_RENDER_* */
+} tt_render_settings;
+
+/* Integrate + accumulate into `target`, as `Integrator::integrate`
+ * (src/integrate/mod.rs:39-61) dispatching integrate.glsl::main
+ * (src/integrate/integrate.glsl:323-347).  Asynchronous. */
+tt_status tt_render(tt_context* context, const tt_scene* scene, const tt_constants* constants,
+                    tt_target* target);
+tt_status tt_render_ex(tt_context* context, const tt_scene* scene, const tt_constants* constants,
+                       const tt_render_settings* settings, tt_target* target);
+
+/* Blocking download of RGBA32F (width*height*16 bytes), as
+ * Renderer::render_to_texture's download (src/lib.rs:201-210). */
+tt_status tt_target_download(tt_context* context, const tt_target* target, float* rgba,
+                             size_t bytes);
+
+/* Named GPU timings in milliseconds, mirroring Renderer::timings
+ * (src/lib.rs:167-178) and Context::insert_timestamp
+ * (crates/tinytrace_backend/src/timing.rs:61-69).  Names: "integrate" (last
+ * render); with TT_RENDER_COLLECT_STATS also "trace", "shade", "generate",
+ * "accumulate" (sums over the last render's launches). */
+tt_status tt_timing_ms(tt_context* context, const char* name, double* out);
+
+typedef struct tt_render_stats {
+    uint64_t paths;
+    uint64_t extension_rays;
+    uint64_t shadow_rays;
+    uint32_t extend_launches;  /* closest-hit launches over extension rays */
+    uint32_t shadow_launches;  /* closest-hit launches over shadow rays */
+    double extend_ms;          /* sum of their durations (HIP events) */
+    double shadow_ms;
+    double integrate_ms;       /* whole render */
+} tt_render_stats;
+/* Statistics of the last render with TT_RENDER_COLLECT_STATS (synchronises). */
+tt_status tt_render_stats_get(tt_context* context, tt_render_stats* out);
+
+/* ---------------------------------------------------------------------- */
+/* Test entry points                                                       */
+/* ---------------------------------------------------------------------- */
+
+/* One ray: world-space origin/direction and the accepted t interval. */
+typedef struct tt_ray {
+    float origin[3];
+    float tmin;
+    float direction[3];
+    float tmax;
+} tt_ray;
+
+/* Closest hit: instance = TT_INVALID_INDEX on a miss. */
+typedef struct tt_hit {
+    float t;
+    uint32_t instance;  /* rayQueryGetIntersectionInstanceCustomIndexEXT */
+    uint32_t primitive; /* rayQueryGetIntersectionPrimitiveIndexEXT */
+    float barycentrics[2];
+} tt_hit;
+
+enum { TT_TRACE_CULL_BACK_FACES = 1u };
+
+/* Closest-hit traversal of host rays, standing in for the rayQueryEXT loops
+ * of integrate.glsl:112-127 (extension rays, back faces culled) and
+ * :168-185 (shadow rays, no culling).  Synchronous. */
+tt_status tt_trace_closest(tt_context* context, const tt_scene* scene, const tt_ray* rays,
+                           tt_hit* hits, uint32_t count, uint32_t flags);
+/* Same on device-resident arrays (no copies, asynchronous). */
+tt_status tt_trace_closest_device(tt_context* context, const tt_scene* scene,
+                                  const tt_ray* device_rays, tt_hit* device_hits, uint32_t count,
+                                  uint32_t flags);
+
+typedef struct tt_bsdf_query {
+    float wi[3];
+    float wo[3];
+} tt_bsdf_query;
+
+typedef struct tt_bsdf_result {
+    float bsdf[3];
+    float density;
+} tt_bsdf_result;
+
+/* bsdf_prepare + bsdf_evaluate for one material at many (wi, wo) pairs, as
+ * the BdsfSampler kernel (src/material/test.rs:38-83): every query starts
+ * from generator state `rng_state`. */
+tt_status tt_bsdf_evaluate(tt_context* context, const tt_material_constants* material,
+                           const tt_bsdf_query* queries, tt_bsdf_result* results, uint32_t count,
+                           uint32_t rng_state);
+
+/* The device's scalar math (op: 0 sin, 1 cos, 2 exp, 3 log, 4 acos, 5 asin,
+ * 6 pow(x, y), 7 sqrt, 8 x / y, 9 1 / x) on host arrays, so tests can check
+ * that the kernels' arithmetic is bit-identical to the host's.  No reference
+ * counterpart (the reference uses GLSL built-ins). */
+tt_status tt_debug_math(tt_context* context, int op, const float* x, const float* y, float* out,
+                        uint32_t count);
+
+/* ---------------------------------------------------------------------- */
+/* Host side of the library (camera, asset processing)                     */
+/* ---------------------------------------------------------------------- */
+
+/* Camera (src/camera.rs:6-26). */
+typedef struct tt_camera {
+    float position[3];
+    float yaw;
+    float pitch;
+    float z_near;
+    float z_far;
+    float fov;
+} tt_camera;
+
+/* Config (src/config.rs:23-43). */
+typedef struct tt_config {
+    uint32_t bounce_count;
+    uint32_t sample_count;
+    uint32_t tonemap;
+    uint32_t light_sampling;
+} tt_config;
+
+void tt_camera_default(tt_camera* out);
+void tt_config_default(tt_config* out);
+
+/* Constants as Renderer::render_to_target builds them (src/lib.rs:125-142):
+ * view = look_at_rh, proj = perspective_infinite_reverse_rh with
+ * z_axis[3] = 1 (src/camera.rs:42-50), inverses by cofactor expansion. */
+tt_status tt_constants_build(const tt_camera* camera, const tt_config* config, uint32_t width,
+                             uint32_t height, uint32_t frame_index,
+                             uint32_t accumulated_frame_count, tt_constants* out);
+
+/* Asset processing: asset::Scene (src/asset/mod.rs:76-205). */
+typedef struct tt_asset_scene tt_asset_scene;
+
+/* asset::Mesh (src/asset/mod.rs:20-27). */
+typedef struct tt_mesh_desc {
+    const float* positions;           /* 3 per vertex */
+    const float* texture_coordinates; /* 2 per vertex */
+    const uint32_t* indices;          /* optional */
+    const float* normals;             /* optional, 3 per vertex */
+    const float* tangents;            /* required here, 4 per vertex */
+    uint32_t vertex_count;
+    uint32_t index_count;
+} tt_mesh_desc;
+
+/* asset::Material with constant parameters (src/asset/material.rs:8-175). */
+typedef struct tt_material_desc {
+    float base_weight;
+    float base_color[3];
+    float base_metalness;
+    float base_diffuse_roughness;
+    float specular_weight;
+    float specular_color[3];
+    float specular_roughness;
+    float specular_roughness_anisotropy;
+    float specular_ior;
+    float specular_rotation;
+    float transmission_weight;
+    float transmission_color[3];
+    float transmission_depth;
+    float coat_weight;
+    float coat_color[3];
+    float coat_roughness;
+    float coat_roughness_anisotropy;
+    float coat_ior;
+    float coat_darkening;
+    float coat_rotation;
+    float fuzz_weight;
+    float fuzz_color[3];
+    float fuzz_roughness;
+    float emission_luminance;
+    float emission_color[3];
+    float geometry_opacity;
+    float geometry_normal[3];
+} tt_material_desc;
+
+/* Defaults of asset::Material (src/asset/material.rs:35-175). */
+void tt_material_desc_default(tt_material_desc* out);
+
+tt_status tt_asset_scene_create(tt_asset_scene** out);
+void tt_asset_scene_destroy(tt_asset_scene* scene);
+/* Scene::insert_mesh (src/asset/mod.rs:89-161). */
+tt_status tt_asset_insert_mesh(tt_asset_scene* scene, const tt_mesh_desc* mesh, uint32_t* index);
+/* Scene::insert_material (src/asset/material.rs:305-357). */
+tt_status tt_asset_insert_material(tt_asset_scene* scene, const tt_material_desc* material,
+                                   uint32_t* index);
+/* Node tree (src/asset/mod.rs:35-40).  Node 0 is the root; `transform` is
+ * column-major. */
+tt_status tt_asset_add_node(tt_asset_scene* scene, uint32_t parent, const float transform[16],
+                            uint32_t* index);
+tt_status tt_asset_add_model(tt_asset_scene* scene, uint32_t node, uint32_t mesh,
+                             uint32_t material);
+/* Depth-first flatten into instances and emissive triangles, as
+ * scene::Scene::new does (src/scene.rs:32-64).  The arrays stay valid until
+ * the asset scene is modified or destroyed. */
+tt_status tt_asset_flatten(tt_asset_scene* scene, tt_scene_desc* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TINYTRACE_H */

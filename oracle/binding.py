@@ -1,0 +1,148 @@
+"""ctypes wrapper of the oracle (liboracle.so).
+
+ORACLE: test infrastructure.  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg may import this module, as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from tinytrace_amd import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-C", HERE, "-B"], check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY_PATH):
+            build()
+        L = C.CDLL(LIBRARY_PATH)
+        vp, u32, f32 = C.c_void_p, C.c_uint32, C.c_float
+        L.oracle_scene_create.restype = vp
+        L.oracle_scene_create.argtypes = [C.POINTER(_abi.SceneDesc), C.c_int, C.c_int]
+        L.oracle_scene_destroy.argtypes = [vp]
+        L.oracle_render.restype = C.c_int
+        L.oracle_render.argtypes = [vp, C.POINTER(_abi.Constants), u32, u32, vp, C.c_int, vp]
+        L.oracle_render_pixels.restype = C.c_int
+        L.oracle_render_pixels.argtypes = [vp, C.POINTER(_abi.Constants), u32, u32, vp, u32, vp, C.c_int, vp]
+        L.oracle_trace.restype = C.c_int
+        L.oracle_trace.argtypes = [vp, vp, vp, u32, u32, C.c_int]
+        L.oracle_bsdf_evaluate.restype = C.c_int
+        L.oracle_bsdf_evaluate.argtypes = [C.POINTER(_abi.MaterialConstants), vp, vp, u32, u32, C.c_int]
+        L.oracle_bsdf_sample.restype = C.c_int
+        L.oracle_bsdf_sample.argtypes = [C.POINTER(_abi.MaterialConstants), vp, vp, u32, vp, C.c_int]
+        L.oracle_rng.restype = None
+        L.oracle_rng.argtypes = [u32, u32, u32, u32, u32, u32, u32, vp, vp]
+        L.oracle_jenkins_hash.restype = u32
+        L.oracle_jenkins_hash.argtypes = [u32]
+        L.oracle_pcg_hash.restype = u32
+        L.oracle_pcg_hash.argtypes = [u32]
+        L.oracle_uint_to_unit_float.restype = f32
+        L.oracle_uint_to_unit_float.argtypes = [u32]
+        L.oracle_math_f32.restype = None
+        L.oracle_math_f32.argtypes = [C.c_int, vp, vp, vp, u32]
+        L.oracle_camera_direction.restype = None
+        L.oracle_camera_direction.argtypes = [C.POINTER(_abi.Constants), f32, f32, vp]
+        L.oracle_fresnel_dielectric.restype = f32
+        L.oracle_fresnel_dielectric.argtypes = [f32, f32]
+        L.oracle_microfacet_distribution.restype = f32
+        L.oracle_microfacet_distribution.argtypes = [f32, f32, vp]
+        L.oracle_sample_triangle.argtypes = [f32, f32, vp]
+        L.oracle_cosine_hemisphere_sample.argtypes = [f32, f32, vp]
+        L.oracle_decode_tangent_frame.argtypes = [u32, vp]
+        _lib = L
+    return _lib
+
+
+class OracleScene:
+    """The oracle's copy of a flattened scene (float or double)."""
+
+    def __init__(self, asset, precision: int = 32, use_bvh: bool = True):
+        self._asset = asset  # keeps the flattened arrays alive
+        self._desc = asset.flatten()
+        self.precision = precision
+        self.handle = lib().oracle_scene_create(C.byref(self._desc), precision, 1 if use_bvh else 0)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().oracle_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def render(self, constants, rng=0, seed=0, previous=None, threads=0):
+        w, h = constants.screen_size[0], constants.screen_size[1]
+        out = np.zeros((h, w, 4), dtype=np.float32) if previous is None else np.array(previous, dtype=np.float32)
+        rays = np.zeros(2, dtype=np.uint64)
+        status = lib().oracle_render(self.handle, C.byref(constants), rng, seed, out.ctypes.data, threads,
+                                     rays.ctypes.data)
+        if status != 0:
+            raise RuntimeError("oracle_render: unsupported settings")
+        return out, rays
+
+    def render_pixels(self, constants, pixels, rng=0, seed=0, threads=0):
+        pixels = np.ascontiguousarray(pixels, dtype=np.uint32)
+        out = np.zeros((pixels.size, 3), dtype=np.float32)
+        rays = np.zeros(2, dtype=np.uint64)
+        status = lib().oracle_render_pixels(self.handle, C.byref(constants), rng, seed, pixels.ctypes.data,
+                                            pixels.size, out.ctypes.data, threads, rays.ctypes.data)
+        if status != 0:
+            raise RuntimeError("oracle_render_pixels: unsupported settings")
+        return out, rays
+
+    def trace(self, rays, cull, threads=0):
+        rays = np.ascontiguousarray(rays, dtype=_abi.RAY_DTYPE)
+        hits = np.zeros(rays.shape[0], dtype=_abi.HIT_DTYPE)
+        lib().oracle_trace(self.handle, rays.ctypes.data, hits.ctypes.data, rays.shape[0],
+                           _abi.TT_TRACE_CULL_BACK_FACES if cull else 0, threads)
+        return hits
+
+
+def bsdf_evaluate(material_constants, wi, wo, rng_state=0, precision=32):
+    q = np.zeros(len(wi), dtype=_abi.BSDF_QUERY_DTYPE)
+    q["wi"], q["wo"] = wi, wo
+    r = np.zeros(len(wi), dtype=_abi.BSDF_RESULT_DTYPE)
+    lib().oracle_bsdf_evaluate(C.byref(material_constants), q.ctypes.data, r.ctypes.data, len(wi), rng_state,
+                               precision)
+    return r
+
+
+def bsdf_sample(material_constants, wi, rng_states, precision=32):
+    """Per query: wo[3], value[3], density, ok, lobe, eval[3], eval_density."""
+    wi = np.ascontiguousarray(wi, dtype=np.float32)
+    states = np.ascontiguousarray(rng_states, dtype=np.uint32)
+    out = np.zeros((len(wi), 13), dtype=np.float32)
+    lib().oracle_bsdf_sample(C.byref(material_constants), wi.ctypes.data, states.ctypes.data, len(wi),
+                             out.ctypes.data, precision)
+    return out
+
+
+def rng_draws(kind, x, y, width, frame, seed, n):
+    out = np.zeros(n, dtype=np.uint32)
+    state = np.zeros(1, dtype=np.uint32)
+    lib().oracle_rng(kind, x, y, width, frame, seed, n, out.ctypes.data, state.ctypes.data)
+    return int(state[0]), out
+
+
+MATH_OPS = {"sin": 0, "cos": 1, "exp": 2, "log": 3, "acos": 4, "asin": 5, "pow": 6, "sqrt": 7}
+
+
+def math_f32(op, x, y=None):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros_like(x) if y is None else np.ascontiguousarray(y, dtype=np.float32)
+    out = np.zeros_like(x)
+    lib().oracle_math_f32(MATH_OPS[op], x.ctypes.data, y.ctypes.data, out.ctypes.data, x.size)
+    return out

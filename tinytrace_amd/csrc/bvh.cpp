@@ -1,0 +1,430 @@
+// Host binned-SAH builder for the two-level BVH described in bvh.h.
+#include "bvh.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <cmath>
+#include <thread>
+
+namespace tt {
+
+namespace {
+
+struct Box {
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const float* p) {
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], p[a]);
+            hi[a] = std::max(hi[a], p[a]);
+        }
+    }
+    void grow(const Box& b) {
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], b.lo[a]);
+            hi[a] = std::max(hi[a], b.hi[a]);
+        }
+    }
+    bool valid() const { return lo[0] <= hi[0]; }
+    double area() const {
+        if (!valid()) return 0.0;
+        double dx = double(hi[0]) - lo[0], dy = double(hi[1]) - lo[1], dz = double(hi[2]) - lo[2];
+        return 2.0 * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+// Outward padding: 2^-20 of the coordinate magnitude (covers the rounding of
+// the watertight triangle test's vertex translation, a few ulp of |v|).
+Box padded(const Box& b) {
+    Box r = b;
+    for (int a = 0; a < 3; a++) {
+        r.lo[a] = b.lo[a] - (std::fabs(b.lo[a]) * 9.5367431640625e-07f + 1e-30f);
+        r.hi[a] = b.hi[a] + (std::fabs(b.hi[a]) * 9.5367431640625e-07f + 1e-30f);
+    }
+    return r;
+}
+
+struct Prim {
+    Box box;
+    float centroid[3];
+    uint32_t id;
+};
+
+struct Builder {
+    std::vector<Prim>& prims;
+    std::vector<BvhNode>& nodes;  // local node array (indices local)
+    uint32_t max_leaf;
+    uint32_t max_depth = 0;
+    // Leaves in creation order: (first prim position, count) -> ref patch.
+    std::vector<std::pair<uint32_t, uint32_t>> leaves;
+
+    Builder(std::vector<Prim>& p, std::vector<BvhNode>& n, uint32_t ml) : prims(p), nodes(n), max_leaf(ml) {}
+
+    // Returns a reference; leaves are encoded as BVH_LEAF | leaf ordinal for
+    // now and patched by the caller.
+    uint32_t build(uint32_t first, uint32_t count, uint32_t depth, Box& out_box) {
+        max_depth = std::max(max_depth, depth);
+        Box box, cbox;
+        for (uint32_t i = first; i < first + count; i++) {
+            box.grow(prims[i].box);
+            cbox.grow(prims[i].centroid);
+        }
+        out_box = box;
+        if (count == 1 || (count <= max_leaf && count <= 2)) return make_leaf(first, count);
+
+        int axis = -1;
+        uint32_t mid = 0;
+        double leaf_cost = double(count);
+        if (depth < BVH_MEDIAN_DEPTH) {
+            const int BINS = 16;
+            double best = INFINITY;
+            int best_axis = -1, best_bin = 0;
+            for (int a = 0; a < 3; a++) {
+                float ext = cbox.hi[a] - cbox.lo[a];
+                if (!(ext > 0.0f)) continue;
+                float scale = float(BINS) / ext;
+                Box bb[BINS];
+                uint32_t bc[BINS] = {};
+                for (uint32_t i = first; i < first + count; i++) {
+                    int b = int((prims[i].centroid[a] - cbox.lo[a]) * scale);
+                    b = std::min(std::max(b, 0), BINS - 1);
+                    bb[b].grow(prims[i].box);
+                    bc[b]++;
+                }
+                double ra[BINS];
+                uint32_t rc[BINS];
+                Box acc;
+                uint32_t n = 0;
+                for (int b = BINS - 1; b > 0; b--) {
+                    acc.grow(bb[b]);
+                    n += bc[b];
+                    ra[b] = acc.area();
+                    rc[b] = n;
+                }
+                acc = Box();
+                n = 0;
+                for (int b = 0; b < BINS - 1; b++) {
+                    acc.grow(bb[b]);
+                    n += bc[b];
+                    if (n == 0 || rc[b + 1] == 0) continue;
+                    double c = acc.area() * n + ra[b + 1] * rc[b + 1];
+                    if (c < best) {
+                        best = c;
+                        best_axis = a;
+                        best_bin = b;
+                    }
+                }
+            }
+            double parent_area = box.area();
+            double split_cost = parent_area > 0.0 ? 1.0 + best / parent_area : INFINITY;
+            if (best_axis >= 0 && !(count <= max_leaf && leaf_cost <= split_cost)) {
+                float ext = cbox.hi[best_axis] - cbox.lo[best_axis];
+                float scale = float(BINS) / ext;
+                auto it = std::partition(prims.begin() + first, prims.begin() + first + count, [&](const Prim& p) {
+                    int b = int((p.centroid[best_axis] - cbox.lo[best_axis]) * scale);
+                    b = std::min(std::max(b, 0), BINS - 1);
+                    return b <= best_bin;
+                });
+                mid = uint32_t(it - prims.begin());
+                if (mid > first && mid < first + count) axis = best_axis;
+            } else if (count <= max_leaf) {
+                return make_leaf(first, count);
+            }
+        } else if (count <= max_leaf) {
+            return make_leaf(first, count);
+        }
+        if (axis < 0) {
+            // Object median on the widest centroid axis.
+            int a = 0;
+            float ext = -1.0f;
+            for (int k = 0; k < 3; k++)
+                if (cbox.hi[k] - cbox.lo[k] > ext) {
+                    ext = cbox.hi[k] - cbox.lo[k];
+                    a = k;
+                }
+            mid = first + count / 2;
+            std::nth_element(prims.begin() + first, prims.begin() + mid, prims.begin() + first + count,
+                             [&](const Prim& x, const Prim& y) {
+                                 if (x.centroid[a] != y.centroid[a]) return x.centroid[a] < y.centroid[a];
+                                 return x.id < y.id;
+                             });
+        }
+        uint32_t index = uint32_t(nodes.size());
+        nodes.emplace_back();
+        Box lb, rb;
+        uint32_t l = build(first, mid - first, depth + 1, lb);
+        uint32_t r = build(mid, first + count - mid, depth + 1, rb);
+        Box pl = padded(lb), pr = padded(rb);
+        BvhNode& node = nodes[index];
+        node.n0[0] = pl.lo[0];
+        node.n0[1] = pl.hi[0];
+        node.n0[2] = pl.lo[1];
+        node.n0[3] = pl.hi[1];
+        node.n1[0] = pr.lo[0];
+        node.n1[1] = pr.hi[0];
+        node.n1[2] = pr.lo[1];
+        node.n1[3] = pr.hi[1];
+        node.n2[0] = pl.lo[2];
+        node.n2[1] = pl.hi[2];
+        node.n2[2] = pr.lo[2];
+        node.n2[3] = pr.hi[2];
+        node.n3[0] = l;
+        node.n3[1] = r;
+        node.n3[2] = 0;
+        node.n3[3] = 0;
+        return index;
+    }
+
+    uint32_t make_leaf(uint32_t first, uint32_t count) {
+        uint32_t ordinal = uint32_t(leaves.size());
+        leaves.emplace_back(first, count);
+        return BVH_LEAF | ordinal;
+    }
+};
+
+// SAH statistics of a subtree: expected visits weighted by surface area.
+struct Stats {
+    double cost = 0.0;
+    double bytes = 0.0;
+};
+
+template <typename LeafFn>
+void subtree_stats(const std::vector<BvhNode>& nodes, uint32_t ref, double p, LeafFn&& leaf, Stats& s) {
+    // p = probability that this reference's box is hit (relative to root).
+    if (ref & BVH_LEAF) {
+        leaf(ref, p, s);
+        return;
+    }
+    const BvhNode& n = nodes[ref];
+    s.cost += p * 1.0;
+    s.bytes += p * double(sizeof(BvhNode));
+    auto area = [](float lx, float hx, float ly, float hy, float lz, float hz) {
+        double dx = double(hx) - lx, dy = double(hy) - ly, dz = double(hz) - lz;
+        return 2.0 * (dx * dy + dy * dz + dz * dx);
+    };
+    double a0 = area(n.n0[0], n.n0[1], n.n0[2], n.n0[3], n.n2[0], n.n2[1]);
+    double a1 = area(n.n1[0], n.n1[1], n.n1[2], n.n1[3], n.n2[2], n.n2[3]);
+    double parent = std::max(a0, a1);
+    // Parent area from the union of the children.
+    double ux = double(std::max(n.n0[1], n.n1[1])) - std::min(n.n0[0], n.n1[0]);
+    double uy = double(std::max(n.n0[3], n.n1[3])) - std::min(n.n0[2], n.n1[2]);
+    double uz = double(std::max(n.n2[1], n.n2[3])) - std::min(n.n2[0], n.n2[2]);
+    parent = std::max(parent, 2.0 * (ux * uy + uy * uz + uz * ux));
+    double q0 = parent > 0 ? a0 / parent : 1.0, q1 = parent > 0 ? a1 / parent : 1.0;
+    subtree_stats(nodes, n.n3[0], p * q0, leaf, s);
+    subtree_stats(nodes, n.n3[1], p * q1, leaf, s);
+}
+
+struct Blas {
+    std::vector<BvhNode> nodes;
+    std::vector<BvhTri> tris;
+    uint32_t root = BVH_LEAF;  // local reference
+    Box box;                   // object-space bounds (unpadded)
+    uint32_t depth = 0;
+    Stats stats;
+    bool empty = true;
+};
+
+void build_blas(const tt_scene_desc& d, uint32_t mesh, Blas& out) {
+    const tt_mesh_record& m = d.meshes[mesh];
+    uint32_t tri_count = m.index_count / 3;
+    if (tri_count == 0) return;
+    std::vector<Prim> prims(tri_count);
+    std::vector<BvhTri> raw(tri_count);
+    for (uint32_t t = 0; t < tri_count; t++) {
+        float v[3][3];
+        for (int c = 0; c < 3; c++) {
+            uint32_t vi = m.vertex_offset + d.indices[m.index_offset + 3 * t + c];
+            for (int a = 0; a < 3; a++) v[c][a] = float(d.positions[3 * size_t(vi) + a]) / 32767.0f;
+        }
+        Prim& p = prims[t];
+        for (int c = 0; c < 3; c++) p.box.grow(v[c]);
+        for (int a = 0; a < 3; a++) p.centroid[a] = (p.box.lo[a] + p.box.hi[a]) * 0.5f;
+        p.id = t;
+        BvhTri& bt = raw[t];
+        for (int a = 0; a < 3; a++) {
+            bt.v0[a] = v[0][a];
+            bt.v1[a] = v[1][a];
+            bt.v2[a] = v[2][a];
+        }
+        uint32_t prim_bits = t;
+        std::memcpy(&bt.v0[3], &prim_bits, 4);
+        bt.v1[3] = 0.0f;
+        bt.v2[3] = 0.0f;
+    }
+    Builder b(prims, out.nodes, 4);
+    uint32_t root = b.build(0, tri_count, 0, out.box);
+    // Lay out triangles in leaf order and patch leaf references.
+    std::vector<uint32_t> leaf_ref(b.leaves.size());
+    for (size_t l = 0; l < b.leaves.size(); l++) {
+        uint32_t first = uint32_t(out.tris.size());
+        uint32_t count = b.leaves[l].second;
+        for (uint32_t k = 0; k < count; k++) out.tris.push_back(raw[prims[b.leaves[l].first + k].id]);
+        leaf_ref[l] = BVH_LEAF | ((count - 1) << 27) | first;
+    }
+    auto patch = [&](uint32_t r) { return (r & BVH_LEAF) ? leaf_ref[r & 0x7fffffffu] : r; };
+    for (auto& n : out.nodes) {
+        n.n3[0] = patch(n.n3[0]);
+        n.n3[1] = patch(n.n3[1]);
+    }
+    out.root = patch(root);
+    out.depth = b.max_depth;
+    out.empty = false;
+    // Expected bytes/cost relative to the BLAS root box.
+    subtree_stats(
+        out.nodes, out.root, 1.0,
+        [&](uint32_t ref, double p, Stats& s) {
+            double n = double(((ref >> 27) & 0xfu) + 1);
+            s.cost += p * n;
+            s.bytes += p * n * double(sizeof(BvhTri));
+        },
+        out.stats);
+}
+
+}  // namespace
+
+bool build_bvh(const tt_scene_desc& d, BvhBuildResult& out) {
+    out = BvhBuildResult();
+    uint64_t total_tris = 0;
+    for (uint32_t m = 0; m < d.mesh_count; m++) total_tris += d.meshes[m].index_count / 3;
+    if (total_tris >= (1ull << 27)) {
+        out.error = "too many triangles for the 27-bit leaf encoding";
+        return false;
+    }
+    std::vector<Blas> blas(d.mesh_count);
+    {
+        std::atomic<uint32_t> next{0};
+        unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (unsigned t = 0; t < nt; t++)
+            pool.emplace_back([&] {
+                for (;;) {
+                    uint32_t m = next.fetch_add(1);
+                    if (m >= d.mesh_count) break;
+                    build_blas(d, m, blas[m]);
+                }
+            });
+        for (auto& t : pool) t.join();
+    }
+
+    // Instance world boxes: 8 transformed corners of the object box in
+    // double, padded by 1e-5 of (|lo| + |hi| + extent).
+    std::vector<Prim> prims;
+    std::vector<Box> world(d.instance_count);
+    for (uint32_t i = 0; i < d.instance_count; i++) {
+        const tt_instance& in = d.instances[i];
+        if (in.mesh_index >= d.mesh_count) {
+            out.error = "instance refers to a missing mesh";
+            return false;
+        }
+        const Blas& b = blas[in.mesh_index];
+        if (b.empty) continue;
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int c = 0; c < 8; c++) {
+            double p[3] = {(c & 1) ? b.box.hi[0] : b.box.lo[0], (c & 2) ? b.box.hi[1] : b.box.lo[1],
+                           (c & 4) ? b.box.hi[2] : b.box.lo[2]};
+            for (int r = 0; r < 3; r++) {
+                double w = double(in.transform[12 + r]);
+                for (int k = 0; k < 3; k++) w += double(in.transform[k * 4 + r]) * p[k];
+                lo[r] = std::min(lo[r], w);
+                hi[r] = std::max(hi[r], w);
+            }
+        }
+        double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+        Prim p;
+        for (int r = 0; r < 3; r++) {
+            double pad = (std::fabs(lo[r]) + std::fabs(hi[r]) + ext) * 1e-5;
+            p.box.lo[r] = float(lo[r] - pad);
+            p.box.hi[r] = float(hi[r] + pad);
+            p.centroid[r] = float(0.5 * (lo[r] + hi[r]));
+        }
+        p.id = i;
+        world[i] = p.box;
+        prims.push_back(p);
+    }
+
+    // TLAS (one instance per leaf).
+    std::vector<BvhNode> tlas_nodes;
+    uint32_t tlas_root = BVH_LEAF | 0x7fffffffu;
+    uint32_t tlas_depth = 0;
+    Box tlas_box;
+    if (!prims.empty()) {
+        Builder b(prims, tlas_nodes, 1);
+        uint32_t root = b.build(0, uint32_t(prims.size()), 0, tlas_box);
+        std::vector<uint32_t> leaf_ref(b.leaves.size());
+        for (size_t l = 0; l < b.leaves.size(); l++) leaf_ref[l] = BVH_LEAF | prims[b.leaves[l].first].id;
+        auto patch = [&](uint32_t r) { return (r & BVH_LEAF) ? leaf_ref[r & 0x7fffffffu] : r; };
+        for (auto& n : tlas_nodes) {
+            n.n3[0] = patch(n.n3[0]);
+            n.n3[1] = patch(n.n3[1]);
+        }
+        tlas_root = patch(root);
+        tlas_depth = b.max_depth;
+        out.empty = false;
+    }
+
+    // Concatenate: TLAS nodes, then each BLAS (references rebased).
+    out.nodes = tlas_nodes;
+    std::vector<uint32_t> blas_root(d.mesh_count, BVH_LEAF);
+    uint32_t deepest = 0;
+    for (uint32_t m = 0; m < d.mesh_count; m++) {
+        Blas& b = blas[m];
+        if (b.empty) continue;
+        uint32_t node_base = uint32_t(out.nodes.size());
+        uint32_t tri_base = uint32_t(out.tris.size());
+        auto rebase = [&](uint32_t r) {
+            if (r & BVH_LEAF) return BVH_LEAF | (r & (0xfu << 27)) | ((r & 0x7ffffffu) + tri_base);
+            return r + node_base;
+        };
+        for (auto n : b.nodes) {
+            n.n3[0] = rebase(n.n3[0]);
+            n.n3[1] = rebase(n.n3[1]);
+            out.nodes.push_back(n);
+        }
+        out.tris.insert(out.tris.end(), b.tris.begin(), b.tris.end());
+        blas_root[m] = rebase(b.root);
+        deepest = std::max(deepest, b.depth);
+    }
+    out.tlas_root = tlas_root;
+    out.max_depth = tlas_depth + deepest;
+    if (out.max_depth + 2 > BVH_STACK_LIMIT) {
+        out.error = "BVH too deep for the traversal stack";
+        return false;
+    }
+
+    out.instances.resize(d.instance_count);
+    for (uint32_t i = 0; i < d.instance_count; i++) {
+        const tt_instance& in = d.instances[i];
+        BvhInstance& bi = out.instances[i];
+        for (int r = 0; r < 3; r++) {
+            bi.c0[r] = in.inverse_transform[0 + r];
+            bi.c1[r] = in.inverse_transform[4 + r];
+            bi.c2[r] = in.inverse_transform[8 + r];
+            bi.c3[r] = in.inverse_transform[12 + r];
+        }
+        uint32_t root = blas[in.mesh_index].empty ? (BVH_LEAF | 0x7fffffffu) : blas_root[in.mesh_index];
+        std::memcpy(&bi.c0[3], &root, 4);
+        bi.c1[3] = bi.c2[3] = bi.c3[3] = 0.0f;
+    }
+
+    // Two-level SAH expectation: TLAS nodes, then per instance leaf the
+    // instance record plus the BLAS expectation.
+    Stats s;
+    if (!out.empty) {
+        subtree_stats(
+            out.nodes, tlas_root, 1.0,
+            [&](uint32_t ref, double p, Stats& st) {
+                uint32_t i = ref & 0x7fffffffu;
+                const Blas& b = blas[d.instances[i].mesh_index];
+                st.cost += p * (1.0 + b.stats.cost);
+                st.bytes += p * (double(sizeof(BvhInstance)) + b.stats.bytes);
+            },
+            s);
+    }
+    out.sah_cost = s.cost;
+    out.expected_bytes = s.bytes;
+    return true;
+}
+
+}  // namespace tt

@@ -1,0 +1,75 @@
+// Two-level BVH in the layout the gfx950 traversal kernel reads.
+//
+// Stands in for the driver's acceleration-structure build that the reference
+// requests per mesh (BLAS, crates/tinytrace_backend/src/resource.rs:804-878,
+// called from src/scene.rs:130-162) and over instances (TLAS,
+// resource.rs:1014-1083, called from src/scene.rs:164-171).
+//
+// HBM layout
+//   nodes : BvhNode[]  64 B, both children's boxes in the parent so one
+//                      16-B-aligned fetch of 4 x dwordx4 tests two boxes
+//                      (Aila & Laine 2009).  TLAS nodes first, then every
+//                      BLAS.
+//   tris  : BvhTri[]   48 B, object-space vertices dequantized from snorm16
+//                      (v / 32767), in leaf order; v0.w holds the mesh-local
+//                      primitive index.
+//   insts : BvhInstance[] 64 B, the instance's f32 inverse_transform columns
+//                      (xyz) and, in c0.w, the BLAS root reference.
+// Node references (u32): bit 31 clear = inner node index; set = leaf.  TLAS
+// leaves hold one instance (bits 0..30); BLAS leaves hold a triangle range:
+// first slot in bits 0..26, count-1 in bits 27..30 (count <= 16).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tinytrace.h"
+
+namespace tt {
+
+struct BvhNode {
+    float n0[4];  // c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y
+    float n1[4];  // c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y
+    float n2[4];  // c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z
+    uint32_t n3[4];  // ref0, ref1, 0, 0
+};
+static_assert(sizeof(BvhNode) == 64, "node size");
+
+struct BvhTri {
+    float v0[4];
+    float v1[4];
+    float v2[4];
+};
+static_assert(sizeof(BvhTri) == 48, "triangle size");
+
+struct BvhInstance {
+    float c0[4];
+    float c1[4];
+    float c2[4];
+    float c3[4];
+};
+static_assert(sizeof(BvhInstance) == 64, "instance size");
+
+constexpr uint32_t BVH_LEAF = 0x80000000u;
+constexpr uint32_t BVH_MAX_LEAF = 16;
+// Depth at which splits switch to object medians; bounds every tree's depth
+// by BVH_MEDIAN_DEPTH + 27 and so the traversal stack.
+constexpr uint32_t BVH_MEDIAN_DEPTH = 20;
+constexpr uint32_t BVH_STACK_LIMIT = 96;
+
+struct BvhBuildResult {
+    std::vector<BvhNode> nodes;
+    std::vector<BvhTri> tris;
+    std::vector<BvhInstance> instances;
+    uint32_t tlas_root = BVH_LEAF;  // BVH_LEAF | 0x7fffffff = empty scene
+    bool empty = true;
+    uint32_t max_depth = 0;
+    double sah_cost = 0.0;
+    double expected_bytes = 0.0;
+    std::string error;
+};
+
+bool build_bvh(const tt_scene_desc& desc, BvhBuildResult& out);
+
+}  // namespace tt

@@ -1,0 +1,272 @@
+// Device view of a resident scene and the closest-hit traversal.
+//
+// Traversal replaces the rayQueryEXT loops of the reference
+// (src/integrate/integrate.glsl:112-127 for extension rays with back-face
+// culling, :168-185 for shadow rays without).  Per ray it walks the
+// two-level BVH of bvh.h with a per-lane stack whose top TRACE_LDS_STACK
+// entries live in LDS (entry e of lane t at lds[e * blockDim + t], so a
+// wave's pushes hit 64 consecutive banks) and whose rare deeper entries spill
+// to a per-lane global region.  Entering an instance pushes a sentinel and
+// switches to the object-space ray (the instance's f32 inverse_transform
+// applied to origin and direction); popping the sentinel switches back.
+//
+// Triangle test: the watertight test of Woop, Benthin & Wald (JCGT 2013)
+// with its double-precision fallback, in object space on the dequantized
+// snorm16 vertices; culling keeps U, V, W >= 0 (the CCW normal faces the
+// origin); hits need tmin <= t <= tmax; the closest hit is the minimum of
+// (t, instance, primitive).  Box tests are conservative: boxes are padded at
+// build time, the slab origin is shifted by e = 64*eps*max|o| (+1e-30) and
+// slab distances widen by a relative 1e-6, so no box rejects a triangle the
+// exact test accepts and the result equals brute force over all triangles.
+#pragma once
+
+#include "device_math.h"
+
+namespace ttd {
+
+constexpr uint32_t LEAF = 0x80000000u;
+constexpr uint32_t SENTINEL = 0xffffffffu;
+constexpr int TRACE_BLOCK = 128;
+constexpr int TRACE_LDS_STACK = 32;
+constexpr int TRACE_SPILL_STACK = 64;  // TRACE_LDS_STACK + this >= BVH_STACK_LIMIT
+
+struct SceneView {
+    const float4* nodes;      // BvhNode, 4 x float4
+    const float4* tris;       // BvhTri, 3 x float4
+    const float4* bvh_insts;  // BvhInstance, 4 x float4
+    uint32_t tlas_root;
+    uint32_t emissive_count;
+    const float* instances;   // tt_instance, 52 floats (208 B)
+    const float4* materials;  // tt_material, 19 x float4 (304 B)
+    const int16_t* positions; // 3 per vertex
+    const uint2* vertices;    // tt_vertex: (f16x2 uv, tangent frame)
+    const uint32_t* indices;
+    const uint32_t* emissive; // tt_emissive_triangle, 9 x u32 (36 B)
+};
+
+struct HitRecord {
+    float t;
+    uint32_t instance, primitive;
+    float b1, b2;
+};
+
+struct TraceRay {
+    f3 org, inv, olo, ohi;  // slab origins shifted by -/+ e
+    // Woop setup
+    int kx, ky, kz;
+    float sx, sy, sz;
+    f3 o, d;
+};
+
+TT_DEV float component(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+TT_DEV void slab_setup(TraceRay& r, f3 o, f3 d) {
+    r.org = o;
+    r.inv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    float mo = fmaxf(fmaxf(abs_(o.x), abs_(o.y)), abs_(o.z));
+    float e = mo * (64.0f * 1.1920929e-7f) + 1e-30f;
+    r.olo = f3(o.x + e, o.y + e, o.z + e);
+    r.ohi = f3(o.x - e, o.y - e, o.z - e);
+}
+
+TT_DEV void woop_setup(TraceRay& r, f3 o, f3 d) {
+    r.o = o;
+    r.d = d;
+    float ax = abs_(d.x), ay = abs_(d.y), az = abs_(d.z);
+    int kz = (ax >= ay) ? (ax >= az ? 0 : 2) : (ay >= az ? 1 : 2);
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    float dz = component(d, kz);
+    if (dz < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    r.sx = component(d, kx) / dz;
+    r.sy = component(d, ky) / dz;
+    r.sz = 1.0f / dz;
+}
+
+// Conservative slab test of one box against [tmin, tmax]; returns the raw
+// entry distance in `tn` for ordering.
+TT_DEV bool box_hit(const TraceRay& r, float lx, float hx, float ly, float hy, float lz, float hz, float tmin,
+                    float tmax, float& tn) {
+    float t0x = (lx - r.olo.x) * r.inv.x, t1x = (hx - r.ohi.x) * r.inv.x;
+    float t0y = (ly - r.olo.y) * r.inv.y, t1y = (hy - r.ohi.y) * r.inv.y;
+    float t0z = (lz - r.olo.z) * r.inv.z, t1z = (hz - r.ohi.z) * r.inv.z;
+    float n = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    float f = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    tn = n;
+    float nc = n - abs_(n) * 1e-6f;
+    float fc = f + abs_(f) * 1e-6f;
+    return fmaxf(nc, tmin) <= fminf(fc, tmax);
+}
+
+TT_DEV bool closer(float t, uint32_t inst, uint32_t prim, const HitRecord& b) {
+    if (t != b.t) return t < b.t;
+    if (inst != b.instance) return inst < b.instance;
+    return prim < b.primitive;
+}
+
+// Woop et al. 2013; U, V, W and det of an accepted hit are kept so the
+// barycentrics are divided once at the end.
+template <bool CULL>
+TT_DEV bool triangle_hit(const TraceRay& r, f3 v0, f3 v1, f3 v2, float tmin, float tmax, float& t_out, float& V_out,
+                         float& W_out, float& det_out) {
+    f3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+    float Akz = component(A, r.kz), Bkz = component(B, r.kz), Ckz = component(C, r.kz);
+    float Ax = component(A, r.kx) - r.sx * Akz;
+    float Ay = component(A, r.ky) - r.sy * Akz;
+    float Bx = component(B, r.kx) - r.sx * Bkz;
+    float By = component(B, r.ky) - r.sy * Bkz;
+    float Cx = component(C, r.kx) - r.sx * Ckz;
+    float Cy = component(C, r.ky) - r.sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if (CULL) {
+        if (U < 0.0f || V < 0.0f || W < 0.0f) return false;
+    } else {
+        if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    }
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float Az = r.sz * Akz, Bz = r.sz * Bkz, Cz = r.sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    float t = T / det;
+    if (!(t >= tmin && t <= tmax)) return false;
+    t_out = t;
+    V_out = V;
+    W_out = W;
+    det_out = det;
+    return true;
+}
+
+struct Stack {
+    uint32_t* lds;    // entry e at lds[e * TRACE_BLOCK]
+    uint32_t* spill;  // entry e at spill[(e - TRACE_LDS_STACK) * spill_stride]
+    uint32_t spill_stride;
+    uint32_t sp;
+    TT_DEV void push(uint32_t v) {
+        if (sp < uint32_t(TRACE_LDS_STACK))
+            lds[sp * TRACE_BLOCK] = v;
+        else
+            spill[(sp - TRACE_LDS_STACK) * spill_stride] = v;
+        sp++;
+    }
+    TT_DEV uint32_t pop() {
+        sp--;
+        if (sp < uint32_t(TRACE_LDS_STACK)) return lds[sp * TRACE_BLOCK];
+        return spill[(sp - TRACE_LDS_STACK) * spill_stride];
+    }
+};
+
+// Closest hit of a world-space ray.  Returns false on a miss.
+template <bool CULL>
+TT_DEV bool trace_closest(const SceneView& s, f3 org, f3 dir, float tmin, float tmax, Stack& stack, HitRecord& best) {
+    best.t = tmax;
+    best.instance = 0xffffffffu;
+    best.primitive = 0xffffffffu;
+    best.b1 = 0.0f;
+    best.b2 = 0.0f;
+    if (s.tlas_root == SENTINEL) return false;
+    bool found = false;
+    float bV = 0.0f, bW = 0.0f, bdet = 1.0f;
+    TraceRay r;
+    slab_setup(r, org, dir);
+    bool in_blas = false;
+    uint32_t instance = 0;
+    uint32_t cur = s.tlas_root;
+    stack.sp = 0;
+    for (;;) {
+        if (!(cur & LEAF)) {
+            const float4* n = s.nodes + 4 * size_t(cur);
+            float4 n0 = n[0], n1 = n[1], n2 = n[2];
+            uint4 n3 = *reinterpret_cast<const uint4*>(n + 3);
+            float limit = found ? best.t : tmax;
+            float tn0, tn1;
+            bool h0 = box_hit(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, tmin, limit, tn0);
+            bool h1 = box_hit(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, tmin, limit, tn1);
+            if (h0 && h1) {
+                bool first0 = tn0 <= tn1;
+                stack.push(first0 ? n3.y : n3.x);
+                cur = first0 ? n3.x : n3.y;
+                continue;
+            }
+            if (h0) {
+                cur = n3.x;
+                continue;
+            }
+            if (h1) {
+                cur = n3.y;
+                continue;
+            }
+        } else if (!in_blas) {
+            instance = cur & 0x7fffffffu;
+            const float4* ip = s.bvh_insts + 4 * size_t(instance);
+            float4 c0 = ip[0], c1 = ip[1], c2 = ip[2], c3 = ip[3];
+            uint32_t root = __float_as_uint(c0.w);
+            if (root != SENTINEL) {
+                f3 o(c0.x * org.x + c1.x * org.y + c2.x * org.z + c3.x, c0.y * org.x + c1.y * org.y + c2.y * org.z + c3.y,
+                     c0.z * org.x + c1.z * org.y + c2.z * org.z + c3.z);
+                f3 d(c0.x * dir.x + c1.x * dir.y + c2.x * dir.z, c0.y * dir.x + c1.y * dir.y + c2.y * dir.z,
+                     c0.z * dir.x + c1.z * dir.y + c2.z * dir.z);
+                slab_setup(r, o, d);
+                woop_setup(r, o, d);
+                stack.push(SENTINEL);
+                in_blas = true;
+                cur = root;
+                continue;
+            }
+        } else {
+            uint32_t first = cur & 0x7ffffffu;
+            uint32_t count = ((cur >> 27) & 0xfu) + 1u;
+            for (uint32_t k = 0; k < count; k++) {
+                const float4* tp = s.tris + 3 * size_t(first + k);
+                float4 a = tp[0], b = tp[1], c = tp[2];
+                float t, V, W, det;
+                float limit = found ? best.t : tmax;
+                if (triangle_hit<CULL>(r, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(c.x, c.y, c.z), tmin, limit, t, V, W,
+                                       det)) {
+                    uint32_t prim = __float_as_uint(a.w);
+                    if (!found || closer(t, instance, prim, best)) {
+                        found = true;
+                        best.t = t;
+                        best.instance = instance;
+                        best.primitive = prim;
+                        bV = V;
+                        bW = W;
+                        bdet = det;
+                    }
+                }
+            }
+        }
+        // Pop the next reference; a sentinel returns to world space.
+        for (;;) {
+            if (stack.sp == 0) {
+                if (found) {
+                    best.b1 = bV / bdet;
+                    best.b2 = bW / bdet;
+                }
+                return found;
+            }
+            cur = stack.pop();
+            if (cur == SENTINEL) {
+                in_blas = false;
+                slab_setup(r, org, dir);
+                continue;
+            }
+            break;
+        }
+    }
+}
+
+}  // namespace ttd

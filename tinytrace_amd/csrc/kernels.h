@@ -1,0 +1,71 @@
+// Host-side launchers of the gfx950 kernels (defined in kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tinytrace.h"
+
+namespace tt {
+
+struct DeviceScene {
+    const void* nodes;
+    const void* tris;
+    const void* bvh_insts;
+    uint32_t tlas_root;
+    uint32_t emissive_count;
+    const void* instances;
+    const void* materials;
+    const void* positions;
+    const void* vertices;
+    const void* indices;
+    const void* emissive;
+};
+
+// Per-render camera/integrator constants passed by value.
+struct FrameArgs {
+    float inverse_view[16];
+    float inverse_proj[16];
+    float camera_position[3];
+    uint32_t width, height;
+    uint32_t sample_count, bounce_count;
+    uint32_t frame_index, accumulated_frame_count;
+    uint32_t light_sampling;
+    uint32_t rng, seed;
+};
+
+// Wavefront buffers; `paths` = width * height * sample_count.
+struct PathBuffers {
+    float4* throughput;   // [paths]
+    float4* radiance;     // [paths]
+    uint32_t* rng;        // [paths]
+    float4* queue[2][2];  // [ping-pong][origin|direction], [paths] each
+    float4* hits;         // [paths]
+    float4* shadow[4];    // origin, direction, direct, emission: [paths] each
+    uint32_t* counters;   // see kernels.hip
+    uint32_t* spill;      // traversal stack spill
+    uint32_t max_bounces;
+};
+
+struct TraceLaunch {
+    int grid;  // persistent trace workgroups
+};
+
+int trace_grid(int cu_count);
+size_t spill_words(int grid);
+uint32_t counter_words(uint32_t max_bounces);
+
+hipError_t launch_zero(hipStream_t s, uint32_t* p, uint32_t words);
+hipError_t launch_trace_rays(hipStream_t s, const DeviceScene& scene, const tt_ray* rays, tt_hit* hits, uint32_t count,
+                             bool cull, uint32_t* work, uint32_t* spill, int grid);
+hipError_t launch_generate(hipStream_t s, const FrameArgs& f, const PathBuffers& b, uint32_t paths);
+hipError_t launch_extend(hipStream_t s, const DeviceScene& scene, const PathBuffers& b, uint32_t bounce, int grid);
+hipError_t launch_shade(hipStream_t s, const DeviceScene& scene, const FrameArgs& f, const PathBuffers& b,
+                        uint32_t bounce, uint32_t paths);
+hipError_t launch_shadow(hipStream_t s, const DeviceScene& scene, const PathBuffers& b, uint32_t bounce, int grid);
+hipError_t launch_accumulate(hipStream_t s, const FrameArgs& f, const PathBuffers& b, float4* target);
+hipError_t launch_bsdf_evaluate(hipStream_t s, const tt_material_constants* material, const tt_bsdf_query* q,
+                                tt_bsdf_result* r, uint32_t count, uint32_t rng_state);
+hipError_t launch_math(hipStream_t s, int op, const float* x, const float* y, float* out, uint32_t count);
+
+}  // namespace tt

@@ -1,0 +1,540 @@
+// gfx950 kernels of the wavefront path integrator.
+//
+// The reference runs one megakernel thread per pixel
+// (src/integrate/integrate.glsl:323-347).  Here one render is a sequence of
+// launches on one stream, per bounce b:
+//   extend  : closest hit of every ray in queue[b % 2] (persistent waves
+//             fetch 64 rays at a time from an atomic cursor)
+//   shade   : one lane per hit: next_path_segment (integrate.glsl:270-321)
+//             up to the two ray queries, appending the continuation ray to
+//             queue[(b+1) % 2] and the shadow ray to the shadow queue with
+//             wave-ballot compaction (one atomic per wave)
+//   shadow  : closest hit of every shadow ray, the reference's visibility
+//             rule (integrate.glsl:175-185) and the radiance update
+// framed by generate (camera rays, integrate.glsl:323-333) and accumulate
+// (sample average + progressive accumulation, integrate.glsl:338-346).
+//
+// Path state is SoA in HBM: throughput/radiance float4, generator u32,
+// queues float4 (origin.xyz + path index bits, direction.xyz); every access
+// is one 16-B load or store per lane, consecutive across the wave.
+#include "device_bsdf.h"
+#include "device_scene.h"
+#include "kernels.h"
+
+using namespace ttd;
+
+namespace {
+
+// Counter words.
+constexpr uint32_t C_EXT = 0;             // [b]: rays in queue consumed at bounce b
+constexpr uint32_t C_SHADOW = 32;         // [b]: shadow rays emitted at bounce b
+constexpr uint32_t C_WORK_EXT = 64;       // [b]: persistent cursor, extend b
+constexpr uint32_t C_WORK_SHADOW = 96;    // [b]: persistent cursor, shadow b
+constexpr uint32_t C_WORDS = 128;
+
+TT_DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+TT_DEV uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+// Wave-level stream compaction: every active lane calls this; lanes with
+// `pred` get consecutive slots from one atomic add per wave.
+TT_DEV uint32_t wave_append(uint32_t* counter, bool pred) {
+    uint64_t active = __ballot(1);
+    uint64_t mask = __ballot(pred);
+    uint32_t leader = uint32_t(__ffsll((unsigned long long)active) - 1);
+    uint32_t total = uint32_t(__popcll(mask));
+    uint32_t base = 0;
+    if (lane_id() == leader && total != 0) base = atomicAdd(counter, total);
+    base = __shfl(base, int(leader));
+    return base + lanes_below(mask);
+}
+
+__device__ __host__ inline SceneView view(const tt::DeviceScene& d) {
+    SceneView s;
+    s.nodes = static_cast<const float4*>(d.nodes);
+    s.tris = static_cast<const float4*>(d.tris);
+    s.bvh_insts = static_cast<const float4*>(d.bvh_insts);
+    s.tlas_root = d.tlas_root;
+    s.emissive_count = d.emissive_count;
+    s.instances = static_cast<const float*>(d.instances);
+    s.materials = static_cast<const float4*>(d.materials);
+    s.positions = static_cast<const int16_t*>(d.positions);
+    s.vertices = static_cast<const uint2*>(d.vertices);
+    s.indices = static_cast<const uint32_t*>(d.indices);
+    s.emissive = static_cast<const uint32_t*>(d.emissive);
+    return s;
+}
+
+TT_DEV Stack make_stack(uint32_t* lds, uint32_t* spill) {
+    Stack st;
+    st.lds = lds + threadIdx.x;
+    uint32_t stride = gridDim.x * TRACE_BLOCK;
+    st.spill = spill + blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    st.spill_stride = stride;
+    st.sp = 0;
+    return st;
+}
+
+// Persistent-wave cursor: lane 0 takes the next 64 items.
+TT_DEV uint32_t next_batch(uint32_t* cursor) {
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(cursor, 64u);
+    return __shfl(base, 0);
+}
+
+// ------------------------------------------------------------ test rays --
+template <bool CULL>
+__global__ __launch_bounds__(TRACE_BLOCK) void trace_rays_kernel(tt::DeviceScene d, const tt_ray* rays, tt_hit* hits,
+                                                                  uint32_t count, uint32_t* work, uint32_t* spill) {
+    __shared__ uint32_t lds[TRACE_LDS_STACK * TRACE_BLOCK];
+    SceneView s = view(d);
+    Stack st = make_stack(lds, spill);
+    for (;;) {
+        uint32_t base = next_batch(work);
+        if (base >= count) break;
+        uint32_t i = base + lane_id();
+        if (i < count) {
+            tt_ray r = rays[i];
+            HitRecord h;
+            bool found = trace_closest<CULL>(s, f3(r.origin[0], r.origin[1], r.origin[2]),
+                                             f3(r.direction[0], r.direction[1], r.direction[2]), r.tmin, r.tmax, st, h);
+            tt_hit o;
+            o.t = found ? h.t : r.tmax;
+            o.instance = found ? h.instance : 0xffffffffu;
+            o.primitive = found ? h.primitive : 0xffffffffu;
+            o.barycentrics[0] = h.b1;
+            o.barycentrics[1] = h.b2;
+            hits[i] = o;
+        }
+    }
+}
+
+// -------------------------------------------------------------- generate --
+// integrate.glsl:129-132 (pixel_ndc), :17-20 (camera_ray_direction),
+// :323-333 (seed, per-sample generator copy).  Path p = pixel * spp + sample.
+__global__ __launch_bounds__(256) void generate_kernel(tt::FrameArgs f, tt::PathBuffers b, uint32_t paths) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= paths) return;
+    uint32_t pixel = p / f.sample_count, sample = p % f.sample_count;
+    uint32_t x = pixel % f.width, y = pixel / f.width;
+    Generator g = init_generator(pixel, f.frame_index, f.rng, f.seed);
+    for (uint32_t k = 0; k < 2u * sample; k++) next_u32(g);
+    float ox = random_float(g);
+    float oy = random_float(g);
+    ox = ox - 0.5f;
+    oy = oy - 0.5f;
+    float ndc_x = ((float(x) + ox) / float(f.width)) * 2.0f - 1.0f;
+    float ndc_y = ((float(y) + oy) / float(f.height)) * 2.0f - 1.0f;
+    const float* ip = f.inverse_proj;
+    f3 v(ip[0] * (-ndc_x) + ip[4] * (-ndc_y) + ip[8] * 1.0f + ip[12] * 1.0f,
+         ip[1] * (-ndc_x) + ip[5] * (-ndc_y) + ip[9] * 1.0f + ip[13] * 1.0f,
+         ip[2] * (-ndc_x) + ip[6] * (-ndc_y) + ip[10] * 1.0f + ip[14] * 1.0f);
+    f3 dir = normalize(mul_point(f.inverse_view, v, 0.0f));
+    b.queue[0][0][p] = make_float4(f.camera_position[0], f.camera_position[1], f.camera_position[2], __uint_as_float(p));
+    b.queue[0][1][p] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+    b.throughput[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    b.radiance[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    b.rng[p] = g.state;
+    if (p == 0) b.counters[C_EXT + 0] = paths;
+}
+
+// ---------------------------------------------------------------- extend --
+__global__ __launch_bounds__(TRACE_BLOCK) void extend_kernel(tt::DeviceScene d, tt::PathBuffers b, uint32_t bounce) {
+    __shared__ uint32_t lds[TRACE_LDS_STACK * TRACE_BLOCK];
+    SceneView s = view(d);
+    Stack st = make_stack(lds, b.spill);
+    const uint32_t count = b.counters[C_EXT + bounce];
+    const float4* qo = b.queue[bounce & 1][0];
+    const float4* qd = b.queue[bounce & 1][1];
+    for (;;) {
+        uint32_t base = next_batch(b.counters + C_WORK_EXT + bounce);
+        if (base >= count) break;
+        uint32_t i = base + lane_id();
+        if (i < count) {
+            float4 o = qo[i], dd = qd[i];
+            HitRecord h;
+            bool found = trace_closest<true>(s, f3(o.x, o.y, o.z), f3(dd.x, dd.y, dd.z), 1.0e-3f, 1000.0f, st, h);
+            b.hits[i] = make_float4(h.b1, h.b2, __uint_as_float(found ? h.instance : 0xffffffffu),
+                                    __uint_as_float(h.primitive));
+        }
+    }
+}
+
+// ----------------------------------------------------------------- shade --
+TT_DEV f3 tt_octahedron_decode(float u, float v) {
+    float sx = u * 2.0f - 1.0f, sy = v * 2.0f - 1.0f;
+    f3 n(sx, sy, 1.0f - abs_(sx) - abs_(sy));
+    float t = clamp_(-n.z, 0.0f, 1.0f);
+    n.x = n.x + (n.x >= 0.0f ? -t : t);
+    n.y = n.y + (n.y >= 0.0f ? -t : t);
+    return normalize(n);
+}
+
+struct TangentFrame {
+    f3 normal, tangent;
+    float bitangent_sign;
+};
+
+// scene.glsl:29-64
+TT_DEV TangentFrame decode_tangent_frame(uint32_t e) {
+    float u = float(e & 0x3ffu) / float(1023);
+    float v = float((e >> 10) & 0x3ffu) / float(1023);
+    TangentFrame f;
+    f.normal = tt_octahedron_decode(u, v);
+    f3 ortho = abs_(f.normal.x) > abs_(f.normal.z) ? f3(-f.normal.y, f.normal.x, 0.0f) : f3(0.0f, -f.normal.z, f.normal.y);
+    float angle = (float((e >> 20) & 0x7ffu) / float(2047)) * TAU;
+    f.tangent = ortho * ttm::cos(angle) + cross(f.normal, ortho) * ttm::sin(angle);
+    f.bitangent_sign = (e >> 31) == 1u ? 1.0f : -1.0f;
+    return f;
+}
+
+struct Basis {
+    f3 normal, tangent, bitangent;
+};
+
+TT_DEV f3 transform_to_basis(const Basis& b, f3 d) { return normalize(d.x * b.tangent + d.y * b.bitangent + d.z * b.normal); }
+TT_DEV f3 transform_from_basis(const Basis& b, f3 d) {
+    return normalize(f3(dot(d, b.tangent), dot(d, b.bitangent), dot(d, b.normal)));
+}
+TT_DEV f3 interpolate(f3 b, f3 x, f3 y, f3 z) { return b.x * x + b.y * y + b.z * z; }
+
+TT_DEV f3 dequantize_snorm16(int16_t x, int16_t y, int16_t z) {
+    return f3(float(x) / 32767.0f, float(y) / 32767.0f, float(z) / 32767.0f);
+}
+
+// The 16 floats of a column-major matrix from 4 float4 loads.
+struct M4 {
+    float m[16];
+};
+TT_DEV M4 load_m4(const float* p) {
+    M4 r;
+    const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        float4 v = q[c];
+        r.m[4 * c + 0] = v.x;
+        r.m[4 * c + 1] = v.y;
+        r.m[4 * c + 2] = v.z;
+        r.m[4 * c + 3] = v.w;
+    }
+    return r;
+}
+
+__global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::FrameArgs f, tt::PathBuffers b,
+                                                    uint32_t bounce) {
+    SceneView s = view(d);
+    const uint32_t count = b.counters[C_EXT + bounce];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;  // whole waves exit together except the last one
+    const bool last = bounce + 1 >= f.bounce_count;
+
+    float4 qo = b.queue[bounce & 1][0][i];
+    float4 qd = b.queue[bounce & 1][1][i];
+    float4 hit = b.hits[i];
+    uint32_t p = __float_as_uint(qo.w);
+    f3 ray_dir(qd.x, qd.y, qd.z);
+    float4 thr4 = b.throughput[p];
+    float4 acc4 = b.radiance[p];
+    f3 throughput(thr4.x, thr4.y, thr4.z);
+    f3 accumulated(acc4.x, acc4.y, acc4.z);
+    Generator g{b.rng[p], f.rng};
+    uint32_t instance = __float_as_uint(hit.z);
+
+    bool emit_shadow = false, emit_ray = false;
+    f3 shadow_org, shadow_dir, direct, emission_after;
+    uint32_t light_instance = 0, light_hash = 0;
+    bool has_emission_after = false;
+    f3 next_dir, next_org;
+
+    if (instance == 0xffffffffu) {
+        // Miss: constant white sky (integrate.glsl:272-276).
+        accumulated = accumulated + f3(1.0f) * throughput;
+    } else {
+        // get_ray_hit (integrate.glsl:22-110) without the texture terms.
+        uint32_t prim = __float_as_uint(hit.w);
+        const float* inst = s.instances + 52 * size_t(instance);
+        M4 transform = load_m4(inst);
+        M4 normal_transform = load_m4(inst + 32);
+        uint4 offs = *reinterpret_cast<const uint4*>(inst + 48);
+        uint32_t vertex_offset = offs.x, index_offset = offs.y, material_index = offs.z;
+        uint32_t base = 3 * prim + index_offset;
+        f3 pos[3];
+        TangentFrame fr[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            uint32_t vi = s.indices[base + c] + vertex_offset;
+            const int16_t* pp = s.positions + 3 * size_t(vi);
+            pos[c] = mul_point1(transform.m, dequantize_snorm16(pp[0], pp[1], pp[2]));
+            fr[c] = decode_tangent_frame(s.vertices[vi].y);
+        }
+        f3 bary(0.0f, hit.x, hit.y);
+        bary.x = 1.0f - bary.y - bary.z;
+        Basis ts;
+        ts.normal = normalize(mul_dir(normal_transform.m, interpolate(bary, fr[0].normal, fr[1].normal, fr[2].normal)));
+        ts.tangent = normalize(mul_dir(normal_transform.m, interpolate(bary, fr[0].tangent, fr[1].tangent, fr[2].tangent)));
+        float sign = sign_(fr[0].bitangent_sign * bary.x + fr[1].bitangent_sign * bary.y + fr[2].bitangent_sign * bary.z);
+        ts.bitangent = normalize(sign * cross(ts.normal, ts.tangent));
+        f3 world_position = bary.x * pos[0] + bary.y * pos[1] + bary.z * pos[2];
+
+        Mat m = load_material(s.materials + 19 * size_t(material_index));
+        // create_surface_basis (integrate.glsl:195-201)
+        Basis sb;
+        sb.normal = transform_to_basis(ts, m.geometry_normal);
+        sb.tangent = normalize(ts.tangent - sb.normal * dot(sb.normal, ts.tangent));
+        sb.bitangent = normalize(cross(sb.normal, sb.tangent));
+        f3 wi = transform_from_basis(sb, -ray_dir);
+        Lobes lobes = bsdf_prepare(m, wi, g);
+
+        bool nee = f.light_sampling == TT_LIGHT_SAMPLING_NEXT_EVENT_ESTIMATION && s.emissive_count > 0 &&
+            m.specular_roughness > 0.15f;
+        f3 emissive = m.emission_color * m.emission_luminance;
+        if (nee) {
+            // sample_random_light (scene.glsl:114-136) and the unshadowed
+            // part of direct_light_contribution (integrate.glsl:156-193).
+            uint32_t li = random_uint(g, s.emissive_count);
+            f2 r2 = random_vec2(g);
+            f3 lb = sample_triangle(r2);
+            const uint32_t* et = s.emissive + 9 * size_t(li);
+            uint32_t w[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) w[k] = et[k];
+            auto half = [&](int k) { return int16_t((w[k >> 1] >> (16 * (k & 1))) & 0xffffu); };
+            light_hash = w[4] >> 16;
+            light_instance = w[8];
+            const float* linst = s.instances + 52 * size_t(light_instance);
+            M4 ltr = load_m4(linst);
+            f3 lp[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                lp[c] = mul_point1(ltr.m, dequantize_snorm16(half(3 * c), half(3 * c + 1), half(3 * c + 2)));
+            float area = fmax_(0.00001f, 0.5f * length(cross(lp[1] - lp[0], lp[2] - lp[0])));
+            f3 lnormal = normalize(cross(lp[1] - lp[0], lp[2] - lp[0]));
+            f3 light_position = interpolate(lb, lp[0], lp[1], lp[2]);
+            float light_probability = 1.0f / float(s.emissive_count);
+            f3 to_light = light_position - world_position;
+            f3 wo = transform_from_basis(sb, to_light);
+            float distance_squared = length_squared(to_light);
+            f3 dir = to_light / sqrt_(distance_squared);
+            float density = (1.0f / area) * distance_squared / saturate(dot(-dir, lnormal)) * light_probability;
+            uint32_t lmat = __float_as_uint(linst[50]);
+            Mat lm = load_material(s.materials + 19 * size_t(lmat));
+            f3 lemissive = lm.emission_color * lm.emission_luminance;
+            float bsdf_density;
+            f3 bsdf = bsdf_evaluate(m, lobes, wi, wo, bsdf_density);
+            f3 contribution = bsdf * lemissive * abs_(cos_theta(wo)) / fmax_(density, PDF_EPSILON);
+            direct = throughput * contribution;
+            shadow_org = world_position;
+            shadow_dir = dir;
+            emit_shadow = true;
+            if (bounce == 0) {
+                emission_after = emissive * throughput;
+                has_emission_after = true;
+            }
+        } else {
+            accumulated = accumulated + emissive * throughput;
+        }
+        if (!last) {
+            f3 wo, value;
+            float density;
+            if (bsdf_sample(m, lobes, wi, g, wo, density, value)) {
+                next_dir = transform_to_basis(sb, wo);
+                next_org = world_position;
+                throughput = throughput * (value / fmax_(density, PDF_EPSILON) * abs_(cos_theta(wo)));
+                emit_ray = true;
+            }
+        }
+    }
+
+    b.radiance[p] = make_float4(accumulated.x, accumulated.y, accumulated.z, 0.0f);
+    b.throughput[p] = make_float4(throughput.x, throughput.y, throughput.z, 0.0f);
+    b.rng[p] = g.state;
+
+    uint32_t slot = wave_append(b.counters + C_EXT + bounce + 1, emit_ray);
+    if (emit_ray) {
+        float4* no = b.queue[(bounce + 1) & 1][0];
+        float4* nd = b.queue[(bounce + 1) & 1][1];
+        no[slot] = make_float4(next_org.x, next_org.y, next_org.z, __uint_as_float(p));
+        nd[slot] = make_float4(next_dir.x, next_dir.y, next_dir.z, 0.0f);
+    }
+    uint32_t sslot = wave_append(b.counters + C_SHADOW + bounce, emit_shadow);
+    if (emit_shadow) {
+        b.shadow[0][sslot] = make_float4(shadow_org.x, shadow_org.y, shadow_org.z, __uint_as_float(p));
+        b.shadow[1][sslot] = make_float4(shadow_dir.x, shadow_dir.y, shadow_dir.z, __uint_as_float(light_instance));
+        b.shadow[2][sslot] = make_float4(direct.x, direct.y, direct.z,
+                                         __uint_as_float(light_hash | (has_emission_after ? 0x10000u : 0u)));
+        b.shadow[3][sslot] = make_float4(emission_after.x, emission_after.y, emission_after.z, 0.0f);
+    }
+}
+
+// ---------------------------------------------------------------- shadow --
+// integrate.glsl:168-192: closest hit without culling in [1e-3, 1e4];
+// visible iff the hit is the sampled light's instance and
+// (primitive + index_offset / 3) % 0xffff equals the light's hash.
+__global__ __launch_bounds__(TRACE_BLOCK) void shadow_kernel(tt::DeviceScene d, tt::PathBuffers b, uint32_t bounce) {
+    __shared__ uint32_t lds[TRACE_LDS_STACK * TRACE_BLOCK];
+    SceneView s = view(d);
+    Stack st = make_stack(lds, b.spill);
+    const uint32_t count = b.counters[C_SHADOW + bounce];
+    for (;;) {
+        uint32_t base = next_batch(b.counters + C_WORK_SHADOW + bounce);
+        if (base >= count) break;
+        uint32_t i = base + lane_id();
+        if (i < count) {
+            float4 o = b.shadow[0][i], dd = b.shadow[1][i];
+            HitRecord h;
+            bool found = trace_closest<false>(s, f3(o.x, o.y, o.z), f3(dd.x, dd.y, dd.z), 1.0e-3f, 10000.0f, st, h);
+            float4 direct = b.shadow[2][i];
+            uint32_t meta = __float_as_uint(direct.w);
+            uint32_t light_instance = __float_as_uint(dd.w);
+            bool visible = false;
+            if (found && h.instance == light_instance) {
+                uint32_t index_offset = __float_as_uint(s.instances[52 * size_t(h.instance) + 49]);
+                uint32_t triangle_index = h.primitive + index_offset / 3u;
+                visible = triangle_index % 0xffffu == (meta & 0xffffu);
+            }
+            bool emission = (meta & 0x10000u) != 0;
+            if (visible || emission) {
+                uint32_t p = __float_as_uint(o.w);
+                float4 a = b.radiance[p];
+                f3 acc(a.x, a.y, a.z);
+                if (visible) acc = acc + f3(direct.x, direct.y, direct.z);
+                if (emission) {
+                    float4 e = b.shadow[3][i];
+                    acc = acc + f3(e.x, e.y, e.z);
+                }
+                b.radiance[p] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ accumulate --
+// integrate.glsl:338-346
+__global__ __launch_bounds__(256) void accumulate_kernel(tt::FrameArgs f, tt::PathBuffers b, float4* target) {
+    uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= f.width * f.height) return;
+    f3 acc;
+    for (uint32_t s = 0; s < f.sample_count; s++) {
+        float4 r = b.radiance[size_t(pixel) * f.sample_count + s];
+        acc = acc + f3(r.x, r.y, r.z);
+    }
+    acc = acc / float(f.sample_count);
+    if (f.accumulated_frame_count != 0) {
+        float4 prev = target[pixel];
+        acc = (f3(prev.x, prev.y, prev.z) * float(f.accumulated_frame_count - 1) + acc) / float(f.accumulated_frame_count);
+    }
+    target[pixel] = make_float4(acc.x, acc.y, acc.z, 1.0f);
+}
+
+// ------------------------------------------------------------------ misc --
+__global__ void zero_kernel(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// BdsfSampler (src/material/test.rs:38-83): prepare from a fixed generator
+// state, then evaluate.
+__global__ void bsdf_evaluate_kernel(tt_material_constants mc, const tt_bsdf_query* q, tt_bsdf_result* out,
+                                     uint32_t count, uint32_t rng_state) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float4 words[12];
+    __builtin_memcpy(words, &mc, sizeof(words));
+    Mat m = load_material(words);
+    Generator g{rng_state, 0u};
+    f3 wi(q[i].wi[0], q[i].wi[1], q[i].wi[2]), wo(q[i].wo[0], q[i].wo[1], q[i].wo[2]);
+    Lobes l = bsdf_prepare(m, wi, g);
+    float density;
+    f3 v = bsdf_evaluate(m, l, wi, wo, density);
+    tt_bsdf_result r;
+    r.bsdf[0] = v.x;
+    r.bsdf[1] = v.y;
+    r.bsdf[2] = v.z;
+    r.density = density;
+    out[i] = r;
+}
+
+__global__ void math_kernel(int op, const float* x, const float* y, float* out, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = x[i];
+    float r;
+    switch (op) {
+        case 0: r = ttm::sin(v); break;
+        case 1: r = ttm::cos(v); break;
+        case 2: r = ttm::exp(v); break;
+        case 3: r = ttm::log(v); break;
+        case 4: r = ttm::acos(v); break;
+        case 5: r = ttm::asin(v); break;
+        case 6: r = ttm::pow(v, y[i]); break;
+        case 7: r = ttm::sqrt(v); break;
+        case 8: r = v / y[i]; break;
+        default: r = 1.0f / v; break;
+    }
+    out[i] = r;
+}
+
+uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
+
+}  // namespace
+
+namespace tt {
+
+int trace_grid(int cu_count) { return cu_count * 8; }
+size_t spill_words(int grid) { return size_t(grid) * TRACE_BLOCK * TRACE_SPILL_STACK; }
+uint32_t counter_words(uint32_t) { return C_WORDS; }
+
+hipError_t launch_zero(hipStream_t s, uint32_t* p, uint32_t words) {
+    zero_kernel<<<blocks(words, 256), 256, 0, s>>>(p, words);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_rays(hipStream_t s, const DeviceScene& scene, const tt_ray* rays, tt_hit* hits, uint32_t count,
+                             bool cull, uint32_t* work, uint32_t* spill, int grid) {
+    if (cull)
+        trace_rays_kernel<true><<<grid, TRACE_BLOCK, 0, s>>>(scene, rays, hits, count, work, spill);
+    else
+        trace_rays_kernel<false><<<grid, TRACE_BLOCK, 0, s>>>(scene, rays, hits, count, work, spill);
+    return hipGetLastError();
+}
+
+hipError_t launch_generate(hipStream_t s, const FrameArgs& f, const PathBuffers& b, uint32_t paths) {
+    generate_kernel<<<blocks(paths, 256), 256, 0, s>>>(f, b, paths);
+    return hipGetLastError();
+}
+
+hipError_t launch_extend(hipStream_t s, const DeviceScene& scene, const PathBuffers& b, uint32_t bounce, int grid) {
+    extend_kernel<<<grid, TRACE_BLOCK, 0, s>>>(scene, b, bounce);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade(hipStream_t s, const DeviceScene& scene, const FrameArgs& f, const PathBuffers& b,
+                        uint32_t bounce, uint32_t paths) {
+    shade_kernel<<<blocks(paths, 256), 256, 0, s>>>(scene, f, b, bounce);
+    return hipGetLastError();
+}
+
+hipError_t launch_shadow(hipStream_t s, const DeviceScene& scene, const PathBuffers& b, uint32_t bounce, int grid) {
+    shadow_kernel<<<grid, TRACE_BLOCK, 0, s>>>(scene, b, bounce);
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate(hipStream_t s, const FrameArgs& f, const PathBuffers& b, float4* target) {
+    accumulate_kernel<<<blocks(f.width * f.height, 256), 256, 0, s>>>(f, b, target);
+    return hipGetLastError();
+}
+
+hipError_t launch_bsdf_evaluate(hipStream_t s, const tt_material_constants* material, const tt_bsdf_query* q,
+                                tt_bsdf_result* r, uint32_t count, uint32_t rng_state) {
+    bsdf_evaluate_kernel<<<blocks(count, 64), 64, 0, s>>>(*material, q, r, count, rng_state);
+    return hipGetLastError();
+}
+
+hipError_t launch_math(hipStream_t s, int op, const float* x, const float* y, float* out, uint32_t count) {
+    math_kernel<<<blocks(count, 256), 256, 0, s>>>(op, x, y, out, count);
+    return hipGetLastError();
+}
+
+}  // namespace tt

@@ -1,0 +1,514 @@
+// C ABI of tinytrace_amd: context, resident scenes, render targets, the
+// render call and the test entry points.  See include/tinytrace.h for the
+// reference interfaces each function replaces.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/tinytrace.h"
+#include "bvh.h"
+#include "kernels.h"
+
+namespace {
+
+struct EventPair {
+    hipEvent_t begin = nullptr, end = nullptr;
+};
+
+}  // namespace
+
+struct tt_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    int cu_count = 0;
+    int grid = 0;
+    // Wavefront buffers, grown on demand.
+    size_t path_capacity = 0;
+    void* path_memory = nullptr;
+    uint32_t* counters = nullptr;
+    uint32_t* spill = nullptr;
+    uint32_t* work = nullptr;
+    // Timing.
+    EventPair integrate;
+    std::vector<EventPair> trace_events;
+    std::vector<uint8_t> trace_event_shadow;
+    uint32_t trace_events_used = 0;
+    bool stats_pending = false;
+    uint64_t last_paths = 0;
+    uint32_t last_bounces = 0;
+    double last_integrate_ms = -1.0;
+};
+
+struct tt_scene {
+    tt_context* context = nullptr;
+    tt::DeviceScene view{};
+    std::vector<void*> allocations;
+    tt_scene_info info{};
+};
+
+struct tt_target {
+    tt_context* context = nullptr;
+    uint32_t width = 0, height = 0;
+    float4* pixels = nullptr;
+};
+
+namespace {
+
+tt_status fail(tt_context* c, tt_status s, const std::string& msg) {
+    if (c) c->error = msg;
+    return s;
+}
+
+tt_status hip_fail(tt_context* c, hipError_t e, const char* what) {
+    return fail(c, e == hipErrorOutOfMemory ? TT_OUT_OF_MEMORY : TT_HIP_ERROR,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define TT_HIP(ctx, expr)                                  \
+    do {                                                   \
+        hipError_t e_ = (expr);                            \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+    } while (0)
+
+tt_status upload(tt_context* c, tt_scene* s, const void* src, size_t bytes, const void** dst) {
+    if (bytes == 0) {
+        *dst = nullptr;
+        return TT_OK;
+    }
+    void* p = nullptr;
+    TT_HIP(c, hipMalloc(&p, bytes));
+    s->allocations.push_back(p);
+    s->info.device_bytes += bytes;
+    TT_HIP(c, hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+    *dst = p;
+    return TT_OK;
+}
+
+// Path buffers: throughput, radiance, 2x2 queue arrays, hits, 4 shadow
+// arrays (float4 each) and the generator words.
+tt_status ensure_paths(tt_context* c, size_t paths, tt::PathBuffers& b) {
+    if (paths > c->path_capacity) {
+        if (c->path_memory) {
+            TT_HIP(c, hipStreamSynchronize(c->stream));
+            TT_HIP(c, hipFree(c->path_memory));
+            c->path_memory = nullptr;
+            c->path_capacity = 0;
+        }
+        size_t bytes = paths * (11 * sizeof(float4) + sizeof(uint32_t));
+        TT_HIP(c, hipMalloc(&c->path_memory, bytes));
+        c->path_capacity = paths;
+    }
+    size_t n = c->path_capacity;
+    float4* f = static_cast<float4*>(c->path_memory);
+    b.throughput = f + 0 * n;
+    b.radiance = f + 1 * n;
+    b.queue[0][0] = f + 2 * n;
+    b.queue[0][1] = f + 3 * n;
+    b.queue[1][0] = f + 4 * n;
+    b.queue[1][1] = f + 5 * n;
+    b.hits = f + 6 * n;
+    b.shadow[0] = f + 7 * n;
+    b.shadow[1] = f + 8 * n;
+    b.shadow[2] = f + 9 * n;
+    b.shadow[3] = f + 10 * n;
+    b.rng = reinterpret_cast<uint32_t*>(f + 11 * n);
+    b.counters = c->counters;
+    b.spill = c->spill;
+    return TT_OK;
+}
+
+tt_status ensure_events(tt_context* c, uint32_t count) {
+    while (c->trace_events.size() < count) {
+        EventPair e;
+        TT_HIP(c, hipEventCreate(&e.begin));
+        TT_HIP(c, hipEventCreate(&e.end));
+        c->trace_events.push_back(e);
+        c->trace_event_shadow.push_back(0);
+    }
+    return TT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+tt_status tt_context_create(const tt_context_desc* desc, tt_context** out) {
+    if (!out) return TT_INVALID_ARGUMENT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return TT_NO_DEVICE;
+    int device = desc ? desc->device : 0;
+    if (device < 0 || device >= count) return TT_NO_DEVICE;
+    tt_context* c = new (std::nothrow) tt_context();
+    if (!c) return TT_OUT_OF_MEMORY;
+    c->device = device;
+    auto cleanup = [&](tt_status s) {
+        tt_context_destroy(c);
+        return s;
+    };
+    if (hipSetDevice(device) != hipSuccess) return cleanup(TT_NO_DEVICE);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return cleanup(TT_HIP_ERROR);
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return cleanup(TT_KERNEL_LOAD);
+    c->cu_count = prop.multiProcessorCount;
+    c->grid = tt::trace_grid(c->cu_count);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(TT_HIP_ERROR);
+    if (hipMalloc(&c->counters, tt::counter_words(0) * sizeof(uint32_t)) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
+    if (hipMalloc(&c->work, 256) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
+    if (hipMalloc(&c->spill, tt::spill_words(c->grid) * sizeof(uint32_t)) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
+    if (hipEventCreate(&c->integrate.begin) != hipSuccess || hipEventCreate(&c->integrate.end) != hipSuccess)
+        return cleanup(TT_HIP_ERROR);
+    *out = c;
+    return TT_OK;
+}
+
+void tt_context_destroy(tt_context* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->path_memory) hipFree(c->path_memory);
+    if (c->counters) hipFree(c->counters);
+    if (c->work) hipFree(c->work);
+    if (c->spill) hipFree(c->spill);
+    if (c->integrate.begin) hipEventDestroy(c->integrate.begin);
+    if (c->integrate.end) hipEventDestroy(c->integrate.end);
+    for (auto& e : c->trace_events) {
+        hipEventDestroy(e.begin);
+        hipEventDestroy(e.end);
+    }
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* tt_last_error(const tt_context* c) { return c ? c->error.c_str() : "no context"; }
+
+tt_status tt_scene_create(tt_context* c, const tt_scene_desc* d, tt_scene** out) {
+    if (!c || !d || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    // Validate the reference arrays before building.
+    for (uint32_t m = 0; m < d->mesh_count; m++) {
+        const tt_mesh_record& r = d->meshes[m];
+        if (r.index_count % 3 != 0 || uint64_t(r.index_offset) + r.index_count > d->index_count ||
+            uint64_t(r.vertex_offset) + r.vertex_count > d->position_count || r.vertex_count > d->vertex_count)
+            return fail(c, TT_INVALID_ARGUMENT, "mesh " + std::to_string(m) + " is out of range");
+        for (uint32_t k = 0; k < r.index_count; k++)
+            if (d->indices[r.index_offset + k] >= r.vertex_count)
+                return fail(c, TT_INVALID_ARGUMENT, "mesh " + std::to_string(m) + " has an index out of range");
+    }
+    for (uint32_t i = 0; i < d->instance_count; i++) {
+        const tt_instance& in = d->instances[i];
+        if (in.mesh_index >= d->mesh_count || in.material_index >= d->material_count)
+            return fail(c, TT_INVALID_ARGUMENT, "instance " + std::to_string(i) + " refers to a missing mesh/material");
+    }
+    if (d->instance_count >= (1u << 24)) return fail(c, TT_INVALID_ARGUMENT, "more than 2^24 instances");
+    for (uint32_t i = 0; i < d->material_count; i++)
+        for (int t = 0; t < 27; t++)
+            if (d->materials[i].textures[t] != TT_INVALID_INDEX)
+                return fail(c, TT_UNSUPPORTED, "textured materials are not supported by this integrator");
+    for (uint32_t e = 0; e < d->emissive_triangle_count; e++)
+        if (d->emissive_triangles[e].instance_index >= d->instance_count)
+            return fail(c, TT_INVALID_ARGUMENT, "emissive triangle refers to a missing instance");
+
+    tt::BvhBuildResult bvh;
+    if (!tt::build_bvh(*d, bvh)) return fail(c, TT_INVALID_ARGUMENT, "BVH build failed: " + bvh.error);
+
+    tt_scene* s = new (std::nothrow) tt_scene();
+    if (!s) return fail(c, TT_OUT_OF_MEMORY, "scene allocation");
+    s->context = c;
+    TT_HIP(c, hipSetDevice(c->device));
+    tt_status st = TT_OK;
+    auto up = [&](const void* src, size_t bytes, const void** dst) {
+        if (st == TT_OK) st = upload(c, s, src, bytes, dst);
+    };
+    up(bvh.nodes.data(), bvh.nodes.size() * sizeof(tt::BvhNode), &s->view.nodes);
+    up(bvh.tris.data(), bvh.tris.size() * sizeof(tt::BvhTri), &s->view.tris);
+    up(bvh.instances.data(), bvh.instances.size() * sizeof(tt::BvhInstance), &s->view.bvh_insts);
+    up(d->instances, size_t(d->instance_count) * sizeof(tt_instance), &s->view.instances);
+    up(d->materials, size_t(d->material_count) * sizeof(tt_material), &s->view.materials);
+    up(d->positions, size_t(d->position_count) * 3 * sizeof(int16_t), &s->view.positions);
+    up(d->vertices, size_t(d->vertex_count) * sizeof(tt_vertex), &s->view.vertices);
+    up(d->indices, size_t(d->index_count) * sizeof(uint32_t), &s->view.indices);
+    up(d->emissive_triangles, size_t(d->emissive_triangle_count) * sizeof(tt_emissive_triangle), &s->view.emissive);
+    if (st != TT_OK) {
+        tt_scene_destroy(s);
+        return st;
+    }
+    s->view.tlas_root = bvh.empty ? 0xffffffffu : bvh.tlas_root;
+    s->view.emissive_count = d->emissive_triangle_count;
+    uint64_t tris = 0, unique = 0;
+    for (uint32_t i = 0; i < d->instance_count; i++) tris += d->meshes[d->instances[i].mesh_index].index_count / 3;
+    for (uint32_t m = 0; m < d->mesh_count; m++) unique += d->meshes[m].index_count / 3;
+    s->info.triangle_count = uint32_t(std::min<uint64_t>(tris, 0xffffffffu));
+    s->info.unique_triangle_count = uint32_t(unique);
+    s->info.node_count = uint32_t(bvh.nodes.size());
+    s->info.max_depth = bvh.max_depth;
+    s->info.sah_cost = bvh.sah_cost;
+    s->info.expected_bytes_per_ray = bvh.expected_bytes;
+    s->info.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = s;
+    return TT_OK;
+}
+
+void tt_scene_destroy(tt_scene* s) {
+    if (!s) return;
+    if (s->context) {
+        hipSetDevice(s->context->device);
+        hipStreamSynchronize(s->context->stream);
+    }
+    for (void* p : s->allocations) hipFree(p);
+    delete s;
+}
+
+tt_status tt_scene_info_get(const tt_scene* s, tt_scene_info* out) {
+    if (!s || !out) return TT_INVALID_ARGUMENT;
+    *out = s->info;
+    return TT_OK;
+}
+
+tt_status tt_target_create(tt_context* c, uint32_t width, uint32_t height, tt_target** out) {
+    if (!c || !out || width == 0 || height == 0) return fail(c, TT_INVALID_ARGUMENT, "bad target size");
+    *out = nullptr;
+    tt_target* t = new (std::nothrow) tt_target();
+    if (!t) return fail(c, TT_OUT_OF_MEMORY, "target allocation");
+    t->context = c;
+    t->width = width;
+    t->height = height;
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess) e = hipMalloc(&t->pixels, size_t(width) * height * sizeof(float4));
+    if (e == hipSuccess) e = hipMemsetAsync(t->pixels, 0, size_t(width) * height * sizeof(float4), c->stream);
+    if (e != hipSuccess) {
+        delete t;
+        return hip_fail(c, e, "target allocation");
+    }
+    *out = t;
+    return TT_OK;
+}
+
+void tt_target_destroy(tt_target* t) {
+    if (!t) return;
+    if (t->context) {
+        hipSetDevice(t->context->device);
+        hipStreamSynchronize(t->context->stream);
+    }
+    if (t->pixels) hipFree(t->pixels);
+    delete t;
+}
+
+tt_status tt_render_ex(tt_context* c, const tt_scene* s, const tt_constants* k, const tt_render_settings* settings,
+                       tt_target* t) {
+    if (!c || !s || !k || !t) return fail(c, TT_INVALID_ARGUMENT, "null argument");
+    if (s->context != c || t->context != c) return fail(c, TT_STALE_HANDLE, "handle from another context");
+    if (k->screen_size[0] != t->width || k->screen_size[1] != t->height)
+        return fail(c, TT_INVALID_ARGUMENT, "constants.screen_size does not match the target");
+    if (k->sample_count == 0 || k->bounce_count == 0 || k->bounce_count > 31)
+        return fail(c, TT_INVALID_ARGUMENT, "sample_count must be >= 1 and bounce_count in [1, 31]");
+    if (k->light_sampling != TT_LIGHT_SAMPLING_ON_HIT && k->light_sampling != TT_LIGHT_SAMPLING_NEXT_EVENT_ESTIMATION)
+        return fail(c, TT_UNSUPPORTED, "light sampling mode not supported");
+    tt_render_settings def{TT_RNG_REFERENCE, 0, 0, 0, 0, 0};
+    const tt_render_settings& st = settings ? *settings : def;
+    if (st.rng > 1) return fail(c, TT_INVALID_ARGUMENT, "unknown generator");
+    uint64_t paths64 = uint64_t(t->width) * t->height * k->sample_count;
+    if (paths64 >= (1ull << 31)) return fail(c, TT_INVALID_ARGUMENT, "too many paths in one render");
+    uint32_t paths = uint32_t(paths64);
+    TT_HIP(c, hipSetDevice(c->device));
+    tt::PathBuffers b;
+    tt_status es = ensure_paths(c, paths, b);
+    if (es != TT_OK) return es;
+    b.max_bounces = k->bounce_count;
+    bool stats = (st.flags & TT_RENDER_COLLECT_STATS) != 0;
+    if (stats) {
+        es = ensure_events(c, 2 * k->bounce_count);
+        if (es != TT_OK) return es;
+    }
+
+    tt::FrameArgs f;
+    std::memcpy(f.inverse_view, k->inverse_view, sizeof(f.inverse_view));
+    std::memcpy(f.inverse_proj, k->inverse_proj, sizeof(f.inverse_proj));
+    f.camera_position[0] = k->camera_position[0];
+    f.camera_position[1] = k->camera_position[1];
+    f.camera_position[2] = k->camera_position[2];
+    f.width = t->width;
+    f.height = t->height;
+    f.sample_count = k->sample_count;
+    f.bounce_count = k->bounce_count;
+    f.frame_index = k->frame_index;
+    f.accumulated_frame_count = k->accumulated_frame_count;
+    f.light_sampling = k->light_sampling;
+    f.rng = st.rng;
+    f.seed = st.seed;
+
+    hipStream_t stream = c->stream;
+    TT_HIP(c, hipEventRecord(c->integrate.begin, stream));
+    TT_HIP(c, tt::launch_zero(stream, c->counters, tt::counter_words(0)));
+    TT_HIP(c, tt::launch_generate(stream, f, b, paths));
+    uint32_t ev = 0;
+    for (uint32_t bounce = 0; bounce < k->bounce_count; bounce++) {
+        if (stats) TT_HIP(c, hipEventRecord(c->trace_events[ev].begin, stream));
+        TT_HIP(c, tt::launch_extend(stream, s->view, b, bounce, c->grid));
+        if (stats) {
+            c->trace_event_shadow[ev] = 0;
+            TT_HIP(c, hipEventRecord(c->trace_events[ev++].end, stream));
+        }
+        TT_HIP(c, tt::launch_shade(stream, s->view, f, b, bounce, paths));
+        if (f.light_sampling == TT_LIGHT_SAMPLING_NEXT_EVENT_ESTIMATION && s->view.emissive_count > 0) {
+            if (stats) TT_HIP(c, hipEventRecord(c->trace_events[ev].begin, stream));
+            TT_HIP(c, tt::launch_shadow(stream, s->view, b, bounce, c->grid));
+            if (stats) {
+                c->trace_event_shadow[ev] = 1;
+                TT_HIP(c, hipEventRecord(c->trace_events[ev++].end, stream));
+            }
+        }
+    }
+    TT_HIP(c, tt::launch_accumulate(stream, f, b, t->pixels));
+    TT_HIP(c, hipEventRecord(c->integrate.end, stream));
+    c->trace_events_used = stats ? ev : 0;
+    c->stats_pending = stats;
+    c->last_paths = paths;
+    c->last_bounces = k->bounce_count;
+    c->last_integrate_ms = -1.0;
+    return TT_OK;
+}
+
+tt_status tt_render(tt_context* c, const tt_scene* s, const tt_constants* k, tt_target* t) {
+    return tt_render_ex(c, s, k, nullptr, t);
+}
+
+tt_status tt_target_download(tt_context* c, const tt_target* t, float* rgba, size_t bytes) {
+    if (!c || !t || !rgba) return fail(c, TT_INVALID_ARGUMENT, "null argument");
+    if (t->context != c) return fail(c, TT_STALE_HANDLE, "target from another context");
+    size_t need = size_t(t->width) * t->height * sizeof(float4);
+    if (bytes < need) return fail(c, TT_INVALID_ARGUMENT, "download buffer too small");
+    TT_HIP(c, hipSetDevice(c->device));
+    TT_HIP(c, hipMemcpyAsync(rgba, t->pixels, need, hipMemcpyDeviceToHost, c->stream));
+    TT_HIP(c, hipStreamSynchronize(c->stream));
+    return TT_OK;
+}
+
+tt_status tt_timing_ms(tt_context* c, const char* name, double* out) {
+    if (!c || !name || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
+    TT_HIP(c, hipEventSynchronize(c->integrate.end));
+    if (std::strcmp(name, "integrate") == 0) {
+        float ms = 0.0f;
+        TT_HIP(c, hipEventElapsedTime(&ms, c->integrate.begin, c->integrate.end));
+        *out = ms;
+        return TT_OK;
+    }
+    if (std::strcmp(name, "trace") == 0) {
+        if (!c->stats_pending) return fail(c, TT_INVALID_ARGUMENT, "render without TT_RENDER_COLLECT_STATS");
+        double sum = 0.0;
+        for (uint32_t i = 0; i < c->trace_events_used; i++) {
+            float ms = 0.0f;
+            TT_HIP(c, hipEventElapsedTime(&ms, c->trace_events[i].begin, c->trace_events[i].end));
+            sum += ms;
+        }
+        *out = sum;
+        return TT_OK;
+    }
+    return fail(c, TT_INVALID_ARGUMENT, std::string("unknown timing ") + name);
+}
+
+tt_status tt_render_stats_get(tt_context* c, tt_render_stats* out) {
+    if (!c || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
+    if (!c->stats_pending) return fail(c, TT_INVALID_ARGUMENT, "render without TT_RENDER_COLLECT_STATS");
+    TT_HIP(c, hipStreamSynchronize(c->stream));
+    uint32_t words[128];
+    TT_HIP(c, hipMemcpy(words, c->counters, sizeof(words), hipMemcpyDeviceToHost));
+    *out = tt_render_stats{};
+    out->paths = c->last_paths;
+    for (uint32_t b = 0; b < c->last_bounces; b++) {
+        out->extension_rays += words[0 + b];
+        out->shadow_rays += words[32 + b];
+    }
+    for (uint32_t i = 0; i < c->trace_events_used; i++) {
+        float ms = 0.0f;
+        TT_HIP(c, hipEventElapsedTime(&ms, c->trace_events[i].begin, c->trace_events[i].end));
+        if (c->trace_event_shadow[i]) {
+            out->shadow_launches++;
+            out->shadow_ms += ms;
+        } else {
+            out->extend_launches++;
+            out->extend_ms += ms;
+        }
+    }
+    float ms = 0.0f;
+    TT_HIP(c, hipEventElapsedTime(&ms, c->integrate.begin, c->integrate.end));
+    out->integrate_ms = ms;
+    return TT_OK;
+}
+
+tt_status tt_trace_closest_device(tt_context* c, const tt_scene* s, const tt_ray* rays, tt_hit* hits, uint32_t count,
+                                  uint32_t flags) {
+    if (!c || !s || (count && (!rays || !hits))) return fail(c, TT_INVALID_ARGUMENT, "null argument");
+    if (s->context != c) return fail(c, TT_STALE_HANDLE, "scene from another context");
+    if (count == 0) return TT_OK;
+    TT_HIP(c, hipSetDevice(c->device));
+    TT_HIP(c, tt::launch_zero(c->stream, c->work, 1));
+    TT_HIP(c, tt::launch_trace_rays(c->stream, s->view, rays, hits, count, (flags & TT_TRACE_CULL_BACK_FACES) != 0,
+                                    c->work, c->spill, c->grid));
+    return TT_OK;
+}
+
+tt_status tt_trace_closest(tt_context* c, const tt_scene* s, const tt_ray* rays, tt_hit* hits, uint32_t count,
+                           uint32_t flags) {
+    if (!c || !s || (count && (!rays || !hits))) return fail(c, TT_INVALID_ARGUMENT, "null argument");
+    if (count == 0) return TT_OK;
+    TT_HIP(c, hipSetDevice(c->device));
+    tt_ray* dr = nullptr;
+    tt_hit* dh = nullptr;
+    TT_HIP(c, hipMalloc(&dr, size_t(count) * sizeof(tt_ray)));
+    hipError_t e = hipMalloc(&dh, size_t(count) * sizeof(tt_hit));
+    if (e == hipSuccess) e = hipMemcpyAsync(dr, rays, size_t(count) * sizeof(tt_ray), hipMemcpyHostToDevice, c->stream);
+    tt_status st = TT_OK;
+    if (e == hipSuccess) st = tt_trace_closest_device(c, s, dr, dh, count, flags);
+    if (e == hipSuccess && st == TT_OK)
+        e = hipMemcpyAsync(hits, dh, size_t(count) * sizeof(tt_hit), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && st == TT_OK) e = hipStreamSynchronize(c->stream);
+    hipFree(dr);
+    if (dh) hipFree(dh);
+    if (e != hipSuccess) return hip_fail(c, e, "tt_trace_closest");
+    return st;
+}
+
+tt_status tt_bsdf_evaluate(tt_context* c, const tt_material_constants* material, const tt_bsdf_query* queries,
+                           tt_bsdf_result* results, uint32_t count, uint32_t rng_state) {
+    if (!c || !material || (count && (!queries || !results))) return fail(c, TT_INVALID_ARGUMENT, "null argument");
+    if (count == 0) return TT_OK;
+    TT_HIP(c, hipSetDevice(c->device));
+    tt_bsdf_query* dq = nullptr;
+    tt_bsdf_result* dr = nullptr;
+    TT_HIP(c, hipMalloc(&dq, size_t(count) * sizeof(tt_bsdf_query)));
+    hipError_t e = hipMalloc(&dr, size_t(count) * sizeof(tt_bsdf_result));
+    if (e == hipSuccess) e = hipMemcpyAsync(dq, queries, size_t(count) * sizeof(tt_bsdf_query), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = tt::launch_bsdf_evaluate(c->stream, material, dq, dr, count, rng_state);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(results, dr, size_t(count) * sizeof(tt_bsdf_result), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(dq);
+    if (dr) hipFree(dr);
+    if (e != hipSuccess) return hip_fail(c, e, "tt_bsdf_evaluate");
+    return TT_OK;
+}
+
+tt_status tt_debug_math(tt_context* c, int op, const float* x, const float* y, float* out, uint32_t count) {
+    if (!c || !x || !y || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
+    if (count == 0) return TT_OK;
+    TT_HIP(c, hipSetDevice(c->device));
+    float* d = nullptr;
+    TT_HIP(c, hipMalloc(&d, size_t(count) * 3 * sizeof(float)));
+    hipError_t e = hipMemcpyAsync(d, x, count * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + count, y, count * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = tt::launch_math(c->stream, op, d, d + count, d + 2 * count, count);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d + 2 * count, count * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(d);
+    if (e != hipSuccess) return hip_fail(c, e, "tt_debug_math");
+    return TT_OK;
+}
+
+}  // extern "C"
