@@ -1,0 +1,170 @@
+"""Host asset pipeline (src/asset/mod.rs, normal.rs, math/mod.rs, scene.rs)
+and the C ABI surface."""
+import ctypes as C
+import math
+import re
+import os
+
+import numpy as np
+
+from oracle import binding
+from tinytrace_amd import AssetScene, Camera, Config, Material, _abi, build_constants, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "tinytrace.h")).read()
+    declared = set(re.findall(r"^\s*(?:tt_status|void|const char\*)\s+(tt_\w+)\s*\(", header, re.M))
+    assert len(declared) >= 25
+    lib = _abi.lib()
+    missing = [name for name in sorted(declared) if not hasattr(lib, name)]
+    assert not missing, missing
+    assert declared == set(_abi.EXPORTED_SYMBOLS)
+
+
+def test_reference_struct_sizes():
+    assert C.sizeof(_abi.Constants) == 368
+    assert C.sizeof(_abi.MaterialConstants) == 192
+    assert _abi.MATERIAL_DTYPE.itemsize == 304
+    assert _abi.INSTANCE_DTYPE.itemsize == 208
+    assert _abi.EMISSIVE_DTYPE.itemsize == 36
+    assert _abi.VERTEX_DTYPE.itemsize == 8
+    assert _abi.MESH_DTYPE.itemsize == 32
+
+
+def _one_mesh_scene(positions, normals, tangents, uv, indices, transform=None, emissive=False):
+    s = AssetScene()
+    mesh = s.insert_mesh(positions, uv, indices, normals, tangents)
+    mat = s.insert_material(Material(emission_luminance=2.0 if emissive else 0.0))
+    s.add_model(s.add_node(transform), mesh, mat)
+    return s, s.arrays()
+
+
+def test_snorm_positions_round_trip_through_the_instance_transform():
+    rng = np.random.default_rng(0)
+    pos = rng.uniform(-3, 5, (300, 3))
+    n = rng.standard_normal((300, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    t = np.concatenate([np.cross(n, [0.3, 0.5, 0.8]), np.ones((300, 1))], axis=1)
+    t[:, :3] /= np.linalg.norm(t[:, :3], axis=1, keepdims=True)
+    node = np.eye(4)
+    node[:3, 3] = [10, -2, 4]
+    s, a = _one_mesh_scene(pos, n, t, rng.random((300, 2)), np.arange(300), node)
+    m = a["meshes"][0]
+    inst = a["instances"][0]
+    transform = inst["transform"].reshape(4, 4).T.astype(np.float64)
+    obj = a["positions"].astype(np.float64) / 32767.0
+    world = obj @ transform[:3, :3].T + transform[:3, 3]
+    assert np.abs(world - (pos + node[:3, 3])).max() <= m["radius"] / 32767.0 * 1.01
+    # transform and inverse_transform are inverses; normal_transform is the
+    # inverse transpose of the node transform (src/scene.rs:39-49).
+    inv = inst["inverse_transform"].reshape(4, 4).T.astype(np.float64)
+    np.testing.assert_allclose(inv @ transform, np.eye(4), atol=1e-5)
+    nt = inst["normal_transform"].reshape(4, 4).T.astype(np.float64)
+    np.testing.assert_allclose(nt, np.linalg.inv(node).T, atol=1e-6)
+    # center/radius: running mean and max distance (src/asset/mod.rs:308-321)
+    np.testing.assert_allclose(m["center"], pos.mean(0), rtol=1e-5, atol=1e-5)
+    assert abs(m["radius"] - np.linalg.norm(pos - pos.mean(0), axis=1).max()) < 1e-4
+
+
+def test_tangent_frame_encoding_round_trip():
+    rng = np.random.default_rng(1)
+    count = 2000
+    n = rng.standard_normal((count, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    helper = rng.standard_normal((count, 3))
+    t = np.cross(n, helper)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    sign = np.where(rng.random(count) < 0.5, 1.0, -1.0)
+    tangents = np.concatenate([t, sign[:, None]], axis=1)
+    pos = rng.standard_normal((count, 3))
+    _, a = _one_mesh_scene(pos, n, tangents, rng.random((count, 2)), np.arange(count - count % 3))
+    out = np.zeros(7, dtype=np.float32)
+    worst_n = worst_t = 0.0
+    for i, enc in enumerate(a["vertices"]["tangent_frame"]):
+        binding.lib().oracle_decode_tangent_frame(int(enc), out.ctypes.data)
+        worst_n = max(worst_n, np.linalg.norm(out[:3] - n[i]))
+        tt = out[3:6] / np.linalg.norm(out[3:6])
+        worst_t = max(worst_t, np.linalg.norm(tt - t[i]))
+        assert out[6] == sign[i]
+    # 10-bit octahedral normal, 11-bit tangent angle.
+    # (octahedral cells are 2/1023 wide, stretched up to ~2x near the fold)
+    assert worst_n < 5e-3
+    assert worst_t < 1e-2
+
+
+def test_texture_coordinates_are_round_to_nearest_even_halves():
+    rng = np.random.default_rng(2)
+    uv = rng.uniform(-4, 4, (99, 2)).astype(np.float32)
+    uv[:3] = [[1e-8, 65504.0], [70000.0, -1e-6], [0.1, 0.2]]
+    pos = rng.standard_normal((99, 3))
+    n = np.tile([0, 0, 1.0], (99, 1))
+    t = np.tile([1.0, 0, 0, 1.0], (99, 1))
+    _, a = _one_mesh_scene(pos, n, t, uv, np.arange(99))
+    with np.errstate(over="ignore"):
+        expected = uv.astype(np.float16).view(np.uint16)
+    np.testing.assert_array_equal(a["vertices"]["texture_coordinates"], expected)
+
+
+def test_emissive_triangles_and_hash():
+    scene, _ = scenes.cornell_box()
+    a = scene.arrays()
+    e = a["emissive_triangles"]
+    assert e.shape[0] == 2
+    inst = a["instances"][e["instance_index"][0]]
+    mesh = a["meshes"][inst["mesh_index"]]
+    # hash = index_offset / 3 + offset % 65535 (src/asset/mod.rs:186)
+    np.testing.assert_array_equal(e["hash"], mesh["index_offset"] // 3 + np.arange(2))
+    idx = a["indices"][mesh["index_offset"]:mesh["index_offset"] + 6].reshape(2, 3) + mesh["vertex_offset"]
+    np.testing.assert_array_equal(e["positions"], a["positions"][idx])
+
+
+def test_depth_first_flatten_order():
+    s = AssetScene()
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], dtype=np.float32)
+    n = np.tile([0, 0, 1.0], (3, 1))
+    t = np.tile([1.0, 0, 0, 1.0], (3, 1))
+    mesh = s.insert_mesh(pos, np.zeros((3, 2)), [0, 1, 2], n, t)
+    mats = [s.insert_material(Material(base_color=(i / 10, 0, 0))) for i in range(5)]
+    a = s.add_node()
+    b = s.add_node(parent=a)
+    c = s.add_node()
+    d = s.add_node(parent=b)
+    for node, mat in ((c, 0), (d, 1), (a, 2), (b, 3)):
+        s.add_model(node, mesh, mats[mat])
+    order = s.arrays()["instances"]["material_index"]
+    # pre-order: a, b, d, c
+    np.testing.assert_array_equal(order, [mats[2], mats[3], mats[1], mats[0]])
+
+
+def test_missing_tangents_is_unsupported():
+    s = AssetScene()
+    try:
+        s.insert_mesh(np.zeros((3, 3)) + np.eye(3), np.zeros((3, 2)), [0, 1, 2], None, None)
+    except _abi.TinytraceError as e:
+        assert e.status == 8
+    else:
+        raise AssertionError("expected TT_UNSUPPORTED")
+
+
+def test_camera_constants_closed_form():
+    """camera_ray_direction(ndc) = normalize(ndc.x a t R - ndc.y t U - F)
+    with t = tan(fov / 2), R = right(), U = the look-at up vector and F =
+    forward() (src/camera.rs:31-50, integrate.glsl:17-20)."""
+    cam = Camera(position=(1.0, 2.0, 3.0), yaw=0.7, pitch=-0.3, fov=0.9)
+    w, h = 320, 200
+    k = build_constants(cam, Config(), w, h, 0, 0)
+    F = cam.forward().astype(np.float64)
+    R = cam.right().astype(np.float64)
+    U = np.cross(-R, F)
+    U /= np.linalg.norm(U)
+    t = math.tan(cam.fov / 2)
+    aspect = w / h
+    out = np.zeros(3, dtype=np.float32)
+    for nx, ny in ((0, 0), (1, 1), (-1, 0.5), (0.3, -0.9)):
+        binding.lib().oracle_camera_direction(C.byref(k), nx, ny, out.ctypes.data)
+        expected = nx * aspect * t * R - ny * t * U - F
+        expected /= np.linalg.norm(expected)
+        np.testing.assert_allclose(out, expected, atol=2e-6)
+    assert list(k.screen_size) == [w, h] and k.camera_position[3] == 0.0

@@ -405,7 +405,14 @@ bool build_bvh(const tt_scene_desc& d, BvhBuildResult& out) {
         }
         uint32_t root = blas[in.mesh_index].empty ? (BVH_LEAF | 0x7fffffffu) : blas_root[in.mesh_index];
         std::memcpy(&bi.c0[3], &root, 4);
-        bi.c1[3] = bi.c2[3] = bi.c3[3] = 0.0f;
+        // c1.w: first global triangle index of the mesh (index_offset / 3);
+        // c2.w: 1 when index_offset / 3 + triangle count <= 65535, i.e. the
+        // emissive hash names exactly one triangle of this instance.
+        uint32_t base = in.index_offset / 3;
+        uint32_t exact = uint64_t(base) + d.meshes[in.mesh_index].index_count / 3 <= 65535u ? 1u : 0u;
+        std::memcpy(&bi.c1[3], &base, 4);
+        std::memcpy(&bi.c2[3], &exact, 4);
+        bi.c3[3] = 0.0f;
     }
 
     // Two-level SAH expectation: TLAS nodes, then per instance leaf the

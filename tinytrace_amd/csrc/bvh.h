@@ -14,7 +14,9 @@
 //                      (v / 32767), in leaf order; v0.w holds the mesh-local
 //                      primitive index.
 //   insts : BvhInstance[] 64 B, the instance's f32 inverse_transform columns
-//                      (xyz) and, in c0.w, the BLAS root reference.
+//                      (xyz); c0.w the BLAS root reference, c1.w the mesh's
+//                      first global triangle (index_offset / 3), c2.w 1 when
+//                      the emissive hash is unambiguous on this instance.
 // Node references (u32): bit 31 clear = inner node index; set = leaf.  TLAS
 // leaves hold one instance (bits 0..30); BLAS leaves hold a triangle range:
 // first slot in bits 0..26, count-1 in bits 27..30 (count <= 16).

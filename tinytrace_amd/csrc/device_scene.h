@@ -169,9 +169,26 @@ struct Stack {
     }
 };
 
+// A shadow query: the ray was aimed at a point sampled on the light
+// triangle identified by (light_instance, hash).  The reference accepts the
+// light iff the closest hit in [tmin, 1e4] is that triangle
+// (integrate.glsl:168-185).  When the light mesh's triangle indices are
+// unambiguous under the hash (index_offset / 3 + triangle count <= 65535),
+// only the sampled triangle passes the check and its hit distance lies within
+// 1e-4 (relative and absolute) of the sample distance d, so the caller may
+// cap tmax at d * (1 + 1e-4) + 1e-4 and stop at the first non-light hit
+// closer than t_exit = d * (1 - 1e-4) - 1e-4: neither changes the decision.
+struct ShadowQuery {
+    uint32_t light_instance;
+    uint32_t hash;
+    float t_exit;   // -inf disables the early exit
+    bool occluded;  // set when a non-light hit closer than t_exit ended the query
+};
+
 // Closest hit of a world-space ray.  Returns false on a miss.
-template <bool CULL>
-TT_DEV bool trace_closest(const SceneView& s, f3 org, f3 dir, float tmin, float tmax, Stack& stack, HitRecord& best) {
+template <bool CULL, bool SHADOW = false>
+TT_DEV bool trace_closest(const SceneView& s, f3 org, f3 dir, float tmin, float tmax, Stack& stack, HitRecord& best,
+                          ShadowQuery* q = nullptr) {
     best.t = tmax;
     best.instance = 0xffffffffu;
     best.primitive = 0xffffffffu;
@@ -184,6 +201,7 @@ TT_DEV bool trace_closest(const SceneView& s, f3 org, f3 dir, float tmin, float 
     slab_setup(r, org, dir);
     bool in_blas = false;
     uint32_t instance = 0;
+    uint32_t triangle_base = 0;  // index_offset / 3 of the current instance
     uint32_t cur = s.tlas_root;
     stack.sp = 0;
     for (;;) {
@@ -223,6 +241,7 @@ TT_DEV bool trace_closest(const SceneView& s, f3 org, f3 dir, float tmin, float 
                 woop_setup(r, o, d);
                 stack.push(SENTINEL);
                 in_blas = true;
+                if (SHADOW) triangle_base = __float_as_uint(c1.w);
                 cur = root;
                 continue;
             }
@@ -237,6 +256,16 @@ TT_DEV bool trace_closest(const SceneView& s, f3 org, f3 dir, float tmin, float 
                 if (triangle_hit<CULL>(r, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(c.x, c.y, c.z), tmin, limit, t, V, W,
                                        det)) {
                     uint32_t prim = __float_as_uint(a.w);
+                    if (SHADOW) {
+                        bool light = instance == q->light_instance && (prim + triangle_base) % 0xffffu == q->hash;
+                        if (!light && t < q->t_exit) {
+                            q->occluded = true;
+                            best.t = t;
+                            best.instance = instance;
+                            best.primitive = prim;
+                            return true;
+                        }
+                    }
                     if (!found || closer(t, instance, prim, best)) {
                         found = true;
                         best.t = t;

@@ -245,7 +245,8 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
     bool emit_shadow = false, emit_ray = false;
     f3 shadow_org, shadow_dir, direct, emission_after;
     uint32_t light_instance = 0, light_hash = 0;
-    bool has_emission_after = false;
+    float shadow_distance = 0.0f;
+    bool light_exact = false, has_emission_after = false;
     f3 next_dir, next_org;
 
     if (instance == 0xffffffffu) {
@@ -316,7 +317,8 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
             f3 to_light = light_position - world_position;
             f3 wo = transform_from_basis(sb, to_light);
             float distance_squared = length_squared(to_light);
-            f3 dir = to_light / sqrt_(distance_squared);
+            float distance = sqrt_(distance_squared);
+            f3 dir = to_light / distance;
             float density = (1.0f / area) * distance_squared / saturate(dot(-dir, lnormal)) * light_probability;
             uint32_t lmat = __float_as_uint(linst[50]);
             Mat lm = load_material(s.materials + 19 * size_t(lmat));
@@ -327,6 +329,8 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
             direct = throughput * contribution;
             shadow_org = world_position;
             shadow_dir = dir;
+            shadow_distance = distance;
+            light_exact = __float_as_uint(s.bvh_insts[4 * size_t(light_instance) + 2].w) != 0u;
             emit_shadow = true;
             if (bounce == 0) {
                 emission_after = emissive * throughput;
@@ -363,15 +367,18 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
         b.shadow[0][sslot] = make_float4(shadow_org.x, shadow_org.y, shadow_org.z, __uint_as_float(p));
         b.shadow[1][sslot] = make_float4(shadow_dir.x, shadow_dir.y, shadow_dir.z, __uint_as_float(light_instance));
         b.shadow[2][sslot] = make_float4(direct.x, direct.y, direct.z,
-                                         __uint_as_float(light_hash | (has_emission_after ? 0x10000u : 0u)));
-        b.shadow[3][sslot] = make_float4(emission_after.x, emission_after.y, emission_after.z, 0.0f);
+                                         __uint_as_float(light_hash | (has_emission_after ? 0x10000u : 0u) |
+                                                         (light_exact ? 0x20000u : 0u)));
+        b.shadow[3][sslot] = make_float4(emission_after.x, emission_after.y, emission_after.z, shadow_distance);
     }
 }
 
 // ---------------------------------------------------------------- shadow --
 // integrate.glsl:168-192: closest hit without culling in [1e-3, 1e4];
 // visible iff the hit is the sampled light's instance and
-// (primitive + index_offset / 3) % 0xffff equals the light's hash.
+// (primitive + index_offset / 3) % 0xffff equals the light's hash.  Where the
+// hash is unambiguous the query is capped just past the light and ends at the
+// first occluder in front of it (ShadowQuery), with the same outcome.
 __global__ __launch_bounds__(TRACE_BLOCK) void shadow_kernel(tt::DeviceScene d, tt::PathBuffers b, uint32_t bounce) {
     __shared__ uint32_t lds[TRACE_LDS_STACK * TRACE_BLOCK];
     SceneView s = view(d);
@@ -383,13 +390,22 @@ __global__ __launch_bounds__(TRACE_BLOCK) void shadow_kernel(tt::DeviceScene d, 
         uint32_t i = base + lane_id();
         if (i < count) {
             float4 o = b.shadow[0][i], dd = b.shadow[1][i];
-            HitRecord h;
-            bool found = trace_closest<false>(s, f3(o.x, o.y, o.z), f3(dd.x, dd.y, dd.z), 1.0e-3f, 10000.0f, st, h);
             float4 direct = b.shadow[2][i];
+            float4 e = b.shadow[3][i];
             uint32_t meta = __float_as_uint(direct.w);
             uint32_t light_instance = __float_as_uint(dd.w);
+            bool exact = (meta & 0x20000u) != 0;
+            ShadowQuery q;
+            q.light_instance = light_instance;
+            q.hash = meta & 0xffffu;
+            q.t_exit = exact ? e.w * (1.0f - 1e-4f) - 1e-4f : -INFINITY;
+            q.occluded = false;
+            float tmax = exact ? fminf(e.w * (1.0f + 1e-4f) + 1e-4f, 10000.0f) : 10000.0f;
+            HitRecord h;
+            bool found =
+                trace_closest<false, true>(s, f3(o.x, o.y, o.z), f3(dd.x, dd.y, dd.z), 1.0e-3f, tmax, st, h, &q);
             bool visible = false;
-            if (found && h.instance == light_instance) {
+            if (found && !q.occluded && h.instance == light_instance) {
                 uint32_t index_offset = __float_as_uint(s.instances[52 * size_t(h.instance) + 49]);
                 uint32_t triangle_index = h.primitive + index_offset / 3u;
                 visible = triangle_index % 0xffffu == (meta & 0xffffu);
@@ -401,7 +417,6 @@ __global__ __launch_bounds__(TRACE_BLOCK) void shadow_kernel(tt::DeviceScene d, 
                 f3 acc(a.x, a.y, a.z);
                 if (visible) acc = acc + f3(direct.x, direct.y, direct.z);
                 if (emission) {
-                    float4 e = b.shadow[3][i];
                     acc = acc + f3(e.x, e.y, e.z);
                 }
                 b.radiance[p] = make_float4(acc.x, acc.y, acc.z, 0.0f);
