@@ -103,6 +103,12 @@ def main() -> int:
     mrays = mpaths * rays_per_path
     bytes_per_ray = RAY_RECORD_BYTES + HIT_RECORD_BYTES + info["expected_bytes_per_ray"]
     achieved_gbs = ext_rays * bytes_per_ray / (ext_ms * 1e-3) / 1e9
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # (scaled by a copy kernel of known size), committed under profiles/.
+    traffic = None
+    traffic_path = os.path.join(ROOT, "profiles", "extend_kernel_traffic.json")
+    if os.path.exists(traffic_path):
+        traffic = round(json.load(open(traffic_path))["bytes_per_launch"])
 
     result = {
         "metric": "Mpaths/s & Mrays/s at fixed RMSE vs FP64 oracle; p50/p99 request latency",
@@ -127,7 +133,7 @@ def main() -> int:
         "rays_per_path": round(rays_per_path, 4),
         "roofline": {"kernel": "extend_kernel", "bound": "hbm", "achieved": round(achieved_gbs, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
-                     "traffic": None, "bytes_per_ray": round(bytes_per_ray, 1),
+                     "traffic": traffic, "bytes_per_ray": round(bytes_per_ray, 1),
                      "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4),
                      "rays_per_launch": round(ext_rays / max(ext_launches, 1), 1)},
     }

@@ -26,9 +26,17 @@ namespace ttd {
 
 constexpr uint32_t LEAF = 0x80000000u;
 constexpr uint32_t SENTINEL = 0xffffffffu;
-constexpr int TRACE_BLOCK = 128;
-constexpr int TRACE_LDS_STACK = 32;
-constexpr int TRACE_SPILL_STACK = 64;  // TRACE_LDS_STACK + this >= BVH_STACK_LIMIT
+#ifndef TT_TRACE_BLOCK
+#define TT_TRACE_BLOCK 128
+#endif
+#ifndef TT_TRACE_LDS_STACK
+#define TT_TRACE_LDS_STACK 32
+#endif
+constexpr int TRACE_BLOCK = TT_TRACE_BLOCK;
+constexpr int TRACE_LDS_STACK = TT_TRACE_LDS_STACK;
+// LDS entries + spill entries cover the deepest two-level path the builder
+// allows (bvh.h BVH_STACK_LIMIT = 96).
+constexpr int TRACE_SPILL_STACK = 96 - TRACE_LDS_STACK;
 
 struct SceneView {
     const float4* nodes;      // BvhNode, 4 x float4

@@ -497,7 +497,10 @@ uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
 namespace tt {
 
-int trace_grid(int cu_count) { return cu_count * 8; }
+#ifndef TT_TRACE_GRID_PER_CU
+#define TT_TRACE_GRID_PER_CU 8
+#endif
+int trace_grid(int cu_count) { return cu_count * TT_TRACE_GRID_PER_CU; }
 size_t spill_words(int grid) { return size_t(grid) * TRACE_BLOCK * TRACE_SPILL_STACK; }
 uint32_t counter_words(uint32_t) { return C_WORDS; }
 
