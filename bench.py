@@ -1,14 +1,16 @@
 """Benchmark: Mpaths/s of the gfx950 path integrator on the 4M-triangle
-sphere field (1920x1080, 1 spp, max depth 8, PCG seed 0).
+sphere field (1920x1080, 1 spp, max depth 8, ReSTIR DI with 32 candidates,
+one temporal and one spatial reuse pass, PCG seed 0).
 
 A step is one render_to_target pass over the whole frame with the scene,
 its BVH and the path buffers already resident in HBM.  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--light-sampling restir|nee]
 """
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -33,6 +35,7 @@ def main() -> int:
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="wall time the CPU baseline keeps rendering whole frames for (at least one frame)")
+    p.add_argument("--light-sampling", choices=("restir", "nee"), default="restir")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="also run a plain device copy of this many MiB (reference kernel for HBM counters)")
     args = p.parse_args()
@@ -46,14 +49,16 @@ def main() -> int:
     from tinytrace_amd import Config, LightSampling, Renderer, RenderSettings, _abi, scenes
 
     scene, camera = scenes.sphere_field(seed=0)
-    settings = RenderSettings(rng=_abi.TT_RNG_PCG, seed=0)
+    restir = args.light_sampling == "restir"
+    mode = LightSampling.RESTIR if restir else LightSampling.NEXT_EVENT_ESTIMATION
+    settings = RenderSettings(rng=_abi.TT_RNG_PCG, seed=0, restir_candidates=32, restir_temporal=1, restir_spatial=1)
     renderer = Renderer((WIDTH, HEIGHT), settings=settings)
     t0 = time.perf_counter()
     renderer.set_scene(scene)
     build_s = time.perf_counter() - t0
     info = renderer.scene.info()
     renderer.camera = camera
-    renderer.set_config(Config(bounce_count=DEPTH, sample_count=1, light_sampling=LightSampling.NEXT_EVENT_ESTIMATION))
+    renderer.set_config(Config(bounce_count=DEPTH, sample_count=1, light_sampling=mode))
 
     import ctypes as C
 
@@ -86,9 +91,9 @@ def main() -> int:
 
     # Ray counts and per-kernel durations of a few more passes (HIP events on
     # the context's stream around every closest-hit launch).
-    renderer.settings = RenderSettings(rng=_abi.TT_RNG_PCG, seed=0, collect_stats=True)
-    ext_rays = shadow_rays = 0
-    ext_ms = shadow_ms = 0.0
+    renderer.settings = dataclasses.replace(settings, collect_stats=True)
+    ext_rays = shadow_rays = restir_rays = 0
+    ext_ms = shadow_ms = restir_ms = 0.0
     ext_launches = 0
     passes = 5
     for _ in range(passes):
@@ -98,8 +103,10 @@ def main() -> int:
         shadow_rays += s["shadow_rays"]
         ext_ms += s["extend_ms"]
         shadow_ms += s["shadow_ms"]
+        restir_rays += s["restir_rays"]
+        restir_ms += s["restir_ms"]
         ext_launches += s["extend_launches"]
-    rays_per_path = (ext_rays + shadow_rays) / (passes * paths)
+    rays_per_path = (ext_rays + shadow_rays + restir_rays) / (passes * paths)
     mrays = mpaths * rays_per_path
     bytes_per_ray = RAY_RECORD_BYTES + HIT_RECORD_BYTES + info["expected_bytes_per_ray"]
     achieved_gbs = ext_rays * bytes_per_ray / (ext_ms * 1e-3) / 1e9
@@ -110,6 +117,8 @@ def main() -> int:
     if os.path.exists(traffic_path):
         traffic = round(json.load(open(traffic_path))["bytes_per_launch"])
 
+    light_desc = ("ReSTIR DI (32 candidates, 1 temporal + 1 spatial reuse)" if restir
+                  else "next-event estimation")
     result = {
         "metric": "Mpaths/s & Mrays/s at fixed RMSE vs FP64 oracle; p50/p99 request latency",
         "value": round(mpaths, 3),
@@ -124,13 +133,16 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (procedural 4M-triangle sphere field, seed 0)",
         "config": {"workload": "sphere field, 800 icospheres (4 096 258 triangles, 256 emissive), 1920x1080, 1 spp, "
-                               "max depth 8, next-event estimation, PCG seed 0",
-                   "width": WIDTH, "height": HEIGHT, "spp": 1, "max_depth": DEPTH, "light_sampling": "nee",
+                               f"max depth 8, {light_desc}, PCG seed 0",
+                   "width": WIDTH, "height": HEIGHT, "spp": 1, "max_depth": DEPTH,
+                   "light_sampling": args.light_sampling,
                    "triangles": info["triangle_count"], "bvh_nodes": info["node_count"],
                    "bvh_expected_bytes_per_ray": round(info["expected_bytes_per_ray"], 1),
                    "scene_build_s": round(build_s, 3)},
         "mrays_per_s": round(mrays, 2),
         "rays_per_path": round(rays_per_path, 4),
+        "kernel_ms_per_step": {"extend": round(ext_ms / passes, 3), "shadow": round(shadow_ms / passes, 3),
+                               "restir": round(restir_ms / passes, 3)},
         "roofline": {"kernel": "extend_kernel", "bound": "hbm", "achieved": round(achieved_gbs, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
                      "traffic": traffic, "bytes_per_ray": round(bytes_per_ray, 1),
@@ -161,13 +173,18 @@ def main() -> int:
         threads = min(len(os.sched_getaffinity(0)), 16)
         oracle = OracleScene(scene, 32)
         pixels = np.arange(paths, dtype=np.uint32)
+        state = oracle.restir_state(WIDTH, HEIGHT) if restir else None
         frames = 0
         cpu_rays = 0
         t0 = time.perf_counter()
         while True:
             k = renderer.constants()
             renderer.frame_index += 1
-            _, rays = oracle.render_pixels(k, pixels, rng=_abi.TT_RNG_PCG, seed=0, threads=threads)
+            if restir:
+                k.accumulated_frame_count = 0
+                _, rays = oracle.render_restir(k, state, rng=_abi.TT_RNG_PCG, seed=0, candidates=32, threads=threads)
+            else:
+                _, rays = oracle.render_pixels(k, pixels, rng=_abi.TT_RNG_PCG, seed=0, threads=threads)
             frames += 1
             cpu_rays += int(rays.sum())
             cpu_s = time.perf_counter() - t0
@@ -176,7 +193,7 @@ def main() -> int:
         result["cpu_baseline"] = {"value": round(frames * paths / cpu_s / 1e6, 4), "unit": "Mpaths/s",
                                   "cores": threads, "kind": "port",
                                   "sample": f"{frames} whole frames of the same workload (1920x1080, 1 spp, depth 8, "
-                                            f"successive frame indices) on {threads} threads, {cpu_s:.1f} s",
+                                            f"{args.light_sampling}, successive frame indices) on {threads} threads, {cpu_s:.1f} s",
                                   "mrays_per_s": round(cpu_rays / cpu_s / 1e6, 3)}
         result["gpu_over_cpu"] = round(result["value"] / result["cpu_baseline"]["value"], 1)
     print(json.dumps(result))

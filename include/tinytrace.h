@@ -271,6 +271,8 @@ typedef struct tt_render_stats {
     double extend_ms;          /* sum of their durations (HIP events) */
     double shadow_ms;
     double integrate_ms;       /* whole render */
+    uint64_t restir_rays;      /* ReSTIR visibility-reuse queries */
+    double restir_ms;          /* ReSTIR passes (candidates, visibility, reuse) */
 } tt_render_stats;
 /* Statistics of the last render with TT_RENDER_COLLECT_STATS (synchronises). */
 tt_status tt_render_stats_get(tt_context* context, tt_render_stats* out);

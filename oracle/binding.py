@@ -37,6 +37,9 @@ def lib():
         L.oracle_render.argtypes = [vp, C.POINTER(_abi.Constants), u32, u32, vp, C.c_int, vp]
         L.oracle_render_pixels.restype = C.c_int
         L.oracle_render_pixels.argtypes = [vp, C.POINTER(_abi.Constants), u32, u32, vp, u32, vp, C.c_int, vp]
+        L.oracle_render_restir.restype = C.c_int
+        L.oracle_render_restir.argtypes = [vp, C.POINTER(_abi.Constants), u32, u32, u32, C.c_int, C.c_int, vp, vp,
+                                           u32, u32, vp, C.c_int, vp]
         L.oracle_trace.restype = C.c_int
         L.oracle_trace.argtypes = [vp, vp, vp, u32, u32, C.c_int]
         L.oracle_bsdf_evaluate.restype = C.c_int
@@ -103,6 +106,27 @@ class OracleScene:
             raise RuntimeError("oracle_render_pixels: unsupported settings")
         return out, rays
 
+    def restir_state(self, width, height):
+        return {"gbuffer": np.zeros((2, height * width, 16), dtype=np.float32),
+                "reservoirs": np.zeros((2, height * width, 4), dtype=np.uint32), "cur": 0, "history": 0}
+
+    def render_restir(self, constants, state, rng=0, seed=0, candidates=32, temporal=True, spatial=True,
+                      previous=None, threads=0):
+        """One ReSTIR DI frame; `state` (from restir_state) carries the
+        G-buffer and reservoir history between frames."""
+        w, h = constants.screen_size[0], constants.screen_size[1]
+        out = np.zeros((h, w, 4), dtype=np.float32) if previous is None else np.array(previous, dtype=np.float32)
+        rays = np.zeros(2, dtype=np.uint64)
+        status = lib().oracle_render_restir(self.handle, C.byref(constants), rng, seed, candidates, int(temporal),
+                                            int(spatial), state["gbuffer"].ctypes.data,
+                                            state["reservoirs"].ctypes.data, state["cur"], state["history"],
+                                            out.ctypes.data, threads, rays.ctypes.data)
+        if status != 0:
+            raise RuntimeError("oracle_render_restir: needs the float oracle and one sample per pixel")
+        state["cur"] ^= 1
+        state["history"] = 1
+        return out, rays
+
     def trace(self, rays, cull, threads=0):
         rays = np.ascontiguousarray(rays, dtype=_abi.RAY_DTYPE)
         hits = np.zeros(rays.shape[0], dtype=_abi.HIT_DTYPE)
@@ -137,7 +161,7 @@ def rng_draws(kind, x, y, width, frame, seed, n):
     return int(state[0]), out
 
 
-MATH_OPS = {"sin": 0, "cos": 1, "exp": 2, "log": 3, "acos": 4, "asin": 5, "pow": 6, "sqrt": 7}
+MATH_OPS = {"sin": 0, "cos": 1, "exp": 2, "log": 3, "acos": 4, "asin": 5, "pow": 6, "sqrt": 7, "half": 8}
 
 
 def math_f32(op, x, y=None):

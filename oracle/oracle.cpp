@@ -19,7 +19,7 @@
 #include <memory>
 #include <thread>
 
-#include "oracle_scene.hpp"
+#include "oracle_restir.hpp"
 
 using namespace oracle;
 
@@ -173,6 +173,171 @@ int oracle_render_pixels(void* handle, const tt_constants* k, uint32_t rng, uint
     return 0;
 }
 
+// One ReSTIR DI frame (float oracle).  gbuffer: 2 x W*H x 16 floats,
+// reservoirs: 2 x W*H x 4 words (final reservoirs); [cur] is written, [cur^1]
+// is last frame's and is read when `history` != 0.  rgba as oracle_render.
+int oracle_render_restir(void* handle, const tt_constants* k, uint32_t rng, uint32_t seed, uint32_t candidates,
+                         int temporal, int spatial, float* gbuffer, uint32_t* reservoirs, uint32_t cur,
+                         uint32_t history, float* rgba, int threads, uint64_t* rays) {
+    auto* os = static_cast<OracleScene*>(handle);
+    if (os->precision != 32 || k->sample_count != 1) return 1;
+    const Scene<float>& s = *os->f;
+    Camera<float> cam = make_camera<float>(*k);
+    uint32_t w = k->screen_size[0], h = k->screen_size[1];
+    uint32_t px = w * h;
+    uint32_t frame = k->frame_index;
+    int nthreads = threads <= 0 ? int(std::thread::hardware_concurrency()) : threads;
+    std::vector<TraceCounts> counts(std::max(1, nthreads));
+    std::vector<Generator> gens(px);
+    std::vector<V3<float>> dirs(px);
+    std::vector<Hit> hits(px);
+    std::vector<uint8_t> found(px);
+    float* gcur = gbuffer + size_t(cur) * px * 16;
+    const float* gprev = gbuffer + size_t(cur ^ 1u) * px * 16;
+    uint32_t* fcur = reservoirs + size_t(cur) * px * 4;
+    const uint32_t* fprev = reservoirs + size_t(cur ^ 1u) * px * 4;
+    std::vector<uint32_t> initial(size_t(px) * 4), temporal_out(size_t(px) * 4);
+    bool has_lights = s.desc->emissive_triangle_count > 0;
+    Settings st{rng, seed};
+
+    // Camera rays and primary hits; G-buffer records.
+    parallel_for(px, 64, nthreads, [&](int tid, uint32_t p) {
+        uint32_t x = p % w, y = p / w;
+        Generator g = init_generator_from_pixel(x, y, w, frame, st.rng, st.seed);
+        float ox = random_float<float>(g);
+        float oy = random_float<float>(g);
+        ox = ox - 0.5f;
+        oy = oy - 0.5f;
+        float ndc_x = ((float(x) + ox) / float(w)) * 2.0f - 1.0f;
+        float ndc_y = ((float(y) + oy) / float(h)) * 2.0f - 1.0f;
+        V3<float> dir = camera_ray_direction(cam, ndc_x, ndc_y);
+        gens[p] = g;
+        dirs[p] = dir;
+        counts[tid].extension++;
+        found[p] = s.closest_hit(cam.position, dir, true, 1.0e-3f, 1000.0f, hits[p]) ? 1 : 0;
+        float* g16 = gcur + size_t(p) * 16;
+        if (!found[p]) {
+            g16[4] = g16[5] = g16[6] = g16[7] = 0.0f;
+            return;
+        }
+        RayHit<float> hit = get_ray_hit(s, hits[p]);
+        ShadingF q;
+        q.m = s.materials[hit.material_index];
+        q.material = hit.material_index;
+        q.position = hit.world_position;
+        q.basis = create_surface_basis(hit.tangent_space, q.m.geometry_normal);
+        q.wi = transform_from_basis(q.basis, -dir);
+        q.valid = has_lights && q.m.specular_roughness > 0.15f;
+        q.depth = length(q.position - cam.position);
+        write_record(g16, q, -dir, frame);
+    });
+    if (has_lights) {
+        // Candidates and visibility.
+        uint32_t* after_candidates = (temporal || spatial) ? initial.data() : fcur;
+        parallel_for(px, 64, nthreads, [&](int tid, uint32_t p) {
+            Res res;
+            const float* g16 = gcur + size_t(p) * 16;
+            if (found[p] && g16[7] != 0.0f) {
+                ShadingF q = read_record(s, g16, p);
+                Generator gen = pass_generator(p, frame, seed, 1u);
+                float sum = 0.0f, p_sel = 0.0f;
+                uint32_t sel_light = 0xffffffffu, sel_bary = 0u;
+                for (uint32_t j = 0; j < candidates; j++) {
+                    uint32_t li = random_uint(gen, s.desc->emissive_triangle_count);
+                    V2<float> r2 = random_vec2<float>(gen);
+                    uint32_t bary = quantize_bary(sample_triangle(r2));
+                    float area;
+                    float ph = restir_target(s, q, li, bary, &area);
+                    float wgt = ph * (float(s.desc->emissive_triangle_count) * area);
+                    sum = sum + wgt;
+                    float u = random_float<float>(gen);
+                    if (u * sum < wgt) {
+                        sel_light = li;
+                        sel_bary = bary;
+                        p_sel = ph;
+                    }
+                }
+                res.M = float(candidates);
+                if (sel_light != 0xffffffffu && p_sel > 0.0f) {
+                    res.light = sel_light;
+                    res.bary = sel_bary;
+                    res.W = (sum / float(candidates)) / p_sel;
+                    LightPointF lp = light_point(s, sel_light, reservoir_bary(sel_bary));
+                    V3<float> to_light = lp.position - q.position;
+                    float dist = Fn<float>::sqrt(length_squared(to_light));
+                    V3<float> dir = to_light / dist;
+                    if (!light_visible(s, q.position, dir, lp, counts[tid])) res.W = 0.0f;
+                }
+            }
+            pack(res, after_candidates + size_t(p) * 4);
+        });
+        const uint32_t* current = after_candidates;
+        if (temporal) {
+            uint32_t* dst = spatial ? temporal_out.data() : fcur;
+            parallel_for(px, 64, nthreads, [&](int, uint32_t p) {
+                Res rq = unpack(current + size_t(p) * 4);
+                ShadingF q = read_record(s, gcur + size_t(p) * 16, p);
+                if (q.valid && history) {
+                    ShadingF o = read_record(s, gprev + size_t(p) * 16, p);
+                    if (similar(q, o)) {
+                        Res ro = unpack(fprev + size_t(p) * 4);
+                        ro.M = rmin(ro.M, 20.0f * float(candidates));
+                        Generator gen = pass_generator(p, frame, seed, 2u);
+                        rq = combine(s, q, rq, o, ro, gen);
+                    }
+                }
+                pack(rq, dst + size_t(p) * 4);
+            });
+            current = dst;
+        }
+        if (spatial) {
+            parallel_for(px, 64, nthreads, [&](int, uint32_t p) {
+                Res rq = unpack(current + size_t(p) * 4);
+                ShadingF q = read_record(s, gcur + size_t(p) * 16, p);
+                if (q.valid) {
+                    Generator gen = pass_generator(p, frame, seed, 3u);
+                    float angle = 2.0f * K<float>::PI * random_float<float>(gen);
+                    float radius = 30.0f * Fn<float>::sqrt(random_float<float>(gen));
+                    int x = int(p % w), y = int(p / w);
+                    int nx = int(std::floor(float(x) + radius * ttm::cos(angle) + 0.5f));
+                    int ny = int(std::floor(float(y) + radius * ttm::sin(angle) + 0.5f));
+                    nx = nx < 0 ? 0 : (nx >= int(w) ? int(w) - 1 : nx);
+                    ny = ny < 0 ? 0 : (ny >= int(h) ? int(h) - 1 : ny);
+                    uint32_t n = uint32_t(ny) * w + uint32_t(nx);
+                    if (n != p) {
+                        ShadingF o = read_record(s, gcur + size_t(n) * 16, n);
+                        if (similar(q, o)) rq = combine(s, q, rq, o, unpack(current + size_t(n) * 4), gen);
+                    }
+                }
+                pack(rq, fcur + size_t(p) * 4);
+            });
+        }
+    }
+    // Paths, with the final reservoir at bounce 0; progressive accumulation.
+    parallel_for(px, 64, nthreads, [&](int tid, uint32_t p) {
+        V3<float> c = trace_path_restir(s, cam, k->bounce_count, dirs[p], gens[p], found[p] != 0, hits[p],
+                                        fcur + size_t(p) * 4, p, frame, counts[tid]);
+        c = c / float(k->sample_count);
+        float* out = rgba + 4 * size_t(p);
+        if (k->accumulated_frame_count != 0) {
+            V3<float> prev{out[0], out[1], out[2]};
+            c = (prev * float(k->accumulated_frame_count - 1) + c) / float(k->accumulated_frame_count);
+        }
+        out[0] = c.x;
+        out[1] = c.y;
+        out[2] = c.z;
+        out[3] = 1.0f;
+    });
+    if (rays) {
+        rays[0] = rays[1] = 0;
+        for (auto& ct : counts) {
+            rays[0] += ct.extension;
+            rays[1] += ct.shadow;
+        }
+    }
+    return 0;
+}
+
 int oracle_trace(void* handle, const tt_ray* rays, tt_hit* hits, uint32_t n, uint32_t flags, int threads) {
     auto* s = static_cast<OracleScene*>(handle);
     bool cull = (flags & TT_TRACE_CULL_BACK_FACES) != 0;
@@ -294,6 +459,7 @@ void oracle_math_f32(int op, const float* x, const float* y, float* out, uint32_
             case 4: out[i] = ttm::acos(v); break;
             case 5: out[i] = ttm::asin(v); break;
             case 6: out[i] = ttm::pow(v, y[i]); break;
+            case 8: out[i] = ttm::half_to_float(ttm::float_to_half(v)); break;
             default: out[i] = ttm::sqrt(v); break;
         }
     }

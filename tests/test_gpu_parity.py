@@ -203,6 +203,47 @@ def test_render_sphere_field_matches_oracle():
     assert_radiance_close(image, reference)
 
 
+def restir_pair(scene, camera, config, width, height, frames, settings):
+    """ReSTIR frames through the Renderer and the oracle; every frame's image
+    is compared, so temporal and spatial reuse see the device's own history."""
+    renderer = Renderer((width, height), settings=settings)
+    renderer.set_scene(scene)
+    renderer.camera = camera
+    renderer.set_config(config)
+    oracle = _oracle().OracleScene(scene, 32)
+    state = oracle.restir_state(width, height)
+    pairs = []
+    for _ in range(frames):
+        k = renderer.constants()
+        renderer.reset_accumulation()
+        image = renderer.render_to_texture()
+        reference, _ = oracle.render_restir(k, state, rng=settings.rng, seed=settings.seed,
+                                            candidates=settings.restir_candidates,
+                                            temporal=bool(settings.restir_temporal),
+                                            spatial=bool(settings.restir_spatial))
+        pairs.append((image, reference))
+    return pairs
+
+
+@pytest.mark.parametrize("reuse", [(False, False), (True, False), (True, True)])
+def test_render_restir_cornell_matches_oracle(reuse):
+    scene, camera = scenes.cornell_box()
+    config = Config(bounce_count=3, sample_count=1, light_sampling=LightSampling.RESTIR)
+    settings = RenderSettings(rng=_abi.TT_RNG_PCG, seed=3, restir_candidates=8, restir_temporal=reuse[0],
+                              restir_spatial=reuse[1])
+    for image, reference in restir_pair(scene, camera, config, 64, 48, 3, settings):
+        assert_radiance_close(image, reference)
+
+
+def test_render_restir_sphere_field_matches_oracle():
+    scene, camera = scenes.sphere_field(seed=1, spheres=80, lights=96, subdivisions=2)
+    config = Config(bounce_count=4, sample_count=1, light_sampling=LightSampling.RESTIR)
+    settings = RenderSettings(rng=_abi.TT_RNG_PCG, seed=0, restir_candidates=32, restir_temporal=True,
+                              restir_spatial=True)
+    for image, reference in restir_pair(scene, camera, config, 96, 64, 3, settings):
+        assert_radiance_close(image, reference)
+
+
 def test_render_is_deterministic():
     scene, camera = scenes.cornell_box()
     renderer = Renderer((64, 64))

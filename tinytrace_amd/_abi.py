@@ -75,7 +75,8 @@ class RenderSettings(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("extend_launches", C.c_uint32), ("shadow_launches", C.c_uint32), ("extend_ms", C.c_double),
-                ("shadow_ms", C.c_double), ("integrate_ms", C.c_double)]
+                ("shadow_ms", C.c_double), ("integrate_ms", C.c_double), ("restir_rays", C.c_uint64),
+                ("restir_ms", C.c_double)]
 
 
 class CameraDesc(C.Structure):

@@ -405,14 +405,21 @@ bool build_bvh(const tt_scene_desc& d, BvhBuildResult& out) {
         }
         uint32_t root = blas[in.mesh_index].empty ? (BVH_LEAF | 0x7fffffffu) : blas_root[in.mesh_index];
         std::memcpy(&bi.c0[3], &root, 4);
-        // c1.w: first global triangle index of the mesh (index_offset / 3);
-        // c2.w: 1 when index_offset / 3 + triangle count <= 65535, i.e. the
-        // emissive hash names exactly one triangle of this instance.
+        // c1.w: first global triangle index of the mesh (index_offset / 3).
         uint32_t base = in.index_offset / 3;
-        uint32_t exact = uint64_t(base) + d.meshes[in.mesh_index].index_count / 3 <= 65535u ? 1u : 0u;
         std::memcpy(&bi.c1[3], &base, 4);
-        std::memcpy(&bi.c2[3], &exact, 4);
+        bi.c2[3] = 0.0f;
         bi.c3[3] = 0.0f;
+    }
+    // Padded world boxes (empty meshes keep lo = +inf > hi = -inf: no ray
+    // enters them).
+    out.instance_boxes.assign(size_t(d.instance_count) * 8, 0.0f);
+    for (uint32_t i = 0; i < d.instance_count; i++) {
+        float* b = &out.instance_boxes[size_t(i) * 8];
+        for (int r = 0; r < 3; r++) {
+            b[r] = world[i].lo[r];
+            b[4 + r] = world[i].hi[r];
+        }
     }
 
     // Two-level SAH expectation: TLAS nodes, then per instance leaf the

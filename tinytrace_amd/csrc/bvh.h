@@ -15,8 +15,11 @@
 //                      primitive index.
 //   insts : BvhInstance[] 64 B, the instance's f32 inverse_transform columns
 //                      (xyz); c0.w the BLAS root reference, c1.w the mesh's
-//                      first global triangle (index_offset / 3), c2.w 1 when
-//                      the emissive hash is unambiguous on this instance.
+//                      first global triangle (index_offset / 3).
+//   boxes : 2 x float4 per instance, its padded world box (lo.xyz, hi.xyz);
+//                      every hit on the instance lies inside, so the
+//                      conservative slab interval of a ray against it bounds
+//                      the hit distances of all its triangles (shadow rays).
 // Node references (u32): bit 31 clear = inner node index; set = leaf.  TLAS
 // leaves hold one instance (bits 0..30); BLAS leaves hold a triangle range:
 // first slot in bits 0..26, count-1 in bits 27..30 (count <= 16).
@@ -64,6 +67,7 @@ struct BvhBuildResult {
     std::vector<BvhNode> nodes;
     std::vector<BvhTri> tris;
     std::vector<BvhInstance> instances;
+    std::vector<float> instance_boxes;  // 8 per instance: lo.xyz, 0, hi.xyz, 0
     uint32_t tlas_root = BVH_LEAF;  // BVH_LEAF | 0x7fffffff = empty scene
     bool empty = true;
     uint32_t max_depth = 0;

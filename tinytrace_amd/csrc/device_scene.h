@@ -42,6 +42,7 @@ struct SceneView {
     const float4* nodes;      // BvhNode, 4 x float4
     const float4* tris;       // BvhTri, 3 x float4
     const float4* bvh_insts;  // BvhInstance, 4 x float4
+    const float4* inst_boxes; // padded world box per instance, 2 x float4
     uint32_t tlas_root;
     uint32_t emissive_count;
     const float* instances;   // tt_instance, 52 floats (208 B)
@@ -177,19 +178,19 @@ struct Stack {
     }
 };
 
-// A shadow query: the ray was aimed at a point sampled on the light
-// triangle identified by (light_instance, hash).  The reference accepts the
-// light iff the closest hit in [tmin, 1e4] is that triangle
-// (integrate.glsl:168-185).  When the light mesh's triangle indices are
-// unambiguous under the hash (index_offset / 3 + triangle count <= 65535),
-// only the sampled triangle passes the check and its hit distance lies within
-// 1e-4 (relative and absolute) of the sample distance d, so the caller may
-// cap tmax at d * (1 + 1e-4) + 1e-4 and stop at the first non-light hit
-// closer than t_exit = d * (1 - 1e-4) - 1e-4: neither changes the decision.
+// A shadow query: the ray was aimed at a point sampled on an emissive
+// triangle of instance light_instance whose emissive hash is `hash`.  The
+// reference accepts the light iff the closest hit in [1e-3, 1e4] lies on that
+// instance and (primitive + index_offset / 3) % 0xffff == hash
+// (integrate.glsl:168-185).  Every hit on the instance lies in the
+// conservative slab interval [t_entry, t_far] of the ray against the
+// instance's padded world box, so the query is capped at t_far and ends at
+// the first hit that fails the check with t < t_entry: a strictly closer hit
+// that is not the light decides "occluded" whatever lies behind it.
 struct ShadowQuery {
     uint32_t light_instance;
     uint32_t hash;
-    float t_exit;   // -inf disables the early exit
+    float t_exit;   // t_entry of the light instance's box (NaN: no early exit)
     bool occluded;  // set when a non-light hit closer than t_exit ended the query
 };
 
@@ -304,6 +305,34 @@ TT_DEV bool trace_closest(const SceneView& s, f3 org, f3 dir, float tmin, float 
             break;
         }
     }
+}
+
+// The reference's shadow-ray rule for a light sample (see ShadowQuery).
+TT_DEV bool light_visible(const SceneView& s, f3 org, f3 dir, uint32_t light_instance, uint32_t hash, Stack& stack) {
+    const float4* box = s.inst_boxes + 2 * size_t(light_instance);
+    float4 lo = box[0], hi = box[1];
+    TraceRay r;
+    slab_setup(r, org, dir);
+    float t0x = (lo.x - r.olo.x) * r.inv.x, t1x = (hi.x - r.ohi.x) * r.inv.x;
+    float t0y = (lo.y - r.olo.y) * r.inv.y, t1y = (hi.y - r.ohi.y) * r.inv.y;
+    float t0z = (lo.z - r.olo.z) * r.inv.z, t1z = (hi.z - r.ohi.z) * r.inv.z;
+    float n = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    float f = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    float t_entry = n - abs_(n) * 1e-6f;
+    float t_far = f + abs_(f) * 1e-6f;
+    const float tmin = 1.0e-3f;
+    float tmax = fminf(t_far, 10000.0f);
+    if (!(fmaxf(t_entry, tmin) <= tmax)) return false;  // the light cannot be hit
+    ShadowQuery q;
+    q.light_instance = light_instance;
+    q.hash = hash;
+    q.t_exit = t_entry;
+    q.occluded = false;
+    HitRecord h;
+    bool found = trace_closest<false, true>(s, org, dir, tmin, tmax, stack, h, &q);
+    if (!found || q.occluded || h.instance != light_instance) return false;
+    uint32_t index_offset = __float_as_uint(s.instances[52 * size_t(h.instance) + 49]);
+    return (h.primitive + index_offset / 3u) % 0xffffu == hash;
 }
 
 }  // namespace ttd

@@ -1,0 +1,245 @@
+// Shared device helpers of the wavefront kernels: wave-level queue
+// primitives, the surface at a hit (get_ray_hit + create_surface_basis,
+// src/integrate/integrate.glsl:22-110, :195-201), emissive-triangle geometry
+// (src/includes/scene.glsl:114-136) and the packed ReSTIR reservoir.
+#pragma once
+
+#include "device_bsdf.h"
+#include "device_scene.h"
+#include "kernels.h"
+
+namespace ttd {
+
+TT_DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+TT_DEV uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+// Wave-level stream compaction: every active lane calls this; lanes with
+// `pred` get consecutive slots from one atomic add per wave.
+TT_DEV uint32_t wave_append(uint32_t* counter, bool pred) {
+    uint64_t active = __ballot(1);
+    uint64_t mask = __ballot(pred);
+    uint32_t leader = uint32_t(__ffsll((unsigned long long)active) - 1);
+    uint32_t total = uint32_t(__popcll(mask));
+    uint32_t base = 0;
+    if (lane_id() == leader && total != 0) base = atomicAdd(counter, total);
+    base = __shfl(base, int(leader));
+    return base + lanes_below(mask);
+}
+
+// Persistent-wave cursor: lane 0 takes the next 64 items.
+TT_DEV uint32_t next_batch(uint32_t* cursor) {
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(cursor, 64u);
+    return __shfl(base, 0);
+}
+
+__device__ __host__ inline SceneView view(const tt::DeviceScene& d) {
+    SceneView s;
+    s.nodes = static_cast<const float4*>(d.nodes);
+    s.tris = static_cast<const float4*>(d.tris);
+    s.bvh_insts = static_cast<const float4*>(d.bvh_insts);
+    s.inst_boxes = static_cast<const float4*>(d.inst_boxes);
+    s.tlas_root = d.tlas_root;
+    s.emissive_count = d.emissive_count;
+    s.instances = static_cast<const float*>(d.instances);
+    s.materials = static_cast<const float4*>(d.materials);
+    s.positions = static_cast<const int16_t*>(d.positions);
+    s.vertices = static_cast<const uint2*>(d.vertices);
+    s.indices = static_cast<const uint32_t*>(d.indices);
+    s.emissive = static_cast<const uint32_t*>(d.emissive);
+    return s;
+}
+
+TT_DEV Stack make_stack(uint32_t* lds, uint32_t* spill) {
+    Stack st;
+    st.lds = lds + threadIdx.x;
+    st.spill = spill + blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    st.spill_stride = gridDim.x * TRACE_BLOCK;
+    st.sp = 0;
+    return st;
+}
+
+// ----------------------------------------------------------- tangent frames --
+TT_DEV f3 tt_octahedron_decode(float u, float v) {
+    float sx = u * 2.0f - 1.0f, sy = v * 2.0f - 1.0f;
+    f3 n(sx, sy, 1.0f - abs_(sx) - abs_(sy));
+    float t = clamp_(-n.z, 0.0f, 1.0f);
+    n.x = n.x + (n.x >= 0.0f ? -t : t);
+    n.y = n.y + (n.y >= 0.0f ? -t : t);
+    return normalize(n);
+}
+
+struct TangentFrame {
+    f3 normal, tangent;
+    float bitangent_sign;
+};
+
+// scene.glsl:29-64
+TT_DEV TangentFrame decode_tangent_frame(uint32_t e) {
+    float u = float(e & 0x3ffu) / float(1023);
+    float v = float((e >> 10) & 0x3ffu) / float(1023);
+    TangentFrame f;
+    f.normal = tt_octahedron_decode(u, v);
+    f3 ortho = abs_(f.normal.x) > abs_(f.normal.z) ? f3(-f.normal.y, f.normal.x, 0.0f) : f3(0.0f, -f.normal.z, f.normal.y);
+    float angle = (float((e >> 20) & 0x7ffu) / float(2047)) * TAU;
+    f.tangent = ortho * ttm::cos(angle) + cross(f.normal, ortho) * ttm::sin(angle);
+    f.bitangent_sign = (e >> 31) == 1u ? 1.0f : -1.0f;
+    return f;
+}
+
+struct Basis {
+    f3 normal, tangent, bitangent;
+};
+
+TT_DEV f3 transform_to_basis(const Basis& b, f3 d) { return normalize(d.x * b.tangent + d.y * b.bitangent + d.z * b.normal); }
+TT_DEV f3 transform_from_basis(const Basis& b, f3 d) {
+    return normalize(f3(dot(d, b.tangent), dot(d, b.bitangent), dot(d, b.normal)));
+}
+TT_DEV f3 interpolate(f3 b, f3 x, f3 y, f3 z) { return b.x * x + b.y * y + b.z * z; }
+
+TT_DEV f3 dequantize_snorm16(int16_t x, int16_t y, int16_t z) {
+    return f3(float(x) / 32767.0f, float(y) / 32767.0f, float(z) / 32767.0f);
+}
+
+struct M4 {
+    float m[16];
+};
+TT_DEV M4 load_m4(const float* p) {
+    M4 r;
+    const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        float4 v = q[c];
+        r.m[4 * c + 0] = v.x;
+        r.m[4 * c + 1] = v.y;
+        r.m[4 * c + 2] = v.z;
+        r.m[4 * c + 3] = v.w;
+    }
+    return r;
+}
+
+// The shading point of a closest hit: get_ray_hit without the texture terms
+// (no texture is ever bound) and create_surface_basis.
+struct Surface {
+    f3 position;
+    Basis basis;  // surface basis (geometry normal applied)
+    f3 wi;        // -ray direction in the surface basis
+    uint32_t material;
+};
+
+TT_DEV Surface surface_at_hit(const SceneView& s, uint32_t instance, uint32_t prim, float b1, float b2, f3 ray_dir,
+                              Mat& m) {
+    const float* inst = s.instances + 52 * size_t(instance);
+    M4 transform = load_m4(inst);
+    M4 normal_transform = load_m4(inst + 32);
+    uint4 offs = *reinterpret_cast<const uint4*>(inst + 48);
+    uint32_t vertex_offset = offs.x, index_offset = offs.y;
+    uint32_t base = 3 * prim + index_offset;
+    f3 pos[3];
+    TangentFrame fr[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        uint32_t vi = s.indices[base + c] + vertex_offset;
+        const int16_t* pp = s.positions + 3 * size_t(vi);
+        pos[c] = mul_point1(transform.m, dequantize_snorm16(pp[0], pp[1], pp[2]));
+        fr[c] = decode_tangent_frame(s.vertices[vi].y);
+    }
+    f3 bary(0.0f, b1, b2);
+    bary.x = 1.0f - bary.y - bary.z;
+    Basis ts;
+    ts.normal = normalize(mul_dir(normal_transform.m, interpolate(bary, fr[0].normal, fr[1].normal, fr[2].normal)));
+    ts.tangent = normalize(mul_dir(normal_transform.m, interpolate(bary, fr[0].tangent, fr[1].tangent, fr[2].tangent)));
+    float sign = sign_(fr[0].bitangent_sign * bary.x + fr[1].bitangent_sign * bary.y + fr[2].bitangent_sign * bary.z);
+    ts.bitangent = normalize(sign * cross(ts.normal, ts.tangent));
+    Surface sf;
+    sf.position = bary.x * pos[0] + bary.y * pos[1] + bary.z * pos[2];
+    sf.material = offs.z;
+    m = load_material(s.materials + 19 * size_t(offs.z));
+    sf.basis.normal = transform_to_basis(ts, m.geometry_normal);
+    sf.basis.tangent = normalize(ts.tangent - sf.basis.normal * dot(sf.basis.normal, ts.tangent));
+    sf.basis.bitangent = normalize(cross(sf.basis.normal, sf.basis.tangent));
+    sf.wi = transform_from_basis(sf.basis, -ray_dir);
+    return sf;
+}
+
+// ---------------------------------------------------------------- lights --
+struct LightPoint {
+    f3 position, normal, emission;
+    float area;
+    uint32_t instance, hash;
+};
+
+// The world-space triangle of emissive record `li` at barycentrics `lb`
+// (sample_random_light, scene.glsl:114-136; material_emissive).
+TT_DEV LightPoint light_point(const SceneView& s, uint32_t li, f3 lb) {
+    const uint32_t* et = s.emissive + 9 * size_t(li);
+    uint32_t w[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) w[k] = et[k];
+    auto half = [&](int k) { return int16_t((w[k >> 1] >> (16 * (k & 1))) & 0xffffu); };
+    LightPoint lp;
+    lp.hash = w[4] >> 16;
+    lp.instance = w[8];
+    const float* linst = s.instances + 52 * size_t(lp.instance);
+    M4 ltr = load_m4(linst);
+    f3 p[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) p[c] = mul_point1(ltr.m, dequantize_snorm16(half(3 * c), half(3 * c + 1), half(3 * c + 2)));
+    lp.area = fmax_(0.00001f, 0.5f * length(cross(p[1] - p[0], p[2] - p[0])));
+    lp.normal = normalize(cross(p[1] - p[0], p[2] - p[0]));
+    lp.position = interpolate(lb, p[0], p[1], p[2]);
+    Mat lm = load_material(s.materials + 19 * size_t(__float_as_uint(linst[50])));
+    lp.emission = lm.emission_color * lm.emission_luminance;
+    return lp;
+}
+
+// ---------------------------------------------------------------- ReSTIR --
+// The lobes (bsdf_prepare's stochastic albedo estimates) at a primary vertex
+// in ReSTIR mode come from this generator, so the reservoir target at a
+// pixel can be rebuilt from its G-buffer record (pixel, frame) anywhere.
+TT_DEV Generator restir_lobe_generator(uint32_t pixel, uint32_t frame) {
+    return Generator{pcg_hash(pixel ^ pcg_hash(frame ^ 0xa511e9b3u)), 1u};
+}
+
+// Reservoir (16 B): x = emissive triangle (0xffffffff = empty), y = the
+// sample's barycentrics (b0, b1) as unorm16, z = W | M as IEEE half, w = 0.
+struct Reservoir {
+    uint32_t light;
+    uint32_t bary;
+    float W, M;
+};
+
+TT_DEV f3 reservoir_bary(uint32_t packed) {
+    float b0 = float(packed & 0xffffu) / 65535.0f;
+    float b1 = float(packed >> 16) / 65535.0f;
+    return f3(b0, b1, 1.0f - b0 - b1);
+}
+
+TT_DEV uint32_t quantize_bary(f3 b) {
+    uint32_t u = uint32_t(b.x * 65535.0f + 0.5f);
+    uint32_t v = uint32_t(b.y * 65535.0f + 0.5f);
+    if (u > 65535u) u = 65535u;
+    if (u + v > 65535u) v = 65535u - u;
+    return u | (v << 16);
+}
+
+TT_DEV uint4 pack_reservoir(const Reservoir& r) {
+    // W saturates at the largest finite half.
+    return make_uint4(r.light, r.bary,
+                      uint32_t(ttm::float_to_half(fmin_(r.W, 65504.0f))) | (uint32_t(ttm::float_to_half(r.M)) << 16),
+                      0u);
+}
+
+TT_DEV Reservoir unpack_reservoir(uint4 v) {
+    Reservoir r;
+    r.light = v.x;
+    r.bary = v.y;
+    r.W = ttm::half_to_float(uint16_t(v.z & 0xffffu));
+    r.M = ttm::half_to_float(uint16_t(v.z >> 16));
+    return r;
+}
+
+}  // namespace ttd

@@ -1,4 +1,5 @@
-// Host-side launchers of the gfx950 kernels (defined in kernels.hip).
+// Host-side launchers of the gfx950 kernels (defined in kernels.hip and
+// restir.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -12,6 +13,7 @@ struct DeviceScene {
     const void* nodes;
     const void* tris;
     const void* bvh_insts;
+    const void* inst_boxes;
     uint32_t tlas_root;
     uint32_t emissive_count;
     const void* instances;
@@ -44,16 +46,34 @@ struct PathBuffers {
     float4* shadow[4];    // origin, direction, direct, emission: [paths] each
     uint32_t* counters;   // see kernels.hip
     uint32_t* spill;      // traversal stack spill
+    const uint4* restir_final;  // per-pixel reservoirs used at the primary vertex
     uint32_t max_bounces;
 };
 
-struct TraceLaunch {
-    int grid;  // persistent trace workgroups
+// Per-target ReSTIR DI state (Bitterli et al. 2020): a G-buffer record
+// (4 x float4) and reservoirs (uint4, FP16 W and M) per pixel.
+struct RestirBuffers {
+    float4* gbuffer[2];     // [cur] this frame, [cur ^ 1] the previous one
+    uint4* initial;         // after candidate RIS + visibility
+    uint4* temporal;        // after temporal reuse
+    uint4* final_[2];       // after spatial reuse; [cur ^ 1] = previous frame
+    uint32_t cur;
+    uint32_t history_valid;
+    uint32_t candidates;
+    uint32_t temporal_reuse;
+    uint32_t spatial_reuse;
 };
 
 int trace_grid(int cu_count);
 size_t spill_words(int grid);
-uint32_t counter_words(uint32_t max_bounces);
+// Counter words (u32, zeroed at the start of every render).
+constexpr uint32_t C_EXT = 0;            // [b]: rays in the queue consumed at bounce b
+constexpr uint32_t C_SHADOW = 32;        // [b]: shadow rays emitted at bounce b
+constexpr uint32_t C_WORK_EXT = 64;      // [b]: persistent cursor, extend b
+constexpr uint32_t C_WORK_SHADOW = 96;   // [b]: persistent cursor, shadow b
+constexpr uint32_t C_RESTIR_VIS = 128;   // ReSTIR visibility queries
+constexpr uint32_t C_WORK_RESTIR = 129;  // their persistent cursor
+constexpr uint32_t C_WORDS = 160;
 
 hipError_t launch_zero(hipStream_t s, uint32_t* p, uint32_t words);
 hipError_t launch_trace_rays(hipStream_t s, const DeviceScene& scene, const tt_ray* rays, tt_hit* hits, uint32_t count,
@@ -67,5 +87,9 @@ hipError_t launch_accumulate(hipStream_t s, const FrameArgs& f, const PathBuffer
 hipError_t launch_bsdf_evaluate(hipStream_t s, const tt_material_constants* material, const tt_bsdf_query* q,
                                 tt_bsdf_result* r, uint32_t count, uint32_t rng_state);
 hipError_t launch_math(hipStream_t s, int op, const float* x, const float* y, float* out, uint32_t count);
+
+// ReSTIR DI passes at the primary vertex (restir.hip).
+hipError_t launch_restir(hipStream_t s, const DeviceScene& scene, const FrameArgs& f, const PathBuffers& b,
+                         const RestirBuffers& r, int grid);
 
 }  // namespace tt

@@ -13,76 +13,22 @@
 //             rule (integrate.glsl:175-185) and the radiance update
 // framed by generate (camera rays, integrate.glsl:323-333) and accumulate
 // (sample average + progressive accumulation, integrate.glsl:338-346).
+// With ReSTIR DI, the passes of restir.hip run between extend(0) and
+// shade(0), and shade(0) takes its light sample from the pixel's reservoir.
 //
 // Path state is SoA in HBM: throughput/radiance float4, generator u32,
 // queues float4 (origin.xyz + path index bits, direction.xyz); every access
 // is one 16-B load or store per lane, consecutive across the wave.
-#include "device_bsdf.h"
-#include "device_scene.h"
-#include "kernels.h"
+#include "device_surface.h"
 
 using namespace ttd;
 
 namespace {
 
-// Counter words.
-constexpr uint32_t C_EXT = 0;             // [b]: rays in queue consumed at bounce b
-constexpr uint32_t C_SHADOW = 32;         // [b]: shadow rays emitted at bounce b
-constexpr uint32_t C_WORK_EXT = 64;       // [b]: persistent cursor, extend b
-constexpr uint32_t C_WORK_SHADOW = 96;    // [b]: persistent cursor, shadow b
-constexpr uint32_t C_WORDS = 128;
-
-TT_DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
-
-TT_DEV uint32_t lanes_below(uint64_t mask) {
-    return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
-}
-
-// Wave-level stream compaction: every active lane calls this; lanes with
-// `pred` get consecutive slots from one atomic add per wave.
-TT_DEV uint32_t wave_append(uint32_t* counter, bool pred) {
-    uint64_t active = __ballot(1);
-    uint64_t mask = __ballot(pred);
-    uint32_t leader = uint32_t(__ffsll((unsigned long long)active) - 1);
-    uint32_t total = uint32_t(__popcll(mask));
-    uint32_t base = 0;
-    if (lane_id() == leader && total != 0) base = atomicAdd(counter, total);
-    base = __shfl(base, int(leader));
-    return base + lanes_below(mask);
-}
-
-__device__ __host__ inline SceneView view(const tt::DeviceScene& d) {
-    SceneView s;
-    s.nodes = static_cast<const float4*>(d.nodes);
-    s.tris = static_cast<const float4*>(d.tris);
-    s.bvh_insts = static_cast<const float4*>(d.bvh_insts);
-    s.tlas_root = d.tlas_root;
-    s.emissive_count = d.emissive_count;
-    s.instances = static_cast<const float*>(d.instances);
-    s.materials = static_cast<const float4*>(d.materials);
-    s.positions = static_cast<const int16_t*>(d.positions);
-    s.vertices = static_cast<const uint2*>(d.vertices);
-    s.indices = static_cast<const uint32_t*>(d.indices);
-    s.emissive = static_cast<const uint32_t*>(d.emissive);
-    return s;
-}
-
-TT_DEV Stack make_stack(uint32_t* lds, uint32_t* spill) {
-    Stack st;
-    st.lds = lds + threadIdx.x;
-    uint32_t stride = gridDim.x * TRACE_BLOCK;
-    st.spill = spill + blockIdx.x * TRACE_BLOCK + threadIdx.x;
-    st.spill_stride = stride;
-    st.sp = 0;
-    return st;
-}
-
-// Persistent-wave cursor: lane 0 takes the next 64 items.
-TT_DEV uint32_t next_batch(uint32_t* cursor) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(cursor, 64u);
-    return __shfl(base, 0);
-}
+using tt::C_EXT;
+using tt::C_SHADOW;
+using tt::C_WORK_EXT;
+using tt::C_WORK_SHADOW;
 
 // ------------------------------------------------------------ test rays --
 template <bool CULL>
@@ -163,65 +109,6 @@ __global__ __launch_bounds__(TRACE_BLOCK) void extend_kernel(tt::DeviceScene d, 
 }
 
 // ----------------------------------------------------------------- shade --
-TT_DEV f3 tt_octahedron_decode(float u, float v) {
-    float sx = u * 2.0f - 1.0f, sy = v * 2.0f - 1.0f;
-    f3 n(sx, sy, 1.0f - abs_(sx) - abs_(sy));
-    float t = clamp_(-n.z, 0.0f, 1.0f);
-    n.x = n.x + (n.x >= 0.0f ? -t : t);
-    n.y = n.y + (n.y >= 0.0f ? -t : t);
-    return normalize(n);
-}
-
-struct TangentFrame {
-    f3 normal, tangent;
-    float bitangent_sign;
-};
-
-// scene.glsl:29-64
-TT_DEV TangentFrame decode_tangent_frame(uint32_t e) {
-    float u = float(e & 0x3ffu) / float(1023);
-    float v = float((e >> 10) & 0x3ffu) / float(1023);
-    TangentFrame f;
-    f.normal = tt_octahedron_decode(u, v);
-    f3 ortho = abs_(f.normal.x) > abs_(f.normal.z) ? f3(-f.normal.y, f.normal.x, 0.0f) : f3(0.0f, -f.normal.z, f.normal.y);
-    float angle = (float((e >> 20) & 0x7ffu) / float(2047)) * TAU;
-    f.tangent = ortho * ttm::cos(angle) + cross(f.normal, ortho) * ttm::sin(angle);
-    f.bitangent_sign = (e >> 31) == 1u ? 1.0f : -1.0f;
-    return f;
-}
-
-struct Basis {
-    f3 normal, tangent, bitangent;
-};
-
-TT_DEV f3 transform_to_basis(const Basis& b, f3 d) { return normalize(d.x * b.tangent + d.y * b.bitangent + d.z * b.normal); }
-TT_DEV f3 transform_from_basis(const Basis& b, f3 d) {
-    return normalize(f3(dot(d, b.tangent), dot(d, b.bitangent), dot(d, b.normal)));
-}
-TT_DEV f3 interpolate(f3 b, f3 x, f3 y, f3 z) { return b.x * x + b.y * y + b.z * z; }
-
-TT_DEV f3 dequantize_snorm16(int16_t x, int16_t y, int16_t z) {
-    return f3(float(x) / 32767.0f, float(y) / 32767.0f, float(z) / 32767.0f);
-}
-
-// The 16 floats of a column-major matrix from 4 float4 loads.
-struct M4 {
-    float m[16];
-};
-TT_DEV M4 load_m4(const float* p) {
-    M4 r;
-    const float4* q = reinterpret_cast<const float4*>(p);
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        float4 v = q[c];
-        r.m[4 * c + 0] = v.x;
-        r.m[4 * c + 1] = v.y;
-        r.m[4 * c + 2] = v.z;
-        r.m[4 * c + 3] = v.w;
-    }
-    return r;
-}
-
 __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::FrameArgs f, tt::PathBuffers b,
                                                     uint32_t bounce) {
     SceneView s = view(d);
@@ -229,9 +116,12 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;  // whole waves exit together except the last one
     const bool last = bounce + 1 >= f.bounce_count;
+    const bool restir = f.light_sampling == TT_LIGHT_SAMPLING_RESTIR && bounce == 0;
+    const bool nee_mode = f.light_sampling == TT_LIGHT_SAMPLING_NEXT_EVENT_ESTIMATION ||
+        (f.light_sampling == TT_LIGHT_SAMPLING_RESTIR && bounce > 0);
 
-    float4 qo = b.queue[bounce & 1][0][i];
     float4 qd = b.queue[bounce & 1][1][i];
+    float4 qo = b.queue[bounce & 1][0][i];
     float4 hit = b.hits[i];
     uint32_t p = __float_as_uint(qo.w);
     f3 ray_dir(qd.x, qd.y, qd.z);
@@ -245,106 +135,87 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
     bool emit_shadow = false, emit_ray = false;
     f3 shadow_org, shadow_dir, direct, emission_after;
     uint32_t light_instance = 0, light_hash = 0;
-    float shadow_distance = 0.0f;
-    bool light_exact = false, has_emission_after = false;
+    bool has_emission_after = false;
     f3 next_dir, next_org;
 
     if (instance == 0xffffffffu) {
         // Miss: constant white sky (integrate.glsl:272-276).
         accumulated = accumulated + f3(1.0f) * throughput;
     } else {
-        // get_ray_hit (integrate.glsl:22-110) without the texture terms.
-        uint32_t prim = __float_as_uint(hit.w);
-        const float* inst = s.instances + 52 * size_t(instance);
-        M4 transform = load_m4(inst);
-        M4 normal_transform = load_m4(inst + 32);
-        uint4 offs = *reinterpret_cast<const uint4*>(inst + 48);
-        uint32_t vertex_offset = offs.x, index_offset = offs.y, material_index = offs.z;
-        uint32_t base = 3 * prim + index_offset;
-        f3 pos[3];
-        TangentFrame fr[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            uint32_t vi = s.indices[base + c] + vertex_offset;
-            const int16_t* pp = s.positions + 3 * size_t(vi);
-            pos[c] = mul_point1(transform.m, dequantize_snorm16(pp[0], pp[1], pp[2]));
-            fr[c] = decode_tangent_frame(s.vertices[vi].y);
+        Mat m;
+        Surface sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, m);
+        f3 wi = sf.wi;
+        Lobes lobes;
+        if (restir) {
+            Generator lg = restir_lobe_generator(p, f.frame_index);
+            lobes = bsdf_prepare(m, wi, lg);
+        } else {
+            lobes = bsdf_prepare(m, wi, g);
         }
-        f3 bary(0.0f, hit.x, hit.y);
-        bary.x = 1.0f - bary.y - bary.z;
-        Basis ts;
-        ts.normal = normalize(mul_dir(normal_transform.m, interpolate(bary, fr[0].normal, fr[1].normal, fr[2].normal)));
-        ts.tangent = normalize(mul_dir(normal_transform.m, interpolate(bary, fr[0].tangent, fr[1].tangent, fr[2].tangent)));
-        float sign = sign_(fr[0].bitangent_sign * bary.x + fr[1].bitangent_sign * bary.y + fr[2].bitangent_sign * bary.z);
-        ts.bitangent = normalize(sign * cross(ts.normal, ts.tangent));
-        f3 world_position = bary.x * pos[0] + bary.y * pos[1] + bary.z * pos[2];
 
-        Mat m = load_material(s.materials + 19 * size_t(material_index));
-        // create_surface_basis (integrate.glsl:195-201)
-        Basis sb;
-        sb.normal = transform_to_basis(ts, m.geometry_normal);
-        sb.tangent = normalize(ts.tangent - sb.normal * dot(sb.normal, ts.tangent));
-        sb.bitangent = normalize(cross(sb.normal, sb.tangent));
-        f3 wi = transform_from_basis(sb, -ray_dir);
-        Lobes lobes = bsdf_prepare(m, wi, g);
-
-        bool nee = f.light_sampling == TT_LIGHT_SAMPLING_NEXT_EVENT_ESTIMATION && s.emissive_count > 0 &&
-            m.specular_roughness > 0.15f;
+        bool gate = s.emissive_count > 0 && m.specular_roughness > 0.15f;
+        bool nee = (nee_mode || restir) && gate;
         f3 emissive = m.emission_color * m.emission_luminance;
-        if (nee) {
+        bool sampled = false;
+        LightPoint lp;
+        f3 contribution;
+        if (nee && nee_mode) {
             // sample_random_light (scene.glsl:114-136) and the unshadowed
             // part of direct_light_contribution (integrate.glsl:156-193).
             uint32_t li = random_uint(g, s.emissive_count);
             f2 r2 = random_vec2(g);
-            f3 lb = sample_triangle(r2);
-            const uint32_t* et = s.emissive + 9 * size_t(li);
-            uint32_t w[9];
-#pragma unroll
-            for (int k = 0; k < 9; k++) w[k] = et[k];
-            auto half = [&](int k) { return int16_t((w[k >> 1] >> (16 * (k & 1))) & 0xffffu); };
-            light_hash = w[4] >> 16;
-            light_instance = w[8];
-            const float* linst = s.instances + 52 * size_t(light_instance);
-            M4 ltr = load_m4(linst);
-            f3 lp[3];
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-                lp[c] = mul_point1(ltr.m, dequantize_snorm16(half(3 * c), half(3 * c + 1), half(3 * c + 2)));
-            float area = fmax_(0.00001f, 0.5f * length(cross(lp[1] - lp[0], lp[2] - lp[0])));
-            f3 lnormal = normalize(cross(lp[1] - lp[0], lp[2] - lp[0]));
-            f3 light_position = interpolate(lb, lp[0], lp[1], lp[2]);
+            lp = light_point(s, li, sample_triangle(r2));
             float light_probability = 1.0f / float(s.emissive_count);
-            f3 to_light = light_position - world_position;
-            f3 wo = transform_from_basis(sb, to_light);
+            f3 to_light = lp.position - sf.position;
+            f3 wo = transform_from_basis(sf.basis, to_light);
             float distance_squared = length_squared(to_light);
             float distance = sqrt_(distance_squared);
             f3 dir = to_light / distance;
-            float density = (1.0f / area) * distance_squared / saturate(dot(-dir, lnormal)) * light_probability;
-            uint32_t lmat = __float_as_uint(linst[50]);
-            Mat lm = load_material(s.materials + 19 * size_t(lmat));
-            f3 lemissive = lm.emission_color * lm.emission_luminance;
+            float density = (1.0f / lp.area) * distance_squared / saturate(dot(-dir, lp.normal)) * light_probability;
             float bsdf_density;
             f3 bsdf = bsdf_evaluate(m, lobes, wi, wo, bsdf_density);
-            f3 contribution = bsdf * lemissive * abs_(cos_theta(wo)) / fmax_(density, PDF_EPSILON);
-            direct = throughput * contribution;
-            shadow_org = world_position;
+            contribution = bsdf * lp.emission * abs_(cos_theta(wo)) / fmax_(density, PDF_EPSILON);
             shadow_dir = dir;
-            shadow_distance = distance;
-            light_exact = __float_as_uint(s.bvh_insts[4 * size_t(light_instance) + 2].w) != 0u;
+            sampled = true;
+        } else if (nee && restir) {
+            // ReSTIR DI: the reservoir's sample y and contribution weight W
+            // estimate the direct light as f(y) Le(y) G(y) W.
+            Reservoir r = unpack_reservoir(b.restir_final[p]);
+            if (r.light != 0xffffffffu && r.W > 0.0f) {
+                lp = light_point(s, r.light, reservoir_bary(r.bary));
+                f3 to_light = lp.position - sf.position;
+                f3 wo = transform_from_basis(sf.basis, to_light);
+                float distance_squared = length_squared(to_light);
+                float distance = sqrt_(distance_squared);
+                f3 dir = to_light / distance;
+                float cos_light = saturate(dot(-dir, lp.normal));
+                float bsdf_density;
+                f3 bsdf = bsdf_evaluate(m, lobes, wi, wo, bsdf_density);
+                float scale = ((abs_(cos_theta(wo)) * cos_light) / distance_squared) * r.W;
+                contribution = (bsdf * lp.emission) * scale;
+                shadow_dir = dir;
+                sampled = true;
+            }
+        }
+        if (sampled) {
+            direct = throughput * contribution;
+            shadow_org = sf.position;
+            light_instance = lp.instance;
+            light_hash = lp.hash;
             emit_shadow = true;
             if (bounce == 0) {
                 emission_after = emissive * throughput;
                 has_emission_after = true;
             }
-        } else {
+        } else if (bounce == 0 || !nee) {
             accumulated = accumulated + emissive * throughput;
         }
         if (!last) {
             f3 wo, value;
             float density;
             if (bsdf_sample(m, lobes, wi, g, wo, density, value)) {
-                next_dir = transform_to_basis(sb, wo);
-                next_org = world_position;
+                next_dir = transform_to_basis(sf.basis, wo);
+                next_org = sf.position;
                 throughput = throughput * (value / fmax_(density, PDF_EPSILON) * abs_(cos_theta(wo)));
                 emit_ray = true;
             }
@@ -367,18 +238,17 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
         b.shadow[0][sslot] = make_float4(shadow_org.x, shadow_org.y, shadow_org.z, __uint_as_float(p));
         b.shadow[1][sslot] = make_float4(shadow_dir.x, shadow_dir.y, shadow_dir.z, __uint_as_float(light_instance));
         b.shadow[2][sslot] = make_float4(direct.x, direct.y, direct.z,
-                                         __uint_as_float(light_hash | (has_emission_after ? 0x10000u : 0u) |
-                                                         (light_exact ? 0x20000u : 0u)));
-        b.shadow[3][sslot] = make_float4(emission_after.x, emission_after.y, emission_after.z, shadow_distance);
+                                         __uint_as_float(light_hash | (has_emission_after ? 0x10000u : 0u)));
+        b.shadow[3][sslot] = make_float4(emission_after.x, emission_after.y, emission_after.z, 0.0f);
     }
 }
 
 // ---------------------------------------------------------------- shadow --
 // integrate.glsl:168-192: closest hit without culling in [1e-3, 1e4];
 // visible iff the hit is the sampled light's instance and
-// (primitive + index_offset / 3) % 0xffff equals the light's hash.  Where the
-// hash is unambiguous the query is capped just past the light and ends at the
-// first occluder in front of it (ShadowQuery), with the same outcome.
+// (primitive + index_offset / 3) % 0xffff equals the light's hash.  The query
+// is confined to the light instance's box interval and ends at the first
+// occluder in front of it (ShadowQuery), with the same outcome.
 __global__ __launch_bounds__(TRACE_BLOCK) void shadow_kernel(tt::DeviceScene d, tt::PathBuffers b, uint32_t bounce) {
     __shared__ uint32_t lds[TRACE_LDS_STACK * TRACE_BLOCK];
     SceneView s = view(d);
@@ -393,32 +263,15 @@ __global__ __launch_bounds__(TRACE_BLOCK) void shadow_kernel(tt::DeviceScene d, 
             float4 direct = b.shadow[2][i];
             float4 e = b.shadow[3][i];
             uint32_t meta = __float_as_uint(direct.w);
-            uint32_t light_instance = __float_as_uint(dd.w);
-            bool exact = (meta & 0x20000u) != 0;
-            ShadowQuery q;
-            q.light_instance = light_instance;
-            q.hash = meta & 0xffffu;
-            q.t_exit = exact ? e.w * (1.0f - 1e-4f) - 1e-4f : -INFINITY;
-            q.occluded = false;
-            float tmax = exact ? fminf(e.w * (1.0f + 1e-4f) + 1e-4f, 10000.0f) : 10000.0f;
-            HitRecord h;
-            bool found =
-                trace_closest<false, true>(s, f3(o.x, o.y, o.z), f3(dd.x, dd.y, dd.z), 1.0e-3f, tmax, st, h, &q);
-            bool visible = false;
-            if (found && !q.occluded && h.instance == light_instance) {
-                uint32_t index_offset = __float_as_uint(s.instances[52 * size_t(h.instance) + 49]);
-                uint32_t triangle_index = h.primitive + index_offset / 3u;
-                visible = triangle_index % 0xffffu == (meta & 0xffffu);
-            }
+            bool visible = light_visible(s, f3(o.x, o.y, o.z), f3(dd.x, dd.y, dd.z), __float_as_uint(dd.w),
+                                         meta & 0xffffu, st);
             bool emission = (meta & 0x10000u) != 0;
             if (visible || emission) {
                 uint32_t p = __float_as_uint(o.w);
                 float4 a = b.radiance[p];
                 f3 acc(a.x, a.y, a.z);
                 if (visible) acc = acc + f3(direct.x, direct.y, direct.z);
-                if (emission) {
-                    acc = acc + f3(e.x, e.y, e.z);
-                }
+                if (emission) acc = acc + f3(e.x, e.y, e.z);
                 b.radiance[p] = make_float4(acc.x, acc.y, acc.z, 0.0f);
             }
         }
@@ -486,6 +339,7 @@ __global__ void math_kernel(int op, const float* x, const float* y, float* out, 
         case 6: r = ttm::pow(v, y[i]); break;
         case 7: r = ttm::sqrt(v); break;
         case 8: r = v / y[i]; break;
+        case 10: r = ttm::half_to_float(ttm::float_to_half(v)); break;
         default: r = 1.0f / v; break;
     }
     out[i] = r;
@@ -502,7 +356,6 @@ namespace tt {
 #endif
 int trace_grid(int cu_count) { return cu_count * TT_TRACE_GRID_PER_CU; }
 size_t spill_words(int grid) { return size_t(grid) * TRACE_BLOCK * TRACE_SPILL_STACK; }
-uint32_t counter_words(uint32_t) { return C_WORDS; }
 
 hipError_t launch_zero(hipStream_t s, uint32_t* p, uint32_t words) {
     zero_kernel<<<blocks(words, 256), 256, 0, s>>>(p, words);

@@ -3,6 +3,8 @@
 // reference interfaces each function replaces.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <new>
@@ -49,12 +51,19 @@ struct tt_scene {
     tt::DeviceScene view{};
     std::vector<void*> allocations;
     tt_scene_info info{};
+    uint64_t serial = 0;  // unique per scene created in this process
 };
 
 struct tt_target {
     tt_context* context = nullptr;
     uint32_t width = 0, height = 0;
     float4* pixels = nullptr;
+    // ReSTIR DI history (allocated on first use): two G-buffers and the
+    // reservoir arrays of tt::RestirBuffers in one allocation.
+    void* restir_memory = nullptr;
+    uint32_t restir_cur = 0;
+    bool restir_history = false;
+    uint64_t restir_scene = 0;  // serial of the scene the history belongs to
 };
 
 namespace {
@@ -158,7 +167,7 @@ tt_status tt_context_create(const tt_context_desc* desc, tt_context** out) {
     c->cu_count = prop.multiProcessorCount;
     c->grid = tt::trace_grid(c->cu_count);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(TT_HIP_ERROR);
-    if (hipMalloc(&c->counters, tt::counter_words(0) * sizeof(uint32_t)) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
+    if (hipMalloc(&c->counters, tt::C_WORDS * sizeof(uint32_t)) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
     if (hipMalloc(&c->work, 256) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
     if (hipMalloc(&c->spill, tt::spill_words(c->grid) * sizeof(uint32_t)) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
     if (hipEventCreate(&c->integrate.begin) != hipSuccess || hipEventCreate(&c->integrate.end) != hipSuccess)
@@ -169,19 +178,20 @@ tt_status tt_context_create(const tt_context_desc* desc, tt_context** out) {
 
 void tt_context_destroy(tt_context* c) {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->path_memory) hipFree(c->path_memory);
-    if (c->counters) hipFree(c->counters);
-    if (c->work) hipFree(c->work);
-    if (c->spill) hipFree(c->spill);
-    if (c->integrate.begin) hipEventDestroy(c->integrate.begin);
-    if (c->integrate.end) hipEventDestroy(c->integrate.end);
+    // Teardown is best effort: errors here have no caller to report to.
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->path_memory) (void)hipFree(c->path_memory);
+    if (c->counters) (void)hipFree(c->counters);
+    if (c->work) (void)hipFree(c->work);
+    if (c->spill) (void)hipFree(c->spill);
+    if (c->integrate.begin) (void)hipEventDestroy(c->integrate.begin);
+    if (c->integrate.end) (void)hipEventDestroy(c->integrate.end);
     for (auto& e : c->trace_events) {
-        hipEventDestroy(e.begin);
-        hipEventDestroy(e.end);
+        (void)hipEventDestroy(e.begin);
+        (void)hipEventDestroy(e.end);
     }
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -219,6 +229,8 @@ tt_status tt_scene_create(tt_context* c, const tt_scene_desc* d, tt_scene** out)
     if (!tt::build_bvh(*d, bvh)) return fail(c, TT_INVALID_ARGUMENT, "BVH build failed: " + bvh.error);
 
     tt_scene* s = new (std::nothrow) tt_scene();
+    static std::atomic<uint64_t> next_serial{1};
+    if (s) s->serial = next_serial.fetch_add(1);
     if (!s) return fail(c, TT_OUT_OF_MEMORY, "scene allocation");
     s->context = c;
     TT_HIP(c, hipSetDevice(c->device));
@@ -229,6 +241,7 @@ tt_status tt_scene_create(tt_context* c, const tt_scene_desc* d, tt_scene** out)
     up(bvh.nodes.data(), bvh.nodes.size() * sizeof(tt::BvhNode), &s->view.nodes);
     up(bvh.tris.data(), bvh.tris.size() * sizeof(tt::BvhTri), &s->view.tris);
     up(bvh.instances.data(), bvh.instances.size() * sizeof(tt::BvhInstance), &s->view.bvh_insts);
+    up(bvh.instance_boxes.data(), bvh.instance_boxes.size() * sizeof(float), &s->view.inst_boxes);
     up(d->instances, size_t(d->instance_count) * sizeof(tt_instance), &s->view.instances);
     up(d->materials, size_t(d->material_count) * sizeof(tt_material), &s->view.materials);
     up(d->positions, size_t(d->position_count) * 3 * sizeof(int16_t), &s->view.positions);
@@ -258,10 +271,10 @@ tt_status tt_scene_create(tt_context* c, const tt_scene_desc* d, tt_scene** out)
 void tt_scene_destroy(tt_scene* s) {
     if (!s) return;
     if (s->context) {
-        hipSetDevice(s->context->device);
-        hipStreamSynchronize(s->context->stream);
+        (void)hipSetDevice(s->context->device);
+        (void)hipStreamSynchronize(s->context->stream);
     }
-    for (void* p : s->allocations) hipFree(p);
+    for (void* p : s->allocations) (void)hipFree(p);
     delete s;
 }
 
@@ -293,10 +306,11 @@ tt_status tt_target_create(tt_context* c, uint32_t width, uint32_t height, tt_ta
 void tt_target_destroy(tt_target* t) {
     if (!t) return;
     if (t->context) {
-        hipSetDevice(t->context->device);
-        hipStreamSynchronize(t->context->stream);
+        (void)hipSetDevice(t->context->device);
+        (void)hipStreamSynchronize(t->context->stream);
     }
-    if (t->pixels) hipFree(t->pixels);
+    if (t->pixels) (void)hipFree(t->pixels);
+    if (t->restir_memory) (void)hipFree(t->restir_memory);
     delete t;
 }
 
@@ -308,10 +322,14 @@ tt_status tt_render_ex(tt_context* c, const tt_scene* s, const tt_constants* k, 
         return fail(c, TT_INVALID_ARGUMENT, "constants.screen_size does not match the target");
     if (k->sample_count == 0 || k->bounce_count == 0 || k->bounce_count > 31)
         return fail(c, TT_INVALID_ARGUMENT, "sample_count must be >= 1 and bounce_count in [1, 31]");
-    if (k->light_sampling != TT_LIGHT_SAMPLING_ON_HIT && k->light_sampling != TT_LIGHT_SAMPLING_NEXT_EVENT_ESTIMATION)
+    if (k->light_sampling != TT_LIGHT_SAMPLING_ON_HIT && k->light_sampling != TT_LIGHT_SAMPLING_NEXT_EVENT_ESTIMATION &&
+        k->light_sampling != TT_LIGHT_SAMPLING_RESTIR)
         return fail(c, TT_UNSUPPORTED, "light sampling mode not supported");
     tt_render_settings def{TT_RNG_REFERENCE, 0, 0, 0, 0, 0};
     const tt_render_settings& st = settings ? *settings : def;
+    const bool restir = k->light_sampling == TT_LIGHT_SAMPLING_RESTIR;
+    if (restir && k->sample_count != 1) return fail(c, TT_UNSUPPORTED, "ReSTIR DI renders one sample per pixel");
+    if (restir && st.restir_candidates > 1024) return fail(c, TT_INVALID_ARGUMENT, "restir_candidates > 1024");
     if (st.rng > 1) return fail(c, TT_INVALID_ARGUMENT, "unknown generator");
     uint64_t paths64 = uint64_t(t->width) * t->height * k->sample_count;
     if (paths64 >= (1ull << 31)) return fail(c, TT_INVALID_ARGUMENT, "too many paths in one render");
@@ -323,8 +341,35 @@ tt_status tt_render_ex(tt_context* c, const tt_scene* s, const tt_constants* k, 
     b.max_bounces = k->bounce_count;
     bool stats = (st.flags & TT_RENDER_COLLECT_STATS) != 0;
     if (stats) {
-        es = ensure_events(c, 2 * k->bounce_count);
+        es = ensure_events(c, 2 * k->bounce_count + 1);
         if (es != TT_OK) return es;
+    }
+
+    tt::RestirBuffers rb{};
+    b.restir_final = nullptr;
+    if (restir) {
+        size_t px = size_t(t->width) * t->height;
+        if (!t->restir_memory) {
+            TT_HIP(c, hipMalloc(&t->restir_memory, px * (8 * sizeof(float4) + 4 * sizeof(uint4))));
+            t->restir_cur = 0;
+            t->restir_history = false;
+        }
+        float4* g = static_cast<float4*>(t->restir_memory);
+        uint4* r = reinterpret_cast<uint4*>(g + 8 * px);
+        rb.gbuffer[0] = g;
+        rb.gbuffer[1] = g + 4 * px;
+        rb.initial = r;
+        rb.temporal = r + px;
+        rb.final_[0] = r + 2 * px;
+        rb.final_[1] = r + 3 * px;
+        // History from another scene would index its lights and materials.
+        if (t->restir_scene != s->serial) t->restir_history = false;
+        rb.cur = t->restir_cur;
+        rb.history_valid = t->restir_history ? 1u : 0u;
+        rb.candidates = st.restir_candidates ? st.restir_candidates : 32u;
+        rb.temporal_reuse = st.restir_temporal ? 1u : 0u;
+        rb.spatial_reuse = st.restir_spatial ? 1u : 0u;
+        b.restir_final = rb.final_[rb.cur];
     }
 
     tt::FrameArgs f;
@@ -345,7 +390,7 @@ tt_status tt_render_ex(tt_context* c, const tt_scene* s, const tt_constants* k, 
 
     hipStream_t stream = c->stream;
     TT_HIP(c, hipEventRecord(c->integrate.begin, stream));
-    TT_HIP(c, tt::launch_zero(stream, c->counters, tt::counter_words(0)));
+    TT_HIP(c, tt::launch_zero(stream, c->counters, tt::C_WORDS));
     TT_HIP(c, tt::launch_generate(stream, f, b, paths));
     uint32_t ev = 0;
     for (uint32_t bounce = 0; bounce < k->bounce_count; bounce++) {
@@ -355,8 +400,16 @@ tt_status tt_render_ex(tt_context* c, const tt_scene* s, const tt_constants* k, 
             c->trace_event_shadow[ev] = 0;
             TT_HIP(c, hipEventRecord(c->trace_events[ev++].end, stream));
         }
+        if (restir && bounce == 0 && s->view.emissive_count > 0) {
+            if (stats) TT_HIP(c, hipEventRecord(c->trace_events[ev].begin, stream));
+            TT_HIP(c, tt::launch_restir(stream, s->view, f, b, rb, c->grid));
+            if (stats) {
+                c->trace_event_shadow[ev] = 2;
+                TT_HIP(c, hipEventRecord(c->trace_events[ev++].end, stream));
+            }
+        }
         TT_HIP(c, tt::launch_shade(stream, s->view, f, b, bounce, paths));
-        if (f.light_sampling == TT_LIGHT_SAMPLING_NEXT_EVENT_ESTIMATION && s->view.emissive_count > 0) {
+        if (f.light_sampling != TT_LIGHT_SAMPLING_ON_HIT && s->view.emissive_count > 0) {
             if (stats) TT_HIP(c, hipEventRecord(c->trace_events[ev].begin, stream));
             TT_HIP(c, tt::launch_shadow(stream, s->view, b, bounce, c->grid));
             if (stats) {
@@ -372,6 +425,13 @@ tt_status tt_render_ex(tt_context* c, const tt_scene* s, const tt_constants* k, 
     c->last_paths = paths;
     c->last_bounces = k->bounce_count;
     c->last_integrate_ms = -1.0;
+    if (restir && s->view.emissive_count > 0) {
+        t->restir_cur ^= 1u;
+        t->restir_history = true;
+        t->restir_scene = s->serial;
+    } else {
+        t->restir_history = false;
+    }
     return TT_OK;
 }
 
@@ -403,6 +463,7 @@ tt_status tt_timing_ms(tt_context* c, const char* name, double* out) {
         if (!c->stats_pending) return fail(c, TT_INVALID_ARGUMENT, "render without TT_RENDER_COLLECT_STATS");
         double sum = 0.0;
         for (uint32_t i = 0; i < c->trace_events_used; i++) {
+            if (c->trace_event_shadow[i] == 2) continue;  // ReSTIR passes
             float ms = 0.0f;
             TT_HIP(c, hipEventElapsedTime(&ms, c->trace_events[i].begin, c->trace_events[i].end));
             sum += ms;
@@ -417,7 +478,7 @@ tt_status tt_render_stats_get(tt_context* c, tt_render_stats* out) {
     if (!c || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     if (!c->stats_pending) return fail(c, TT_INVALID_ARGUMENT, "render without TT_RENDER_COLLECT_STATS");
     TT_HIP(c, hipStreamSynchronize(c->stream));
-    uint32_t words[128];
+    uint32_t words[tt::C_WORDS];
     TT_HIP(c, hipMemcpy(words, c->counters, sizeof(words), hipMemcpyDeviceToHost));
     *out = tt_render_stats{};
     out->paths = c->last_paths;
@@ -425,10 +486,13 @@ tt_status tt_render_stats_get(tt_context* c, tt_render_stats* out) {
         out->extension_rays += words[0 + b];
         out->shadow_rays += words[32 + b];
     }
+    out->restir_rays = words[tt::C_RESTIR_VIS];
     for (uint32_t i = 0; i < c->trace_events_used; i++) {
         float ms = 0.0f;
         TT_HIP(c, hipEventElapsedTime(&ms, c->trace_events[i].begin, c->trace_events[i].end));
-        if (c->trace_event_shadow[i]) {
+        if (c->trace_event_shadow[i] == 2) {
+            out->restir_ms += ms;
+        } else if (c->trace_event_shadow[i]) {
             out->shadow_launches++;
             out->shadow_ms += ms;
         } else {
@@ -469,8 +533,8 @@ tt_status tt_trace_closest(tt_context* c, const tt_scene* s, const tt_ray* rays,
     if (e == hipSuccess && st == TT_OK)
         e = hipMemcpyAsync(hits, dh, size_t(count) * sizeof(tt_hit), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess && st == TT_OK) e = hipStreamSynchronize(c->stream);
-    hipFree(dr);
-    if (dh) hipFree(dh);
+    (void)hipFree(dr);
+    if (dh) (void)hipFree(dh);
     if (e != hipSuccess) return hip_fail(c, e, "tt_trace_closest");
     return st;
 }
@@ -489,8 +553,8 @@ tt_status tt_bsdf_evaluate(tt_context* c, const tt_material_constants* material,
     if (e == hipSuccess)
         e = hipMemcpyAsync(results, dr, size_t(count) * sizeof(tt_bsdf_result), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(dq);
-    if (dr) hipFree(dr);
+    (void)hipFree(dq);
+    if (dr) (void)hipFree(dr);
     if (e != hipSuccess) return hip_fail(c, e, "tt_bsdf_evaluate");
     return TT_OK;
 }
@@ -506,7 +570,7 @@ tt_status tt_debug_math(tt_context* c, int op, const float* x, const float* y, f
     if (e == hipSuccess) e = tt::launch_math(c->stream, op, d, d + count, d + 2 * count, count);
     if (e == hipSuccess) e = hipMemcpyAsync(out, d + 2 * count, count * sizeof(float), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(d);
+    (void)hipFree(d);
     if (e != hipSuccess) return hip_fail(c, e, "tt_debug_math");
     return TT_OK;
 }

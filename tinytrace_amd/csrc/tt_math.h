@@ -240,6 +240,47 @@ TT_HD float acos(float x) {
     return PIO2 - asin(x);
 }
 
+// IEEE binary32 -> binary16 bits, round to nearest even (overflow to inf).
+TT_HD uint16_t float_to_half(float f) {
+    uint32_t x = float_bits(f);
+    uint32_t sign = (x >> 16) & 0x8000u;
+    uint32_t e = (x >> 23) & 0xffu;
+    uint32_t man = x & 0x7fffffu;
+    if (e == 0xffu) return uint16_t(sign | 0x7c00u | (man ? 0x200u : 0u));
+    int he = int(e) - 127 + 15;
+    if (he >= 0x1f) return uint16_t(sign | 0x7c00u);
+    if (he <= 0) {
+        if (he < -10) return uint16_t(sign);
+        man |= 0x800000u;
+        int shift = 14 - he;
+        uint32_t hm = man >> shift;
+        uint32_t rem = man & ((1u << shift) - 1u);
+        uint32_t halfway = 1u << (shift - 1);
+        if (rem > halfway || (rem == halfway && (hm & 1u))) hm++;
+        return uint16_t(sign | hm);
+    }
+    uint32_t hm = man >> 13;
+    uint32_t rem = man & 0x1fffu;
+    uint32_t h = sign | (uint32_t(he) << 10) | hm;
+    if (rem > 0x1000u || (rem == 0x1000u && (hm & 1u))) h++;
+    return uint16_t(h);
+}
+
+// IEEE binary16 bits -> binary32 (exact).
+TT_HD float half_to_float(uint16_t h) {
+    uint32_t sign = uint32_t(h & 0x8000u) << 16;
+    uint32_t e = (h >> 10) & 0x1fu;
+    uint32_t man = h & 0x3ffu;
+    if (e == 0) {
+        if (man == 0) return bits_float(sign);
+        // subnormal half: man * 2^-24
+        float v = float(man) * 5.9604644775390625e-8f;
+        return sign ? -v : v;
+    }
+    if (e == 31) return bits_float(sign | 0x7f800000u | (man << 13));
+    return bits_float(sign | ((e + 112u) << 23) | (man << 13));
+}
+
 // Power for x >= 0.  GLSL leaves pow(x<0, y) undefined; this returns NaN.
 TT_HD float pow(float x, float y) {
     if (x == 0.0f) return y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : bits_float(0x7f800000u));
