@@ -36,6 +36,8 @@ def main() -> int:
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="wall time the CPU baseline keeps rendering whole frames for (at least one frame)")
     p.add_argument("--light-sampling", choices=("restir", "nee"), default="restir")
+    p.add_argument("--quality-frames", type=int, default=16,
+                   help="frames rendered for the RMSE-vs-FP64 leg (0 skips it)")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="also run a plain device copy of this many MiB (reference kernel for HBM counters)")
     args = p.parse_args()
@@ -150,6 +152,9 @@ def main() -> int:
                      "rays_per_launch": round(ext_rays / max(ext_launches, 1), 1)},
     }
 
+    if args.quality_frames > 0:
+        result["quality"] = quality(renderer, settings, restir, args.quality_frames, step, sync, args.steps)
+
     if args.copy_calibration_mib:
         # A device-to-device hipMemcpy of known size runs as the runtime's
         # copy kernel: the reference point for the HBM byte counters.
@@ -198,6 +203,56 @@ def main() -> int:
         result["gpu_over_cpu"] = round(result["value"] / result["cpu_baseline"]["value"], 1)
     print(json.dumps(result))
     return 0
+
+
+def quality(renderer, settings, restir, frames, step, sync, steps):
+    """RMSE of 1, 4, 16 ... accumulated frames against the FP64 reference
+    fixture (4096-spp double-precision oracle radiance on a 60x34 pixel
+    lattice of the same workload); with ReSTIR also the next-event
+    estimator's RMSE and frame time, so equal-time quality can be compared."""
+    import numpy as np
+
+    from tinytrace_amd import LightSampling
+
+    ref_path = os.path.join(ROOT, "tests", "golden", "fp64_sphere_field_1080p.npz")
+    with np.load(ref_path, allow_pickle=False) as f:
+        pixels, reference, ref_spp = f["pixels"], f["radiance"], int(f["spp"])
+
+    def run(mode):
+        renderer.config = dataclasses.replace(renderer.config, light_sampling=mode)
+        renderer.settings = settings
+        renderer.frame_index = 100000
+        samples = []
+        for _ in range(frames):
+            renderer.reset_accumulation()
+            image = renderer.render_to_texture().reshape(-1, 4)
+            samples.append(image[pixels, :3].astype(np.float64))
+        samples = np.array(samples)
+        out = {"rmse": {}, "mean_rel_error": round(float(samples.mean() / reference.mean() - 1.0), 5)}
+        n = 1
+        while n <= frames:
+            err = samples[:n].mean(0) - reference
+            out["rmse"][str(n)] = round(float(np.sqrt((err * err).mean())), 5)
+            n *= 4
+        # Frame time of this mode (the timed loop measured the benchmark mode).
+        for _ in range(2):
+            step()
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        sync()
+        out["ms_per_frame"] = round((time.perf_counter() - t0) / steps * 1e3, 3)
+        return out
+
+    result = {"reference": f"FP64 oracle, {ref_spp} spp next-event estimation, {pixels.size}-pixel lattice "
+                           "(stride 32) of the same workload",
+              "frames": frames}
+    result["restir" if restir else "nee"] = run(LightSampling.RESTIR if restir else LightSampling.NEXT_EVENT_ESTIMATION)
+    if restir:
+        result["nee"] = run(LightSampling.NEXT_EVENT_ESTIMATION)
+        renderer.config = dataclasses.replace(renderer.config, light_sampling=LightSampling.RESTIR)
+    return result
 
 
 if __name__ == "__main__":

@@ -1,0 +1,68 @@
+"""Image quality of the gfx950 path against the FP64 reference fixtures
+(4096-spp double-precision oracle on pixel lattices):
+
+  * next-event estimation on the Cornell box (512x512, depth 4, 64 frames)
+    and on the 4M-triangle sphere field (1920x1080, depth 8, 128 frames) is
+    unbiased against FP64;
+  * ReSTIR DI on the sphere field (32 candidates, temporal + spatial reuse)
+    stays within 3 % of the FP64 mean and has lower lattice RMSE than
+    next-event estimation after the same number of frames.
+"""
+import numpy as np
+import pytest
+
+from fp64_reference import assert_unbiased, load, rmse
+from tinytrace_amd import Config, LightSampling, Renderer, RenderSettings, _abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def lattice_frames(scene, camera, ref, frames, light_sampling, settings, first_frame=1000):
+    width, height = int(ref["width"]), int(ref["height"])
+    renderer = Renderer((width, height), settings=settings)
+    renderer.set_scene(scene)
+    renderer.camera = camera
+    renderer.set_config(Config(bounce_count=int(ref["depth"]), sample_count=1, light_sampling=light_sampling))
+    renderer.frame_index = first_frame
+    px = ref["pixels"]
+    out = []
+    for _ in range(frames):
+        renderer.reset_accumulation()
+        image = renderer.render_to_texture().reshape(-1, 4)
+        out.append(image[px, :3].astype(np.float64))
+    return np.array(out)
+
+
+def test_cornell_nee_unbiased_against_fp64():
+    ref = load("fp64_cornell_512")
+    scene, camera = scenes.cornell_box()
+    frames = lattice_frames(scene, camera, ref, 64, LightSampling.NEXT_EVENT_ESTIMATION,
+                            RenderSettings(rng=_abi.TT_RNG_PCG, seed=9))
+    assert_unbiased(frames, ref)
+    assert rmse(frames.mean(0), ref) < 0.25 * rmse(frames[0], ref)
+
+
+@pytest.fixture(scope="module")
+def sphere_field():
+    return scenes.sphere_field(seed=0)
+
+
+def test_sphere_field_nee_unbiased_against_fp64(sphere_field):
+    ref = load("fp64_sphere_field_1080p")
+    scene, camera = sphere_field
+    frames = lattice_frames(scene, camera, ref, 128, LightSampling.NEXT_EVENT_ESTIMATION,
+                            RenderSettings(rng=_abi.TT_RNG_PCG, seed=4))
+    assert_unbiased(frames, ref)
+
+
+def test_sphere_field_restir_quality(sphere_field):
+    ref = load("fp64_sphere_field_1080p")
+    scene, camera = sphere_field
+    n = 16
+    restir = lattice_frames(scene, camera, ref, n, LightSampling.RESTIR,
+                            RenderSettings(rng=_abi.TT_RNG_PCG, seed=4, restir_candidates=32))
+    nee = lattice_frames(scene, camera, ref, n, LightSampling.NEXT_EVENT_ESTIMATION,
+                         RenderSettings(rng=_abi.TT_RNG_PCG, seed=4))
+    assert abs(restir.mean() / ref["radiance"].mean() - 1.0) < 0.03
+    assert rmse(restir.mean(0), ref) < rmse(nee.mean(0), ref)
+    assert rmse(restir[-1], ref) < rmse(nee[-1], ref)
