@@ -228,14 +228,14 @@ tt_status tt_target_create(tt_context* context, uint32_t width, uint32_t height,
 void tt_target_destroy(tt_target* target);
 
 enum { TT_RNG_REFERENCE = 0, /* jenkins_hash + xorshift32, random.glsl:4-42 */
-       TT_RNG_PCG = 1 /* PCG-RXS-M-XS 32 (O'Neill 2014) */ };
+       TT_RNG_PCG = 1 /* pcg32 (PCG-XSH-RR 64/32, O'Neill 2014), SplitMix64-seeded */ };
 
 enum { TT_RENDER_COLLECT_STATS = 1u /* count rays, time stages with events */ };
 
 /* Settings the reference does not have (new with this library). */
 typedef struct tt_render_settings {
     uint32_t rng;                /* TT_RNG_* */
-    uint32_t seed;               /* mixed into the PCG pixel seed */
+    uint32_t seed;               /* mixed into the pcg32 path seed */
     uint32_t restir_candidates;  /* RIS candidates (light_sampling == RESTIR) */
     uint32_t restir_temporal;    /* 0 or 1 temporal reuse pass */
     uint32_t restir_spatial;     /* 0 or 1 spatial reuse pass */
@@ -258,8 +258,8 @@ tt_status tt_target_download(tt_context* context, const tt_target* target, float
 /* Named GPU timings in milliseconds, mirroring Renderer::timings
  * (src/lib.rs:167-178) and Context::insert_timestamp
  * (crates/tinytrace_backend/src/timing.rs:61-69).  Names: "integrate" (last
- * render); with TT_RENDER_COLLECT_STATS also "trace", "shade", "generate",
- * "accumulate" (sums over the last render's launches). */
+ * render); with TT_RENDER_COLLECT_STATS also "trace" (sum over the last
+ * render's extension and shadow closest-hit launches). */
 tt_status tt_timing_ms(tt_context* context, const char* name, double* out);
 
 typedef struct tt_render_stats {

@@ -50,8 +50,10 @@ def lib():
         L.oracle_rng.argtypes = [u32, u32, u32, u32, u32, u32, u32, vp, vp]
         L.oracle_jenkins_hash.restype = u32
         L.oracle_jenkins_hash.argtypes = [u32]
-        L.oracle_pcg_hash.restype = u32
-        L.oracle_pcg_hash.argtypes = [u32]
+        L.oracle_pcg32.restype = None
+        L.oracle_pcg32.argtypes = [C.c_uint64, C.c_uint64, u32, vp]
+        L.oracle_splitmix64.restype = C.c_uint64
+        L.oracle_splitmix64.argtypes = [C.c_uint64]
         L.oracle_uint_to_unit_float.restype = f32
         L.oracle_uint_to_unit_float.argtypes = [u32]
         L.oracle_math_f32.restype = None
@@ -156,9 +158,15 @@ def bsdf_sample(material_constants, wi, rng_states, precision=32):
 
 def rng_draws(kind, x, y, width, frame, seed, n):
     out = np.zeros(n, dtype=np.uint32)
-    state = np.zeros(1, dtype=np.uint32)
+    state = np.zeros(1, dtype=np.uint64)
     lib().oracle_rng(kind, x, y, width, frame, seed, n, out.ctypes.data, state.ctypes.data)
     return int(state[0]), out
+
+
+def pcg32_draws(initstate, initseq, n):
+    out = np.zeros(n, dtype=np.uint32)
+    lib().oracle_pcg32(initstate, initseq, n, out.ctypes.data)
+    return out
 
 
 MATH_OPS = {"sin": 0, "cos": 1, "exp": 2, "log": 3, "acos": 4, "asin": 5, "pow": 6, "sqrt": 7, "half": 8}

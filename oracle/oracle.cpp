@@ -437,14 +437,23 @@ int oracle_bsdf_sample(const tt_material_constants* mc, const float* wi_in, cons
 // Generator draws: seeds with init_generator_from_pixel(x, y, width, frame)
 // and writes `n` u32 draws.
 void oracle_rng(uint32_t kind, uint32_t x, uint32_t y, uint32_t width, uint32_t frame, uint32_t seed, uint32_t n,
-                uint32_t* out, uint32_t* initial_state) {
+                uint32_t* out, uint64_t* initial_state) {
     Generator g = init_generator_from_pixel(x, y, width, frame, kind, seed);
     if (initial_state) *initial_state = g.state;
     for (uint32_t i = 0; i < n; i++) out[i] = next_u32(g);
 }
 
 uint32_t oracle_jenkins_hash(uint32_t x) { return jenkins_hash(x); }
-uint32_t oracle_pcg_hash(uint32_t x) { return pcg_hash(x); }
+// pcg32_srandom_r(initstate, initseq) followed by n pcg32_random_r draws.
+void oracle_pcg32(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t* out) {
+    uint64_t inc = (initseq << 1u) | 1u;
+    uint64_t s = pcg32_srandom_state(initstate, inc);
+    for (uint32_t i = 0; i < n; i++) {
+        out[i] = pcg32_output(s);
+        s = s * PCG32_MULT + inc;
+    }
+}
+uint64_t oracle_splitmix64(uint64_t x) { return splitmix64(x); }
 float oracle_uint_to_unit_float(uint32_t v) { return uint_to_unit_float(v); }
 
 // op: 0 sin, 1 cos, 2 exp, 3 log, 4 acos, 5 asin, 6 pow(x, y), 7 sqrt

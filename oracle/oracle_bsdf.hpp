@@ -34,44 +34,70 @@ inline uint32_t jenkins_hash(uint32_t x) {
     return x;
 }
 
-// PCG-RXS-M-XS 32 output permutation on a 32-bit LCG state (O'Neill 2014,
-// "PCG: A Family of Simple Fast Space-Efficient Statistically Good
-// Algorithms for Random Number Generation", section 6.3.4; constants as in
-// Jarzynski & Olano 2020, "Hash Functions for GPU Rendering").
-inline uint32_t pcg_output(uint32_t state) {
-    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
-    return (word >> 22u) ^ word;
-}
-inline uint32_t pcg_step(uint32_t state) { return state * 747796405u + 2891336453u; }
-inline uint32_t pcg_hash(uint32_t v) { return pcg_output(pcg_step(v)); }
+// pcg32 = PCG-XSH-RR 64/32 (O'Neill 2014, "PCG: A Family of Simple Fast
+// Space-Efficient Statistically Good Algorithms for Random Number
+// Generation", section 6.3.1): 64-bit LCG, multiplier 6364136223846793005,
+// output from the pre-step state.  `inc` is the (odd) stream increment;
+// pcg32_srandom(initstate, initseq) uses inc = 2 * initseq + 1.
+constexpr uint64_t PCG32_MULT = 6364136223846793005ull;
+constexpr uint64_t PCG32_INC = 0xda3e39cb94b95bdbull;  // pcg32's default stream
 
-// Generator: `kind` 0 = the reference's xorshift32 (random.glsl:29-34),
-// kind 1 = PCG.  Both produce one u32 per draw; the draw order is the
-// reference's.
+inline uint32_t pcg32_output(uint64_t old) {
+    uint32_t xs = uint32_t(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = uint32_t(old >> 59u);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+}
+inline uint64_t pcg32_srandom_state(uint64_t initstate, uint64_t inc) {
+    uint64_t s = 0;
+    s = s * PCG32_MULT + inc;
+    s += initstate;
+    return s * PCG32_MULT + inc;
+}
+
+// SplitMix64 (Steele, Lea & Flood 2014; the finaliser of Vigna's
+// splitmix64.c).
+inline uint64_t splitmix64(uint64_t x) {
+    uint64_t z = x + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// Generator: `kind` 0 = the reference's xorshift32 (random.glsl:29-34) in the
+// low 32 bits of `state`, kind 1 = pcg32 on the default stream.  Both produce
+// one u32 per draw; the draw order is the reference's.
 struct Generator {
-    uint32_t state;
+    uint64_t state;
     uint32_t kind;
 };
 
-// random.glsl:17-23 (reference), and the PCG seeding of this library:
-// state = pcg_hash(index ^ pcg_hash(frame ^ pcg_hash(seed))).
+inline Generator pcg32_seeded(uint64_t initstate) { return Generator{pcg32_srandom_state(initstate, PCG32_INC), 1}; }
+inline uint64_t pcg32_key(uint32_t index, uint32_t frame) { return (uint64_t(frame) << 32) | index; }
+
+// random.glsl:17-23 (reference), and the pcg32 seeding of this library:
+// initstate = splitmix64(((frame << 32) | index) ^ splitmix64(seed)).
+inline Generator init_generator_from_index(uint32_t index, uint32_t frame, uint32_t kind, uint32_t seed) {
+    if (kind == 1) return pcg32_seeded(splitmix64(pcg32_key(index, frame) ^ splitmix64(seed)));
+    return Generator{jenkins_hash(index ^ jenkins_hash(frame)), 0};
+}
 inline Generator init_generator_from_pixel(uint32_t x, uint32_t y, uint32_t width, uint32_t frame,
                                            uint32_t kind, uint32_t seed) {
-    uint32_t index = x + y * width;
-    if (kind == 1) return Generator{pcg_hash(index ^ pcg_hash(frame ^ pcg_hash(seed))), 1};
-    return Generator{jenkins_hash(index ^ jenkins_hash(frame)), 0};
+    return init_generator_from_index(x + y * width, frame, kind, seed);
 }
 
 inline uint32_t next_u32(Generator& g) {
     if (g.kind == 1) {
-        g.state = pcg_step(g.state);
-        return pcg_output(g.state);
+        uint64_t old = g.state;
+        g.state = old * PCG32_MULT + PCG32_INC;
+        return pcg32_output(old);
     }
     // random.glsl:29-34
-    g.state ^= g.state << 13;
-    g.state ^= g.state >> 17;
-    g.state ^= g.state << 5;
-    return g.state;
+    uint32_t s = uint32_t(g.state);
+    s ^= s << 13;
+    s ^= s >> 17;
+    s ^= s << 5;
+    g.state = s;
+    return s;
 }
 
 // random.glsl:25-27

@@ -101,12 +101,12 @@ struct ShadingF {
 // Lobes at a primary vertex in ReSTIR mode come from this generator (per
 // pixel and frame), so the target can be rebuilt from a G-buffer record.
 inline Generator lobe_generator(uint32_t pixel, uint32_t frame) {
-    return Generator{pcg_hash(pixel ^ pcg_hash(frame ^ 0xa511e9b3u)), 1};
+    return pcg32_seeded(splitmix64(pcg32_key(pixel, frame) ^ 0xa511e9b3c0ffee11ull));
 }
 
+// One generator per ReSTIR pass (salt 1 candidates, 2 temporal, 3 spatial).
 inline Generator pass_generator(uint32_t pixel, uint32_t frame, uint32_t seed, uint32_t salt) {
-    uint32_t index = pixel;
-    return Generator{pcg_hash(index ^ pcg_hash(frame ^ pcg_hash(seed ^ (salt * 0x9e3779b9u)))), 1};
+    return init_generator_from_index(pixel, frame, 1, seed ^ (salt * 0x9e3779b9u));
 }
 
 // G-buffer record: 16 floats (position, material bits | normal, valid |

@@ -3,9 +3,10 @@
 The reference generator is pinned by integer known-answer vectors computed
 from the text of random.glsl:4-38 in Python integer arithmetic (jenkins_hash,
 seeding from pixel and frame, Marsaglia xorshift32 (13, 17, 5), the
-0x3f800000 | v >> 9 float).  PCG is pinned by the invertibility of its
-output permutation, the Hull-Dobell full-period conditions of its LCG, and
-uniformity.
+0x3f800000 | v >> 9 float).  The PCG sampler is pcg32 (PCG-XSH-RR 64/32),
+pinned by the published output of the PCG reference implementation's demo;
+its per-path seeding goes through SplitMix64, pinned by splitmix64.c's
+published sequence.
 """
 import numpy as np
 import pytest
@@ -42,33 +43,32 @@ def test_frame_one_pixel_zero():
     assert [hex(d) for d in draws] == ["0x9f15277e", "0x44a5aae3", "0xcecff1ff"]
 
 
-def _pcg_output_inverse(word_out):
-    # Invert out = (w >> 22) ^ w, then w = (x ^ (s >> shift)) * M with
-    # M = 277803737, x = s, shift = (s >> 28) + 4.
-    w = word_out
-    w = w ^ (w >> 22)
-    w &= 0xFFFFFFFF
-    m_inv = pow(277803737, -1, 1 << 32)
-    x = (w * m_inv) & 0xFFFFFFFF
-    # x = s ^ (s >> k) with k = (s >> 28) + 4 >= 4: the top 4 bits of s are
-    # unchanged, so k is known; then undo the xorshift.
-    k = (x >> 28) + 4
-    s = x
-    for _ in range(32 // k + 1):
-        s = x ^ (s >> k)
-    return s & 0xFFFFFFFF
+# pcg32 demo of the PCG reference implementation (pcg-c-basic, pcg32-demo):
+# pcg32_srandom_r(&rng, 42u, 54u), then six pcg32_random_r draws.
+PCG32_42_54 = [0xA15C02B7, 0x7B47F409, 0xBA1D3330, 0x83D2F293, 0xBFA4784B, 0xCBED606E]
+# splitmix64.c (Vigna) seeded with 0: the first three next() outputs, i.e.
+# the finaliser at k * 0x9e3779b97f4a7c15 for k = 0, 1, 2.
+SPLITMIX64_0 = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
 
 
-def test_pcg_output_is_a_bijection():
-    rng = np.random.default_rng(0)
-    for s in rng.integers(0, 2**32, 2000, dtype=np.uint64):
-        out = binding.lib().oracle_pcg_hash((int(s) - 2891336453) * pow(747796405, -1, 1 << 32) & 0xFFFFFFFF)
-        assert _pcg_output_inverse(out) == int(s)
+def test_pcg32_known_answers():
+    assert [int(v) for v in binding.pcg32_draws(42, 54, 6)] == PCG32_42_54
 
 
-def test_pcg_lcg_has_full_period_constants():
-    # Hull-Dobell for modulus 2^32: odd increment, multiplier = 1 mod 4.
-    assert 2891336453 % 2 == 1 and 747796405 % 4 == 1
+def test_splitmix64_known_answers():
+    for k, expected in enumerate(SPLITMIX64_0):
+        assert binding.lib().oracle_splitmix64((k * 0x9E3779B97F4A7C15) & (2**64 - 1)) == expected
+
+
+def test_path_generator_is_pcg32_on_the_default_stream():
+    """kind 1 draws = pcg32 seeded with initstate splitmix64(((frame << 32) |
+    index) ^ splitmix64(seed)) on the default stream (initseq such that
+    2 * initseq + 1 = 0xda3e39cb94b95bdb)."""
+    index, frame, seed = 3 + 7 * 512, 5, 1
+    initstate = binding.lib().oracle_splitmix64(((frame << 32) | index) ^ binding.lib().oracle_splitmix64(seed))
+    _, draws = binding.rng_draws(1, 3, 7, 512, frame, seed, 16)
+    expected = binding.pcg32_draws(initstate, 0xDA3E39CB94B95BDB >> 1, 16)
+    np.testing.assert_array_equal(draws, expected)
 
 
 @pytest.mark.parametrize("kind", [0, 1])

@@ -19,9 +19,12 @@
 namespace ttd {
 
 // ---------------------------------------------------------------- random --
+// kind 0: the reference's xorshift32 (random.glsl:29-34) in the low 32 bits
+// of `state`; kind 1: pcg32 (PCG-XSH-RR 64/32, O'Neill 2014) on the 64-bit
+// LCG state with the pcg32 default stream.
 struct Generator {
-    uint32_t state;
-    uint32_t kind;  // 0 = reference xorshift32 (random.glsl:29-34), 1 = PCG
+    uint64_t state;
+    uint32_t kind;
 };
 
 TT_DEV uint32_t jenkins_hash(uint32_t x) {
@@ -33,25 +36,47 @@ TT_DEV uint32_t jenkins_hash(uint32_t x) {
     return x;
 }
 
-// PCG-RXS-M-XS 32 (O'Neill 2014, section 6.3.4) on a 32-bit LCG.
-TT_DEV uint32_t pcg_output(uint32_t s) {
-    uint32_t word = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
-    return (word >> 22u) ^ word;
+constexpr uint64_t PCG32_MULT = 6364136223846793005ull;
+constexpr uint64_t PCG32_INC = 0xda3e39cb94b95bdbull;  // pcg32's default stream
+
+// XSH-RR output of the pre-step state.
+TT_DEV uint32_t pcg32_output(uint64_t old) {
+    uint32_t xs = uint32_t(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = uint32_t(old >> 59u);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
 }
-TT_DEV uint32_t pcg_step(uint32_t s) { return s * 747796405u + 2891336453u; }
-TT_DEV uint32_t pcg_hash(uint32_t v) { return pcg_output(pcg_step(v)); }
+
+// SplitMix64 finaliser (Steele, Lea & Flood 2014): turns (pixel, frame, seed)
+// keys into well-spread pcg32 initial states.
+TT_DEV uint64_t splitmix64(uint64_t x) {
+    uint64_t z = x + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// pcg32_srandom with the default stream.
+TT_DEV Generator pcg32_seeded(uint64_t initstate) {
+    uint64_t s = PCG32_INC;  // 0 * MULT + INC
+    s = s + initstate;
+    s = s * PCG32_MULT + PCG32_INC;
+    return Generator{s, 1u};
+}
+
+TT_DEV uint64_t pcg32_key(uint32_t index, uint32_t frame) { return (uint64_t(frame) << 32) | index; }
 
 TT_DEV Generator init_generator(uint32_t index, uint32_t frame, uint32_t kind, uint32_t seed) {
-    if (kind == 1u) return Generator{pcg_hash(index ^ pcg_hash(frame ^ pcg_hash(seed))), 1u};
+    if (kind == 1u) return pcg32_seeded(splitmix64(pcg32_key(index, frame) ^ splitmix64(seed)));
     return Generator{jenkins_hash(index ^ jenkins_hash(frame)), 0u};
 }
 
 TT_DEV uint32_t next_u32(Generator& g) {
     if (g.kind == 1u) {
-        g.state = pcg_step(g.state);
-        return pcg_output(g.state);
+        uint64_t old = g.state;
+        g.state = old * PCG32_MULT + PCG32_INC;
+        return pcg32_output(old);
     }
-    uint32_t s = g.state;
+    uint32_t s = uint32_t(g.state);
     s ^= s << 13;
     s ^= s >> 17;
     s ^= s << 5;

@@ -201,7 +201,7 @@ TT_DEV LightPoint light_point(const SceneView& s, uint32_t li, f3 lb) {
 // in ReSTIR mode come from this generator, so the reservoir target at a
 // pixel can be rebuilt from its G-buffer record (pixel, frame) anywhere.
 TT_DEV Generator restir_lobe_generator(uint32_t pixel, uint32_t frame) {
-    return Generator{pcg_hash(pixel ^ pcg_hash(frame ^ 0xa511e9b3u)), 1u};
+    return pcg32_seeded(splitmix64(pcg32_key(pixel, frame) ^ 0xa511e9b3c0ffee11ull));
 }
 
 // Reservoir (16 B): x = emissive triangle (0xffffffff = empty), y = the

@@ -40,7 +40,7 @@ struct FrameArgs {
 struct PathBuffers {
     float4* throughput;   // [paths]
     float4* radiance;     // [paths]
-    uint32_t* rng;        // [paths]
+    uint64_t* rng;        // [paths] generator state
     float4* queue[2][2];  // [ping-pong][origin|direction], [paths] each
     float4* hits;         // [paths]
     float4* shadow[4];    // origin, direction, direct, emission: [paths] each

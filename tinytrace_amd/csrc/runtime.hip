@@ -108,7 +108,7 @@ tt_status ensure_paths(tt_context* c, size_t paths, tt::PathBuffers& b) {
             c->path_memory = nullptr;
             c->path_capacity = 0;
         }
-        size_t bytes = paths * (11 * sizeof(float4) + sizeof(uint32_t));
+        size_t bytes = paths * (11 * sizeof(float4) + sizeof(uint64_t));
         TT_HIP(c, hipMalloc(&c->path_memory, bytes));
         c->path_capacity = paths;
     }
@@ -125,7 +125,7 @@ tt_status ensure_paths(tt_context* c, size_t paths, tt::PathBuffers& b) {
     b.shadow[1] = f + 8 * n;
     b.shadow[2] = f + 9 * n;
     b.shadow[3] = f + 10 * n;
-    b.rng = reinterpret_cast<uint32_t*>(f + 11 * n);
+    b.rng = reinterpret_cast<uint64_t*>(f + 11 * n);
     b.counters = c->counters;
     b.spill = c->spill;
     return TT_OK;
