@@ -38,6 +38,9 @@ def main() -> int:
     p.add_argument("--light-sampling", choices=("restir", "nee"), default="restir")
     p.add_argument("--quality-frames", type=int, default=16,
                    help="frames rendered for the RMSE-vs-FP64 leg (0 skips it)")
+    p.add_argument("--serving-requests", type=int, default=40,
+                   help="requests in the serving-latency leg (0 skips it)")
+    p.add_argument("--serving-rate", type=float, default=5.0, help="Poisson arrival rate of that leg, requests/s")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="also run a plain device copy of this many MiB (reference kernel for HBM counters)")
     args = p.parse_args()
@@ -155,6 +158,11 @@ def main() -> int:
     if args.quality_frames > 0:
         result["quality"] = quality(renderer, settings, restir, args.quality_frames, step, sync, args.steps)
 
+    if args.serving_requests > 0:
+        result["serving"] = serving(renderer.context, args.serving_requests, args.serving_rate)
+        result["p50_ms"] = result["serving"]["p50_ms"]
+        result["p99_ms"] = result["serving"]["p99_ms"]
+
     if args.copy_calibration_mib:
         # A device-to-device hipMemcpy of known size runs as the runtime's
         # copy kernel: the reference point for the HBM byte counters.
@@ -203,6 +211,27 @@ def main() -> int:
         result["gpu_over_cpu"] = round(result["value"] / result["cpu_baseline"]["value"], 1)
     print(json.dumps(result))
     return 0
+
+
+def serving(context, requests, rate):
+    """Request latency of the serving loop: four resident Cornell-box scenes
+    (512x512-class C1 scenes, depth 4, next-event estimation), Poisson
+    arrivals, resolutions 256^2 / 512^2 / 1920x1080 / 3840x2160 with
+    probabilities .4 / .3 / .2 / .1, 1 / 4 / 16 / 64 samples per pixel,
+    cameras on a sphere of radius 3.5 around the box.  Latency runs from
+    arrival to the end of the image download."""
+    from tinytrace_amd import Config, DeviceScene, LightSampling, scenes
+    from tinytrace_amd.serving import Server, request_mix, serve_poisson
+
+    server = Server(context, pool_paths=1 << 22)
+    slots = [server.add_scene(DeviceScene(context, scenes.cornell_box(v)[0])) for v in range(4)]
+    specs = request_mix(requests, rate_per_s=rate, scenes=4, seed=0)
+    config = Config(bounce_count=4, sample_count=1, light_sampling=LightSampling.NEXT_EVENT_ESTIMATION)
+    out = serve_poisson(server, slots, specs, config)
+    server.close()
+    return {"workload": f"{requests} requests, Poisson {rate}/s, 4 resident Cornell scenes, depth 4, NEE, "
+                        "resolution mix 256^2/512^2/1080p/2160p (.4/.3/.2/.1), spp 1/4/16/64, pool 4 Mi paths",
+            **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in out.items()}}
 
 
 def quality(renderer, settings, restir, frames, step, sync, steps):

@@ -17,8 +17,9 @@
  * Conventions
  *  - Every function returns tt_status; no C++ exception crosses the ABI.
  *  - Host arrays passed in are copied before the call returns.
- *  - Handles are owned by the context that created them; destroy scenes and
- *    targets before their context.  One context per host thread.
+ *  - Handles are owned by the context that created them.  They may be
+ *    destroyed in any order: a destroyed context stays alive until its last
+ *    scene, target and server is destroyed.  One context per host thread.
  *  - All device work of a context is ordered on one HIP stream;
  *    tt_target_download is the synchronisation point.
  *  - Structs marked "reference layout" are byte-identical to the reference's
@@ -276,6 +277,71 @@ typedef struct tt_render_stats {
 } tt_render_stats;
 /* Statistics of the last render with TT_RENDER_COLLECT_STATS (synchronises). */
 tt_status tt_render_stats_get(tt_context* context, tt_render_stats* out);
+
+/* ---------------------------------------------------------------------- */
+/* Serving loop (new with this library; the reference renders one request  */
+/* at a time through Renderer::render_to_texture, src/lib.rs:201-210)      */
+/* ---------------------------------------------------------------------- */
+
+/* A server keeps scenes resident and renders many requests concurrently:
+ * every step runs one batch, a shared pool of path slots filled with
+ * one-sample chunks (pass k, pixel range) of the admitted requests of one
+ * scene and integrator configuration, and accumulates each chunk into its
+ * request's target in pass order.  A request's image therefore depends only
+ * on the request (keyed generators: pixel, pass, seed) and is bit-identical
+ * whether it runs alone or in any mix.  Requests wait in arrival order until
+ * their render target fits the HBM budget next to the path pool. */
+typedef struct tt_server tt_server;
+
+typedef struct tt_server_desc {
+    uint64_t hbm_budget_bytes; /* path pool + resident targets (0: 64 GiB) */
+    uint32_t pool_paths;       /* path slots per batch (0: 4 Mi) */
+    uint32_t reserved;
+} tt_server_desc;
+
+typedef struct tt_server_request {
+    uint32_t scene;          /* slot from tt_server_add_scene */
+    uint32_t rng;            /* TT_RNG_* */
+    uint32_t seed;
+    uint32_t passes;         /* samples per pixel, one per pass */
+    tt_constants constants;  /* camera, screen size, bounce_count,
+                                light_sampling (ON_HIT or NEE); frame_index
+                                and accumulated_frame_count are set per pass
+                                (k and k + 1: the result is the exact mean) */
+} tt_server_request;
+
+enum { TT_REQUEST_QUEUED = 0, TT_REQUEST_ADMITTED = 1, TT_REQUEST_DONE = 2 };
+
+typedef struct tt_request_status {
+    uint32_t state;          /* TT_REQUEST_* */
+    uint32_t passes_done;
+    uint64_t submitted_ns;   /* steady clock, ns since the server was created */
+    uint64_t admitted_ns;
+    uint64_t done_ns;        /* image downloaded to host memory */
+} tt_request_status;
+
+typedef struct tt_server_stats {
+    uint64_t batches;
+    uint64_t paths;          /* path samples rendered */
+    uint64_t queued, active, done;
+    uint64_t budget_used_bytes;
+    double busy_ms;          /* wall time spent inside tt_server_step batches */
+} tt_server_stats;
+
+tt_status tt_server_create(tt_context* context, const tt_server_desc* desc, tt_server** out);
+void tt_server_destroy(tt_server* server);
+/* Keeps `scene` resident for requests (the scene must outlive the server). */
+tt_status tt_server_add_scene(tt_server* server, const tt_scene* scene, uint32_t* slot);
+/* TT_OVER_BUDGET when the request's target can never fit next to the pool. */
+tt_status tt_server_submit(tt_server* server, const tt_server_request* request, uint64_t* id);
+/* Admits what fits, renders one batch (blocking) and downloads the requests
+ * it completed; *completed = their number (0 and no work when idle). */
+tt_status tt_server_step(tt_server* server, uint32_t* completed);
+tt_status tt_server_status(tt_server* server, uint64_t id, tt_request_status* out);
+/* Copies a done request's RGBA32F image (width*height*16 bytes) and forgets
+ * the request. */
+tt_status tt_server_take_result(tt_server* server, uint64_t id, float* rgba, size_t bytes);
+tt_status tt_server_stats_get(tt_server* server, tt_server_stats* out);
 
 /* ---------------------------------------------------------------------- */
 /* Test entry points                                                       */

@@ -6,8 +6,10 @@ fallback path.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -110,6 +112,28 @@ class MaterialDesc(C.Structure):
     _fields_ = [(n, C.c_float if k == 1 else C.c_float * 3) for n, k in MATERIAL_DESC_FIELDS]
 
 
+class ServerDesc(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("pool_paths", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class ServerRequest(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("rng", C.c_uint32), ("seed", C.c_uint32), ("passes", C.c_uint32),
+                ("constants", Constants)]
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [("state", C.c_uint32), ("passes_done", C.c_uint32), ("submitted_ns", C.c_uint64),
+                ("admitted_ns", C.c_uint64), ("done_ns", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("batches", C.c_uint64), ("paths", C.c_uint64), ("queued", C.c_uint64), ("active", C.c_uint64),
+                ("done", C.c_uint64), ("budget_used_bytes", C.c_uint64), ("busy_ms", C.c_double)]
+
+
+TT_REQUEST_QUEUED, TT_REQUEST_ADMITTED, TT_REQUEST_DONE = 0, 1, 2
+
+
 # numpy dtypes of the reference byte layouts (little-endian, packed as C).
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("direction", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("instance", "<u4"), ("primitive", "<u4"), ("barycentrics", "<f4", 2)])
@@ -152,6 +176,14 @@ _PROTOTYPES = [
     ("tt_target_download", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ("tt_timing_ms", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
     ("tt_render_stats_get", C.c_int, [C.c_void_p, C.POINTER(RenderStats)]),
+    ("tt_server_create", C.c_int, [C.c_void_p, C.POINTER(ServerDesc), C.POINTER(C.c_void_p)]),
+    ("tt_server_destroy", None, [C.c_void_p]),
+    ("tt_server_add_scene", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)]),
+    ("tt_server_submit", C.c_int, [C.c_void_p, C.POINTER(ServerRequest), C.POINTER(C.c_uint64)]),
+    ("tt_server_step", C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    ("tt_server_status", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(RequestStatus)]),
+    ("tt_server_take_result", C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_size_t]),
+    ("tt_server_stats_get", C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     ("tt_trace_closest", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     ("tt_trace_closest_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     ("tt_bsdf_evaluate", C.c_int, [C.c_void_p, C.POINTER(MaterialConstants), C.c_void_p, C.c_void_p, C.c_uint32,
@@ -172,6 +204,27 @@ _PROTOTYPES = [
 ]
 
 EXPORTED_SYMBOLS = [p[0] for p in _PROTOTYPES]
+
+
+# Live native handles by teardown order (servers, then scenes / targets /
+# asset scenes, then contexts).  They are closed at interpreter exit, before
+# the HIP runtime's own static teardown, whatever order the garbage collector
+# would pick.
+_LIVE = (weakref.WeakSet(), weakref.WeakSet(), weakref.WeakSet())
+
+
+def track(obj, level: int) -> None:
+    _LIVE[level].add(obj)
+
+
+@atexit.register
+def _close_live_handles() -> None:
+    for live in _LIVE:
+        for obj in list(live):
+            try:
+                obj.close()
+            except Exception:
+                pass
 
 
 def lib():

@@ -252,6 +252,7 @@ class Context:
         desc = _abi.ContextDesc(device, 0)
         check(lib().tt_context_create(C.byref(desc), C.byref(h)), None, "tt_context_create")
         self.handle = h
+        _abi.track(self, 2)
 
     def close(self):
         if getattr(self, "handle", None):
@@ -271,6 +272,7 @@ class DeviceScene:
         check(lib().tt_scene_create(context.handle, C.byref(asset.flatten()), C.byref(h)), context.handle,
               "tt_scene_create")
         self.handle = h
+        _abi.track(self, 1)
 
     def info(self) -> dict:
         i = _abi.SceneInfo()
@@ -300,6 +302,7 @@ class Target:
         h = C.c_void_p()
         check(lib().tt_target_create(context.handle, width, height, C.byref(h)), context.handle, "tt_target_create")
         self.handle = h
+        _abi.track(self, 1)
 
     def download(self) -> np.ndarray:
         out = np.zeros((self.height, self.width, 4), dtype=np.float32)

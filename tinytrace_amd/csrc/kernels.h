@@ -84,6 +84,12 @@ hipError_t launch_shade(hipStream_t s, const DeviceScene& scene, const FrameArgs
                         uint32_t bounce, uint32_t paths);
 hipError_t launch_shadow(hipStream_t s, const DeviceScene& scene, const PathBuffers& b, uint32_t bounce, int grid);
 hipError_t launch_accumulate(hipStream_t s, const FrameArgs& f, const PathBuffers& b, float4* target);
+// Serving batches: one sample of pixels [pixel0, pixel0 + count) in path
+// slots [base, base + count).
+hipError_t launch_generate_chunk(hipStream_t s, const FrameArgs& f, const PathBuffers& b, uint32_t base,
+                                 uint32_t pixel0, uint32_t count);
+hipError_t launch_accumulate_chunk(hipStream_t s, const FrameArgs& f, const PathBuffers& b, uint32_t base,
+                                   uint32_t pixel0, uint32_t count, float4* target);
 hipError_t launch_bsdf_evaluate(hipStream_t s, const tt_material_constants* material, const tt_bsdf_query* q,
                                 tt_bsdf_result* r, uint32_t count, uint32_t rng_state);
 hipError_t launch_math(hipStream_t s, int op, const float* x, const float* y, float* out, uint32_t count);

@@ -60,10 +60,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void trace_rays_kernel(tt::DeviceScene
 // -------------------------------------------------------------- generate --
 // integrate.glsl:129-132 (pixel_ndc), :17-20 (camera_ray_direction),
 // :323-333 (seed, per-sample generator copy).  Path p = pixel * spp + sample.
-__global__ __launch_bounds__(256) void generate_kernel(tt::FrameArgs f, tt::PathBuffers b, uint32_t paths) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= paths) return;
-    uint32_t pixel = p / f.sample_count, sample = p % f.sample_count;
+// Path p of pixel `pixel`, sample `sample` of this pass.
+TT_DEV void generate_path(const tt::FrameArgs& f, const tt::PathBuffers& b, uint32_t p, uint32_t pixel,
+                          uint32_t sample) {
     uint32_t x = pixel % f.width, y = pixel / f.width;
     Generator g = init_generator(pixel, f.frame_index, f.rng, f.seed);
     for (uint32_t k = 0; k < 2u * sample; k++) next_u32(g);
@@ -83,7 +82,23 @@ __global__ __launch_bounds__(256) void generate_kernel(tt::FrameArgs f, tt::Path
     b.throughput[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     b.radiance[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     b.rng[p] = g.state;
+}
+
+__global__ __launch_bounds__(256) void generate_kernel(tt::FrameArgs f, tt::PathBuffers b, uint32_t paths) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= paths) return;
+    generate_path(f, b, p, p / f.sample_count, p % f.sample_count);
     if (p == 0) b.counters[C_EXT + 0] = paths;
+}
+
+// Serving: one sample of pixels [pixel0, pixel0 + count) into path slots
+// [base, base + count); the batch's path count accumulates in C_EXT[0].
+__global__ __launch_bounds__(256) void generate_chunk_kernel(tt::FrameArgs f, tt::PathBuffers b, uint32_t base,
+                                                             uint32_t pixel0, uint32_t count) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    generate_path(f, b, base + i, pixel0 + i, 0u);
+    if (i == 0) atomicAdd(b.counters + C_EXT + 0, count);
 }
 
 // ---------------------------------------------------------------- extend --
@@ -279,7 +294,16 @@ __global__ __launch_bounds__(TRACE_BLOCK) void shadow_kernel(tt::DeviceScene d, 
 }
 
 // ------------------------------------------------------------ accumulate --
-// integrate.glsl:338-346
+// integrate.glsl:338-346: with n = accumulated_frame_count != 0 the target
+// becomes (previous * (n - 1) + new) / n.
+TT_DEV void accumulate_pixel(const tt::FrameArgs& f, float4* target, uint32_t pixel, f3 acc) {
+    if (f.accumulated_frame_count != 0) {
+        float4 prev = target[pixel];
+        acc = (f3(prev.x, prev.y, prev.z) * float(f.accumulated_frame_count - 1) + acc) / float(f.accumulated_frame_count);
+    }
+    target[pixel] = make_float4(acc.x, acc.y, acc.z, 1.0f);
+}
+
 __global__ __launch_bounds__(256) void accumulate_kernel(tt::FrameArgs f, tt::PathBuffers b, float4* target) {
     uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
     if (pixel >= f.width * f.height) return;
@@ -289,11 +313,16 @@ __global__ __launch_bounds__(256) void accumulate_kernel(tt::FrameArgs f, tt::Pa
         acc = acc + f3(r.x, r.y, r.z);
     }
     acc = acc / float(f.sample_count);
-    if (f.accumulated_frame_count != 0) {
-        float4 prev = target[pixel];
-        acc = (f3(prev.x, prev.y, prev.z) * float(f.accumulated_frame_count - 1) + acc) / float(f.accumulated_frame_count);
-    }
-    target[pixel] = make_float4(acc.x, acc.y, acc.z, 1.0f);
+    accumulate_pixel(f, target, pixel, acc);
+}
+
+// Serving: the chunk generated by generate_chunk_kernel (one sample).
+__global__ __launch_bounds__(256) void accumulate_chunk_kernel(tt::FrameArgs f, tt::PathBuffers b, uint32_t base,
+                                                               uint32_t pixel0, uint32_t count, float4* target) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float4 r = b.radiance[base + i];
+    accumulate_pixel(f, target, pixel0 + i, f3(r.x, r.y, r.z) / 1.0f);
 }
 
 // ------------------------------------------------------------------ misc --
@@ -373,6 +402,18 @@ hipError_t launch_trace_rays(hipStream_t s, const DeviceScene& scene, const tt_r
 
 hipError_t launch_generate(hipStream_t s, const FrameArgs& f, const PathBuffers& b, uint32_t paths) {
     generate_kernel<<<blocks(paths, 256), 256, 0, s>>>(f, b, paths);
+    return hipGetLastError();
+}
+
+hipError_t launch_generate_chunk(hipStream_t s, const FrameArgs& f, const PathBuffers& b, uint32_t base,
+                                 uint32_t pixel0, uint32_t count) {
+    generate_chunk_kernel<<<blocks(count, 256), 256, 0, s>>>(f, b, base, pixel0, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate_chunk(hipStream_t s, const FrameArgs& f, const PathBuffers& b, uint32_t base,
+                                   uint32_t pixel0, uint32_t count, float4* target) {
+    accumulate_chunk_kernel<<<blocks(count, 256), 256, 0, s>>>(f, b, base, pixel0, count, target);
     return hipGetLastError();
 }
 

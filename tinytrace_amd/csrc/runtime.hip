@@ -14,75 +14,12 @@
 #include "../../include/tinytrace.h"
 #include "bvh.h"
 #include "kernels.h"
+#include "runtime_internal.h"
+
+using tt::fail;
+using tt::hip_fail;
 
 namespace {
-
-struct EventPair {
-    hipEvent_t begin = nullptr, end = nullptr;
-};
-
-}  // namespace
-
-struct tt_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string error;
-    int cu_count = 0;
-    int grid = 0;
-    // Wavefront buffers, grown on demand.
-    size_t path_capacity = 0;
-    void* path_memory = nullptr;
-    uint32_t* counters = nullptr;
-    uint32_t* spill = nullptr;
-    uint32_t* work = nullptr;
-    // Timing.
-    EventPair integrate;
-    std::vector<EventPair> trace_events;
-    std::vector<uint8_t> trace_event_shadow;
-    uint32_t trace_events_used = 0;
-    bool stats_pending = false;
-    uint64_t last_paths = 0;
-    uint32_t last_bounces = 0;
-    double last_integrate_ms = -1.0;
-};
-
-struct tt_scene {
-    tt_context* context = nullptr;
-    tt::DeviceScene view{};
-    std::vector<void*> allocations;
-    tt_scene_info info{};
-    uint64_t serial = 0;  // unique per scene created in this process
-};
-
-struct tt_target {
-    tt_context* context = nullptr;
-    uint32_t width = 0, height = 0;
-    float4* pixels = nullptr;
-    // ReSTIR DI history (allocated on first use): two G-buffers and the
-    // reservoir arrays of tt::RestirBuffers in one allocation.
-    void* restir_memory = nullptr;
-    uint32_t restir_cur = 0;
-    bool restir_history = false;
-    uint64_t restir_scene = 0;  // serial of the scene the history belongs to
-};
-
-namespace {
-
-tt_status fail(tt_context* c, tt_status s, const std::string& msg) {
-    if (c) c->error = msg;
-    return s;
-}
-
-tt_status hip_fail(tt_context* c, hipError_t e, const char* what) {
-    return fail(c, e == hipErrorOutOfMemory ? TT_OUT_OF_MEMORY : TT_HIP_ERROR,
-                std::string(what) + ": " + hipGetErrorString(e));
-}
-
-#define TT_HIP(ctx, expr)                                  \
-    do {                                                   \
-        hipError_t e_ = (expr);                            \
-        if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
-    } while (0)
 
 tt_status upload(tt_context* c, tt_scene* s, const void* src, size_t bytes, const void** dst) {
     if (bytes == 0) {
@@ -98,9 +35,24 @@ tt_status upload(tt_context* c, tt_scene* s, const void* src, size_t bytes, cons
     return TT_OK;
 }
 
+tt_status ensure_events(tt_context* c, uint32_t count) {
+    while (c->trace_events.size() < count) {
+        tt::EventPair e;
+        TT_HIP(c, hipEventCreate(&e.begin));
+        TT_HIP(c, hipEventCreate(&e.end));
+        c->trace_events.push_back(e);
+        c->trace_event_shadow.push_back(0);
+    }
+    return TT_OK;
+}
+
+}  // namespace
+
+namespace tt {
+
 // Path buffers: throughput, radiance, 2x2 queue arrays, hits, 4 shadow
 // arrays (float4 each) and the generator words.
-tt_status ensure_paths(tt_context* c, size_t paths, tt::PathBuffers& b) {
+tt_status ensure_paths(tt_context* c, size_t paths, PathBuffers& b) {
     if (paths > c->path_capacity) {
         if (c->path_memory) {
             TT_HIP(c, hipStreamSynchronize(c->stream));
@@ -108,7 +60,7 @@ tt_status ensure_paths(tt_context* c, size_t paths, tt::PathBuffers& b) {
             c->path_memory = nullptr;
             c->path_capacity = 0;
         }
-        size_t bytes = paths * (11 * sizeof(float4) + sizeof(uint64_t));
+        size_t bytes = paths * PATH_BYTES;
         TT_HIP(c, hipMalloc(&c->path_memory, bytes));
         c->path_capacity = paths;
     }
@@ -131,18 +83,26 @@ tt_status ensure_paths(tt_context* c, size_t paths, tt::PathBuffers& b) {
     return TT_OK;
 }
 
-tt_status ensure_events(tt_context* c, uint32_t count) {
-    while (c->trace_events.size() < count) {
-        EventPair e;
-        TT_HIP(c, hipEventCreate(&e.begin));
-        TT_HIP(c, hipEventCreate(&e.end));
-        c->trace_events.push_back(e);
-        c->trace_event_shadow.push_back(0);
-    }
-    return TT_OK;
+FrameArgs frame_args(const tt_constants& k, uint32_t width, uint32_t height, uint32_t rng, uint32_t seed) {
+    FrameArgs f;
+    std::memcpy(f.inverse_view, k.inverse_view, sizeof(f.inverse_view));
+    std::memcpy(f.inverse_proj, k.inverse_proj, sizeof(f.inverse_proj));
+    f.camera_position[0] = k.camera_position[0];
+    f.camera_position[1] = k.camera_position[1];
+    f.camera_position[2] = k.camera_position[2];
+    f.width = width;
+    f.height = height;
+    f.sample_count = k.sample_count;
+    f.bounce_count = k.bounce_count;
+    f.frame_index = k.frame_index;
+    f.accumulated_frame_count = k.accumulated_frame_count;
+    f.light_sampling = k.light_sampling;
+    f.rng = rng;
+    f.seed = seed;
+    return f;
 }
 
-}  // namespace
+}  // namespace tt
 
 extern "C" {
 
@@ -176,8 +136,11 @@ tt_status tt_context_create(const tt_context_desc* desc, tt_context** out) {
     return TT_OK;
 }
 
-void tt_context_destroy(tt_context* c) {
-    if (!c) return;
+}  // extern "C"
+
+namespace {
+
+void context_free(tt_context* c) {
     // Teardown is best effort: errors here have no caller to report to.
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -193,6 +156,25 @@ void tt_context_destroy(tt_context* c) {
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+}  // namespace
+
+void tt::context_retain(tt_context* c) { c->children++; }
+
+void tt::context_release(tt_context* c) {
+    if (--c->children == 0 && c->closing) context_free(c);
+}
+
+extern "C" {
+
+void tt_context_destroy(tt_context* c) {
+    if (!c) return;
+    if (c->children > 0) {
+        c->closing = true;
+        return;
+    }
+    context_free(c);
 }
 
 const char* tt_last_error(const tt_context* c) { return c ? c->error.c_str() : "no context"; }
@@ -233,8 +215,10 @@ tt_status tt_scene_create(tt_context* c, const tt_scene_desc* d, tt_scene** out)
     if (s) s->serial = next_serial.fetch_add(1);
     if (!s) return fail(c, TT_OUT_OF_MEMORY, "scene allocation");
     s->context = c;
-    TT_HIP(c, hipSetDevice(c->device));
+    tt::context_retain(c);
     tt_status st = TT_OK;
+    hipError_t de = hipSetDevice(c->device);
+    if (de != hipSuccess) st = hip_fail(c, de, "hipSetDevice");
     auto up = [&](const void* src, size_t bytes, const void** dst) {
         if (st == TT_OK) st = upload(c, s, src, bytes, dst);
     };
@@ -275,7 +259,9 @@ void tt_scene_destroy(tt_scene* s) {
         (void)hipStreamSynchronize(s->context->stream);
     }
     for (void* p : s->allocations) (void)hipFree(p);
+    tt_context* c = s->context;
     delete s;
+    if (c) tt::context_release(c);
 }
 
 tt_status tt_scene_info_get(const tt_scene* s, tt_scene_info* out) {
@@ -290,13 +276,14 @@ tt_status tt_target_create(tt_context* c, uint32_t width, uint32_t height, tt_ta
     tt_target* t = new (std::nothrow) tt_target();
     if (!t) return fail(c, TT_OUT_OF_MEMORY, "target allocation");
     t->context = c;
+    tt::context_retain(c);
     t->width = width;
     t->height = height;
     hipError_t e = hipSetDevice(c->device);
     if (e == hipSuccess) e = hipMalloc(&t->pixels, size_t(width) * height * sizeof(float4));
     if (e == hipSuccess) e = hipMemsetAsync(t->pixels, 0, size_t(width) * height * sizeof(float4), c->stream);
     if (e != hipSuccess) {
-        delete t;
+        tt_target_destroy(t);
         return hip_fail(c, e, "target allocation");
     }
     *out = t;
@@ -311,7 +298,9 @@ void tt_target_destroy(tt_target* t) {
     }
     if (t->pixels) (void)hipFree(t->pixels);
     if (t->restir_memory) (void)hipFree(t->restir_memory);
+    tt_context* c = t->context;
     delete t;
+    if (c) tt::context_release(c);
 }
 
 tt_status tt_render_ex(tt_context* c, const tt_scene* s, const tt_constants* k, const tt_render_settings* settings,
@@ -336,7 +325,7 @@ tt_status tt_render_ex(tt_context* c, const tt_scene* s, const tt_constants* k, 
     uint32_t paths = uint32_t(paths64);
     TT_HIP(c, hipSetDevice(c->device));
     tt::PathBuffers b;
-    tt_status es = ensure_paths(c, paths, b);
+    tt_status es = tt::ensure_paths(c, paths, b);
     if (es != TT_OK) return es;
     b.max_bounces = k->bounce_count;
     bool stats = (st.flags & TT_RENDER_COLLECT_STATS) != 0;
@@ -372,21 +361,7 @@ tt_status tt_render_ex(tt_context* c, const tt_scene* s, const tt_constants* k, 
         b.restir_final = rb.final_[rb.cur];
     }
 
-    tt::FrameArgs f;
-    std::memcpy(f.inverse_view, k->inverse_view, sizeof(f.inverse_view));
-    std::memcpy(f.inverse_proj, k->inverse_proj, sizeof(f.inverse_proj));
-    f.camera_position[0] = k->camera_position[0];
-    f.camera_position[1] = k->camera_position[1];
-    f.camera_position[2] = k->camera_position[2];
-    f.width = t->width;
-    f.height = t->height;
-    f.sample_count = k->sample_count;
-    f.bounce_count = k->bounce_count;
-    f.frame_index = k->frame_index;
-    f.accumulated_frame_count = k->accumulated_frame_count;
-    f.light_sampling = k->light_sampling;
-    f.rng = st.rng;
-    f.seed = st.seed;
+    tt::FrameArgs f = tt::frame_args(*k, t->width, t->height, st.rng, st.seed);
 
     hipStream_t stream = c->stream;
     TT_HIP(c, hipEventRecord(c->integrate.begin, stream));

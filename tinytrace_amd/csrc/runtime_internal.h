@@ -1,0 +1,99 @@
+// Internal state behind the C ABI, shared by the render path (runtime.hip)
+// and the serving loop (server.hip).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tinytrace.h"
+#include "kernels.h"
+
+namespace tt {
+
+struct EventPair {
+    hipEvent_t begin = nullptr, end = nullptr;
+};
+
+}  // namespace tt
+
+struct tt_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    int cu_count = 0;
+    int grid = 0;
+    // Wavefront buffers, grown on demand.
+    size_t path_capacity = 0;
+    void* path_memory = nullptr;
+    uint32_t* counters = nullptr;
+    uint32_t* spill = nullptr;
+    uint32_t* work = nullptr;
+    // Timing.
+    tt::EventPair integrate;
+    std::vector<tt::EventPair> trace_events;
+    std::vector<uint8_t> trace_event_shadow;
+    uint32_t trace_events_used = 0;
+    bool stats_pending = false;
+    uint64_t last_paths = 0;
+    uint32_t last_bounces = 0;
+    double last_integrate_ms = -1.0;
+    // Scenes, targets and servers alive on this context.  Destroying the
+    // context while any remain only marks it; the last of them frees it, so
+    // handles may be destroyed in any order.
+    uint32_t children = 0;
+    bool closing = false;
+};
+
+struct tt_scene {
+    tt_context* context = nullptr;
+    tt::DeviceScene view{};
+    std::vector<void*> allocations;
+    tt_scene_info info{};
+    uint64_t serial = 0;  // unique per scene created in this process
+};
+
+struct tt_target {
+    tt_context* context = nullptr;
+    uint32_t width = 0, height = 0;
+    float4* pixels = nullptr;
+    // ReSTIR DI history (allocated on first use): two G-buffers and the
+    // reservoir arrays of tt::RestirBuffers in one allocation.
+    void* restir_memory = nullptr;
+    uint32_t restir_cur = 0;
+    bool restir_history = false;
+    uint64_t restir_scene = 0;  // serial of the scene the history belongs to
+};
+
+namespace tt {
+
+inline tt_status fail(tt_context* c, tt_status s, const std::string& msg) {
+    if (c) c->error = msg;
+    return s;
+}
+
+inline tt_status hip_fail(tt_context* c, hipError_t e, const char* what) {
+    return fail(c, e == hipErrorOutOfMemory ? TT_OUT_OF_MEMORY : TT_HIP_ERROR,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void context_retain(tt_context* c);
+void context_release(tt_context* c);
+
+// Path buffers of at least `paths` paths (grown on demand, kept in the
+// context); fills the pointers of `b`.
+tt_status ensure_paths(tt_context* c, size_t paths, PathBuffers& b);
+// Bytes of path state per path slot.
+constexpr size_t PATH_BYTES = 11 * sizeof(float4) + sizeof(uint64_t);
+// Per-render arguments of the kernels from the reference Constants.
+FrameArgs frame_args(const tt_constants& k, uint32_t width, uint32_t height, uint32_t rng, uint32_t seed);
+
+}  // namespace tt
+
+#define TT_HIP(ctx, expr)                                          \
+    do {                                                           \
+        hipError_t e_ = (expr);                                    \
+        if (e_ != hipSuccess) return tt::hip_fail(ctx, e_, #expr); \
+    } while (0)

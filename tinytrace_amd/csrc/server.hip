@@ -1,0 +1,321 @@
+// Serving loop: resident scenes, HBM-budget admission and continuous
+// batching of render requests over one shared pool of path slots.
+//
+// A request of P passes over W x H pixels is a sequence of one-sample work
+// units (pass k, pixel range) in the order k = 0 .. P-1, pixels ascending.
+// Every step builds one batch:
+//   1. admit queued requests in arrival order while their target
+//      (16 B/pixel) fits the HBM budget next to the path pool;
+//   2. pick the integrator configuration (scene, bounce_count,
+//      light_sampling, generator) of the oldest unfinished request and gather
+//      the unfinished requests that share it;
+//   3. give each of them an equal share of the pool's slots in arrival
+//      order, then hand what is left to the oldest first; each share is taken
+//      from the request's cursor as chunks (pass k, pixels [a, b));
+//   4. run the wavefront kernels once over the pool (generate per chunk,
+//      extend / shade / shadow per bounce for all chunks together,
+//      accumulate per chunk in the order taken);
+//   5. download finished requests and release their targets.
+// A path's radiance depends only on (scene, camera, pixel, pass, seed,
+// configuration) and each pixel accumulates its passes in order with
+// (previous * k + new) / (k + 1), so a request's image is the same bits
+// alone or in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "runtime_internal.h"
+
+namespace {
+
+struct Request {
+    uint64_t id = 0;
+    tt_server_request desc{};
+    uint32_t width = 0, height = 0;
+    uint32_t state = TT_REQUEST_QUEUED;
+    // Scheduling cursor: next unit is (next_pass, next_pixel).
+    uint32_t next_pass = 0, next_pixel = 0;
+    uint32_t passes_done = 0;  // passes fully accumulated
+    float4* target = nullptr;
+    std::vector<float> image;  // host copy once done
+    uint64_t submitted_ns = 0, admitted_ns = 0, done_ns = 0;
+
+    uint64_t pixels() const { return uint64_t(width) * height; }
+    uint64_t target_bytes() const { return pixels() * sizeof(float4); }
+    bool scheduled_all() const { return next_pass >= desc.passes; }
+};
+
+struct Chunk {
+    Request* request;
+    uint32_t pass, pixel0, count, base;
+};
+
+bool same_config(const Request& a, const Request& b) {
+    return a.desc.scene == b.desc.scene && a.desc.rng == b.desc.rng &&
+           a.desc.constants.bounce_count == b.desc.constants.bounce_count &&
+           a.desc.constants.light_sampling == b.desc.constants.light_sampling;
+}
+
+}  // namespace
+
+struct tt_server {
+    tt_context* context = nullptr;
+    uint64_t budget = 0;
+    uint32_t pool_paths = 0;
+    std::vector<const tt_scene*> scenes;
+    std::deque<Request*> queued;
+    std::vector<Request*> active;  // arrival order
+    std::map<uint64_t, std::unique_ptr<Request>> requests;
+    uint64_t next_id = 1;
+    uint64_t used = 0;  // targets of admitted requests
+    std::chrono::steady_clock::time_point origin;
+    tt_server_stats stats{};
+
+    uint64_t now_ns() const {
+        return uint64_t(std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - origin)
+                            .count());
+    }
+    uint64_t pool_bytes() const { return uint64_t(pool_paths) * tt::PATH_BYTES; }
+};
+
+namespace {
+
+using tt::fail;
+
+// Takes up to `want` path slots of work from r's cursor.
+void take(Request* r, uint32_t want, uint32_t& base, std::vector<Chunk>& out) {
+    const uint32_t px = uint32_t(r->pixels());
+    while (want > 0 && !r->scheduled_all()) {
+        uint32_t n = std::min<uint32_t>(want, px - r->next_pixel);
+        out.push_back(Chunk{r, r->next_pass, r->next_pixel, n, base});
+        base += n;
+        want -= n;
+        r->next_pixel += n;
+        if (r->next_pixel == px) {
+            r->next_pixel = 0;
+            r->next_pass++;
+        }
+    }
+}
+
+uint64_t remaining_paths(const Request* r) {
+    if (r->scheduled_all()) return 0;
+    return uint64_t(r->desc.passes - r->next_pass) * r->pixels() - r->next_pixel;
+}
+
+tt_status admit(tt_server* sv) {
+    tt_context* c = sv->context;
+    while (!sv->queued.empty()) {
+        Request* r = sv->queued.front();
+        if (sv->used + r->target_bytes() + sv->pool_bytes() > sv->budget) break;
+        TT_HIP(c, hipMalloc(&r->target, r->target_bytes()));
+        sv->used += r->target_bytes();
+        // Pass 0 computes (previous * 0 + new) / 1: start from zeros.
+        TT_HIP(c, tt::launch_zero(c->stream, reinterpret_cast<uint32_t*>(r->target), uint32_t(r->pixels() * 4)));
+        r->state = TT_REQUEST_ADMITTED;
+        r->admitted_ns = sv->now_ns();
+        sv->queued.pop_front();
+        sv->active.push_back(r);
+    }
+    return TT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+tt_status tt_server_create(tt_context* c, const tt_server_desc* desc, tt_server** out) {
+    if (!c || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    tt_server* sv = new (std::nothrow) tt_server();
+    if (!sv) return fail(c, TT_OUT_OF_MEMORY, "server allocation");
+    sv->context = c;
+    tt::context_retain(c);
+    sv->budget = desc && desc->hbm_budget_bytes ? desc->hbm_budget_bytes : (64ull << 30);
+    sv->pool_paths = desc && desc->pool_paths ? desc->pool_paths : (4u << 20);
+    sv->origin = std::chrono::steady_clock::now();
+    if (sv->pool_bytes() > sv->budget) {
+        tt_server_destroy(sv);
+        return fail(c, TT_OVER_BUDGET, "the path pool alone exceeds the HBM budget");
+    }
+    *out = sv;
+    return TT_OK;
+}
+
+void tt_server_destroy(tt_server* sv) {
+    if (!sv) return;
+    (void)hipSetDevice(sv->context->device);
+    (void)hipStreamSynchronize(sv->context->stream);
+    for (auto& kv : sv->requests)
+        if (kv.second->target) (void)hipFree(kv.second->target);
+    tt_context* c = sv->context;
+    delete sv;
+    tt::context_release(c);
+}
+
+tt_status tt_server_add_scene(tt_server* sv, const tt_scene* s, uint32_t* slot) {
+    if (!sv || !s || !slot) return fail(sv ? sv->context : nullptr, TT_INVALID_ARGUMENT, "null argument");
+    if (s->context != sv->context) return fail(sv->context, TT_STALE_HANDLE, "scene from another context");
+    *slot = uint32_t(sv->scenes.size());
+    sv->scenes.push_back(s);
+    return TT_OK;
+}
+
+tt_status tt_server_submit(tt_server* sv, const tt_server_request* q, uint64_t* id) {
+    if (!sv || !q || !id) return fail(sv ? sv->context : nullptr, TT_INVALID_ARGUMENT, "null argument");
+    tt_context* c = sv->context;
+    const tt_constants& k = q->constants;
+    if (q->scene >= sv->scenes.size()) return fail(c, TT_INVALID_ARGUMENT, "unknown scene slot");
+    if (q->passes == 0 || k.screen_size[0] == 0 || k.screen_size[1] == 0)
+        return fail(c, TT_INVALID_ARGUMENT, "empty request");
+    if (k.bounce_count == 0 || k.bounce_count > 31) return fail(c, TT_INVALID_ARGUMENT, "bounce_count in [1, 31]");
+    if (k.light_sampling != TT_LIGHT_SAMPLING_ON_HIT && k.light_sampling != TT_LIGHT_SAMPLING_NEXT_EVENT_ESTIMATION)
+        return fail(c, TT_UNSUPPORTED, "serving renders ON_HIT or NEXT_EVENT_ESTIMATION");
+    if (q->rng > 1) return fail(c, TT_INVALID_ARGUMENT, "unknown generator");
+    auto r = std::make_unique<Request>();
+    r->desc = *q;
+    r->width = k.screen_size[0];
+    r->height = k.screen_size[1];
+    if (r->pixels() >= (1ull << 30)) return fail(c, TT_INVALID_ARGUMENT, "request too large");
+    if (r->target_bytes() + sv->pool_bytes() > sv->budget)
+        return fail(c, TT_OVER_BUDGET, "the request's target does not fit the HBM budget next to the path pool");
+    r->id = sv->next_id++;
+    r->submitted_ns = sv->now_ns();
+    sv->queued.push_back(r.get());
+    *id = r->id;
+    sv->requests[r->id] = std::move(r);
+    return TT_OK;
+}
+
+tt_status tt_server_step(tt_server* sv, uint32_t* completed) {
+    if (!sv) return TT_INVALID_ARGUMENT;
+    tt_context* c = sv->context;
+    if (completed) *completed = 0;
+    TT_HIP(c, hipSetDevice(c->device));
+    tt_status st = admit(sv);
+    if (st != TT_OK) return st;
+
+    // The oldest unfinished request sets the batch's configuration.
+    Request* lead = nullptr;
+    for (Request* r : sv->active)
+        if (!r->scheduled_all()) {
+            lead = r;
+            break;
+        }
+    if (!lead) return TT_OK;
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<Request*> group;
+    for (Request* r : sv->active)
+        if (!r->scheduled_all() && same_config(*r, *lead)) group.push_back(r);
+
+    std::vector<Chunk> chunks;
+    uint32_t base = 0;
+    uint32_t share = std::max<uint32_t>(sv->pool_paths / uint32_t(group.size()), 1u);
+    for (Request* r : group) take(r, uint32_t(std::min<uint64_t>(share, remaining_paths(r))), base, chunks);
+    for (Request* r : group) take(r, sv->pool_paths - base, base, chunks);
+    const uint32_t paths = base;
+
+    tt::PathBuffers b;
+    st = tt::ensure_paths(c, sv->pool_paths, b);
+    if (st != TT_OK) return st;
+    const tt_scene* scene = sv->scenes[lead->desc.scene];
+    const tt_constants& lk = lead->desc.constants;
+    b.max_bounces = lk.bounce_count;
+    b.restir_final = nullptr;
+    auto chunk_args = [&](const Chunk& ch) {
+        tt_constants k = ch.request->desc.constants;
+        k.sample_count = 1;
+        k.frame_index = ch.pass;
+        k.accumulated_frame_count = ch.pass + 1;
+        return tt::frame_args(k, ch.request->width, ch.request->height, ch.request->desc.rng, ch.request->desc.seed);
+    };
+
+    hipStream_t stream = c->stream;
+    TT_HIP(c, tt::launch_zero(stream, c->counters, tt::C_WORDS));
+    for (const Chunk& ch : chunks)
+        TT_HIP(c, tt::launch_generate_chunk(stream, chunk_args(ch), b, ch.base, ch.pixel0, ch.count));
+    tt::FrameArgs f = chunk_args(chunks.front());  // configuration fields are the group's
+    for (uint32_t bounce = 0; bounce < lk.bounce_count; bounce++) {
+        TT_HIP(c, tt::launch_extend(stream, scene->view, b, bounce, c->grid));
+        TT_HIP(c, tt::launch_shade(stream, scene->view, f, b, bounce, paths));
+        if (lk.light_sampling != TT_LIGHT_SAMPLING_ON_HIT && scene->view.emissive_count > 0)
+            TT_HIP(c, tt::launch_shadow(stream, scene->view, b, bounce, c->grid));
+    }
+    for (const Chunk& ch : chunks)
+        TT_HIP(c, tt::launch_accumulate_chunk(stream, chunk_args(ch), b, ch.base, ch.pixel0, ch.count,
+                                              ch.request->target));
+    TT_HIP(c, hipStreamSynchronize(stream));
+    sv->stats.batches++;
+    sv->stats.paths += paths;
+
+    // Passes completed by this batch, then finished requests.
+    for (const Chunk& ch : chunks)
+        if (ch.pixel0 + ch.count == ch.request->pixels()) ch.request->passes_done = ch.pass + 1;
+    uint32_t done = 0;
+    std::vector<Request*> still;
+    for (Request* r : sv->active) {
+        if (r->passes_done < r->desc.passes) {
+            still.push_back(r);
+            continue;
+        }
+        r->image.resize(size_t(r->pixels()) * 4);
+        TT_HIP(c, hipMemcpy(r->image.data(), r->target, r->target_bytes(), hipMemcpyDeviceToHost));
+        TT_HIP(c, hipFree(r->target));
+        r->target = nullptr;
+        sv->used -= r->target_bytes();
+        r->state = TT_REQUEST_DONE;
+        r->done_ns = sv->now_ns();
+        done++;
+    }
+    sv->active.swap(still);
+    sv->stats.busy_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (completed) *completed = done;
+    return TT_OK;
+}
+
+tt_status tt_server_status(tt_server* sv, uint64_t id, tt_request_status* out) {
+    if (!sv || !out) return fail(sv ? sv->context : nullptr, TT_INVALID_ARGUMENT, "null argument");
+    auto it = sv->requests.find(id);
+    if (it == sv->requests.end()) return fail(sv->context, TT_STALE_HANDLE, "unknown request");
+    const Request& r = *it->second;
+    out->state = r.state;
+    out->passes_done = r.passes_done;
+    out->submitted_ns = r.submitted_ns;
+    out->admitted_ns = r.admitted_ns;
+    out->done_ns = r.done_ns;
+    return TT_OK;
+}
+
+tt_status tt_server_take_result(tt_server* sv, uint64_t id, float* rgba, size_t bytes) {
+    if (!sv || !rgba) return fail(sv ? sv->context : nullptr, TT_INVALID_ARGUMENT, "null argument");
+    auto it = sv->requests.find(id);
+    if (it == sv->requests.end()) return fail(sv->context, TT_STALE_HANDLE, "unknown request");
+    Request& r = *it->second;
+    if (r.state != TT_REQUEST_DONE) return fail(sv->context, TT_INVALID_ARGUMENT, "request not done");
+    if (bytes < r.target_bytes()) return fail(sv->context, TT_INVALID_ARGUMENT, "result buffer too small");
+    std::memcpy(rgba, r.image.data(), r.target_bytes());
+    sv->requests.erase(it);
+    return TT_OK;
+}
+
+tt_status tt_server_stats_get(tt_server* sv, tt_server_stats* out) {
+    if (!sv || !out) return fail(sv ? sv->context : nullptr, TT_INVALID_ARGUMENT, "null argument");
+    *out = sv->stats;
+    out->queued = sv->queued.size();
+    out->active = sv->active.size();
+    uint64_t done = 0;
+    for (auto& kv : sv->requests)
+        if (kv.second->state == TT_REQUEST_DONE) done++;
+    out->done = done;
+    out->budget_used_bytes = sv->used + sv->pool_bytes();
+    return TT_OK;
+}
+
+}  // extern "C"

@@ -75,7 +75,10 @@ def _insert_flat(scene: AssetScene, positions, normals, tangents, indices) -> in
     return scene.insert_mesh(positions, uv, indices, normals, tangents)
 
 
-def cornell_box() -> Tuple[AssetScene, Camera]:
+def cornell_box(variant: int = 0) -> Tuple[AssetScene, Camera]:
+    """The Cornell box (36 triangles, 2 emissive).  variant > 0 turns the two
+    boxes by a further 11 degrees per step and swaps the wall colours on odd
+    variants (distinct resident scenes for the serving mix)."""
     scene = AssetScene()
     base_only = dict(specular_weight=0.0, specular_roughness=1.0)
     light = scene.insert_material(Material(base_color=(0.73, 0.73, 0.73), emission_luminance=15.0,
@@ -83,6 +86,9 @@ def cornell_box() -> Tuple[AssetScene, Camera]:
     white = scene.insert_material(Material(base_color=(0.73, 0.73, 0.73), **base_only))
     red = scene.insert_material(Material(base_color=(0.65, 0.05, 0.05), **base_only))
     green = scene.insert_material(Material(base_color=(0.12, 0.45, 0.15), **base_only))
+    if variant % 2 == 1:
+        red, green = green, red
+    turn = math.radians(11.0 * variant)
 
     def wall(center, normal, u, a, b, material):
         p, n, t = _quad(center, normal, u, a, b)
@@ -96,8 +102,8 @@ def cornell_box() -> Tuple[AssetScene, Camera]:
     wall((0, 1, -1), (0, 0, 1), (1, 0, 0), 1, 1, white)     # back
     wall((-1, 1, 0), (1, 0, 0), (0, 0, 1), 1, 1, red)       # left
     wall((1, 1, 0), (-1, 0, 0), (0, 0, 1), 1, 1, green)     # right
-    for size, angle, center in (((0.6, 1.2, 0.6), math.radians(17.0), (-0.33, 0.6, -0.3)),
-                                ((0.6, 0.6, 0.6), math.radians(-17.0), (0.33, 0.3, 0.3))):
+    for size, angle, center in (((0.6, 1.2, 0.6), math.radians(17.0) + turn, (-0.33, 0.6, -0.3)),
+                                ((0.6, 0.6, 0.6), math.radians(-17.0) - turn, (0.33, 0.3, 0.3))):
         p, n, t = _box_faces(size, angle, center)
         mesh = _insert_flat(scene, p, n, t, _quad_indices(6))
         scene.add_model(scene.add_node(), mesh, white)
