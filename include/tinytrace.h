@@ -344,6 +344,57 @@ tt_status tt_server_take_result(tt_server* server, uint64_t id, float* rgba, siz
 tt_status tt_server_stats_get(tt_server* server, tt_server_stats* out);
 
 /* ---------------------------------------------------------------------- */
+/* Radiance cache (new with this library): a 64-5x64-3 MLP trained online   */
+/* ---------------------------------------------------------------------- */
+
+/* Network: 64 encoded features (position frequencies, direction, normal,
+ * base colour, roughness, metalness, bounce) -> 5 x (64, ReLU) -> 3
+ * (softplus) = outgoing radiance.  20672 fp32 master parameters (no biases),
+ * bf16 copies on the matrix cores, Adam.  A cached render ends each path at
+ * its first vertex after the primary one whose specular roughness exceeds
+ * 0.3 and adds throughput x cache; training paths run in full and teach the
+ * cache the radiance their tails carry. */
+typedef struct tt_mlp tt_mlp;
+
+typedef struct tt_mlp_desc {
+    uint64_t seed;        /* weight initialisation (He-uniform) */
+    float learning_rate;  /* Adam, 0: 1e-3 (beta1 0.9, beta2 0.99, eps 1e-8) */
+    uint32_t reserved;
+} tt_mlp_desc;
+
+tt_status tt_mlp_create(tt_context* context, const tt_mlp_desc* desc, tt_mlp** out);
+void tt_mlp_destroy(tt_mlp* mlp);
+/* World box that positions are normalised to. */
+tt_status tt_mlp_set_bounds(tt_mlp* mlp, const float lo[3], const float hi[3]);
+/* Read (`get`) and/or replace (`set`) the 20672 fp32 parameters. */
+tt_status tt_mlp_params(tt_mlp* mlp, float* get, const float* set, uint32_t count);
+/* Host-array operations on n raw records (16 floats each: position,
+ * direction, normal, base colour, roughness, metalness, bounce, unused):
+ *   op 0: encode -> out[n x 64];  op 1: infer -> out[n x 3];
+ *   op 2: loss gradient for targets[n x 4] -> out[20672], *loss;
+ *   op 3: one Adam step on targets -> *loss (before the step). */
+tt_status tt_mlp_run(tt_mlp* mlp, int op, const float* records, const float* targets, uint32_t n, float* out,
+                     double* loss);
+
+typedef struct tt_cache_settings {
+    uint32_t train_paths;  /* training paths per frame (0: 65536) */
+    uint32_t train_steps;  /* Adam steps per frame over their samples (0: 4) */
+} tt_cache_settings;
+
+typedef struct tt_cache_stats {
+    uint32_t queries;      /* paths ended at a cache query */
+    uint32_t samples;      /* training samples recorded */
+    double loss;           /* mean training loss of the frame's first step */
+} tt_cache_stats;
+
+/* tt_render_ex with the radiance cache: renders (NEE or ON_HIT, one sample
+ * per pixel), queries the cache for the paths it ends, then trains it on the
+ * frame's training samples. */
+tt_status tt_render_cached(tt_context* context, const tt_scene* scene, const tt_constants* constants,
+                           const tt_render_settings* settings, tt_mlp* cache, const tt_cache_settings* cache_settings,
+                           tt_target* target, tt_cache_stats* stats);
+
+/* ---------------------------------------------------------------------- */
 /* Test entry points                                                       */
 /* ---------------------------------------------------------------------- */
 

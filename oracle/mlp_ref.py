@@ -1,0 +1,166 @@
+"""ORACLE (test infrastructure only): float64 restatement of the radiance-cache
+MLP of tinytrace_amd/csrc/mlp.h and mlp.hip, for checking the GPU kernels.
+
+The reference has no neural cache, so this is a textbook statement of the
+network and its gradient.  The analytic gradient is checked against central
+finite differences in float64.  With `bf16=True` the forward and backward
+passes round to bfloat16 at exactly the points where the kernels do:
+
+  * the encoded input;
+  * every post-ReLU activation;
+  * the output delta;
+  * every backward delta.
+
+Accumulation is exact (float64), where the kernels accumulate in fp32.
+
+Network: a_0 = encode(record) (64); a_{l+1} = relu(W_l a_l) for l = 0..4
+(W_l: 64 x 64); z = W_5 a_5 (3 x 64); y = softplus(z).  The loss is the mean
+over samples and channels of (y - t)^2 / (y^2 + 0.01), with the denominator
+held constant.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+PARAMS = 5 * 64 * 64 + 3 * 64
+LOSS_EPS = 0.01
+
+
+def bf16(x):
+    """Round float32 values to bfloat16 (nearest even), returned as float64."""
+    a = np.asarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    nan = (u & 0x7FFFFFFF) > 0x7F800000
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF
+    r = np.where(nan, (u >> 16) | 0x40, r).astype(np.uint32) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def unpack(params):
+    """20672 parameters -> [W_0 .. W_4 (64 x 64), W_5 (3 x 64)] (float64)."""
+    p = np.asarray(params, dtype=np.float64)
+    ws = [p[k * 4096:(k + 1) * 4096].reshape(64, 64) for k in range(5)]
+    ws.append(p[20480:].reshape(3, 64))
+    return ws
+
+
+def pack(ws):
+    return np.concatenate([w.reshape(-1) for w in ws])
+
+
+def encode(records, lo, hi):
+    """Feature encoding of mlp.h (float64): records (n x 16) -> (n x 64)."""
+    r = np.asarray(records, dtype=np.float64)
+    n = r.shape[0]
+    f = np.zeros((n, 64))
+    ext = np.asarray(hi, dtype=np.float64) - np.asarray(lo, dtype=np.float64)
+    inv = np.where(ext > 0, 1.0 / np.where(ext > 0, ext, 1.0), 0.0)
+    for c in range(3):
+        u = np.clip((r[:, c] - lo[c]) * inv[c], 0.0, 1.0)
+        for i in range(6):
+            f[:, 12 * c + 2 * i] = np.sin(np.pi * 2 ** i * u)
+            f[:, 12 * c + 2 * i + 1] = np.cos(np.pi * 2 ** i * u)
+    f[:, 36:39] = r[:, 3:6]
+    f[:, 39:42] = r[:, 6:9]
+    f[:, 42:45] = r[:, 9:12]
+    f[:, 45] = r[:, 12]
+    f[:, 46] = r[:, 13]
+    f[:, 47] = r[:, 14] / 8.0
+    for c in range(3):
+        d = r[:, 3 + c]
+        f[:, 48 + 4 * c: 52 + 4 * c] = np.stack([np.sin(np.pi * d), np.cos(np.pi * d), np.sin(2 * np.pi * d),
+                                                 np.cos(2 * np.pi * d)], axis=1)
+    f[:, 60] = 1.0
+    return f
+
+
+def softplus(z):
+    return np.where(z > 20.0, z, np.log1p(np.exp(np.minimum(z, 20.0))))
+
+
+def sigmoid(z):
+    return 1.0 / (1.0 + np.exp(-z))
+
+
+def forward(params, features, bf16_points=False):
+    """Returns (activations a_0..a_5 as (n x 64) arrays, z (n x 3))."""
+    rnd = bf16 if bf16_points else (lambda v: np.asarray(v, dtype=np.float64))
+    ws = unpack(params)
+    if bf16_points:
+        ws = [bf16(w.astype(np.float32)) for w in ws]
+    a = [rnd(features)]
+    for k in range(5):
+        z = a[-1] @ ws[k].T
+        a.append(rnd(np.maximum(z, 0.0)))
+    z = a[-1] @ ws[5].T
+    return a, z
+
+
+def infer(params, features, bf16_points=False):
+    return softplus(forward(params, features, bf16_points)[1])
+
+
+def loss(params, features, targets, bf16_points=False):
+    _, z = forward(params, features, bf16_points)
+    y = softplus(z)
+    t = np.asarray(targets, dtype=np.float64)[:, :3]
+    return float(((y - t) ** 2 / (y * y + LOSS_EPS)).mean())
+
+
+def gradient(params, features, targets, bf16_points=False):
+    """Analytic gradient of `loss` (denominator held constant) w.r.t. the
+    20672 parameters, and the loss."""
+    rnd = bf16 if bf16_points else (lambda v: np.asarray(v, dtype=np.float64))
+    ws = unpack(params)
+    if bf16_points:
+        ws = [bf16(w.astype(np.float32)) for w in ws]
+    a, z = forward(params, features, bf16_points)
+    y = softplus(z)
+    t = np.asarray(targets, dtype=np.float64)[:, :3]
+    n = z.shape[0]
+    den = y * y + LOSS_EPS
+    if bf16_points:
+        # The kernels evaluate the output delta in fp32 before rounding it.
+        g = (2.0 * (y - t) / den).astype(np.float32) * np.float32(1.0 / (3.0 * n))
+        d = rnd(g.astype(np.float64) * sigmoid(z))
+    else:
+        d = 2.0 * (y - t) / den / (3.0 * n) * sigmoid(z)
+    grads = [None] * 6
+    grads[5] = d.T @ a[5]
+    for k in range(4, -1, -1):
+        d = rnd((d @ ws[k + 1]) * (a[k + 1] > 0))
+        grads[k] = d.T @ a[k]
+    return pack(grads), float((((y - t) ** 2) / den).mean())
+
+
+def finite_difference(params, features, targets, indices, h=1e-6):
+    """Central differences of the loss with the denominator held at its
+    value for the unperturbed parameters (the analytic convention)."""
+    p0 = np.asarray(params, dtype=np.float64)
+    _, z0 = forward(p0, features)
+    den = softplus(z0) ** 2 + LOSS_EPS
+    t = np.asarray(targets, dtype=np.float64)[:, :3]
+
+    def fixed_loss(p):
+        y = softplus(forward(p, features)[1])
+        return ((y - t) ** 2 / den).mean()
+
+    out = []
+    for i in indices:
+        pp, pm = p0.copy(), p0.copy()
+        pp[i] += h
+        pm[i] -= h
+        out.append((fixed_loss(pp) - fixed_loss(pm)) / (2 * h))
+    return np.array(out)
+
+
+def adam(params, grad, m, v, step, lr=1e-3, beta1=0.9, beta2=0.99, eps=1e-8):
+    """One Adam step (Kingma & Ba 2015) in float32, as the kernel does it."""
+    f = np.float32
+    g = np.asarray(grad, dtype=np.float32)
+    m = f(beta1) * np.asarray(m, dtype=np.float32) + f(1 - beta1) * g
+    v = f(beta2) * np.asarray(v, dtype=np.float32) + f(1 - beta2) * (g * g)
+    c1 = f(1.0 / (1.0 - beta1 ** step))
+    c2 = f(1.0 / (1.0 - beta2 ** step))
+    p = np.asarray(params, dtype=np.float32) - f(lr) * (m * c1) / (np.sqrt(v * c2) + f(eps))
+    return p, m, v

@@ -1,0 +1,119 @@
+"""Radiance-cache MLP on the GPU against the float64 oracle
+(oracle/mlp_ref.py), which rounds to bfloat16 where the kernels do:
+
+  * encoding: within 2e-5 of the float64 formula (the float32 argument
+    2^i pi u of the highest octave, up to 32 pi, carries ~1e-5);
+  * inference: within 1% (a bf16 activation can round the other way when
+    fp32 and float64 accumulation straddle a rounding boundary);
+  * loss gradient: within 1% in norm per layer of the float64 gradient,
+    which finite differences pin on the CPU;
+  * Adam: the fp32 update of the oracle from the device gradient;
+  * training fits a smooth target, and the cached render stays within a few
+    percent of the FP64 Cornell reference.
+"""
+import numpy as np
+import pytest
+
+from fp64_reference import load, rmse
+from oracle import mlp_ref
+from tinytrace_amd import Config, Context, LightSampling, Renderer, RenderSettings, _abi, scenes
+from tinytrace_amd.cache import RadianceCache, render_cached
+
+pytestmark = pytest.mark.gpu
+
+LO, HI = (-2.0, -2.0, -2.0), (2.0, 2.0, 2.0)
+
+
+def _records(n, seed):
+    rng = np.random.default_rng(seed)
+    r = np.zeros((n, 16), dtype=np.float32)
+    r[:, 0:3] = rng.uniform(-2.5, 2.5, (n, 3))
+    d = rng.normal(size=(n, 3))
+    r[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    nn = rng.normal(size=(n, 3))
+    r[:, 6:9] = nn / np.linalg.norm(nn, axis=1, keepdims=True)
+    r[:, 9:12] = rng.uniform(0, 1, (n, 3))
+    r[:, 12] = rng.uniform(0.3, 1, n)
+    r[:, 13] = rng.integers(0, 2, n)
+    r[:, 14] = rng.integers(1, 4, n)
+    return r
+
+
+@pytest.fixture(scope="module")
+def cache():
+    c = RadianceCache(Context(), seed=3)
+    c.set_bounds(LO, HI)
+    return c
+
+
+def test_encoding_matches_formula(cache):
+    r = _records(500, 0)
+    np.testing.assert_allclose(cache.encode(r), mlp_ref.encode(r, LO, HI), atol=2e-5)
+
+
+def test_inference_matches_oracle(cache):
+    r = _records(1000, 1)
+    feats = cache.encode(r)
+    got = cache.infer(r)
+    want = mlp_ref.infer(cache.params, feats, bf16_points=True)
+    np.testing.assert_allclose(got, want, rtol=1e-2, atol=1e-4)
+
+
+def test_gradient_matches_oracle(cache):
+    r = _records(3000, 2)
+    t = np.random.default_rng(3).uniform(0, 3, (3000, 3))
+    g, loss = cache.gradient(r, t)
+    want, want_loss = mlp_ref.gradient(cache.params, cache.encode(r), t, bf16_points=True)
+    assert abs(loss - want_loss) <= 2e-3 * want_loss
+    for k, (a, b) in enumerate([(0, 4096), (4096, 8192), (8192, 12288), (12288, 16384), (16384, 20480),
+                                (20480, 20672)]):
+        err = np.linalg.norm(g[a:b] - want[a:b]) / np.linalg.norm(want[a:b])
+        assert err < 1e-2, (k, err)
+    exact, _ = mlp_ref.gradient(cache.params, mlp_ref.encode(r, LO, HI), t)
+    assert np.linalg.norm(g - exact) / np.linalg.norm(exact) < 0.05
+
+
+def test_adam_step_matches_oracle():
+    c = RadianceCache(Context(), seed=5, learning_rate=1e-3)
+    c.set_bounds(LO, HI)
+    r = _records(700, 4)
+    t = np.random.default_rng(5).uniform(0, 3, (700, 3))
+    p0 = c.params
+    g, _ = c.gradient(r, t)
+    c.train_step(r, t)
+    want, _, _ = mlp_ref.adam(p0, g, np.zeros_like(p0), np.zeros_like(p0), 1)
+    np.testing.assert_allclose(c.params, want, rtol=1e-6, atol=1e-9)
+
+
+def test_training_fits_a_smooth_target():
+    c = RadianceCache(Context(), seed=7, learning_rate=3e-3)
+    c.set_bounds(LO, HI)
+    r = _records(1 << 14, 6)
+    t = np.stack([1.0 + 0.5 * np.sin(2 * r[:, 0]), 0.5 + 0.25 * r[:, 1] ** 2, 0.2 + 0.1 * r[:, 6]], axis=1)
+    first = c.train_step(r, t)
+    for _ in range(300):
+        last = c.train_step(r, t)
+    assert last < 0.1 * first, (first, last)
+    pred = c.infer(r[:512])
+    assert np.abs(pred - t[:512]).mean() < 0.1
+
+
+def test_cached_render_close_to_reference():
+    ref = load("fp64_cornell_512")
+    scene, camera = scenes.cornell_box()
+    renderer = Renderer((512, 512), settings=RenderSettings(rng=_abi.TT_RNG_PCG, seed=2))
+    renderer.set_scene(scene)
+    renderer.camera = camera
+    renderer.set_config(Config(bounce_count=4, sample_count=1, light_sampling=LightSampling.NEXT_EVENT_ESTIMATION))
+    cache = RadianceCache(renderer.context, seed=1)
+    for _ in range(24):  # warm up: train on 24 frames
+        stats = render_cached(renderer, cache)
+    assert stats["samples"] > 30000 and stats["queries"] > 100000
+    assert np.isfinite(stats["loss"])
+    renderer.reset_accumulation()
+    for _ in range(16):
+        render_cached(renderer, cache)
+    image = renderer.target.download().reshape(-1, 4)[ref["pixels"], :3]
+    assert np.all(np.isfinite(image))
+    assert abs(image.mean() / ref["radiance"].mean() - 1) < 0.05
+    assert rmse(image, ref) < 0.1

@@ -1,0 +1,65 @@
+"""The radiance-cache MLP oracle (oracle/mlp_ref.py): its analytic gradient
+against central finite differences in float64, bfloat16 rounding, and the
+encoding's closed-form properties."""
+import numpy as np
+
+from oracle import mlp_ref
+
+
+def _problem(n=24, seed=0):
+    rng = np.random.default_rng(seed)
+    params = rng.uniform(-1, 1, mlp_ref.PARAMS) * np.sqrt(6 / 64)
+    records = rng.uniform(-1, 1, (n, 16))
+    records[:, 12:15] = rng.uniform(0, 1, (n, 3))
+    feats = mlp_ref.encode(records, (-1, -1, -1), (1, 1, 1))
+    targets = rng.uniform(0, 2, (n, 3))
+    return params, feats, targets
+
+
+def test_gradient_matches_finite_differences():
+    params, feats, targets = _problem()
+    grad, _ = mlp_ref.gradient(params, feats, targets)
+    rng = np.random.default_rng(1)
+    # Parameters of every layer, including the output layer.
+    idx = np.concatenate([rng.integers(k * 4096, (k + 1) * 4096, 6) for k in range(5)] + [rng.integers(20480, 20672, 6)])
+    fd = mlp_ref.finite_difference(params, feats, targets, idx)
+    np.testing.assert_allclose(grad[idx], fd, rtol=2e-5, atol=1e-9)
+
+
+def test_bf16_points_change_the_gradient_only_slightly():
+    params, feats, targets = _problem(n=64)
+    g64, l64 = mlp_ref.gradient(params, feats, targets)
+    g16, l16 = mlp_ref.gradient(params, feats, targets, bf16_points=True)
+    assert np.linalg.norm(g16 - g64) / np.linalg.norm(g64) < 0.05
+    assert abs(l16 - l64) / l64 < 0.05
+
+
+def test_bf16_round_to_nearest_even():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -7, -2.5, 3.0e38, np.inf], dtype=np.float32)
+    # 3e38 = 1.7632 * 2^127 -> 1 + 98/128 in bf16's 7 stored mantissa bits.
+    expected = [1.0, 1.0, 1.0 + 2 ** -6, 1.0 + 2 ** -7, -2.5, (1 + 98 / 128) * 2.0 ** 127, np.inf]
+    np.testing.assert_allclose(mlp_ref.bf16(x), expected, rtol=1e-7)
+
+
+def test_encoding_properties():
+    rng = np.random.default_rng(2)
+    rec = rng.uniform(-3, 3, (100, 16))
+    f = mlp_ref.encode(rec, (-2, -2, -2), (2, 2, 2))
+    for c in range(3):
+        for i in range(6):
+            s, co = f[:, 12 * c + 2 * i], f[:, 12 * c + 2 * i + 1]
+            np.testing.assert_allclose(s * s + co * co, 1.0, atol=1e-12)
+    u = np.clip((rec[:, 0] + 2) / 4, 0, 1)
+    np.testing.assert_allclose(f[:, 0], np.sin(np.pi * u))
+    assert np.all(f[:, 60] == 1.0) and np.all(f[:, 61:] == 0.0)
+    np.testing.assert_array_equal(f[:, 36:39], rec[:, 3:6])
+
+
+def test_adam_first_step_is_signed_learning_rate():
+    """With zero state, the first bias-corrected Adam step moves every
+    parameter by lr * g / (|g| + eps) ~ lr * sign(g)."""
+    p = np.zeros(10, dtype=np.float32)
+    g = np.linspace(-1, 1, 10).astype(np.float32)
+    g[4] = 0.25
+    q, _, _ = mlp_ref.adam(p, g, np.zeros(10), np.zeros(10), 1)
+    np.testing.assert_allclose(q, -1e-3 * np.sign(g), rtol=1e-4)

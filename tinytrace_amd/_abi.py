@@ -134,6 +134,21 @@ class ServerStats(C.Structure):
 TT_REQUEST_QUEUED, TT_REQUEST_ADMITTED, TT_REQUEST_DONE = 0, 1, 2
 
 
+class MlpDesc(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("learning_rate", C.c_float), ("reserved", C.c_uint32)]
+
+
+class CacheSettings(C.Structure):
+    _fields_ = [("train_paths", C.c_uint32), ("train_steps", C.c_uint32)]
+
+
+class CacheStats(C.Structure):
+    _fields_ = [("queries", C.c_uint32), ("samples", C.c_uint32), ("loss", C.c_double)]
+
+
+MLP_PARAMS = 20672
+
+
 # numpy dtypes of the reference byte layouts (little-endian, packed as C).
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("direction", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("instance", "<u4"), ("primitive", "<u4"), ("barycentrics", "<f4", 2)])
@@ -184,6 +199,14 @@ _PROTOTYPES = [
     ("tt_server_status", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(RequestStatus)]),
     ("tt_server_take_result", C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_size_t]),
     ("tt_server_stats_get", C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    ("tt_mlp_create", C.c_int, [C.c_void_p, C.POINTER(MlpDesc), C.POINTER(C.c_void_p)]),
+    ("tt_mlp_destroy", None, [C.c_void_p]),
+    ("tt_mlp_set_bounds", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("tt_mlp_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
+    ("tt_mlp_run", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                             C.POINTER(C.c_double)]),
+    ("tt_render_cached", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Constants), C.POINTER(RenderSettings),
+                                   C.c_void_p, C.POINTER(CacheSettings), C.c_void_p, C.POINTER(CacheStats)]),
     ("tt_trace_closest", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     ("tt_trace_closest_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     ("tt_bsdf_evaluate", C.c_int, [C.c_void_p, C.POINTER(MaterialConstants), C.c_void_p, C.c_void_p, C.c_uint32,

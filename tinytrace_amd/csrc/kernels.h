@@ -34,6 +34,21 @@ struct FrameArgs {
     uint32_t frame_index, accumulated_frame_count;
     uint32_t light_sampling;
     uint32_t rng, seed;
+    // Radiance cache: 0 off; else paths end at their first vertex after the
+    // primary one with specular roughness > 0.3 (a cache query), except
+    // training paths (hash of (path, frame) below train_threshold / 2^32).
+    uint32_t cache_mode;
+    uint32_t train_threshold;
+};
+
+// Radiance-cache queues of one render (null when the cache is off).
+struct CacheBuffers {
+    uint32_t* train_slot;   // [paths] training record of the path or ~0
+    float4* query_rec;      // [paths x 4] raw query records (mlp.h)
+    float4* query_scale;    // [paths] (throughput, path index)
+    float4* train_rec;      // [paths x 4] raw training records
+    float4* train_thr;      // [paths] (throughput at the record, path index)
+    float4* train_target;   // [paths] radiance the path carries from there on
 };
 
 // Wavefront buffers; `paths` = width * height * sample_count.
@@ -48,6 +63,7 @@ struct PathBuffers {
     uint32_t* spill;      // traversal stack spill
     const uint4* restir_final;  // per-pixel reservoirs used at the primary vertex
     uint32_t max_bounces;
+    CacheBuffers cache;
 };
 
 // Per-target ReSTIR DI state (Bitterli et al. 2020): a G-buffer record
@@ -73,6 +89,8 @@ constexpr uint32_t C_WORK_EXT = 64;      // [b]: persistent cursor, extend b
 constexpr uint32_t C_WORK_SHADOW = 96;   // [b]: persistent cursor, shadow b
 constexpr uint32_t C_RESTIR_VIS = 128;   // ReSTIR visibility queries
 constexpr uint32_t C_WORK_RESTIR = 129;  // their persistent cursor
+constexpr uint32_t C_CACHE_QUERY = 130;  // radiance-cache queries
+constexpr uint32_t C_CACHE_TRAIN = 131;  // radiance-cache training records
 constexpr uint32_t C_WORDS = 160;
 
 hipError_t launch_zero(hipStream_t s, uint32_t* p, uint32_t words);

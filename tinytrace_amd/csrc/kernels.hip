@@ -82,6 +82,7 @@ TT_DEV void generate_path(const tt::FrameArgs& f, const tt::PathBuffers& b, uint
     b.throughput[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     b.radiance[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     b.rng[p] = g.state;
+    if (b.cache.train_slot) b.cache.train_slot[p] = 0xffffffffu;
 }
 
 __global__ __launch_bounds__(256) void generate_kernel(tt::FrameArgs f, tt::PathBuffers b, uint32_t paths) {
@@ -124,6 +125,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void extend_kernel(tt::DeviceScene d, 
 }
 
 // ----------------------------------------------------------------- shade --
+// Radiance increment a seen from a vertex with throughput t (0 where t is 0).
+TT_DEV f3 cache_ratio(f3 a, f3 t) {
+    return f3(t.x > 0.0f ? a.x / t.x : 0.0f, t.y > 0.0f ? a.y / t.y : 0.0f, t.z > 0.0f ? a.z / t.z : 0.0f);
+}
+
 __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::FrameArgs f, tt::PathBuffers b,
                                                     uint32_t bounce) {
     SceneView s = view(d);
@@ -147,7 +153,18 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
     Generator g{b.rng[p], f.rng};
     uint32_t instance = __float_as_uint(hit.z);
 
+    // Radiance cache: the path's training record (if any), the radiance this
+    // vertex adds (for training targets) and a query / record at this vertex.
+    const bool cache = f.cache_mode != 0u;
+    const f3 thr_in = throughput;
+    const uint32_t tslot = cache ? b.cache.train_slot[p] : 0xffffffffu;
+    bool emit_query = false, emit_train = false;
+    f3 added;
+    float rec[16];
+
     bool emit_shadow = false, emit_ray = false;
+    Mat m;
+    Surface sf;
     f3 shadow_org, shadow_dir, direct, emission_after;
     uint32_t light_instance = 0, light_hash = 0;
     bool has_emission_after = false;
@@ -155,10 +172,29 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
 
     if (instance == 0xffffffffu) {
         // Miss: constant white sky (integrate.glsl:272-276).
-        accumulated = accumulated + f3(1.0f) * throughput;
+        added = f3(1.0f) * throughput;
+        accumulated = accumulated + added;
     } else {
-        Mat m;
-        Surface sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, m);
+        sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, m);
+        if (cache && bounce >= 1 && m.specular_roughness > 0.3f && tslot == 0xffffffffu) {
+            // The cache vertex: a query ends the path here; a training path
+            // records the vertex and runs on.
+            f3 v[5] = {sf.position, -ray_dir, sf.basis.normal, m.base_color, f3()};
+            for (int k = 0; k < 4; k++) {
+                rec[3 * k + 0] = v[k].x;
+                rec[3 * k + 1] = v[k].y;
+                rec[3 * k + 2] = v[k].z;
+            }
+            rec[12] = m.specular_roughness;
+            rec[13] = m.base_metalness;
+            rec[14] = float(bounce);
+            rec[15] = 0.0f;
+            uint32_t h = pcg32_output(splitmix64(pcg32_key(p, f.frame_index) ^ 0x6a09e667f3bcc909ull));
+            if (h < f.train_threshold) emit_train = true;
+            else emit_query = true;
+        }
+    }
+    if (instance != 0xffffffffu && !emit_query) {
         f3 wi = sf.wi;
         Lobes lobes;
         if (restir) {
@@ -223,7 +259,8 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
                 has_emission_after = true;
             }
         } else if (bounce == 0 || !nee) {
-            accumulated = accumulated + emissive * throughput;
+            added = emissive * throughput;
+            accumulated = accumulated + added;
         }
         if (!last) {
             f3 wo, value;
@@ -256,6 +293,29 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
                                          __uint_as_float(light_hash | (has_emission_after ? 0x10000u : 0u)));
         b.shadow[3][sslot] = make_float4(emission_after.x, emission_after.y, emission_after.z, 0.0f);
     }
+    if (cache) {
+        const tt::CacheBuffers& cb = b.cache;
+        uint32_t qs = wave_append(b.counters + tt::C_CACHE_QUERY, emit_query);
+        if (emit_query) {
+            for (int k = 0; k < 4; k++)
+                cb.query_rec[4 * size_t(qs) + k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
+            cb.query_scale[qs] = make_float4(thr_in.x, thr_in.y, thr_in.z, __uint_as_float(p));
+        }
+        uint32_t ts = wave_append(b.counters + tt::C_CACHE_TRAIN, emit_train);
+        if (emit_train) {
+            for (int k = 0; k < 4; k++)
+                cb.train_rec[4 * size_t(ts) + k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
+            cb.train_thr[ts] = make_float4(thr_in.x, thr_in.y, thr_in.z, __uint_as_float(p));
+            f3 t = cache_ratio(added, thr_in);
+            cb.train_target[ts] = make_float4(t.x, t.y, t.z, 0.0f);
+            cb.train_slot[p] = ts;
+        } else if (tslot != 0xffffffffu) {
+            float4 T = cb.train_thr[tslot];
+            float4 t = cb.train_target[tslot];
+            f3 d = cache_ratio(added, f3(T.x, T.y, T.z));
+            cb.train_target[tslot] = make_float4(t.x + d.x, t.y + d.y, t.z + d.z, 0.0f);
+        }
+    }
 }
 
 // ---------------------------------------------------------------- shadow --
@@ -281,6 +341,17 @@ __global__ __launch_bounds__(TRACE_BLOCK) void shadow_kernel(tt::DeviceScene d, 
             bool visible = light_visible(s, f3(o.x, o.y, o.z), f3(dd.x, dd.y, dd.z), __float_as_uint(dd.w),
                                          meta & 0xffffu, st);
             bool emission = (meta & 0x10000u) != 0;
+            if (visible && b.cache.train_slot) {
+                // Direct light reaching a training path after its record.
+                uint32_t p = __float_as_uint(o.w);
+                uint32_t ts = b.cache.train_slot[p];
+                if (ts != 0xffffffffu) {
+                    float4 T = b.cache.train_thr[ts];
+                    float4 t = b.cache.train_target[ts];
+                    f3 dlt = cache_ratio(f3(direct.x, direct.y, direct.z), f3(T.x, T.y, T.z));
+                    b.cache.train_target[ts] = make_float4(t.x + dlt.x, t.y + dlt.y, t.z + dlt.z, 0.0f);
+                }
+            }
             if (visible || emission) {
                 uint32_t p = __float_as_uint(o.w);
                 float4 a = b.radiance[p];

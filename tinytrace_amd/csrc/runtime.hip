@@ -80,6 +80,30 @@ tt_status ensure_paths(tt_context* c, size_t paths, PathBuffers& b) {
     b.rng = reinterpret_cast<uint64_t*>(f + 11 * n);
     b.counters = c->counters;
     b.spill = c->spill;
+    b.cache = CacheBuffers{};
+    return TT_OK;
+}
+
+// Radiance-cache queues for `paths` paths (grown on demand).
+tt_status ensure_cache(tt_context* c, size_t paths, CacheBuffers& cb) {
+    if (paths > c->cache_capacity) {
+        if (c->cache_memory) {
+            TT_HIP(c, hipStreamSynchronize(c->stream));
+            TT_HIP(c, hipFree(c->cache_memory));
+            c->cache_memory = nullptr;
+            c->cache_capacity = 0;
+        }
+        TT_HIP(c, hipMalloc(&c->cache_memory, paths * (11 * sizeof(float4) + sizeof(uint32_t))));
+        c->cache_capacity = paths;
+    }
+    size_t n = c->cache_capacity;
+    float4* f = static_cast<float4*>(c->cache_memory);
+    cb.query_rec = f;
+    cb.query_scale = f + 4 * n;
+    cb.train_rec = f + 5 * n;
+    cb.train_thr = f + 9 * n;
+    cb.train_target = f + 10 * n;
+    cb.train_slot = reinterpret_cast<uint32_t*>(f + 11 * n);
     return TT_OK;
 }
 
@@ -99,6 +123,8 @@ FrameArgs frame_args(const tt_constants& k, uint32_t width, uint32_t height, uin
     f.light_sampling = k.light_sampling;
     f.rng = rng;
     f.seed = seed;
+    f.cache_mode = 0;
+    f.train_threshold = 0;
     return f;
 }
 
@@ -145,6 +171,7 @@ void context_free(tt_context* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->path_memory) (void)hipFree(c->path_memory);
+    if (c->cache_memory) (void)hipFree(c->cache_memory);
     if (c->counters) (void)hipFree(c->counters);
     if (c->work) (void)hipFree(c->work);
     if (c->spill) (void)hipFree(c->spill);
@@ -237,6 +264,15 @@ tt_status tt_scene_create(tt_context* c, const tt_scene_desc* d, tt_scene** out)
         return st;
     }
     s->view.tlas_root = bvh.empty ? 0xffffffffu : bvh.tlas_root;
+    for (int a = 0; a < 3; a++) {
+        float lo = INFINITY, hi = -INFINITY;
+        for (size_t i = 0; i < bvh.instance_boxes.size(); i += 8) {
+            lo = std::min(lo, bvh.instance_boxes[i + a]);
+            hi = std::max(hi, bvh.instance_boxes[i + 4 + a]);
+        }
+        s->bounds_lo[a] = lo <= hi ? lo : 0.0f;
+        s->bounds_hi[a] = lo <= hi ? hi : 0.0f;
+    }
     s->view.emissive_count = d->emissive_triangle_count;
     uint64_t tris = 0, unique = 0;
     for (uint32_t i = 0; i < d->instance_count; i++) tris += d->meshes[d->instances[i].mesh_index].index_count / 3;
@@ -303,8 +339,13 @@ void tt_target_destroy(tt_target* t) {
     if (c) tt::context_release(c);
 }
 
-tt_status tt_render_ex(tt_context* c, const tt_scene* s, const tt_constants* k, const tt_render_settings* settings,
-                       tt_target* t) {
+}  // extern "C"
+
+namespace {
+
+// tt_render_ex, optionally with the radiance cache (tt_render_cached).
+tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const tt_render_settings* settings,
+                 tt_target* t, tt_mlp* cache, const tt_cache_settings* cache_settings, tt_cache_stats* cache_stats) {
     if (!c || !s || !k || !t) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     if (s->context != c || t->context != c) return fail(c, TT_STALE_HANDLE, "handle from another context");
     if (k->screen_size[0] != t->width || k->screen_size[1] != t->height)
@@ -362,6 +403,21 @@ tt_status tt_render_ex(tt_context* c, const tt_scene* s, const tt_constants* k, 
     }
 
     tt::FrameArgs f = tt::frame_args(*k, t->width, t->height, st.rng, st.seed);
+    uint32_t train_steps = 0;
+    if (cache) {
+        if (tt::mlp_context(cache) != c) return fail(c, TT_STALE_HANDLE, "cache from another context");
+        if (k->light_sampling == TT_LIGHT_SAMPLING_RESTIR)
+            return fail(c, TT_UNSUPPORTED, "the radiance cache renders with NEE or ON_HIT");
+        tt_status cs = tt::ensure_cache(c, paths, b.cache);
+        if (cs != TT_OK) return cs;
+        uint32_t train_paths = cache_settings && cache_settings->train_paths ? cache_settings->train_paths : 65536u;
+        train_steps = cache_settings && cache_settings->train_steps ? cache_settings->train_steps : 4u;
+        double frac = std::min(1.0, double(train_paths) / double(paths));
+        f.cache_mode = 1;
+        f.train_threshold = frac >= 1.0 ? 0xffffffffu : uint32_t(frac * 4294967296.0);
+        cs = tt::mlp_set_scene_bounds(cache, s->bounds_lo, s->bounds_hi);
+        if (cs != TT_OK) return cs;
+    }
 
     hipStream_t stream = c->stream;
     TT_HIP(c, hipEventRecord(c->integrate.begin, stream));
@@ -393,8 +449,40 @@ tt_status tt_render_ex(tt_context* c, const tt_scene* s, const tt_constants* k, 
             }
         }
     }
+    if (cache) {
+        // Cached radiance for the paths that ended at a query.
+        tt_status cs = tt::mlp_infer_device(cache, b.cache.query_rec, b.cache.query_scale,
+                                            c->counters + tt::C_CACHE_QUERY, 0, nullptr, b.radiance);
+        if (cs != TT_OK) return cs;
+    }
     TT_HIP(c, tt::launch_accumulate(stream, f, b, t->pixels));
     TT_HIP(c, hipEventRecord(c->integrate.end, stream));
+    if (cache) {
+        // Train on this frame's samples: train_steps Adam steps over
+        // consecutive slices of the records.
+        uint32_t counts[2];
+        TT_HIP(c, hipMemcpyAsync(counts, c->counters + tt::C_CACHE_QUERY, sizeof(counts), hipMemcpyDeviceToHost,
+                                 stream));
+        TT_HIP(c, hipStreamSynchronize(stream));
+        uint32_t n = counts[1];
+        double loss = 0.0;
+        uint32_t steps = std::min(train_steps, n);
+        for (uint32_t i = 0; i < steps; i++) {
+            uint32_t a = uint32_t(uint64_t(n) * i / steps), e = uint32_t(uint64_t(n) * (i + 1) / steps);
+            tt_status cs = tt::mlp_train_device(cache, b.cache.train_rec + 4 * size_t(a), b.cache.train_target + a,
+                                                e - a, true, nullptr);
+            if (cs != TT_OK) return cs;
+            if (i == 0 && cache_stats) {
+                cs = tt::mlp_last_loss(cache, e - a, &loss);
+                if (cs != TT_OK) return cs;
+            }
+        }
+        if (cache_stats) {
+            cache_stats->queries = counts[0];
+            cache_stats->samples = n;
+            cache_stats->loss = loss;
+        }
+    }
     c->trace_events_used = stats ? ev : 0;
     c->stats_pending = stats;
     c->last_paths = paths;
@@ -408,6 +496,22 @@ tt_status tt_render_ex(tt_context* c, const tt_scene* s, const tt_constants* k, 
         t->restir_history = false;
     }
     return TT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+tt_status tt_render_ex(tt_context* c, const tt_scene* s, const tt_constants* k, const tt_render_settings* settings,
+                       tt_target* t) {
+    return render(c, s, k, settings, t, nullptr, nullptr, nullptr);
+}
+
+tt_status tt_render_cached(tt_context* c, const tt_scene* s, const tt_constants* k, const tt_render_settings* settings,
+                           tt_mlp* cache, const tt_cache_settings* cache_settings, tt_target* t,
+                           tt_cache_stats* stats) {
+    if (!cache) return fail(c, TT_INVALID_ARGUMENT, "null cache");
+    return render(c, s, k, settings, t, cache, cache_settings, stats);
 }
 
 tt_status tt_render(tt_context* c, const tt_scene* s, const tt_constants* k, tt_target* t) {

@@ -45,6 +45,9 @@ struct tt_context {
     // handles may be destroyed in any order.
     uint32_t children = 0;
     bool closing = false;
+    // Radiance-cache queues, grown on demand.
+    size_t cache_capacity = 0;
+    void* cache_memory = nullptr;
 };
 
 struct tt_scene {
@@ -53,6 +56,7 @@ struct tt_scene {
     std::vector<void*> allocations;
     tt_scene_info info{};
     uint64_t serial = 0;  // unique per scene created in this process
+    float bounds_lo[3] = {0.0f, 0.0f, 0.0f}, bounds_hi[3] = {0.0f, 0.0f, 0.0f};  // world box
 };
 
 struct tt_target {
@@ -81,6 +85,17 @@ inline tt_status hip_fail(tt_context* c, hipError_t e, const char* what) {
 
 void context_retain(tt_context* c);
 void context_release(tt_context* c);
+
+tt_status ensure_cache(tt_context* c, size_t paths, CacheBuffers& cb);
+
+// Radiance-cache MLP (mlp.hip).
+tt_context* mlp_context(const tt_mlp* m);
+tt_status mlp_set_scene_bounds(tt_mlp* m, const float lo[3], const float hi[3]);
+tt_status mlp_infer_device(tt_mlp* m, const float4* records, const float4* scale_path, const uint32_t* count_dev,
+                           uint32_t count_host, float* out_rgb, float4* radiance);
+tt_status mlp_train_device(tt_mlp* m, const float4* records, const float4* targets, uint32_t n, bool apply,
+                           float* grad_out);
+tt_status mlp_last_loss(tt_mlp* m, uint32_t n, double* out);
 
 // Path buffers of at least `paths` paths (grown on demand, kept in the
 // context); fills the pointers of `b`.
