@@ -41,6 +41,8 @@ def main() -> int:
     p.add_argument("--serving-requests", type=int, default=40,
                    help="requests in the serving-latency leg (0 skips it)")
     p.add_argument("--serving-rate", type=float, default=5.0, help="Poisson arrival rate of that leg, requests/s")
+    p.add_argument("--cache-frames", type=int, default=16,
+                   help="frames of the radiance-cache leg on the Cornell box (0 skips it)")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="also run a plain device copy of this many MiB (reference kernel for HBM counters)")
     args = p.parse_args()
@@ -163,6 +165,9 @@ def main() -> int:
         result["p50_ms"] = result["serving"]["p50_ms"]
         result["p99_ms"] = result["serving"]["p99_ms"]
 
+    if args.cache_frames > 0:
+        result["radiance_cache"] = radiance_cache(args.cache_frames)
+
     if args.copy_calibration_mib:
         # A device-to-device hipMemcpy of known size runs as the runtime's
         # copy kernel: the reference point for the HBM byte counters.
@@ -211,6 +216,78 @@ def main() -> int:
         result["gpu_over_cpu"] = round(result["value"] / result["cpu_baseline"]["value"], 1)
     print(json.dumps(result))
     return 0
+
+
+def radiance_cache(frames):
+    """Cornell box 512x512, depth 4, next-event estimation: the plain
+    integrator against the one with the MLP radiance cache (65536 training
+    paths and 4 Adam steps per frame, after 32 warm-up frames), each over
+    `frames` accumulated frames; RMSE against the FP64 Cornell fixture at
+    equal frames and at the plain renderer's wall time."""
+    import numpy as np
+
+    from tinytrace_amd import Config, LightSampling, Renderer, RenderSettings, _abi, scenes
+    from tinytrace_amd.cache import RadianceCache, render_cached
+
+    with np.load(os.path.join(ROOT, "tests", "golden", "fp64_cornell_512.npz"), allow_pickle=False) as f:
+        pixels, reference = f["pixels"], f["radiance"]
+
+    def rmse(renderer):
+        img = renderer.target.download().reshape(-1, 4)[pixels, :3].astype(np.float64)
+        return float(np.sqrt(((img - reference) ** 2).mean()))
+
+    scene, camera = scenes.cornell_box()
+    renderer = Renderer((512, 512), settings=RenderSettings(rng=_abi.TT_RNG_PCG, seed=0))
+    renderer.set_scene(scene)
+    renderer.camera = camera
+    renderer.set_config(Config(bounce_count=4, sample_count=1, light_sampling=LightSampling.NEXT_EVENT_ESTIMATION))
+
+    def sync():
+        renderer.target.download()
+
+    def ms_per_frame(step, n):
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            step()
+        sync()
+        return (time.perf_counter() - t0) / n * 1e3
+
+    def curve(step, counts):
+        """RMSE after each count of accumulated frames (untimed)."""
+        renderer.reset_accumulation()
+        out, done = {}, 0
+        for n in sorted(counts):
+            while done < n:
+                step()
+                done += 1
+            out[str(n)] = round(rmse(renderer), 5)
+        return out
+
+    plain_ms = ms_per_frame(renderer.render_to_target, 4 * frames)
+    cache = RadianceCache(renderer.context, seed=0)
+    for _ in range(32):
+        render_cached(renderer, cache)
+    stats = {}
+
+    def cached_step():
+        stats.update(render_cached(renderer, cache))
+
+    cached_ms = ms_per_frame(cached_step, frames)
+    counts = [n for n in (1, 2, 4, 8, 16, 32, 64) if n <= frames]
+    cached_rmse = curve(cached_step, counts)
+    equal_frames = max(1, int(frames * cached_ms / plain_ms))
+    plain_rmse = curve(renderer.render_to_target, counts + [equal_frames])
+    cache.close()
+    return {"workload": "Cornell box 512x512, depth 4, NEE, PCG seed 0; cache: 64-5x64-3 bf16 MLP, 65536 training "
+                        "paths + 4 Adam steps per frame, 32 warm-up frames",
+            "frames": frames,
+            "plain": {"ms_per_frame": round(plain_ms, 3), "rmse": plain_rmse},
+            "cached": {"ms_per_frame": round(cached_ms, 3), "rmse": cached_rmse,
+                       "queries_per_frame": stats.get("queries"), "training_samples_per_frame": stats.get("samples"),
+                       "training_loss": round(stats.get("loss", 0.0), 5)},
+            "equal_time": {"ms": round(frames * cached_ms, 3), "rmse_cached": cached_rmse[str(frames)],
+                           "plain_frames": equal_frames, "rmse_plain": plain_rmse[str(equal_frames)]}}
 
 
 def serving(context, requests, rate):

@@ -387,8 +387,9 @@ typedef struct tt_cache_stats {
     double loss;           /* mean training loss of the frame's first step */
 } tt_cache_stats;
 
-/* tt_render_ex with the radiance cache: renders (NEE or ON_HIT, one sample
- * per pixel), queries the cache for the paths it ends, then trains it on the
+/* tt_render_ex with the radiance cache: renders (any light sampling; with
+ * ReSTIR the primary vertex uses the reservoirs and the cache ends paths
+ * later), queries the cache for the paths it ends, then trains it on the
  * frame's training samples. */
 tt_status tt_render_cached(tt_context* context, const tt_scene* scene, const tt_constants* constants,
                            const tt_render_settings* settings, tt_mlp* cache, const tt_cache_settings* cache_settings,

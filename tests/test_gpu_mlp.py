@@ -117,3 +117,23 @@ def test_cached_render_close_to_reference():
     assert np.all(np.isfinite(image))
     assert abs(image.mean() / ref["radiance"].mean() - 1) < 0.05
     assert rmse(image, ref) < 0.1
+
+
+def test_cached_restir_render_on_sphere_field():
+    """ReSTIR at the primary vertex with the cache ending paths afterwards:
+    the image stays finite and near the FP64 reference."""
+    ref = load("fp64_sphere_field_1080p")
+    scene, camera = scenes.sphere_field(seed=0)
+    renderer = Renderer((1920, 1080), settings=RenderSettings(rng=_abi.TT_RNG_PCG, seed=3, restir_candidates=32))
+    renderer.set_scene(scene)
+    renderer.camera = camera
+    renderer.set_config(Config(bounce_count=8, sample_count=1, light_sampling=LightSampling.RESTIR))
+    cache = RadianceCache(renderer.context, seed=2)
+    for _ in range(16):
+        render_cached(renderer, cache)
+    renderer.reset_accumulation()
+    for _ in range(8):
+        stats = render_cached(renderer, cache)
+    image = renderer.target.download().reshape(-1, 4)[ref["pixels"], :3]
+    assert np.all(np.isfinite(image)) and stats["queries"] > 0
+    assert abs(image.mean() / ref["radiance"].mean() - 1) < 0.15

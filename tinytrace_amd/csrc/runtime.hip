@@ -406,8 +406,6 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     uint32_t train_steps = 0;
     if (cache) {
         if (tt::mlp_context(cache) != c) return fail(c, TT_STALE_HANDLE, "cache from another context");
-        if (k->light_sampling == TT_LIGHT_SAMPLING_RESTIR)
-            return fail(c, TT_UNSUPPORTED, "the radiance cache renders with NEE or ON_HIT");
         tt_status cs = tt::ensure_cache(c, paths, b.cache);
         if (cs != TT_OK) return cs;
         uint32_t train_paths = cache_settings && cache_settings->train_paths ? cache_settings->train_paths : 65536u;

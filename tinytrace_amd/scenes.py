@@ -208,3 +208,110 @@ def sphere_field(seed: int = 0, spheres: int = 800, lights: int = 256, subdivisi
         scene.add_model(scene.add_node(transform), mesh, material)
     camera = Camera.looking_at((-25.0, 60.0, -25.0), (55.0, 5.0, 55.0), fov=math.pi / 3)
     return scene, camera
+
+
+def torus(major: float, minor: float, nu: int, nv: int):
+    """Torus around y: (positions, normals, tangents, uv, indices), CCW seen
+    from outside, 2 nu nv triangles."""
+    u = np.arange(nu) * (2 * math.pi / nu)
+    v = np.arange(nv) * (2 * math.pi / nv)
+    uu, vv = np.meshgrid(u, v, indexing="ij")
+    cu, su, cv, sv = np.cos(uu), np.sin(uu), np.cos(vv), np.sin(vv)
+    ring = major + minor * cv
+    pos = np.stack([ring * cu, minor * sv, ring * su], axis=-1).reshape(-1, 3)
+    nrm = np.stack([cv * cu, sv, cv * su], axis=-1).reshape(-1, 3)
+    tan = np.stack([-su, np.zeros_like(su), cu], axis=-1).reshape(-1, 3)
+    tan = np.concatenate([tan, np.ones((tan.shape[0], 1))], axis=1)
+    uv = np.stack([uu / (2 * math.pi), vv / (2 * math.pi)], axis=-1).reshape(-1, 2)
+    i = np.arange(nu)[:, None]
+    j = np.arange(nv)[None, :]
+    a = i * nv + j
+    b = ((i + 1) % nu) * nv + j
+    c = ((i + 1) % nu) * nv + (j + 1) % nv
+    d = i * nv + (j + 1) % nv
+    idx = np.stack([a, d, c, a, c, b], axis=-1).reshape(-1).astype(np.uint32)
+    return pos, nrm, tan, uv, idx
+
+
+_ICOSPHERES = {}
+
+
+def displaced_icosphere(subdivisions: int, rng: np.random.Generator, amplitude: float = 0.15):
+    """Icosphere with smooth radial displacement 1 + amplitude * sum of three
+    random sinusoids; normals from the displaced faces."""
+    if subdivisions not in _ICOSPHERES:
+        _ICOSPHERES[subdivisions] = icosphere(subdivisions)
+    unit, faces = _ICOSPHERES[subdivisions]
+    k = rng.normal(size=(3, 3)) * 3.0
+    phase = rng.uniform(0, 2 * math.pi, 3)
+    r = 1.0 + amplitude / 3.0 * np.sin(unit @ k.T + phase).sum(axis=1)
+    pos = unit * r[:, None]
+    fn = np.cross(pos[faces[:, 1]] - pos[faces[:, 0]], pos[faces[:, 2]] - pos[faces[:, 0]])
+    nrm = np.zeros_like(pos)
+    for c in range(3):
+        np.add.at(nrm, faces[:, c], fn)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    _, tangents, uv = _sphere_attributes(pos)
+    t = tangents[:, :3] - nrm * (tangents[:, :3] * nrm).sum(axis=1, keepdims=True)
+    bad = np.linalg.norm(t, axis=1) < 1e-6
+    t[bad] = np.cross(nrm[bad], [0.0, 0.0, 1.0])
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    tan = np.concatenate([t, np.ones((t.shape[0], 1))], axis=1)
+    return pos, nrm, tan, uv, faces.reshape(-1)
+
+
+def instanced_field(seed: int = 0, meshes: int = 64, instances: int = 12000, lights: int = 16,
+                    min_triangles: int = 5000, max_triangles: int = 200000, extent: float = 200.0
+                    ) -> Tuple[AssetScene, Camera]:
+    """The instanced benchmark scene: `meshes` distinct meshes (even: displaced
+    icospheres with 20 * 4^s triangles, s in 4..6; odd: tori with ~T
+    triangles) whose target sizes T are log-uniform in [min, max],
+    `instances` instances of them (mesh uniform, uniform random rotation,
+    scale log-uniform in [0.5, 3], position U[0, extent]^3), a ground quad,
+    and `lights` instances of one 256-triangle emissive torus (4096 emissive
+    triangles for 16 lights, luminance log-uniform in [2, 50]).  Seed 0:
+    3 215 512 distinct triangles, 596 951 286 instanced (12 017 instances);
+    meshes average ~50k triangles, so 12 000 uniformly chosen instances carry
+    far more than 64M triangles."""
+    rng = np.random.default_rng(seed)
+    scene = AssetScene()
+    # Lights first (global triangle indices 0 .. 255 for the emissive mesh).
+    lp, ln, lt, luv, li = torus(1.0, 0.3, 16, 8)
+    light_mesh = scene.insert_mesh(lp, luv, li, ln, lt)
+    for _ in range(lights):
+        lum = math.exp(rng.uniform(math.log(2.0), math.log(50.0)))
+        mat = scene.insert_material(Material(base_color=(0.8, 0.8, 0.8), emission_luminance=lum,
+                                             specular_roughness=0.5))
+        transform = np.eye(4)
+        transform[:3, :3] = _random_rotation(rng) * math.exp(rng.uniform(math.log(1.0), math.log(4.0)))
+        transform[:3, 3] = rng.uniform(0.0, extent, 3)
+        scene.add_model(scene.add_node(transform), light_mesh, mat)
+    p, n, t = _quad((extent / 2, 0.0, extent / 2), (0, 1, 0), (1, 0, 0), extent, extent)
+    ground = _insert_flat(scene, p, np.tile(n, (4, 1)), np.tile(np.append(t, 1.0), (4, 1)), _quad_indices(1))
+    scene.add_model(scene.add_node(), ground, scene.insert_material(Material(base_color=(0.5, 0.5, 0.5),
+                                                                              specular_roughness=0.5)))
+    mesh_ids, mesh_mats, mesh_tris = [], [], []
+    for k in range(meshes):
+        target = math.exp(rng.uniform(math.log(min_triangles), math.log(max_triangles)))
+        if k % 2 == 0:
+            s = int(min(6, max(4, round(math.log(target / 20.0, 4)))))
+            pos, nrm, tan, uv, idx = displaced_icosphere(s, rng)
+        else:
+            nu = max(8, int(round(math.sqrt(target))))
+            nv = max(4, int(round(target / (2 * nu))))
+            pos, nrm, tan, uv, idx = torus(1.0, 0.35, nu, nv)
+        mesh_ids.append(scene.insert_mesh(pos, uv, idx, nrm, tan))
+        mesh_tris.append(idx.size // 3)
+        mesh_mats.append(scene.insert_material(Material(
+            base_color=tuple(rng.uniform(0.05, 0.95, 3)), specular_roughness=float(rng.uniform(0.0, 1.0)),
+            base_metalness=1.0 if rng.random() < 0.3 else 0.0, specular_ior=1.5,
+            coat_weight=1.0 if rng.random() < 0.2 else 0.0)))
+    for _ in range(instances):
+        m = int(rng.integers(meshes))
+        transform = np.eye(4)
+        transform[:3, :3] = _random_rotation(rng) * math.exp(rng.uniform(math.log(0.5), math.log(3.0)))
+        transform[:3, 3] = rng.uniform(0.0, extent, 3)
+        scene.add_model(scene.add_node(transform), mesh_ids[m], mesh_mats[m])
+    camera = Camera.looking_at((-0.25 * extent, 0.6 * extent, -0.25 * extent), (0.55 * extent, 0.05 * extent,
+                                                                                  0.55 * extent), fov=math.pi / 3)
+    return scene, camera
