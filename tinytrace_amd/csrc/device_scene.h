@@ -51,6 +51,7 @@ struct SceneView {
     const uint2* vertices;    // tt_vertex: (f16x2 uv, tangent frame)
     const uint32_t* indices;
     const uint32_t* emissive; // tt_emissive_triangle, 9 x u32 (36 B)
+    const float4* lights;     // light records, 5 x float4 per emissive triangle
 };
 
 struct HitRecord {

@@ -50,6 +50,7 @@ __device__ __host__ inline SceneView view(const tt::DeviceScene& d) {
     s.vertices = static_cast<const uint2*>(d.vertices);
     s.indices = static_cast<const uint32_t*>(d.indices);
     s.emissive = static_cast<const uint32_t*>(d.emissive);
+    s.lights = static_cast<const float4*>(d.lights);
     return s;
 }
 
@@ -172,27 +173,57 @@ struct LightPoint {
     uint32_t instance, hash;
 };
 
-// The world-space triangle of emissive record `li` at barycentrics `lb`
-// (sample_random_light, scene.glsl:114-136; material_emissive).
-TT_DEV LightPoint light_point(const SceneView& s, uint32_t li, f3 lb) {
+// The world-space triangle of emissive record `li` (sample_random_light,
+// scene.glsl:114-136; material_emissive): vertices, normal, area, emission.
+struct LightTriangle {
+    f3 p[3], normal, emission;
+    float area;
+    uint32_t instance, hash;
+};
+
+TT_DEV LightTriangle light_triangle(const SceneView& s, uint32_t li) {
     const uint32_t* et = s.emissive + 9 * size_t(li);
     uint32_t w[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) w[k] = et[k];
     auto half = [&](int k) { return int16_t((w[k >> 1] >> (16 * (k & 1))) & 0xffffu); };
-    LightPoint lp;
-    lp.hash = w[4] >> 16;
-    lp.instance = w[8];
-    const float* linst = s.instances + 52 * size_t(lp.instance);
+    LightTriangle t;
+    t.hash = w[4] >> 16;
+    t.instance = w[8];
+    const float* linst = s.instances + 52 * size_t(t.instance);
     M4 ltr = load_m4(linst);
-    f3 p[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) p[c] = mul_point1(ltr.m, dequantize_snorm16(half(3 * c), half(3 * c + 1), half(3 * c + 2)));
-    lp.area = fmax_(0.00001f, 0.5f * length(cross(p[1] - p[0], p[2] - p[0])));
-    lp.normal = normalize(cross(p[1] - p[0], p[2] - p[0]));
-    lp.position = interpolate(lb, p[0], p[1], p[2]);
+    for (int c = 0; c < 3; c++)
+        t.p[c] = mul_point1(ltr.m, dequantize_snorm16(half(3 * c), half(3 * c + 1), half(3 * c + 2)));
+    t.area = fmax_(0.00001f, 0.5f * length(cross(t.p[1] - t.p[0], t.p[2] - t.p[0])));
+    t.normal = normalize(cross(t.p[1] - t.p[0], t.p[2] - t.p[0]));
     Mat lm = load_material(s.materials + 19 * size_t(__float_as_uint(linst[50])));
-    lp.emission = lm.emission_color * lm.emission_luminance;
+    t.emission = lm.emission_color * lm.emission_luminance;
+    return t;
+}
+
+// Light record (80 B, built once per scene from light_triangle):
+//   (p0, area) (p1, normal.x) (p2, normal.y) (emission, normal.z)
+//   (instance, hash, 0, 0)
+TT_DEV void store_light_record(float4* r, const LightTriangle& t) {
+    r[0] = make_float4(t.p[0].x, t.p[0].y, t.p[0].z, t.area);
+    r[1] = make_float4(t.p[1].x, t.p[1].y, t.p[1].z, t.normal.x);
+    r[2] = make_float4(t.p[2].x, t.p[2].y, t.p[2].z, t.normal.y);
+    r[3] = make_float4(t.emission.x, t.emission.y, t.emission.z, t.normal.z);
+    r[4] = make_float4(__uint_as_float(t.instance), __uint_as_float(t.hash), 0.0f, 0.0f);
+}
+
+// The point at barycentrics `lb` of emissive triangle `li`.
+TT_DEV LightPoint light_point(const SceneView& s, uint32_t li, f3 lb) {
+    const float4* r = s.lights + 5 * size_t(li);
+    float4 a = r[0], b = r[1], c = r[2], d = r[3], e = r[4];
+    LightPoint lp;
+    lp.area = a.w;
+    lp.normal = f3(b.w, c.w, d.w);
+    lp.position = interpolate(lb, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(c.x, c.y, c.z));
+    lp.emission = f3(d.x, d.y, d.z);
+    lp.instance = __float_as_uint(e.x);
+    lp.hash = __float_as_uint(e.y);
     return lp;
 }
 

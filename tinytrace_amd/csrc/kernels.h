@@ -22,6 +22,7 @@ struct DeviceScene {
     const void* vertices;
     const void* indices;
     const void* emissive;
+    const void* lights;  // per emissive triangle, 5 x float4 (device_surface.h)
 };
 
 // Per-render camera/integrator constants passed by value.
@@ -62,6 +63,7 @@ struct PathBuffers {
     uint32_t* counters;   // see kernels.hip
     uint32_t* spill;      // traversal stack spill
     const uint4* restir_final;  // per-pixel reservoirs used at the primary vertex
+    const uint4* restir_initial;  // the pixel's own candidate after the visibility pass
     uint32_t max_bounces;
     CacheBuffers cache;
 };
@@ -110,6 +112,8 @@ hipError_t launch_accumulate_chunk(hipStream_t s, const FrameArgs& f, const Path
                                    uint32_t pixel0, uint32_t count, float4* target);
 hipError_t launch_bsdf_evaluate(hipStream_t s, const tt_material_constants* material, const tt_bsdf_query* q,
                                 tt_bsdf_result* r, uint32_t count, uint32_t rng_state);
+// World-space light records of every emissive triangle (scene creation).
+hipError_t launch_light_records(hipStream_t s, const DeviceScene& scene, void* out);
 hipError_t launch_math(hipStream_t s, int op, const float* x, const float* y, float* out, uint32_t count);
 
 // ReSTIR DI passes at the primary vertex (restir.hip).

@@ -207,7 +207,7 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
         bool gate = s.emissive_count > 0 && m.specular_roughness > 0.15f;
         bool nee = (nee_mode || restir) && gate;
         f3 emissive = m.emission_color * m.emission_luminance;
-        bool sampled = false;
+        bool sampled = false, known_visible = false;
         LightPoint lp;
         f3 contribution;
         if (nee && nee_mode) {
@@ -246,9 +246,19 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
                 contribution = (bsdf * lp.emission) * scale;
                 shadow_dir = dir;
                 sampled = true;
+                // The pixel's own candidate that passed the visibility pass
+                // is the same ray from the same point: its answer stands.
+                uint4 own = b.restir_initial[p];
+                known_visible = own.x == r.light && own.y == r.bary && (own.z & 0xffffu) != 0u;
             }
         }
-        if (sampled) {
+        if (sampled && known_visible) {
+            // As the shadow kernel would: direct light, then the deferred
+            // emission of the primary vertex.
+            direct = throughput * contribution;
+            accumulated = accumulated + direct;
+            accumulated = accumulated + emissive * throughput;
+        } else if (sampled) {
             direct = throughput * contribution;
             shadow_org = sf.position;
             light_instance = lp.instance;
@@ -396,6 +406,14 @@ __global__ __launch_bounds__(256) void accumulate_chunk_kernel(tt::FrameArgs f, 
     accumulate_pixel(f, target, pixel0 + i, f3(r.x, r.y, r.z) / 1.0f);
 }
 
+// ---------------------------------------------------------------- lights --
+__global__ void light_records_kernel(tt::DeviceScene d, float4* out) {
+    SceneView s = view(d);
+    uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= s.emissive_count) return;
+    store_light_record(out + 5 * size_t(li), light_triangle(s, li));
+}
+
 // ------------------------------------------------------------------ misc --
 __global__ void zero_kernel(uint32_t* p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -485,6 +503,12 @@ hipError_t launch_generate_chunk(hipStream_t s, const FrameArgs& f, const PathBu
 hipError_t launch_accumulate_chunk(hipStream_t s, const FrameArgs& f, const PathBuffers& b, uint32_t base,
                                    uint32_t pixel0, uint32_t count, float4* target) {
     accumulate_chunk_kernel<<<blocks(count, 256), 256, 0, s>>>(f, b, base, pixel0, count, target);
+    return hipGetLastError();
+}
+
+hipError_t launch_light_records(hipStream_t s, const DeviceScene& scene, void* out) {
+    if (scene.emissive_count == 0) return hipSuccess;
+    light_records_kernel<<<blocks(scene.emissive_count, 64), 64, 0, s>>>(scene, static_cast<float4*>(out));
     return hipGetLastError();
 }
 

@@ -81,6 +81,8 @@ tt_status ensure_paths(tt_context* c, size_t paths, PathBuffers& b) {
     b.counters = c->counters;
     b.spill = c->spill;
     b.cache = CacheBuffers{};
+    b.restir_final = nullptr;
+    b.restir_initial = nullptr;
     return TT_OK;
 }
 
@@ -274,6 +276,23 @@ tt_status tt_scene_create(tt_context* c, const tt_scene_desc* d, tt_scene** out)
         s->bounds_hi[a] = lo <= hi ? hi : 0.0f;
     }
     s->view.emissive_count = d->emissive_triangle_count;
+    if (d->emissive_triangle_count > 0) {
+        // World-space light records, computed once on the device.
+        size_t bytes = size_t(d->emissive_triangle_count) * 5 * sizeof(float4);
+        void* lights = nullptr;
+        hipError_t e = hipMalloc(&lights, bytes);
+        if (e == hipSuccess) {
+            s->allocations.push_back(lights);
+            s->info.device_bytes += bytes;
+            e = tt::launch_light_records(c->stream, s->view, lights);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            tt_scene_destroy(s);
+            return hip_fail(c, e, "light records");
+        }
+        s->view.lights = lights;
+    }
     uint64_t tris = 0, unique = 0;
     for (uint32_t i = 0; i < d->instance_count; i++) tris += d->meshes[d->instances[i].mesh_index].index_count / 3;
     for (uint32_t m = 0; m < d->mesh_count; m++) unique += d->meshes[m].index_count / 3;
@@ -400,6 +419,7 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
         rb.temporal_reuse = st.restir_temporal ? 1u : 0u;
         rb.spatial_reuse = st.restir_spatial ? 1u : 0u;
         b.restir_final = rb.final_[rb.cur];
+        b.restir_initial = (rb.temporal_reuse || rb.spatial_reuse) ? rb.initial : rb.final_[rb.cur];
     }
 
     tt::FrameArgs f = tt::frame_args(*k, t->width, t->height, st.rng, st.seed);
