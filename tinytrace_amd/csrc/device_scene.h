@@ -61,7 +61,8 @@ struct HitRecord {
 };
 
 struct TraceRay {
-    f3 org, inv, olo, ohi;  // slab origins shifted by -/+ e
+    f3 inv;           // 1 / direction (inf where a component is 0)
+    f3 olo_inv, ohi_inv;  // slab origins shifted by +e / -e, times inv
     // Woop setup
     int kx, ky, kz;
     float sx, sy, sz;
@@ -71,12 +72,11 @@ struct TraceRay {
 TT_DEV float component(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
 TT_DEV void slab_setup(TraceRay& r, f3 o, f3 d) {
-    r.org = o;
     r.inv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     float mo = fmaxf(fmaxf(abs_(o.x), abs_(o.y)), abs_(o.z));
     float e = mo * (64.0f * 1.1920929e-7f) + 1e-30f;
-    r.olo = f3(o.x + e, o.y + e, o.z + e);
-    r.ohi = f3(o.x - e, o.y - e, o.z - e);
+    r.olo_inv = f3((o.x + e) * r.inv.x, (o.y + e) * r.inv.y, (o.z + e) * r.inv.z);
+    r.ohi_inv = f3((o.x - e) * r.inv.x, (o.y - e) * r.inv.y, (o.z - e) * r.inv.z);
 }
 
 TT_DEV void woop_setup(TraceRay& r, f3 o, f3 d) {
@@ -100,18 +100,28 @@ TT_DEV void woop_setup(TraceRay& r, f3 o, f3 d) {
     r.sz = 1.0f / dz;
 }
 
-// Conservative slab test of one box against [tmin, tmax]; returns the raw
-// entry distance in `tn` for ordering.
+// Conservative slab interval of a box: plane distances as one fused
+// multiply-add from the shifted origins (the shift e covers the rounding of
+// origin * inv many times over), widened by a relative 1e-6.  A zero
+// direction component makes that axis' distances NaN, which min/max ignore:
+// the axis then does not cull, so the test stays conservative.
+TT_DEV void box_interval(const TraceRay& r, float lx, float hx, float ly, float hy, float lz, float hz, float& n,
+                         float& f) {
+    float t0x = __builtin_fmaf(lx, r.inv.x, -r.olo_inv.x), t1x = __builtin_fmaf(hx, r.inv.x, -r.ohi_inv.x);
+    float t0y = __builtin_fmaf(ly, r.inv.y, -r.olo_inv.y), t1y = __builtin_fmaf(hy, r.inv.y, -r.ohi_inv.y);
+    float t0z = __builtin_fmaf(lz, r.inv.z, -r.olo_inv.z), t1z = __builtin_fmaf(hz, r.inv.z, -r.ohi_inv.z);
+    n = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    f = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+}
+
+// Box test against [tmin, tmax]; `tn` = raw entry distance for ordering.
 TT_DEV bool box_hit(const TraceRay& r, float lx, float hx, float ly, float hy, float lz, float hz, float tmin,
                     float tmax, float& tn) {
-    float t0x = (lx - r.olo.x) * r.inv.x, t1x = (hx - r.ohi.x) * r.inv.x;
-    float t0y = (ly - r.olo.y) * r.inv.y, t1y = (hy - r.ohi.y) * r.inv.y;
-    float t0z = (lz - r.olo.z) * r.inv.z, t1z = (hz - r.ohi.z) * r.inv.z;
-    float n = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
-    float f = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    float n, f;
+    box_interval(r, lx, hx, ly, hy, lz, hz, n, f);
     tn = n;
-    float nc = n - abs_(n) * 1e-6f;
-    float fc = f + abs_(f) * 1e-6f;
+    float nc = __builtin_fmaf(-abs_(n), 1e-6f, n);
+    float fc = __builtin_fmaf(abs_(f), 1e-6f, f);
     return fmaxf(nc, tmin) <= fminf(fc, tmax);
 }
 
@@ -314,13 +324,10 @@ TT_DEV bool light_visible(const SceneView& s, f3 org, f3 dir, uint32_t light_ins
     float4 lo = box[0], hi = box[1];
     TraceRay r;
     slab_setup(r, org, dir);
-    float t0x = (lo.x - r.olo.x) * r.inv.x, t1x = (hi.x - r.ohi.x) * r.inv.x;
-    float t0y = (lo.y - r.olo.y) * r.inv.y, t1y = (hi.y - r.ohi.y) * r.inv.y;
-    float t0z = (lo.z - r.olo.z) * r.inv.z, t1z = (hi.z - r.ohi.z) * r.inv.z;
-    float n = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
-    float f = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-    float t_entry = n - abs_(n) * 1e-6f;
-    float t_far = f + abs_(f) * 1e-6f;
+    float n, f;
+    box_interval(r, lo.x, hi.x, lo.y, hi.y, lo.z, hi.z, n, f);
+    float t_entry = __builtin_fmaf(-abs_(n), 1e-6f, n);
+    float t_far = __builtin_fmaf(abs_(f), 1e-6f, f);
     const float tmin = 1.0e-3f;
     float tmax = fminf(t_far, 10000.0f);
     if (!(fmaxf(t_entry, tmin) <= tmax)) return false;  // the light cannot be hit
