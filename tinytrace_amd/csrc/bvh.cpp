@@ -7,6 +7,15 @@
 #include <cmath>
 #include <thread>
 
+// SAH cost of one traversal step relative to one triangle test, and the
+// largest BLAS leaf.
+#ifndef TT_SAH_TRAVERSAL_COST
+#define TT_SAH_TRAVERSAL_COST 1.0
+#endif
+#ifndef TT_BLAS_MAX_LEAF
+#define TT_BLAS_MAX_LEAF 4
+#endif
+
 namespace tt {
 
 namespace {
@@ -117,7 +126,7 @@ struct Builder {
                 }
             }
             double parent_area = box.area();
-            double split_cost = parent_area > 0.0 ? 1.0 + best / parent_area : INFINITY;
+            double split_cost = parent_area > 0.0 ? TT_SAH_TRAVERSAL_COST + best / parent_area : INFINITY;
             if (best_axis >= 0 && !(count <= max_leaf && leaf_cost <= split_cost)) {
                 float ext = cbox.hi[best_axis] - cbox.lo[best_axis];
                 float scale = float(BINS) / ext;
@@ -253,7 +262,7 @@ void build_blas(const tt_scene_desc& d, uint32_t mesh, Blas& out) {
         bt.v1[3] = 0.0f;
         bt.v2[3] = 0.0f;
     }
-    Builder b(prims, out.nodes, 4);
+    Builder b(prims, out.nodes, TT_BLAS_MAX_LEAF);
     uint32_t root = b.build(0, tri_count, 0, out.box);
     // Lay out triangles in leaf order and patch leaf references.
     std::vector<uint32_t> leaf_ref(b.leaves.size());

@@ -170,22 +170,33 @@ TT_DEV bool triangle_hit(const TraceRay& r, f3 v0, f3 v1, f3 v2, float tmin, flo
     return true;
 }
 
+// Top TRACE_LDS_STACK entries of every lane's stack: entry e of lane t at
+// trace_stack[e * TRACE_BLOCK + t].  A file-scope LDS array, so pushes and
+// pops compile to ds_write / ds_read rather than generic flat accesses.
+static __shared__ uint32_t trace_stack[TRACE_LDS_STACK * TRACE_BLOCK];
+
 struct Stack {
-    uint32_t* lds;    // entry e at lds[e * TRACE_BLOCK]
+    uint32_t lane;    // threadIdx.x
     uint32_t* spill;  // entry e at spill[(e - TRACE_LDS_STACK) * spill_stride]
     uint32_t spill_stride;
     uint32_t sp;
     TT_DEV void push(uint32_t v) {
         if (sp < uint32_t(TRACE_LDS_STACK))
-            lds[sp * TRACE_BLOCK] = v;
+            trace_stack[sp * TRACE_BLOCK + lane] = v;
         else
-            spill[(sp - TRACE_LDS_STACK) * spill_stride] = v;
+            static_cast<volatile uint32_t*>(spill)[(sp - TRACE_LDS_STACK) * spill_stride] = v;
         sp++;
     }
     TT_DEV uint32_t pop() {
         sp--;
-        if (sp < uint32_t(TRACE_LDS_STACK)) return lds[sp * TRACE_BLOCK];
-        return spill[(sp - TRACE_LDS_STACK) * spill_stride];
+        // The LDS read is unconditional (clamped slot) so it stays a ds_read;
+        // only the rare spilled entry comes from global memory.
+        uint32_t slot = sp < uint32_t(TRACE_LDS_STACK) ? sp : uint32_t(TRACE_LDS_STACK - 1);
+        uint32_t v = trace_stack[slot * TRACE_BLOCK + lane];
+        // volatile keeps the spill read a separate global load (otherwise it
+        // is folded with the LDS read into one generic flat load).
+        if (sp >= uint32_t(TRACE_LDS_STACK)) v = static_cast<volatile uint32_t*>(spill)[(sp - TRACE_LDS_STACK) * spill_stride];
+        return v;
     }
 };
 

@@ -54,9 +54,9 @@ __device__ __host__ inline SceneView view(const tt::DeviceScene& d) {
     return s;
 }
 
-TT_DEV Stack make_stack(uint32_t* lds, uint32_t* spill) {
+TT_DEV Stack make_stack(uint32_t* spill) {
     Stack st;
-    st.lds = lds + threadIdx.x;
+    st.lane = threadIdx.x;
     st.spill = spill + blockIdx.x * TRACE_BLOCK + threadIdx.x;
     st.spill_stride = gridDim.x * TRACE_BLOCK;
     st.sp = 0;

@@ -34,9 +34,8 @@ using tt::C_WORK_SHADOW;
 template <bool CULL>
 __global__ __launch_bounds__(TRACE_BLOCK) void trace_rays_kernel(tt::DeviceScene d, const tt_ray* rays, tt_hit* hits,
                                                                   uint32_t count, uint32_t* work, uint32_t* spill) {
-    __shared__ uint32_t lds[TRACE_LDS_STACK * TRACE_BLOCK];
     SceneView s = view(d);
-    Stack st = make_stack(lds, spill);
+    Stack st = make_stack(spill);
     for (;;) {
         uint32_t base = next_batch(work);
         if (base >= count) break;
@@ -104,9 +103,8 @@ __global__ __launch_bounds__(256) void generate_chunk_kernel(tt::FrameArgs f, tt
 
 // ---------------------------------------------------------------- extend --
 __global__ __launch_bounds__(TRACE_BLOCK) void extend_kernel(tt::DeviceScene d, tt::PathBuffers b, uint32_t bounce) {
-    __shared__ uint32_t lds[TRACE_LDS_STACK * TRACE_BLOCK];
     SceneView s = view(d);
-    Stack st = make_stack(lds, b.spill);
+    Stack st = make_stack(b.spill);
     const uint32_t count = b.counters[C_EXT + bounce];
     const float4* qo = b.queue[bounce & 1][0];
     const float4* qd = b.queue[bounce & 1][1];
@@ -335,9 +333,8 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
 // is confined to the light instance's box interval and ends at the first
 // occluder in front of it (ShadowQuery), with the same outcome.
 __global__ __launch_bounds__(TRACE_BLOCK) void shadow_kernel(tt::DeviceScene d, tt::PathBuffers b, uint32_t bounce) {
-    __shared__ uint32_t lds[TRACE_LDS_STACK * TRACE_BLOCK];
     SceneView s = view(d);
-    Stack st = make_stack(lds, b.spill);
+    Stack st = make_stack(b.spill);
     const uint32_t count = b.counters[C_SHADOW + bounce];
     for (;;) {
         uint32_t base = next_batch(b.counters + C_WORK_SHADOW + bounce);

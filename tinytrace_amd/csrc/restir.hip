@@ -206,9 +206,8 @@ __global__ __launch_bounds__(256) void restir_candidates_kernel(tt::DeviceScene 
 // candidate winner; occluded samples keep M and get W = 0.
 __global__ __launch_bounds__(TRACE_BLOCK) void restir_visibility_kernel(tt::DeviceScene d, tt::PathBuffers b,
                                                                         uint4* res) {
-    __shared__ uint32_t lds[TRACE_LDS_STACK * TRACE_BLOCK];
     SceneView s = view(d);
-    Stack st = make_stack(lds, b.spill);
+    Stack st = make_stack(b.spill);
     const uint32_t count = b.counters[C_RESTIR_VIS];
     for (;;) {
         uint32_t base = next_batch(b.counters + C_WORK_RESTIR);
