@@ -206,6 +206,19 @@ const char* tt_last_error(const tt_context* context);
  * BLAS/TLAS build (resource.rs:804-878, :1014-1083): uploads the arrays and
  * builds one BLAS per mesh plus a TLAS over the instances. */
 tt_status tt_scene_create(tt_context* context, const tt_scene_desc* desc, tt_scene** out);
+
+/* BVH builders: binned SAH on the host threads (best traversal speed) or
+ * LBVH on the device (Karras 2012: Morton codes, radix sort, parallel
+ * hierarchy, bottom-up boxes; builds in milliseconds).  The TLAS over
+ * instances is built on the host either way. */
+enum { TT_BVH_BUILDER_HOST_SAH = 0, TT_BVH_BUILDER_DEVICE_LBVH = 1 };
+typedef struct tt_scene_options {
+    uint32_t bvh_builder; /* TT_BVH_BUILDER_* */
+    uint32_t reserved;
+} tt_scene_options;
+/* tt_scene_create with options (NULL: the defaults). */
+tt_status tt_scene_create_ex(tt_context* context, const tt_scene_desc* desc, const tt_scene_options* options,
+                             tt_scene** out);
 void tt_scene_destroy(tt_scene* scene);
 
 typedef struct tt_scene_info {

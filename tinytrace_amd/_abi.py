@@ -112,6 +112,13 @@ class MaterialDesc(C.Structure):
     _fields_ = [(n, C.c_float if k == 1 else C.c_float * 3) for n, k in MATERIAL_DESC_FIELDS]
 
 
+class SceneOptions(C.Structure):
+    _fields_ = [("bvh_builder", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+TT_BVH_BUILDER_HOST_SAH, TT_BVH_BUILDER_DEVICE_LBVH = 0, 1
+
+
 class ServerDesc(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("pool_paths", C.c_uint32), ("reserved", C.c_uint32)]
 
@@ -182,6 +189,8 @@ _PROTOTYPES = [
     ("tt_context_destroy", None, [C.c_void_p]),
     ("tt_last_error", C.c_char_p, [C.c_void_p]),
     ("tt_scene_create", C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
+    ("tt_scene_create_ex", C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(SceneOptions),
+                                     C.POINTER(C.c_void_p)]),
     ("tt_scene_destroy", None, [C.c_void_p]),
     ("tt_scene_info_get", C.c_int, [C.c_void_p, C.POINTER(SceneInfo)]),
     ("tt_target_create", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),

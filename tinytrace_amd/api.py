@@ -264,13 +264,17 @@ class Context:
 
 
 class DeviceScene:
-    """A resident scene (tt_scene): arrays uploaded and BVH built."""
+    """A resident scene (tt_scene): arrays uploaded and BVH built, with the
+    host binned-SAH builder ("sah") or the device LBVH builder ("lbvh")."""
 
-    def __init__(self, context: Context, asset: AssetScene):
+    BUILDERS = {"sah": 0, "lbvh": 1}
+
+    def __init__(self, context: Context, asset: AssetScene, builder: str = "sah"):
         self.context = context
         h = C.c_void_p()
-        check(lib().tt_scene_create(context.handle, C.byref(asset.flatten()), C.byref(h)), context.handle,
-              "tt_scene_create")
+        opts = _abi.SceneOptions(self.BUILDERS[builder], 0)
+        check(lib().tt_scene_create_ex(context.handle, C.byref(asset.flatten()), C.byref(opts), C.byref(h)),
+              context.handle, "tt_scene_create_ex")
         self.handle = h
         _abi.track(self, 1)
 
@@ -355,10 +359,10 @@ class Renderer:
         self._asset = AssetScene()
         self.scene = DeviceScene(self.context, self._asset)
 
-    def set_scene(self, scene: AssetScene) -> None:
+    def set_scene(self, scene: AssetScene, builder: str = "sah") -> None:
         self.scene.close()
         self._asset = scene
-        self.scene = DeviceScene(self.context, scene)
+        self.scene = DeviceScene(self.context, scene, builder)
 
     def set_config(self, config: Config) -> None:
         if config != self.config:

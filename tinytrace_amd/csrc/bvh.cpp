@@ -199,13 +199,15 @@ struct Stats {
 };
 
 template <typename LeafFn>
-void subtree_stats(const std::vector<BvhNode>& nodes, uint32_t ref, double p, LeafFn&& leaf, Stats& s) {
+void subtree_stats(const std::vector<BvhNode>& nodes, uint32_t ref, uint32_t node_base, double p, LeafFn&& leaf,
+                   Stats& s) {
+    // nodes[k] holds node node_base + k.
     // p = probability that this reference's box is hit (relative to root).
     if (ref & BVH_LEAF) {
         leaf(ref, p, s);
         return;
     }
-    const BvhNode& n = nodes[ref];
+    const BvhNode& n = nodes[ref - node_base];
     s.cost += p * 1.0;
     s.bytes += p * double(sizeof(BvhNode));
     auto area = [](float lx, float hx, float ly, float hy, float lz, float hz) {
@@ -221,8 +223,8 @@ void subtree_stats(const std::vector<BvhNode>& nodes, uint32_t ref, double p, Le
     double uz = double(std::max(n.n2[1], n.n2[3])) - std::min(n.n2[0], n.n2[2]);
     parent = std::max(parent, 2.0 * (ux * uy + uy * uz + uz * ux));
     double q0 = parent > 0 ? a0 / parent : 1.0, q1 = parent > 0 ? a1 / parent : 1.0;
-    subtree_stats(nodes, n.n3[0], p * q0, leaf, s);
-    subtree_stats(nodes, n.n3[1], p * q1, leaf, s);
+    subtree_stats(nodes, n.n3[0], node_base, p * q0, leaf, s);
+    subtree_stats(nodes, n.n3[1], node_base, p * q1, leaf, s);
 }
 
 struct Blas {
@@ -282,7 +284,7 @@ void build_blas(const tt_scene_desc& d, uint32_t mesh, Blas& out) {
     out.empty = false;
     // Expected bytes/cost relative to the BLAS root box.
     subtree_stats(
-        out.nodes, out.root, 1.0,
+        out.nodes, out.root, 0, 1.0,
         [&](uint32_t ref, double p, Stats& s) {
             double n = double(((ref >> 27) & 0xfu) + 1);
             s.cost += p * n;
@@ -292,6 +294,125 @@ void build_blas(const tt_scene_desc& d, uint32_t mesh, Blas& out) {
 }
 
 }  // namespace
+
+// TLAS over the instances, instance records, world boxes and the two-level
+// SAH statistics, given every mesh's BLAS (roots as global references).  The
+// TLAS nodes are appended to out.nodes and numbered from node_base on.
+bool build_tlas(const tt_scene_desc& d, const std::vector<BlasSummary>& blas, uint32_t node_base,
+                BvhBuildResult& out) {
+    // Instance world boxes: 8 transformed corners of the object box in
+    // double, padded by 1e-5 of (|lo| + |hi| + extent).
+    std::vector<Prim> prims;
+    std::vector<Box> world(d.instance_count);
+    for (uint32_t i = 0; i < d.instance_count; i++) {
+        const tt_instance& in = d.instances[i];
+        if (in.mesh_index >= d.mesh_count) {
+            out.error = "instance refers to a missing mesh";
+            return false;
+        }
+        const BlasSummary& b = blas[in.mesh_index];
+        if (b.empty) continue;
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int c = 0; c < 8; c++) {
+            double p[3] = {(c & 1) ? b.hi[0] : b.lo[0], (c & 2) ? b.hi[1] : b.lo[1], (c & 4) ? b.hi[2] : b.lo[2]};
+            for (int r = 0; r < 3; r++) {
+                double w = double(in.transform[12 + r]);
+                for (int k = 0; k < 3; k++) w += double(in.transform[k * 4 + r]) * p[k];
+                lo[r] = std::min(lo[r], w);
+                hi[r] = std::max(hi[r], w);
+            }
+        }
+        double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+        Prim p;
+        for (int r = 0; r < 3; r++) {
+            double pad = (std::fabs(lo[r]) + std::fabs(hi[r]) + ext) * 1e-5;
+            p.box.lo[r] = float(lo[r] - pad);
+            p.box.hi[r] = float(hi[r] + pad);
+            p.centroid[r] = float(0.5 * (lo[r] + hi[r]));
+        }
+        p.id = i;
+        world[i] = p.box;
+        prims.push_back(p);
+    }
+
+    // TLAS (one instance per leaf), after the BLAS nodes.
+    std::vector<BvhNode> tlas_nodes;
+    uint32_t tlas_root = BVH_LEAF | 0x7fffffffu;
+    uint32_t tlas_depth = 0;
+    Box tlas_box;
+    if (!prims.empty()) {
+        Builder b(prims, tlas_nodes, 1);
+        uint32_t root = b.build(0, uint32_t(prims.size()), 0, tlas_box);
+        std::vector<uint32_t> leaf_ref(b.leaves.size());
+        for (size_t l = 0; l < b.leaves.size(); l++) leaf_ref[l] = BVH_LEAF | prims[b.leaves[l].first].id;
+        auto patch = [&](uint32_t r) { return (r & BVH_LEAF) ? leaf_ref[r & 0x7fffffffu] : r + node_base; };
+        for (auto& n : tlas_nodes) {
+            n.n3[0] = patch(n.n3[0]);
+            n.n3[1] = patch(n.n3[1]);
+        }
+        tlas_root = patch(root);
+        tlas_depth = b.max_depth;
+        out.empty = false;
+    }
+    std::vector<BvhNode> tlas_copy = tlas_nodes;
+    out.nodes.insert(out.nodes.end(), tlas_copy.begin(), tlas_copy.end());
+    uint32_t deepest = 0;
+    for (const BlasSummary& b : blas)
+        if (!b.empty) deepest = std::max(deepest, b.depth);
+    out.tlas_root = tlas_root;
+    out.max_depth = tlas_depth + deepest;
+    if (out.max_depth + 2 > BVH_STACK_LIMIT) {
+        out.error = "BVH too deep for the traversal stack";
+        return false;
+    }
+
+    out.instances.resize(d.instance_count);
+    for (uint32_t i = 0; i < d.instance_count; i++) {
+        const tt_instance& in = d.instances[i];
+        BvhInstance& bi = out.instances[i];
+        for (int r = 0; r < 3; r++) {
+            bi.c0[r] = in.inverse_transform[0 + r];
+            bi.c1[r] = in.inverse_transform[4 + r];
+            bi.c2[r] = in.inverse_transform[8 + r];
+            bi.c3[r] = in.inverse_transform[12 + r];
+        }
+        uint32_t root = blas[in.mesh_index].empty ? (BVH_LEAF | 0x7fffffffu) : blas[in.mesh_index].root;
+        std::memcpy(&bi.c0[3], &root, 4);
+        // c1.w: first global triangle index of the mesh (index_offset / 3).
+        uint32_t base = in.index_offset / 3;
+        std::memcpy(&bi.c1[3], &base, 4);
+        bi.c2[3] = 0.0f;
+        bi.c3[3] = 0.0f;
+    }
+    // Padded world boxes (empty meshes keep lo = +inf > hi = -inf: no ray
+    // enters them).
+    out.instance_boxes.assign(size_t(d.instance_count) * 8, 0.0f);
+    for (uint32_t i = 0; i < d.instance_count; i++) {
+        float* b = &out.instance_boxes[size_t(i) * 8];
+        for (int r = 0; r < 3; r++) {
+            b[r] = world[i].lo[r];
+            b[4 + r] = world[i].hi[r];
+        }
+    }
+
+    // Two-level SAH expectation: TLAS nodes, then per instance leaf the
+    // instance record plus the BLAS expectation.
+    Stats s;
+    if (!out.empty) {
+        subtree_stats(
+            tlas_copy, tlas_root, node_base, 1.0,
+            [&](uint32_t ref, double p, Stats& st) {
+                uint32_t i = ref & 0x7fffffffu;
+                const BlasSummary& b = blas[d.instances[i].mesh_index];
+                st.cost += p * (1.0 + b.cost);
+                st.bytes += p * (double(sizeof(BvhInstance)) + b.bytes);
+            },
+            s);
+    }
+    out.sah_cost = s.cost;
+    out.expected_bytes = s.bytes;
+    return true;
+}
 
 bool build_bvh(const tt_scene_desc& d, BvhBuildResult& out) {
     out = BvhBuildResult();
@@ -316,69 +437,12 @@ bool build_bvh(const tt_scene_desc& d, BvhBuildResult& out) {
             });
         for (auto& t : pool) t.join();
     }
-
-    // Instance world boxes: 8 transformed corners of the object box in
-    // double, padded by 1e-5 of (|lo| + |hi| + extent).
-    std::vector<Prim> prims;
-    std::vector<Box> world(d.instance_count);
-    for (uint32_t i = 0; i < d.instance_count; i++) {
-        const tt_instance& in = d.instances[i];
-        if (in.mesh_index >= d.mesh_count) {
-            out.error = "instance refers to a missing mesh";
-            return false;
-        }
-        const Blas& b = blas[in.mesh_index];
-        if (b.empty) continue;
-        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (int c = 0; c < 8; c++) {
-            double p[3] = {(c & 1) ? b.box.hi[0] : b.box.lo[0], (c & 2) ? b.box.hi[1] : b.box.lo[1],
-                           (c & 4) ? b.box.hi[2] : b.box.lo[2]};
-            for (int r = 0; r < 3; r++) {
-                double w = double(in.transform[12 + r]);
-                for (int k = 0; k < 3; k++) w += double(in.transform[k * 4 + r]) * p[k];
-                lo[r] = std::min(lo[r], w);
-                hi[r] = std::max(hi[r], w);
-            }
-        }
-        double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
-        Prim p;
-        for (int r = 0; r < 3; r++) {
-            double pad = (std::fabs(lo[r]) + std::fabs(hi[r]) + ext) * 1e-5;
-            p.box.lo[r] = float(lo[r] - pad);
-            p.box.hi[r] = float(hi[r] + pad);
-            p.centroid[r] = float(0.5 * (lo[r] + hi[r]));
-        }
-        p.id = i;
-        world[i] = p.box;
-        prims.push_back(p);
-    }
-
-    // TLAS (one instance per leaf).
-    std::vector<BvhNode> tlas_nodes;
-    uint32_t tlas_root = BVH_LEAF | 0x7fffffffu;
-    uint32_t tlas_depth = 0;
-    Box tlas_box;
-    if (!prims.empty()) {
-        Builder b(prims, tlas_nodes, 1);
-        uint32_t root = b.build(0, uint32_t(prims.size()), 0, tlas_box);
-        std::vector<uint32_t> leaf_ref(b.leaves.size());
-        for (size_t l = 0; l < b.leaves.size(); l++) leaf_ref[l] = BVH_LEAF | prims[b.leaves[l].first].id;
-        auto patch = [&](uint32_t r) { return (r & BVH_LEAF) ? leaf_ref[r & 0x7fffffffu] : r; };
-        for (auto& n : tlas_nodes) {
-            n.n3[0] = patch(n.n3[0]);
-            n.n3[1] = patch(n.n3[1]);
-        }
-        tlas_root = patch(root);
-        tlas_depth = b.max_depth;
-        out.empty = false;
-    }
-
-    // Concatenate: TLAS nodes, then each BLAS (references rebased).
-    out.nodes = tlas_nodes;
-    std::vector<uint32_t> blas_root(d.mesh_count, BVH_LEAF);
-    uint32_t deepest = 0;
+    // Concatenate every BLAS (references rebased), then the TLAS.
+    std::vector<BlasSummary> summary(d.mesh_count);
     for (uint32_t m = 0; m < d.mesh_count; m++) {
         Blas& b = blas[m];
+        BlasSummary& bs = summary[m];
+        bs.empty = b.empty;
         if (b.empty) continue;
         uint32_t node_base = uint32_t(out.nodes.size());
         uint32_t tri_base = uint32_t(out.tris.size());
@@ -392,62 +456,16 @@ bool build_bvh(const tt_scene_desc& d, BvhBuildResult& out) {
             out.nodes.push_back(n);
         }
         out.tris.insert(out.tris.end(), b.tris.begin(), b.tris.end());
-        blas_root[m] = rebase(b.root);
-        deepest = std::max(deepest, b.depth);
-    }
-    out.tlas_root = tlas_root;
-    out.max_depth = tlas_depth + deepest;
-    if (out.max_depth + 2 > BVH_STACK_LIMIT) {
-        out.error = "BVH too deep for the traversal stack";
-        return false;
-    }
-
-    out.instances.resize(d.instance_count);
-    for (uint32_t i = 0; i < d.instance_count; i++) {
-        const tt_instance& in = d.instances[i];
-        BvhInstance& bi = out.instances[i];
-        for (int r = 0; r < 3; r++) {
-            bi.c0[r] = in.inverse_transform[0 + r];
-            bi.c1[r] = in.inverse_transform[4 + r];
-            bi.c2[r] = in.inverse_transform[8 + r];
-            bi.c3[r] = in.inverse_transform[12 + r];
+        bs.root = rebase(b.root);
+        for (int a = 0; a < 3; a++) {
+            bs.lo[a] = b.box.lo[a];
+            bs.hi[a] = b.box.hi[a];
         }
-        uint32_t root = blas[in.mesh_index].empty ? (BVH_LEAF | 0x7fffffffu) : blas_root[in.mesh_index];
-        std::memcpy(&bi.c0[3], &root, 4);
-        // c1.w: first global triangle index of the mesh (index_offset / 3).
-        uint32_t base = in.index_offset / 3;
-        std::memcpy(&bi.c1[3], &base, 4);
-        bi.c2[3] = 0.0f;
-        bi.c3[3] = 0.0f;
+        bs.depth = b.depth;
+        bs.cost = b.stats.cost;
+        bs.bytes = b.stats.bytes;
     }
-    // Padded world boxes (empty meshes keep lo = +inf > hi = -inf: no ray
-    // enters them).
-    out.instance_boxes.assign(size_t(d.instance_count) * 8, 0.0f);
-    for (uint32_t i = 0; i < d.instance_count; i++) {
-        float* b = &out.instance_boxes[size_t(i) * 8];
-        for (int r = 0; r < 3; r++) {
-            b[r] = world[i].lo[r];
-            b[4 + r] = world[i].hi[r];
-        }
-    }
-
-    // Two-level SAH expectation: TLAS nodes, then per instance leaf the
-    // instance record plus the BLAS expectation.
-    Stats s;
-    if (!out.empty) {
-        subtree_stats(
-            out.nodes, tlas_root, 1.0,
-            [&](uint32_t ref, double p, Stats& st) {
-                uint32_t i = ref & 0x7fffffffu;
-                const Blas& b = blas[d.instances[i].mesh_index];
-                st.cost += p * (1.0 + b.stats.cost);
-                st.bytes += p * (double(sizeof(BvhInstance)) + b.stats.bytes);
-            },
-            s);
-    }
-    out.sah_cost = s.cost;
-    out.expected_bytes = s.bytes;
-    return true;
+    return build_tlas(d, summary, uint32_t(out.nodes.size()), out);
 }
 
 }  // namespace tt

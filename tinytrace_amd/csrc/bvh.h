@@ -8,8 +8,8 @@
 // HBM layout
 //   nodes : BvhNode[]  64 B, both children's boxes in the parent so one
 //                      16-B-aligned fetch of 4 x dwordx4 tests two boxes
-//                      (Aila & Laine 2009).  TLAS nodes first, then every
-//                      BLAS.
+//                      (Aila & Laine 2009).  Every BLAS first, then the
+//                      TLAS.
 //   tris  : BvhTri[]   48 B, object-space vertices dequantized from snorm16
 //                      (v / 32767), in leaf order; v0.w holds the mesh-local
 //                      primitive index.
@@ -76,6 +76,21 @@ struct BvhBuildResult {
     std::string error;
 };
 
+// Host binned-SAH build of every BLAS and the TLAS.
 bool build_bvh(const tt_scene_desc& desc, BvhBuildResult& out);
+
+// One mesh's BLAS as the TLAS build needs it.
+struct BlasSummary {
+    bool empty = true;
+    uint32_t root = BVH_LEAF;      // global node / leaf reference
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // object-space bounds (unpadded)
+    uint32_t depth = 0;
+    double cost = 0.0, bytes = 0.0;  // SAH expectation relative to the root box
+};
+
+// TLAS, instance records, world boxes and statistics over the given BLASes;
+// appends the TLAS nodes to out.nodes, numbered from node_base on.
+bool build_tlas(const tt_scene_desc& desc, const std::vector<BlasSummary>& blas, uint32_t node_base,
+                BvhBuildResult& out);
 
 }  // namespace tt

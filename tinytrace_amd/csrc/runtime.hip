@@ -209,6 +209,10 @@ void tt_context_destroy(tt_context* c) {
 const char* tt_last_error(const tt_context* c) { return c ? c->error.c_str() : "no context"; }
 
 tt_status tt_scene_create(tt_context* c, const tt_scene_desc* d, tt_scene** out) {
+    return tt_scene_create_ex(c, d, nullptr, out);
+}
+
+tt_status tt_scene_create_ex(tt_context* c, const tt_scene_desc* d, const tt_scene_options* options, tt_scene** out) {
     if (!c || !d || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     auto t0 = std::chrono::steady_clock::now();
@@ -236,8 +240,15 @@ tt_status tt_scene_create(tt_context* c, const tt_scene_desc* d, tt_scene** out)
         if (d->emissive_triangles[e].instance_index >= d->instance_count)
             return fail(c, TT_INVALID_ARGUMENT, "emissive triangle refers to a missing instance");
 
+    const uint32_t builder = options ? options->bvh_builder : TT_BVH_BUILDER_HOST_SAH;
+    if (builder != TT_BVH_BUILDER_HOST_SAH && builder != TT_BVH_BUILDER_DEVICE_LBVH)
+        return fail(c, TT_INVALID_ARGUMENT, "unknown BVH builder");
+    uint64_t total_tris = 0;
+    for (uint32_t m = 0; m < d->mesh_count; m++) total_tris += d->meshes[m].index_count / 3;
+    if (total_tris >= (1ull << 27)) return fail(c, TT_INVALID_ARGUMENT, "too many triangles for the 27-bit leaf encoding");
     tt::BvhBuildResult bvh;
-    if (!tt::build_bvh(*d, bvh)) return fail(c, TT_INVALID_ARGUMENT, "BVH build failed: " + bvh.error);
+    if (builder == TT_BVH_BUILDER_HOST_SAH && !tt::build_bvh(*d, bvh))
+        return fail(c, TT_INVALID_ARGUMENT, "BVH build failed: " + bvh.error);
 
     tt_scene* s = new (std::nothrow) tt_scene();
     static std::atomic<uint64_t> next_serial{1};
@@ -251,16 +262,54 @@ tt_status tt_scene_create(tt_context* c, const tt_scene_desc* d, tt_scene** out)
     auto up = [&](const void* src, size_t bytes, const void** dst) {
         if (st == TT_OK) st = upload(c, s, src, bytes, dst);
     };
-    up(bvh.nodes.data(), bvh.nodes.size() * sizeof(tt::BvhNode), &s->view.nodes);
-    up(bvh.tris.data(), bvh.tris.size() * sizeof(tt::BvhTri), &s->view.tris);
-    up(bvh.instances.data(), bvh.instances.size() * sizeof(tt::BvhInstance), &s->view.bvh_insts);
-    up(bvh.instance_boxes.data(), bvh.instance_boxes.size() * sizeof(float), &s->view.inst_boxes);
     up(d->instances, size_t(d->instance_count) * sizeof(tt_instance), &s->view.instances);
     up(d->materials, size_t(d->material_count) * sizeof(tt_material), &s->view.materials);
     up(d->positions, size_t(d->position_count) * 3 * sizeof(int16_t), &s->view.positions);
     up(d->vertices, size_t(d->vertex_count) * sizeof(tt_vertex), &s->view.vertices);
     up(d->indices, size_t(d->index_count) * sizeof(uint32_t), &s->view.indices);
     up(d->emissive_triangles, size_t(d->emissive_triangle_count) * sizeof(tt_emissive_triangle), &s->view.emissive);
+    uint32_t node_count = 0;
+    if (st == TT_OK && builder == TT_BVH_BUILDER_DEVICE_LBVH) {
+        // Every BLAS on the device, then the TLAS on the host after them.
+        uint32_t max_internal = uint32_t(total_tris);
+        size_t node_bytes = (size_t(max_internal) + 2 * size_t(d->instance_count) + 2) * sizeof(tt::BvhNode);
+        size_t tri_bytes = std::max<size_t>(total_tris, 1) * sizeof(tt::BvhTri);
+        void* nodes = nullptr;
+        void* tris_dev = nullptr;
+        hipError_t e = hipMalloc(&nodes, node_bytes);
+        if (e == hipSuccess) {
+            s->allocations.push_back(nodes);
+            e = hipMalloc(&tris_dev, tri_bytes);
+        }
+        if (e == hipSuccess) s->allocations.push_back(tris_dev);
+        if (e != hipSuccess) st = hip_fail(c, e, "BVH allocation");
+        std::vector<tt::BlasSummary> summary;
+        uint32_t blas_nodes = 0;
+        if (st == TT_OK)
+            st = tt::build_blas_device(c, *d, static_cast<const int16_t*>(s->view.positions),
+                                       static_cast<const uint32_t*>(s->view.indices), static_cast<tt::BvhNode*>(nodes),
+                                       static_cast<tt::BvhTri*>(tris_dev), summary, blas_nodes);
+        if (st == TT_OK && !tt::build_tlas(*d, summary, blas_nodes, bvh))
+            st = fail(c, TT_INVALID_ARGUMENT, "BVH build failed: " + bvh.error);
+        if (st == TT_OK && !bvh.nodes.empty()) {
+            e = hipMemcpy(static_cast<tt::BvhNode*>(nodes) + blas_nodes, bvh.nodes.data(),
+                          bvh.nodes.size() * sizeof(tt::BvhNode), hipMemcpyHostToDevice);
+            if (e != hipSuccess) st = hip_fail(c, e, "TLAS upload");
+        }
+        if (st == TT_OK) {
+            s->view.nodes = nodes;
+            s->view.tris = tris_dev;
+            s->info.device_bytes += size_t(blas_nodes + bvh.nodes.size()) * sizeof(tt::BvhNode) +
+                                    size_t(total_tris) * sizeof(tt::BvhTri);
+            node_count = blas_nodes + uint32_t(bvh.nodes.size());
+        }
+    } else {
+        up(bvh.nodes.data(), bvh.nodes.size() * sizeof(tt::BvhNode), &s->view.nodes);
+        up(bvh.tris.data(), bvh.tris.size() * sizeof(tt::BvhTri), &s->view.tris);
+        node_count = uint32_t(bvh.nodes.size());
+    }
+    up(bvh.instances.data(), bvh.instances.size() * sizeof(tt::BvhInstance), &s->view.bvh_insts);
+    up(bvh.instance_boxes.data(), bvh.instance_boxes.size() * sizeof(float), &s->view.inst_boxes);
     if (st != TT_OK) {
         tt_scene_destroy(s);
         return st;
@@ -298,7 +347,7 @@ tt_status tt_scene_create(tt_context* c, const tt_scene_desc* d, tt_scene** out)
     for (uint32_t m = 0; m < d->mesh_count; m++) unique += d->meshes[m].index_count / 3;
     s->info.triangle_count = uint32_t(std::min<uint64_t>(tris, 0xffffffffu));
     s->info.unique_triangle_count = uint32_t(unique);
-    s->info.node_count = uint32_t(bvh.nodes.size());
+    s->info.node_count = node_count;
     s->info.max_depth = bvh.max_depth;
     s->info.sah_cost = bvh.sah_cost;
     s->info.expected_bytes_per_ray = bvh.expected_bytes;

@@ -88,6 +88,13 @@ void context_release(tt_context* c);
 
 tt_status ensure_cache(tt_context* c, size_t paths, CacheBuffers& cb);
 
+// Device LBVH build of every BLAS (bvh_gpu.hip).
+struct BvhNode;
+struct BvhTri;
+struct BlasSummary;
+tt_status build_blas_device(tt_context* c, const tt_scene_desc& d, const int16_t* positions, const uint32_t* indices,
+                            BvhNode* nodes, BvhTri* tris, std::vector<BlasSummary>& summary, uint32_t& node_count);
+
 // Radiance-cache MLP (mlp.hip).
 tt_context* mlp_context(const tt_mlp* m);
 tt_status mlp_set_scene_bounds(tt_mlp* m, const float lo[3], const float hi[3]);
