@@ -200,16 +200,20 @@ struct Stats {
 
 template <typename LeafFn>
 void subtree_stats(const std::vector<BvhNode>& nodes, uint32_t ref, uint32_t node_base, double p, LeafFn&& leaf,
-                   Stats& s) {
+                   Stats& s, uint32_t depth = 0) {
     // nodes[k] holds node node_base + k.
     // p = probability that this reference's box is hit (relative to root).
+    // Binary nodes at even depth are the wide nodes traversal visits (one
+    // 128-B fetch, four box tests); odd-depth ones are folded into them.
     if (ref & BVH_LEAF) {
         leaf(ref, p, s);
         return;
     }
     const BvhNode& n = nodes[ref - node_base];
-    s.cost += p * 1.0;
-    s.bytes += p * double(sizeof(BvhNode));
+    if (depth % 2 == 0) {
+        s.cost += p * 1.0;
+        s.bytes += p * double(sizeof(BvhWide));
+    }
     auto area = [](float lx, float hx, float ly, float hy, float lz, float hz) {
         double dx = double(hx) - lx, dy = double(hy) - ly, dz = double(hz) - lz;
         return 2.0 * (dx * dy + dy * dz + dz * dx);
@@ -223,8 +227,8 @@ void subtree_stats(const std::vector<BvhNode>& nodes, uint32_t ref, uint32_t nod
     double uz = double(std::max(n.n2[1], n.n2[3])) - std::min(n.n2[0], n.n2[2]);
     parent = std::max(parent, 2.0 * (ux * uy + uy * uz + uz * ux));
     double q0 = parent > 0 ? a0 / parent : 1.0, q1 = parent > 0 ? a1 / parent : 1.0;
-    subtree_stats(nodes, n.n3[0], node_base, p * q0, leaf, s);
-    subtree_stats(nodes, n.n3[1], node_base, p * q1, leaf, s);
+    subtree_stats(nodes, n.n3[0], node_base, p * q0, leaf, s, depth + 1);
+    subtree_stats(nodes, n.n3[1], node_base, p * q1, leaf, s, depth + 1);
 }
 
 struct Blas {

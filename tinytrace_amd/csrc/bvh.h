@@ -23,6 +23,17 @@
 // Node references (u32): bit 31 clear = inner node index; set = leaf.  TLAS
 // leaves hold one instance (bits 0..30); BLAS leaves hold a triangle range:
 // first slot in bits 0..26, count-1 in bits 27..30 (count <= 16).
+//
+// The builders produce these binary nodes; the scene then collapses them on
+// the device into the 4-wide nodes traversal reads (bvh_wide.hip):
+//   wide  : BvhWide[]  128 B, one cache line: the four children's boxes as
+//                      SoA (lo.x[4], hi.x[4], lo.y[4], hi.y[4], lo.z[4],
+//                      hi.z[4]) and their four references.  Every binary
+//                      node at even depth below its tree's root becomes one
+//                      wide node whose children are its grandchildren (or
+//                      its children where those are leaves).  Boxes are
+//                      copied unchanged; unused slots hold +inf boxes, which
+//                      no ray enters, and the reference 0xffffffff.
 #pragma once
 
 #include <cstdint>
@@ -40,6 +51,15 @@ struct BvhNode {
     uint32_t n3[4];  // ref0, ref1, 0, 0
 };
 static_assert(sizeof(BvhNode) == 64, "node size");
+
+struct BvhWide {
+    float lo_x[4], hi_x[4];
+    float lo_y[4], hi_y[4];
+    float lo_z[4], hi_z[4];
+    uint32_t child[4];
+    uint32_t pad[4];
+};
+static_assert(sizeof(BvhWide) == 128, "wide node size");
 
 struct BvhTri {
     float v0[4];
@@ -62,6 +82,10 @@ constexpr uint32_t BVH_MAX_LEAF = 16;
 // by BVH_MEDIAN_DEPTH + 27 and so the traversal stack.
 constexpr uint32_t BVH_MEDIAN_DEPTH = 20;
 constexpr uint32_t BVH_STACK_LIMIT = 96;
+// The 4-wide traversal pushes at most 3 references per wide level; a
+// two-level path of binary depth D <= BVH_STACK_LIMIT - 2 has at most
+// D / 2 + 2 wide levels, plus the instance sentinel.
+constexpr uint32_t BVH_WIDE_STACK_LIMIT = 3 * ((BVH_STACK_LIMIT - 2) / 2 + 2) + 1;  // 148
 
 struct BvhBuildResult {
     std::vector<BvhNode> nodes;
@@ -76,7 +100,10 @@ struct BvhBuildResult {
     std::string error;
 };
 
-// Host binned-SAH build of every BLAS and the TLAS.
+// Host binned-SAH build of every BLAS and the TLAS.  The statistics
+// (sah_cost, expected_bytes) describe the 4-wide tree the device makes of
+// it: binary nodes at even depth cost one 128-B wide node visit, those at
+// odd depth nothing.
 bool build_bvh(const tt_scene_desc& desc, BvhBuildResult& out);
 
 // One mesh's BLAS as the TLAS build needs it.

@@ -396,8 +396,10 @@ tt_status build_blas_device(tt_context* c, const tt_scene_desc& d, const int16_t
         double root_area = 2.0 * (dx * dy + dy * dz + dz * dx);
         double ai = h_area[m], al = h_area[meshes + m];
         if (root_area > 0.0) {
-            b.cost = (ai + al) / root_area;
-            b.bytes = (ai * double(sizeof(BvhNode)) + al * double(sizeof(BvhTri))) / root_area;
+            b.cost = (0.5 * ai + al) / root_area;
+            // Wide nodes are the even-depth half of the binary ones; their
+            // area is taken as half of the binary nodes' total.
+            b.bytes = (0.5 * ai * double(sizeof(BvhWide)) + al * double(sizeof(BvhTri))) / root_area;
         }
     }
     return done(TT_OK);

@@ -35,11 +35,13 @@ constexpr uint32_t SENTINEL = 0xffffffffu;
 constexpr int TRACE_BLOCK = TT_TRACE_BLOCK;
 constexpr int TRACE_LDS_STACK = TT_TRACE_LDS_STACK;
 // LDS entries + spill entries cover the deepest two-level path the builder
-// allows (bvh.h BVH_STACK_LIMIT = 96).
-constexpr int TRACE_SPILL_STACK = 96 - TRACE_LDS_STACK;
+// allows: 3 pending children per wide level plus the instance sentinel
+// (bvh.h BVH_WIDE_STACK_LIMIT = 148).
+constexpr int TRACE_STACK_LIMIT = 148;
+constexpr int TRACE_SPILL_STACK = TRACE_STACK_LIMIT - TRACE_LDS_STACK;
 
 struct SceneView {
-    const float4* nodes;      // BvhNode, 4 x float4
+    const float4* nodes;      // BvhWide, 8 x float4
     const float4* tris;       // BvhTri, 3 x float4
     const float4* bvh_insts;  // BvhInstance, 4 x float4
     const float4* inst_boxes; // padded world box per instance, 2 x float4
@@ -123,6 +125,19 @@ TT_DEV bool box_hit(const TraceRay& r, float lx, float hx, float ly, float hy, f
     float nc = __builtin_fmaf(-abs_(n), 1e-6f, n);
     float fc = __builtin_fmaf(abs_(f), 1e-6f, f);
     return fmaxf(nc, tmin) <= fminf(fc, tmax);
+}
+
+constexpr float INF_F = __builtin_huge_valf();
+
+// Compare-exchange of (key, reference) pairs: the smaller key first.
+TT_DEV void sort_pair(float& ka, uint32_t& ra, float& kb, uint32_t& rb) {
+    bool swap = kb < ka;
+    float k = swap ? kb : ka;
+    kb = swap ? ka : kb;
+    ka = k;
+    uint32_t v = swap ? rb : ra;
+    rb = swap ? ra : rb;
+    ra = v;
 }
 
 TT_DEV bool closer(float t, uint32_t inst, uint32_t prim, const HitRecord& b) {
@@ -237,25 +252,39 @@ TT_DEV bool trace_closest(const SceneView& s, f3 org, f3 dir, float tmin, float 
     stack.sp = 0;
     for (;;) {
         if (!(cur & LEAF)) {
-            const float4* n = s.nodes + 4 * size_t(cur);
-            float4 n0 = n[0], n1 = n[1], n2 = n[2];
-            uint4 n3 = *reinterpret_cast<const uint4*>(n + 3);
+            // Wide node: SoA boxes of four children, then their references.
+            const float4* n = s.nodes + 8 * size_t(cur);
+            float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
+            uint4 ch = *reinterpret_cast<const uint4*>(n + 6);
             float limit = found ? best.t : tmax;
-            float tn0, tn1;
-            bool h0 = box_hit(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, tmin, limit, tn0);
-            bool h1 = box_hit(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, tmin, limit, tn1);
-            if (h0 && h1) {
-                bool first0 = tn0 <= tn1;
-                stack.push(first0 ? n3.y : n3.x);
-                cur = first0 ? n3.x : n3.y;
-                continue;
+            float k0, k1, k2, k3;
+            // Sort keys: entry distance of a hit child (made finite), +inf
+            // for a miss or an unused slot (reference SENTINEL, only in
+            // slots 2 and 3; its box is not trusted to miss, since the
+            // conservative widening can accept a box at infinity).
+            uint32_t r0 = ch.x, r1 = ch.y, r2 = ch.z, r3 = ch.w;
+            {
+                float tn;
+                bool h0 = box_hit(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tmin, limit, tn);
+                k0 = h0 ? fminf(tn, 3.0e38f) : INF_F;
+                bool h1 = box_hit(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tmin, limit, tn);
+                k1 = h1 ? fminf(tn, 3.0e38f) : INF_F;
+                bool h2 = box_hit(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tmin, limit, tn) && r2 != SENTINEL;
+                k2 = h2 ? fminf(tn, 3.0e38f) : INF_F;
+                bool h3 = box_hit(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tmin, limit, tn) && r3 != SENTINEL;
+                k3 = h3 ? fminf(tn, 3.0e38f) : INF_F;
             }
-            if (h0) {
-                cur = n3.x;
-                continue;
-            }
-            if (h1) {
-                cur = n3.y;
+            // Five compare-exchanges sort the four (key, reference) pairs.
+            sort_pair(k0, r0, k1, r1);
+            sort_pair(k2, r2, k3, r3);
+            sort_pair(k0, r0, k2, r2);
+            sort_pair(k1, r1, k3, r3);
+            sort_pair(k1, r1, k2, r2);
+            if (k3 != INF_F) stack.push(r3);
+            if (k2 != INF_F) stack.push(r2);
+            if (k1 != INF_F) stack.push(r1);
+            if (k0 != INF_F) {
+                cur = r0;
                 continue;
             }
         } else if (!in_blas) {

@@ -19,9 +19,12 @@
 // Path state is SoA in HBM: throughput/radiance float4, generator u32,
 // queues float4 (origin.xyz + path index bits, direction.xyz); every access
 // is one 16-B load or store per lane, consecutive across the wave.
+#include "bvh.h"
 #include "device_surface.h"
 
 using namespace ttd;
+
+static_assert(TRACE_STACK_LIMIT == int(tt::BVH_WIDE_STACK_LIMIT), "traversal stack covers the deepest wide path");
 
 namespace {
 

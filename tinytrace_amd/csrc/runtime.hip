@@ -310,11 +310,34 @@ tt_status tt_scene_create_ex(tt_context* c, const tt_scene_desc* d, const tt_sce
     }
     up(bvh.instances.data(), bvh.instances.size() * sizeof(tt::BvhInstance), &s->view.bvh_insts);
     up(bvh.instance_boxes.data(), bvh.instance_boxes.size() * sizeof(float), &s->view.inst_boxes);
+    // Traversal reads 4-wide nodes: collapse the binary ones (renumbering
+    // the instance records' BLAS roots and the TLAS root), then free them.
+    uint32_t tlas_root = bvh.empty ? 0xffffffffu : bvh.tlas_root;
+    if (st == TT_OK) {
+        const void* binary = s->view.nodes;
+        tt::BvhWide* wide = nullptr;
+        uint32_t wide_count = 0;
+        st = tt::collapse_wide(c, static_cast<const tt::BvhNode*>(binary), binary ? node_count : 0u,
+                               const_cast<void*>(s->view.bvh_insts), uint32_t(bvh.instances.size()), tlas_root, &wide,
+                               wide_count);
+        if (st == TT_OK) {
+            auto it = std::find(s->allocations.begin(), s->allocations.end(), binary);
+            if (it != s->allocations.end()) {
+                s->allocations.erase(it);
+                (void)hipFree(const_cast<void*>(binary));
+            }
+            if (wide) s->allocations.push_back(wide);
+            s->view.nodes = wide;
+            s->info.device_bytes -= size_t(node_count) * sizeof(tt::BvhNode);
+            s->info.device_bytes += size_t(wide_count) * sizeof(tt::BvhWide);
+            node_count = wide_count;
+        }
+    }
     if (st != TT_OK) {
         tt_scene_destroy(s);
         return st;
     }
-    s->view.tlas_root = bvh.empty ? 0xffffffffu : bvh.tlas_root;
+    s->view.tlas_root = tlas_root;
     for (int a = 0; a < 3; a++) {
         float lo = INFINITY, hi = -INFINITY;
         for (size_t i = 0; i < bvh.instance_boxes.size(); i += 8) {

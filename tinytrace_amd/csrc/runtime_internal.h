@@ -95,6 +95,13 @@ struct BlasSummary;
 tt_status build_blas_device(tt_context* c, const tt_scene_desc& d, const int16_t* positions, const uint32_t* indices,
                             BvhNode* nodes, BvhTri* tris, std::vector<BlasSummary>& summary, uint32_t& node_count);
 
+// 4-wide nodes from the binary ones (bvh_wide.hip): renumbers the BLAS
+// roots of the device instance records and tlas_root; *out is a new device
+// allocation of wide_count nodes (nullptr when count == 0).
+struct BvhWide;
+tt_status collapse_wide(tt_context* c, const BvhNode* nodes, uint32_t count, void* instances,
+                        uint32_t instance_count, uint32_t& tlas_root, BvhWide** out, uint32_t& wide_count);
+
 // Radiance-cache MLP (mlp.hip).
 tt_context* mlp_context(const tt_mlp* m);
 tt_status mlp_set_scene_bounds(tt_mlp* m, const float lo[3], const float hi[3]);
