@@ -162,10 +162,17 @@ TT_DEV bool triangle_hit(const TraceRay& r, f3 v0, f3 v1, f3 v2, float tmin, flo
     float U = Cx * By - Cy * Bx;
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
-    if (U == 0.0f || V == 0.0f || W == 0.0f) {
-        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
-        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
-        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    // Double-precision recomputation on an exact zero.  Without culling a
+    // wave-uniform branch keeps it off the common path (as a per-lane branch
+    // the compiler evaluates it for every triangle and selects); with
+    // culling the per-lane form measured faster.
+    bool degenerate = U == 0.0f || V == 0.0f || W == 0.0f;
+    if (CULL || __builtin_amdgcn_ballot_w64(degenerate) != 0ull) {
+        if (degenerate) {
+            U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+            V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+            W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+        }
     }
     if (CULL) {
         if (U < 0.0f || V < 0.0f || W < 0.0f) return false;
