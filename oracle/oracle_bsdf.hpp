@@ -319,14 +319,14 @@ inline R average_dielectric_fresnel(R ior) {
 }
 
 // --------------------------------------------------------------- diffuse --
-// diffuse_brdf.glsl:9-16
+// diffuse_brdf.glsl:9-16; the GLSL's pow(s, 3.0) is evaluated as s * s * s.
 template <typename R>
 inline R oren_nayar(R c, R s2) {
     R a = R(1) - R(0.5) * (s2 / (s2 + R(0.33)));
     R b = R(0.45) * s2 / (s2 + R(0.09));
     R s = Fn<R>::sqrt(R(1) - pow2(c));
     R g = s * (Fn<R>::acos(c) - s * c) +
-        (s / rmax(R(1e-7), c)) * (R(1) - Fn<R>::pow(s, R(3))) * R(2) / R(3);
+        (s / rmax(R(1e-7), c)) * (R(1) - s * s * s) * R(2) / R(3);
     return a + (b * K<R>::INVERSE_PI) * g;
 }
 

@@ -236,38 +236,48 @@ TT_DEV float average_dielectric_fresnel(float ior) {
 }
 
 // --------------------------------------------------------------- diffuse --
+// diffuse_brdf.glsl:9-16; the GLSL's pow(s, 3.0) is evaluated as s * s * s.
 TT_DEV float oren_nayar(float c, float s2) {
     float a = 1.0f - 0.5f * (s2 / (s2 + 0.33f));
     float b = 0.45f * s2 / (s2 + 0.09f);
     float s = sqrt_(1.0f - pow2(c));
-    float g = s * (ttm::acos(c) - s * c) + (s / fmax_(1e-7f, c)) * (1.0f - ttm::pow(s, 3.0f)) * 2.0f / 3.0f;
+    float g = s * (ttm::acos(c) - s * c) + (s / fmax_(1e-7f, c)) * (1.0f - s * s * s) * 2.0f / 3.0f;
     return a + (b * INVERSE_PI) * g;
 }
-TT_DEV f3 energy_compensated_oren_nayar(f3 rho, float sigma, f3 wi, f3 wo) {
+// The factor max(1e-7, 1 - oren_nayar) of one direction in the energy
+// compensation; the incoming direction's is computed once per shading point
+// (Lobes::diffuse_wi) and reused by every evaluation there.
+TT_DEV float oren_nayar_complement(const Mat& m, f3 w) {
+    float s2 = pow2(PI / 2.0f * m.base_diffuse_roughness);
+    return fmax_(1e-7f, 1.0f - oren_nayar(cos_theta(w), s2));
+}
+// diffuse_brdf.glsl:18-32, with wi_term = oren_nayar_complement(m, wi).
+TT_DEV f3 energy_compensated_oren_nayar(f3 rho, float sigma, f3 wi, f3 wo, float wi_term) {
     float s2 = pow2(sigma);
     float s = dot(wi, wo) - cos_theta(wi) * cos_theta(wo);
     float s_over_t = s > 0.0f ? s / fmax_(cos_theta(wi), cos_theta(wo)) : 0.0f;
     float a = 1.0f - 0.5f * (s2 / (s2 + 0.33f));
     float b = 0.45f * s2 / (s2 + 0.09f);
     float on_o = oren_nayar(cos_theta(wo), s2);
-    float on_i = oren_nayar(cos_theta(wi), s2);
     float average_albedo = a + (2.0f / 3.0f - 64.0f / (45.0f * PI)) * b;
     f3 rho_ms = rho * rho * average_albedo / (1.0f - rho * fmax_(0.0f, 1.0f - average_albedo));
     return (rho * INVERSE_PI) * (a + b * s_over_t) +
-        (rho_ms * INVERSE_PI) * fmax_(1e-7f, 1.0f - on_o) * fmax_(1e-7f, 1.0f - on_i) / fmax_(1e-7f, 1.0f - average_albedo);
+        (rho_ms * INVERSE_PI) * fmax_(1e-7f, 1.0f - on_o) * wi_term / fmax_(1e-7f, 1.0f - average_albedo);
 }
-TT_DEV f3 diffuse_brdf_evaluate(const Mat& m, f3 wi, f3 wo, float& density) {
+TT_DEV f3 diffuse_brdf_evaluate(const Mat& m, f3 wi, f3 wo, float wi_term, float& density) {
     if (cos_theta(wi) < DENOM_TOLERANCE || cos_theta(wo) < DENOM_TOLERANCE) return f3();
     density = cosine_hemisphere_density(cos_theta(wo));
-    return energy_compensated_oren_nayar(m.base_color * m.base_weight, PI / 2.0f * m.base_diffuse_roughness, wi, wo);
+    return energy_compensated_oren_nayar(m.base_color * m.base_weight, PI / 2.0f * m.base_diffuse_roughness, wi, wo,
+                                         wi_term);
 }
-TT_DEV f3 diffuse_brdf_sample(const Mat& m, f3 wi, Generator& g, f3& wo, float& density, bool& ok) {
+TT_DEV f3 diffuse_brdf_sample(const Mat& m, f3 wi, float wi_term, Generator& g, f3& wo, float& density, bool& ok) {
     ok = false;
     if (cos_theta(wi) < DENOM_TOLERANCE) return f3();
     wo = cosine_hemisphere_sample(random_vec2(g));
     ok = true;
     density = cosine_hemisphere_density(cos_theta(wo));
-    return energy_compensated_oren_nayar(m.base_color * m.base_weight, PI / 2.0f * m.base_diffuse_roughness, wi, wo);
+    return energy_compensated_oren_nayar(m.base_color * m.base_weight, PI / 2.0f * m.base_diffuse_roughness, wi, wo,
+                                         wi_term);
 }
 TT_DEV f3 diffuse_brdf_albedo(const Mat& m, f3 wi) {
     if (cos_theta(wi) < DENOM_TOLERANCE) return f3();
@@ -589,6 +599,7 @@ enum { FUZZ = 0, COAT = 1, METAL = 2, SPECULAR_BRDF = 3, SPECULAR_BTDF = 4, DIFF
 struct Lobes {
     f3 weights[LOBE_COUNT];
     float probs[LOBE_COUNT];
+    float diffuse_wi;  // oren_nayar_complement(m, wi) when the diffuse lobe is active
 };
 
 // bsdf.glsl:33-113 (the albedos are only needed while preparing).
@@ -603,6 +614,7 @@ TT_DEV Lobes bsdf_prepare(const Mat& m, f3 wi, Generator& g) {
     albedos[SPECULAR_BRDF] = !fully_metallic ? specular_brdf_albedo(m, wi, g) : f3();
     albedos[SPECULAR_BTDF] = !fully_metallic && m.transmission_weight > 0.0f ? specular_btdf_albedo(m, wi, g) : f3();
     albedos[DIFFUSE] = !fully_metallic && !fully_transmissive ? diffuse_brdf_albedo(m, wi) : f3();
+    l.diffuse_wi = !fully_metallic && !fully_transmissive ? oren_nayar_complement(m, wi) : 1e-7f;
 
     l.weights[FUZZ] = f3(m.fuzz_weight);
     f3 coated_base_weight = mixv(f3(1.0f), 1.0f - albedos[FUZZ], m.fuzz_weight);
@@ -649,7 +661,7 @@ TT_DEV f3 bsdf_evaluate_lobes(const Mat& m, const Lobes& l, f3 wi, f3 wo, int sk
     if (skip != SPECULAR_BRDF && l.probs[SPECULAR_BRDF] > 0.0f)
         bsdf = bsdf + l.weights[SPECULAR_BRDF] * specular_brdf_evaluate(m, wi, wo, d[SPECULAR_BRDF]);
     if (skip != DIFFUSE && l.probs[DIFFUSE] > 0.0f)
-        bsdf = bsdf + l.weights[DIFFUSE] * diffuse_brdf_evaluate(m, wi, wo, d[DIFFUSE]);
+        bsdf = bsdf + l.weights[DIFFUSE] * diffuse_brdf_evaluate(m, wi, wo, l.diffuse_wi, d[DIFFUSE]);
     if (skip != SPECULAR_BTDF && l.probs[SPECULAR_BTDF] > 0.0f)
         bsdf = bsdf + l.weights[SPECULAR_BTDF] * specular_btdf_evaluate(m, wi, wo, d[SPECULAR_BTDF]);
     return bsdf;
@@ -691,7 +703,7 @@ TT_DEV bool bsdf_sample(const Mat& m, const Lobes& l, f3 wi, Generator& g, f3& w
         case METAL: lobe_bsdf = metal_brdf_sample(m, wi, g, wo, lobe_density, ok); break;
         case SPECULAR_BRDF: lobe_bsdf = specular_brdf_sample(m, wi, g, wo, lobe_density, ok); break;
         case SPECULAR_BTDF: lobe_bsdf = specular_btdf_sample(m, wi, g, wo, lobe_density, ok); break;
-        default: lobe_bsdf = diffuse_brdf_sample(m, wi, g, wo, lobe_density, ok); break;
+        default: lobe_bsdf = diffuse_brdf_sample(m, wi, l.diffuse_wi, g, wo, lobe_density, ok); break;
     }
     if (!ok) return false;
     float d[LOBE_COUNT] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
