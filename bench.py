@@ -43,6 +43,9 @@ def main() -> int:
     p.add_argument("--serving-rate", type=float, default=5.0, help="Poisson arrival rate of that leg, requests/s")
     p.add_argument("--cache-frames", type=int, default=16,
                    help="frames of the radiance-cache leg on the Cornell box (0 skips it)")
+    p.add_argument("--c5-frames", type=int, default=4,
+                   help="frames of the instanced-scene leg (3840x2160, depth 12, ReSTIR, with and without the "
+                        "radiance cache); 0 skips it")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="also run a plain device copy of this many MiB (reference kernel for HBM counters)")
     args = p.parse_args()
@@ -168,6 +171,9 @@ def main() -> int:
     if args.cache_frames > 0:
         result["radiance_cache"] = radiance_cache(args.cache_frames)
 
+    if args.c5_frames > 0:
+        result["instanced"] = instanced(args.c5_frames)
+
     if args.copy_calibration_mib:
         # A device-to-device hipMemcpy of known size runs as the runtime's
         # copy kernel: the reference point for the HBM byte counters.
@@ -288,6 +294,64 @@ def radiance_cache(frames):
                        "training_loss": round(stats.get("loss", 0.0), 5)},
             "equal_time": {"ms": round(frames * cached_ms, 3), "rmse_cached": cached_rmse[str(frames)],
                            "plain_frames": equal_frames, "rmse_plain": plain_rmse[str(equal_frames)]}}
+
+
+def instanced(frames):
+    """The instanced scene (64 displaced icosphere/torus meshes of 5k-200k
+    triangles, 12 017 instances = 597 M instanced triangles, 4096 emissive)
+    at 3840x2160, 1 spp, depth 12, ReSTIR DI: `frames` timed frames plain and
+    `frames` with the radiance cache (65536 training paths and 4 Adam steps
+    per frame, after 8 warm-up frames)."""
+    from tinytrace_amd import Config, LightSampling, Renderer, RenderSettings, _abi, scenes
+    from tinytrace_amd.cache import RadianceCache, render_cached
+
+    width, height, depth = 3840, 2160, 12
+    scene, camera = scenes.instanced_field(seed=0)
+    settings = RenderSettings(rng=_abi.TT_RNG_PCG, seed=0, restir_candidates=32, collect_stats=True)
+    renderer = Renderer((width, height), settings=settings)
+    t0 = time.perf_counter()
+    renderer.set_scene(scene)
+    build_s = time.perf_counter() - t0
+    info = renderer.scene.info()
+    renderer.camera = camera
+    renderer.set_config(Config(bounce_count=depth, sample_count=1, light_sampling=LightSampling.RESTIR))
+
+    def sync():
+        renderer.target.download()
+
+    def timed(step):
+        sync()
+        t = time.perf_counter()
+        rays = 0
+        for _ in range(frames):
+            step()
+            st = renderer.stats()
+            rays += st["extension_rays"] + st["shadow_rays"] + st["restir_rays"]
+        sync()
+        return (time.perf_counter() - t) / frames * 1e3, rays / frames
+
+    renderer.render_to_target()
+    plain_ms, plain_rays = timed(renderer.render_to_target)
+    cache = RadianceCache(renderer.context, seed=0)
+    for _ in range(8):
+        render_cached(renderer, cache)
+    stats = {}
+
+    def cached_step():
+        stats.update(render_cached(renderer, cache))
+
+    cached_ms, cached_rays = timed(cached_step)
+    cache.close()
+    paths = width * height
+    return {"workload": f"instanced field seed 0 ({info['unique_triangle_count']} unique / "
+                        f"{info['triangle_count']} instanced triangles), {width}x{height}, 1 spp, depth {depth}, "
+                        "ReSTIR DI (32 candidates, 1 temporal + 1 spatial reuse), PCG seed 0",
+            "scene_build_s": round(build_s, 3), "bvh_nodes": info["node_count"],
+            "plain": {"ms_per_frame": round(plain_ms, 3), "mpaths_per_s": round(paths / plain_ms / 1e3, 2),
+                      "mrays_per_s": round(plain_rays / plain_ms / 1e3, 2)},
+            "cached": {"ms_per_frame": round(cached_ms, 3), "mpaths_per_s": round(paths / cached_ms / 1e3, 2),
+                       "mrays_per_s": round(cached_rays / cached_ms / 1e3, 2),
+                       "queries_per_frame": stats.get("queries"), "training_samples_per_frame": stats.get("samples")}}
 
 
 def serving(context, requests, rate):
