@@ -505,7 +505,7 @@ typedef struct tt_mesh_desc {
     const float* texture_coordinates; /* 2 per vertex */
     const uint32_t* indices;          /* optional */
     const float* normals;             /* optional, 3 per vertex */
-    const float* tangents;            /* required here, 4 per vertex */
+    const float* tangents;            /* optional, 4 per vertex (generated when absent) */
     uint32_t vertex_count;
     uint32_t index_count;
 } tt_mesh_desc;
@@ -557,6 +557,16 @@ tt_status tt_asset_add_node(tt_asset_scene* scene, uint32_t parent, const float 
                             uint32_t* index);
 tt_status tt_asset_add_model(tt_asset_scene* scene, uint32_t node, uint32_t mesh,
                              uint32_t material);
+/* Replaces GltfImporter::new(path) + insert_in_scene (src/asset/gltf.rs:50-72,
+ * 419-452): appends the default scene (else the first) of a .gltf file with
+ * external or base64 data-URI buffers under the root node, as the reference
+ * does (meshes per primitive, materials per index, node tree depth first).
+ * TT_UNSUPPORTED for binary .glb, textures, non-triangle primitives and
+ * accessor formats the reference rejects; TT_INVALID_ARGUMENT for unreadable
+ * files and malformed documents.  On failure the scene may hold part of the
+ * import. */
+tt_status tt_asset_import_gltf(tt_asset_scene* scene, const char* path);
+
 /* Depth-first flatten into instances and emissive triangles, as
  * scene::Scene::new does (src/scene.rs:32-64).  The arrays stay valid until
  * the asset scene is modified or destroyed. */

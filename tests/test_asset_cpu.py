@@ -138,14 +138,16 @@ def test_depth_first_flatten_order():
     np.testing.assert_array_equal(order, [mats[2], mats[3], mats[1], mats[0]])
 
 
-def test_missing_tangents_is_unsupported():
+def test_missing_tangents_are_generated():
+    """generate_tangents (src/asset/mod.rs:244-262) stands in when a mesh has
+    none: a unit tangent orthogonal to the normal (tests/test_gltf_cpu.py
+    checks its direction and sign)."""
     s = AssetScene()
-    try:
-        s.insert_mesh(np.zeros((3, 3)) + np.eye(3), np.zeros((3, 2)), [0, 1, 2], None, None)
-    except _abi.TinytraceError as e:
-        assert e.status == 8
-    else:
-        raise AssertionError("expected TT_UNSUPPORTED")
+    s.insert_mesh(np.zeros((3, 3)) + np.eye(3), np.zeros((3, 2)), [0, 1, 2], None, None)
+    frame = np.zeros(7, dtype=np.float32)
+    binding.lib().oracle_decode_tangent_frame(int(s.arrays()["vertices"]["tangent_frame"][0]), frame.ctypes.data)
+    t = frame[3:6] / np.linalg.norm(frame[3:6])
+    assert abs(float(np.dot(t, frame[:3]))) < 1e-2
 
 
 def test_camera_constants_closed_form():

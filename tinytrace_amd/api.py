@@ -17,6 +17,7 @@ import ctypes as C
 import dataclasses
 import enum
 import math
+import os
 from typing import List, Optional, Sequence
 
 import numpy as np
@@ -159,6 +160,24 @@ class Material:
         return m
 
 
+class GltfImporter:
+    """gltf::GltfImporter (src/asset/gltf.rs:44-72, 419-452): a .gltf file
+    (external or base64 data-URI buffers) imported into an AssetScene."""
+
+    def __init__(self, path):
+        self.path = os.fspath(path)
+        if not os.path.isfile(self.path):
+            raise FileNotFoundError(self.path)
+
+    def insert_in_scene(self, scene: "AssetScene") -> None:
+        scene.import_gltf(self.path)
+
+    def new_scene(self) -> "AssetScene":
+        scene = AssetScene()
+        self.insert_in_scene(scene)
+        return scene
+
+
 class AssetScene:
     """asset::Scene: processed meshes, materials and a node tree (node 0 = root)."""
 
@@ -213,6 +232,11 @@ class AssetScene:
 
     def add_model(self, node: int, mesh: int, material: int) -> None:
         check(lib().tt_asset_add_model(self._handle, node, mesh, material), None, "add_model")
+        self._desc = None
+
+    def import_gltf(self, path) -> None:
+        """Append a .gltf file's default scene under the root (GltfImporter::insert_in_scene)."""
+        check(lib().tt_asset_import_gltf(self._handle, os.fsencode(path)), None, "tt_asset_import_gltf")
         self._desc = None
 
     def flatten(self) -> _abi.SceneDesc:

@@ -6,6 +6,7 @@
 //   emissive_triangles_in_model src/asset/mod.rs:163-193
 //   depth_first_node_tree       src/asset/mod.rs:207-224
 //   generate_normals            src/asset/mod.rs:226-242
+//   generate_tangents           src/asset/mod.rs:244-262 (see insert_mesh)
 //   bounding_sphere             src/asset/mod.rs:308-321
 //   TangentFrame::new           src/asset/normal.rs:18-36
 //   angle_around_normal         src/asset/normal.rs:38-52
@@ -248,7 +249,6 @@ void tt_asset_scene_destroy(tt_asset_scene* scene) { delete scene; }
 
 tt_status tt_asset_insert_mesh(tt_asset_scene* scene, const tt_mesh_desc* mesh, uint32_t* index) {
     if (!scene || !mesh || !mesh->positions || !mesh->texture_coordinates) return TT_INVALID_ARGUMENT;
-    if (!mesh->tangents) return TT_UNSUPPORTED;  // mikktspace generation is not part of this library
     uint32_t vc = mesh->vertex_count;
     if (vc == 0) return TT_INVALID_ARGUMENT;
     if (mesh->indices) {
@@ -306,12 +306,57 @@ tt_status tt_asset_insert_mesh(tt_asset_scene* scene, const tt_mesh_desc* mesh, 
         for (auto& n : normals) n = normalize(n);
     }
 
+    // generate_tangents (src/asset/mod.rs:244-262) when absent.  The
+    // reference runs MikkTSpace (mikktspace 0.3.0) and negates the sign it
+    // returns.  This restates MikkTSpace's result for smooth, non-degenerate
+    // texture mappings: per vertex, the sum of the triangles' tangents and
+    // bitangents from the texture-coordinate derivatives (Lengyel 2001),
+    // Gram-Schmidt against the normal, sign = +1 when cross(n, t) points
+    // along the bitangent, then negated as the reference does.  Where the
+    // texture coordinates span no area (e.g. none were given) the tangent is
+    // orthonormal(n) with the negated sign -1.
+    std::vector<float> generated;
+    const float* tangents = mesh->tangents;
+    if (!tangents) {
+        std::vector<Vec3> tsum(vc), bsum(vc);
+        const uint32_t* idx = scene->indices.data() + index_offset;
+        const float* uv = mesh->texture_coordinates;
+        for (uint32_t t = 0; t + 2 < index_count; t += 3) {
+            uint32_t i0 = idx[t], i1 = idx[t + 1], i2 = idx[t + 2];
+            Vec3 e1 = pos(i1) - pos(i0), e2 = pos(i2) - pos(i0);
+            float du1 = uv[2 * i1] - uv[2 * i0], dv1 = uv[2 * i1 + 1] - uv[2 * i0 + 1];
+            float du2 = uv[2 * i2] - uv[2 * i0], dv2 = uv[2 * i2 + 1] - uv[2 * i0 + 1];
+            float r = du1 * dv2 - du2 * dv1;
+            if (!(std::fabs(r) > 0.0f) || !std::isfinite(r)) continue;
+            float f = 1.0f / r;
+            Vec3 tv = (e1 * dv2 - e2 * dv1) * f, bv = (e2 * du1 - e1 * du2) * f;
+            for (uint32_t k : {i0, i1, i2}) {
+                tsum[k] = tsum[k] + tv;
+                bsum[k] = bsum[k] + bv;
+            }
+        }
+        generated.resize(4 * size_t(vc));
+        for (uint32_t i = 0; i < vc; i++) {
+            Vec3 n = normals[i];
+            Vec3 t = tsum[i] - n * dot(n, tsum[i]);
+            float len = length(t);
+            bool ok = len > 1e-20f && std::isfinite(len);
+            t = ok ? t / len : orthonormal(n);
+            float sign = ok && dot(cross(n, t), bsum[i]) < 0.0f ? -1.0f : 1.0f;
+            generated[4 * size_t(i) + 0] = t.x;
+            generated[4 * size_t(i) + 1] = t.y;
+            generated[4 * size_t(i) + 2] = t.z;
+            generated[4 * size_t(i) + 3] = -sign;
+        }
+        tangents = generated.data();
+    }
+
     scene->positions.insert(scene->positions.end(), quantized.begin(), quantized.end());
     for (uint32_t i = 0; i < vc; i++) {
         tt_vertex v;
         v.texture_coordinates[0] = float_to_half(mesh->texture_coordinates[2 * i]);
         v.texture_coordinates[1] = float_to_half(mesh->texture_coordinates[2 * i + 1]);
-        v.tangent_frame = tangent_frame(normals[i], mesh->tangents + 4 * size_t(i));
+        v.tangent_frame = tangent_frame(normals[i], tangents + 4 * size_t(i));
         scene->vertices.push_back(v);
     }
     tt_mesh_record rec;
