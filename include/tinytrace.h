@@ -269,11 +269,21 @@ tt_status tt_render_ex(tt_context* context, const tt_scene* scene, const tt_cons
 tt_status tt_target_download(tt_context* context, const tt_target* target, float* rgba,
                              size_t bytes);
 
+/* Replaces Renderer::prepare_to_present's post-process dispatch
+ * (src/lib.rs:180-193, src/post_process/display.glsl:6-12) for headless use:
+ * the target's radiance through neutral_tonemap when constants->tonemap != 0
+ * (src/post_process/tonemap.glsl:1-26), sRGB-encoded and rounded to RGBA8
+ * (alpha 255) as a store to an sRGB swapchain image would, copied to `rgba8`
+ * (width*height*4 bytes; blocking).  Timed as "post_process". */
+tt_status tt_target_display(tt_context* context, const tt_target* target, const tt_constants* constants,
+                            uint8_t* rgba8, size_t bytes);
+
 /* Named GPU timings in milliseconds, mirroring Renderer::timings
  * (src/lib.rs:167-178) and Context::insert_timestamp
  * (crates/tinytrace_backend/src/timing.rs:61-69).  Names: "integrate" (last
- * render); with TT_RENDER_COLLECT_STATS also "trace" (sum over the last
- * render's extension and shadow closest-hit launches). */
+ * render), "post_process" (last tt_target_display); with
+ * TT_RENDER_COLLECT_STATS also "trace" (sum over the last render's extension
+ * and shadow closest-hit launches). */
 tt_status tt_timing_ms(tt_context* context, const char* name, double* out);
 
 typedef struct tt_render_stats {

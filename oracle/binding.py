@@ -67,6 +67,7 @@ def lib():
         L.oracle_sample_triangle.argtypes = [f32, f32, vp]
         L.oracle_cosine_hemisphere_sample.argtypes = [f32, f32, vp]
         L.oracle_decode_tangent_frame.argtypes = [u32, vp]
+        L.oracle_display.argtypes = [vp, u32, u32, vp]
         _lib = L
     return _lib
 
@@ -177,4 +178,12 @@ def math_f32(op, x, y=None):
     y = np.zeros_like(x) if y is None else np.ascontiguousarray(y, dtype=np.float32)
     out = np.zeros_like(x)
     lib().oracle_math_f32(MATH_OPS[op], x.ctypes.data, y.ctypes.data, out.ctypes.data, x.size)
+    return out
+
+
+def display(rgba, tonemap=True):
+    """RGBA32F (H, W, 4) -> the display pass's RGBA8 (H, W, 4)."""
+    a = np.ascontiguousarray(rgba, dtype=np.float32)
+    out = np.zeros(a.shape[:-1] + (4,), dtype=np.uint8)
+    lib().oracle_display(a.ctypes.data, a.size // 4, 1 if tonemap else 0, out.ctypes.data)
     return out

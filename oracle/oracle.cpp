@@ -505,4 +505,40 @@ void oracle_decode_tangent_frame(uint32_t encoded, float* out) {
     std::memcpy(out, v, sizeof(v));
 }
 
+// Post-process display (src/post_process/display.glsl:6-12): neutral_tonemap
+// (src/post_process/tonemap.glsl:1-26) when `tonemap` != 0, then the sRGB
+// transfer function and 8-bit rounding of an sRGB store; RGBA8, alpha 255.
+void oracle_display(const float* rgba, uint32_t count, uint32_t tonemap, uint8_t* out) {
+    auto maxf = [](float a, float b) { return a < b ? b : a; };
+    auto curve = [](float v) { return 1.0f - ttm::exp(-v); };
+    auto mix = [](float x, float y, float a) { return x * (1.0f - a) + y * a; };
+    for (uint32_t i = 0; i < count; i++) {
+        float c[3] = {rgba[4 * i], rgba[4 * i + 1], rgba[4 * i + 2]};
+        if (tonemap) {
+            // rgb_to_ycbcr: color * mat3(columns) = dot with each column.
+            float y = c[0] * 0.2126f + c[1] * 0.7152f + c[2] * 0.0722f;
+            float cb = c[0] * -0.1146f + c[1] * -0.3854f + c[2] * 0.5f;
+            float cr = c[0] * 0.5f + c[1] * -0.4542f + c[2] * -0.0458f;
+            float bt = curve(ttm::sqrt(cb * cb + cr * cr) * 2.4f);
+            float desat = maxf((bt - 0.7f) * 0.8f, 0.0f);
+            desat = desat * desat;
+            float tm_lum = curve(y);
+            float lum = 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2];
+            float scale = maxf(0.0f, tm_lum / maxf(1e-5f, lum));
+            float w = bt * bt;
+            for (int k = 0; k < 3; k++) {
+                float tm0 = c[k] * scale;
+                float tm1 = curve(mix(c[k], y, desat));
+                c[k] = mix(tm0, tm1, w) * 0.97f;
+            }
+        }
+        for (int k = 0; k < 3; k++) {
+            float v = c[k] > 0.0f ? (c[k] < 1.0f ? c[k] : 1.0f) : 0.0f;
+            float srgb = v <= 0.0031308f ? 12.92f * v : 1.055f * ttm::pow(v, 1.0f / 2.4f) - 0.055f;
+            out[4 * i + k] = uint8_t(uint32_t(srgb * 255.0f + 0.5f));
+        }
+        out[4 * i + 3] = 255;
+    }
+}
+
 }  // extern "C"

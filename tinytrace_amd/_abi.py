@@ -234,6 +234,7 @@ _PROTOTYPES = [
     ("tt_asset_add_model", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
     ("tt_asset_flatten", C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
     ("tt_asset_import_gltf", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("tt_target_display", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Constants), C.c_void_p, C.c_size_t]),
 ]
 
 EXPORTED_SYMBOLS = [p[0] for p in _PROTOTYPES]

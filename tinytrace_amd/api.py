@@ -417,9 +417,20 @@ class Renderer:
         self.render_to_target()
         return self.target.download()
 
+    def display(self) -> np.ndarray:
+        """The post-process pass on the target (display.glsl, prepare_to_present):
+        neutral_tonemap when config.tonemap, sRGB-encoded RGBA8 (H, W, 4)."""
+        k = self.constants()
+        out = np.empty((self.target.height, self.target.width, 4), dtype=np.uint8)
+        check(lib().tt_target_display(self.context.handle, self.target.handle, C.byref(k), out.ctypes.data,
+                                      out.nbytes), self.context.handle, "tt_target_display")
+        return out
+
     def timings(self) -> dict:
+        """Renderer::timings (src/lib.rs:167-178) in milliseconds: integrate,
+        post_process (after display()) and, with collect_stats, trace."""
         out = {}
-        for name in ("integrate", "trace"):
+        for name in ("integrate", "post_process", "trace"):
             v = C.c_double()
             if lib().tt_timing_ms(self.context.handle, name.encode(), C.byref(v)) == _abi.TT_OK:
                 out[name] = v.value

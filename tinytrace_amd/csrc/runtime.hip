@@ -174,6 +174,9 @@ void context_free(tt_context* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->path_memory) (void)hipFree(c->path_memory);
     if (c->cache_memory) (void)hipFree(c->cache_memory);
+    if (c->display_memory) (void)hipFree(c->display_memory);
+    if (c->post_process.begin) (void)hipEventDestroy(c->post_process.begin);
+    if (c->post_process.end) (void)hipEventDestroy(c->post_process.end);
     if (c->counters) (void)hipFree(c->counters);
     if (c->work) (void)hipFree(c->work);
     if (c->spill) (void)hipFree(c->spill);
@@ -625,6 +628,14 @@ tt_status tt_timing_ms(tt_context* c, const char* name, double* out) {
     if (std::strcmp(name, "integrate") == 0) {
         float ms = 0.0f;
         TT_HIP(c, hipEventElapsedTime(&ms, c->integrate.begin, c->integrate.end));
+        *out = ms;
+        return TT_OK;
+    }
+    if (std::strcmp(name, "post_process") == 0) {
+        if (!c->post_process_recorded) return fail(c, TT_INVALID_ARGUMENT, "no tt_target_display yet");
+        TT_HIP(c, hipEventSynchronize(c->post_process.end));
+        float ms = 0.0f;
+        TT_HIP(c, hipEventElapsedTime(&ms, c->post_process.begin, c->post_process.end));
         *out = ms;
         return TT_OK;
     }

@@ -48,6 +48,12 @@ struct tt_context {
     // Radiance-cache queues, grown on demand.
     size_t cache_capacity = 0;
     void* cache_memory = nullptr;
+    // Display pass (tt_target_display): RGBA8 staging, grown on demand, and
+    // its timing events (created on first use).
+    size_t display_capacity = 0;
+    void* display_memory = nullptr;
+    tt::EventPair post_process;
+    bool post_process_recorded = false;
 };
 
 struct tt_scene {
