@@ -177,7 +177,34 @@ typedef struct tt_scene_desc {
     uint32_t instance_count;
     const tt_emissive_triangle* emissive_triangles;
     uint32_t emissive_triangle_count;
+    /* Processed textures (scene.textures, src/asset/texture.rs), indexed by
+     * tt_material.textures; none when texture_count == 0. */
+    const struct tt_texture_record* textures;
+    uint32_t texture_count;
+    const uint8_t* texture_data;
+    uint64_t texture_data_bytes;
 } tt_scene_desc;
+
+/* Texel formats of processed textures (ProcessedTexture::from_*,
+ * src/asset/texture.rs:57-119), stored uncompressed: colour as RGBA8 read
+ * through the sRGB curve (the reference's BC1 sRGB), normal maps as RG8
+ * octahedral (its BC5), scalars as R8 unorm or R16 float. */
+enum {
+    TT_TEXTURE_FORMAT_RGBA8_SRGB = 0,
+    TT_TEXTURE_FORMAT_RG8_UNORM = 1,
+    TT_TEXTURE_FORMAT_R8_UNORM = 2,
+    TT_TEXTURE_FORMAT_R16_FLOAT = 3
+};
+
+/* One processed texture: its mip chain (generate_mip_chain,
+ * texture.rs:121-143) in texture_data; level k is (width >> k) x
+ * (height >> k) texels, rows from the top (v = 0). */
+typedef struct tt_texture_record {
+    uint32_t format;
+    uint32_t width, height;
+    uint32_t levels;
+    uint32_t offsets[16]; /* byte offset of each level in texture_data */
+} tt_texture_record;
 
 /* ---------------------------------------------------------------------- */
 /* Context                                                                 */
@@ -554,6 +581,48 @@ typedef struct tt_material_desc {
 /* Defaults of asset::Material (src/asset/material.rs:35-175). */
 void tt_material_desc_default(tt_material_desc* out);
 
+/* Texture channels of a material: the fields of MaterialTextures in order
+ * (src/asset/material.rs:203-231), i.e. the order of tt_material.textures. */
+enum {
+    TT_TEXTURE_BASE_COLOR = 0,
+    TT_TEXTURE_SPECULAR_COLOR = 1,
+    TT_TEXTURE_TRANSMISSION_COLOR = 2,
+    TT_TEXTURE_COAT_COLOR = 3,
+    TT_TEXTURE_FUZZ_COLOR = 4,
+    TT_TEXTURE_EMISSION_COLOR = 5,
+    TT_TEXTURE_GEOMETRY_NORMAL = 6,
+    TT_TEXTURE_BASE_WEIGHT = 7,
+    TT_TEXTURE_BASE_METALNESS = 8,
+    TT_TEXTURE_BASE_DIFFUSE_ROUGHNESS = 9,
+    TT_TEXTURE_SPECULAR_WEIGHT = 10,
+    TT_TEXTURE_SPECULAR_ROUGHNESS = 11,
+    TT_TEXTURE_SPECULAR_ROUGHNESS_ANISOTROPY = 12,
+    TT_TEXTURE_SPECULAR_IOR = 13,
+    TT_TEXTURE_SPECULAR_ROTATION = 14,
+    TT_TEXTURE_TRANSMISSION_WEIGHT = 15,
+    TT_TEXTURE_TRANSMISSION_DEPTH = 16,
+    TT_TEXTURE_COAT_WEIGHT = 17,
+    TT_TEXTURE_COAT_ROUGHNESS = 18,
+    TT_TEXTURE_COAT_ROUGHNESS_ANISOTROPY = 19,
+    TT_TEXTURE_COAT_IOR = 20,
+    TT_TEXTURE_COAT_DARKENING = 21,
+    TT_TEXTURE_COAT_ROTATION = 22,
+    TT_TEXTURE_FUZZ_WEIGHT = 23,
+    TT_TEXTURE_FUZZ_ROUGHNESS = 24,
+    TT_TEXTURE_EMISSION_LUMINANCE = 25,
+    TT_TEXTURE_GEOMETRY_OPACITY = 26,
+    TT_TEXTURE_CHANNELS = 27
+};
+
+/* A material parameter given as a texture (Param::Texture): width x height
+ * texels, rows from the top (v = 0); 3 floats per texel for the colour and
+ * normal channels (0..6), 1 float otherwise. */
+typedef struct tt_texture_desc {
+    uint32_t channel; /* TT_TEXTURE_* */
+    uint32_t width, height;
+    const float* texels;
+} tt_texture_desc;
+
 tt_status tt_asset_scene_create(tt_asset_scene** out);
 void tt_asset_scene_destroy(tt_asset_scene* scene);
 /* Scene::insert_mesh (src/asset/mod.rs:89-161). */
@@ -561,6 +630,12 @@ tt_status tt_asset_insert_mesh(tt_asset_scene* scene, const tt_mesh_desc* mesh, 
 /* Scene::insert_material (src/asset/material.rs:305-357). */
 tt_status tt_asset_insert_material(tt_asset_scene* scene, const tt_material_desc* material,
                                    uint32_t* index);
+/* The same with textured parameters: each texture is processed into its mip
+ * chain and format and replaces its channel's constant, which becomes the
+ * channel type's default (zero) as Param::constant_or_default gives it. */
+tt_status tt_asset_insert_material_textured(tt_asset_scene* scene, const tt_material_desc* material,
+                                            const tt_texture_desc* textures, uint32_t texture_count,
+                                            uint32_t* index);
 /* Node tree (src/asset/mod.rs:35-40).  Node 0 is the root; `transform` is
  * column-major. */
 tt_status tt_asset_add_node(tt_asset_scene* scene, uint32_t parent, const float transform[16],

@@ -68,6 +68,8 @@ def lib():
         L.oracle_cosine_hemisphere_sample.argtypes = [f32, f32, vp]
         L.oracle_decode_tangent_frame.argtypes = [u32, vp]
         L.oracle_display.argtypes = [vp, u32, u32, vp]
+        L.oracle_sample_texture.argtypes = [vp, u32, C.c_float, C.c_float, C.c_float, vp]
+        L.oracle_primary_lod.argtypes = [vp, vp, C.c_float, C.c_float, vp]
         _lib = L
     return _lib
 
@@ -129,6 +131,19 @@ class OracleScene:
         state["cur"] ^= 1
         state["history"] = 1
         return out, rays
+
+    def sample_texture(self, index, u, v, lod):
+        """textureLod of texture `index` as the integrator samples it (RGBA)."""
+        out = np.zeros(4, dtype=np.float32)
+        assert lib().oracle_sample_texture(self.handle, index, u, v, lod, out.ctypes.data) == 0
+        return out
+
+    def primary_lod(self, constants, ndc_x, ndc_y):
+        """Texture level of detail of the first hit of a primary ray (None on a miss)."""
+        out = np.zeros(1, dtype=np.float32)
+        if lib().oracle_primary_lod(self.handle, C.byref(constants), ndc_x, ndc_y, out.ctypes.data) != 0:
+            return None
+        return float(out[0])
 
     def trace(self, rays, cull, threads=0):
         rays = np.ascontiguousarray(rays, dtype=_abi.RAY_DTYPE)

@@ -182,6 +182,7 @@ int oracle_render_restir(void* handle, const tt_constants* k, uint32_t rng, uint
     auto* os = static_cast<OracleScene*>(handle);
     if (os->precision != 32 || k->sample_count != 1) return 1;
     const Scene<float>& s = *os->f;
+    if (s.textured()) return 1;  // ReSTIR DI on textured scenes: not supported
     Camera<float> cam = make_camera<float>(*k);
     uint32_t w = k->screen_size[0], h = k->screen_size[1];
     uint32_t px = w * h;
@@ -539,6 +540,27 @@ void oracle_display(const float* rgba, uint32_t count, uint32_t tonemap, uint8_t
         }
         out[4 * i + 3] = 255;
     }
+}
+
+// Test entry points of the texture path: one sample of texture `index`
+// (float precision), and the level of detail of the first hit of the
+// primary ray at (ndc_x, ndc_y); 0 on success, 1 without a hit.
+int oracle_sample_texture(void* handle, uint32_t index, float u, float v, float lod, float* out) {
+    auto* os = static_cast<OracleScene*>(handle);
+    if (index >= os->desc.texture_count) return 1;
+    os->f->sample(index, u, v, lod, out);
+    return 0;
+}
+
+int oracle_primary_lod(void* handle, const tt_constants* k, float ndc_x, float ndc_y, float* lod) {
+    auto* os = static_cast<OracleScene*>(handle);
+    const Scene<float>& s = *os->f;
+    Camera<float> cam = make_camera<float>(*k);
+    V3<float> dir = camera_ray_direction(cam, ndc_x, ndc_y);
+    Hit h;
+    if (!s.closest_hit(cam.position, dir, true, 1.0e-3f, 1000.0f, h)) return 1;
+    *lod = first_hit_lod(cam, get_ray_hit(s, h), ndc_x, ndc_y);
+    return 0;
 }
 
 }  // extern "C"

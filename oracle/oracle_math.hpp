@@ -33,6 +33,7 @@ struct Fn<float> {
     static float cos(float x) { return ttm::cos(x); }
     static float acos(float x) { return ttm::acos(x); }
     static float log(float x) { return ttm::log(x); }
+    static float log2(float x) { return ttm::log2(x); }
     static float exp(float x) { return ttm::exp(x); }
     static float pow(float x, float y) { return ttm::pow(x, y); }
     static bool isinf(float x) { return ttm::isinf(x); }
@@ -46,6 +47,7 @@ struct Fn<double> {
     static double cos(double x) { return std::cos(x); }
     static double acos(double x) { return std::acos(x); }
     static double log(double x) { return std::log(x); }
+    static double log2(double x) { return std::log2(x); }
     static double exp(double x) { return std::exp(x); }
     static double pow(double x, double y) { return std::pow(x, y); }
     static bool isinf(double x) { return std::isinf(x); }

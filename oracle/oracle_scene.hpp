@@ -390,6 +390,8 @@ struct Scene {
     std::vector<Bvh<R>> blas;
     Bvh<R> tlas;
     bool use_bvh = true;
+    // sRGB code -> linear (c <= 0.04045 ? c / 12.92 : ((c + 0.055) / 1.055)^2.4).
+    float srgb[256];
 
     void load(const tt_scene_desc* d, bool build_bvh) {
         desc = d;
@@ -414,6 +416,118 @@ struct Scene {
                                  dequantize_snorm16<R>(d->positions[3 * i + 2]));
         use_bvh = build_bvh;
         if (build_bvh) build_acceleration();
+        for (int i = 0; i < 256; i++) {
+            double c = double(i) / 255.0;
+            srgb[i] = float(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+        }
+    }
+
+    bool textured() const { return desc->texture_count > 0; }
+
+    // One texel of texture level `data` as the sampler returns it: RGBA8
+    // through the sRGB table, RG8 and R8 as unorm, R16 as half.
+    void texel(uint32_t format, const uint8_t* data, uint32_t i, R out[4]) const {
+        if (format == TT_TEXTURE_FORMAT_RGBA8_SRGB) {
+            const uint8_t* t = data + 4 * size_t(i);
+            out[0] = R(srgb[t[0]]);
+            out[1] = R(srgb[t[1]]);
+            out[2] = R(srgb[t[2]]);
+            out[3] = R(float(t[3]) / 255.0f);
+        } else if (format == TT_TEXTURE_FORMAT_RG8_UNORM) {
+            out[0] = R(float(data[2 * size_t(i)]) / 255.0f);
+            out[1] = R(float(data[2 * size_t(i) + 1]) / 255.0f);
+            out[2] = R(0);
+            out[3] = R(1);
+        } else if (format == TT_TEXTURE_FORMAT_R8_UNORM) {
+            out[0] = R(float(data[i]) / 255.0f);
+            out[1] = out[2] = R(0);
+            out[3] = R(1);
+        } else {
+            uint16_t h = uint16_t(data[2 * size_t(i)] | (data[2 * size_t(i) + 1] << 8));
+            out[0] = R(ttm::half_to_float(h));
+            out[1] = out[2] = R(0);
+            out[3] = R(1);
+        }
+    }
+
+    // Repeat addressing of one axis: lower and upper texel, weight of upper.
+    static void wrap(R c, uint32_t n, uint32_t& i0, uint32_t& i1, R& a) {
+        R f = c - std::floor(c);
+        if (!(f >= R(0) && f < R(1))) f = R(0);
+        R x = f * R(float(n)) - R(0.5);
+        R fx = std::floor(x);
+        a = x - fx;
+        int i = int(fx);
+        i0 = i < 0 ? n - 1u : uint32_t(i);
+        i1 = i0 + 1u == n ? 0u : i0 + 1u;
+    }
+
+    void bilinear(uint32_t format, const uint8_t* data, uint32_t w, uint32_t h, R u, R v, R out[4]) const {
+        uint32_t x0, x1, y0, y1;
+        R ax, ay;
+        wrap(u, w, x0, x1, ax);
+        wrap(v, h, y0, y1, ay);
+        R c00[4], c10[4], c01[4], c11[4];
+        texel(format, data, y0 * w + x0, c00);
+        texel(format, data, y0 * w + x1, c10);
+        texel(format, data, y1 * w + x0, c01);
+        texel(format, data, y1 * w + x1, c11);
+        for (int k = 0; k < 4; k++) {
+            R top = c00[k] * (R(1) - ax) + c10[k] * ax;
+            R bottom = c01[k] * (R(1) - ax) + c11[k] * ax;
+            out[k] = top * (R(1) - ay) + bottom * ay;
+        }
+    }
+
+    // textureLod with linear filtering between the two levels around the
+    // clamped level of detail (the sampler of scene.rs:121-128 without its
+    // anisotropic filtering).
+    void sample(uint32_t index, R u, R v, R lod, R out[4]) const {
+        const tt_texture_record& r = desc->textures[index];
+        R top = R(float(r.levels - 1u));
+        R l = lod > R(0) ? (lod < top ? lod : top) : R(0);
+        uint32_t l0 = uint32_t(std::floor(l));
+        uint32_t l1 = l0 + 1u < r.levels ? l0 + 1u : r.levels - 1u;
+        R t = l - R(float(l0));
+        R a[4], b[4];
+        bilinear(r.format, desc->texture_data + r.offsets[l0], r.width >> l0, r.height >> l0, u, v, a);
+        bilinear(r.format, desc->texture_data + r.offsets[l1], r.width >> l1, r.height >> l1, u, v, b);
+        for (int k = 0; k < 4; k++) out[k] = a[k] * (R(1) - t) + b[k] * t;
+    }
+
+    // integrate.glsl:219-267: the material constants with its textured
+    // channels replaced (the channels load_textures reads, in its order).
+    Material<R> material_at(uint32_t index, R u, R v, R lod) const {
+        Material<R> m = materials[index];
+        if (!textured()) return m;
+        const uint32_t* t = desc->materials[index].textures;
+        R c[4];
+        auto fetch = [&](int ch) {
+            if (t[ch] == TT_INVALID_INDEX) return false;
+            sample(t[ch], u, v, lod, c);
+            return true;
+        };
+        if (fetch(TT_TEXTURE_BASE_COLOR)) m.base_color = V3<R>(c[0], c[1], c[2]);
+        if (fetch(TT_TEXTURE_SPECULAR_COLOR)) m.specular_color = V3<R>(c[0], c[1], c[2]);
+        if (fetch(TT_TEXTURE_TRANSMISSION_COLOR)) m.transmission_color = V3<R>(c[0], c[1], c[2]);
+        if (fetch(TT_TEXTURE_COAT_COLOR)) m.coat_color = V3<R>(c[0], c[1], c[2]);
+        if (fetch(TT_TEXTURE_FUZZ_COLOR)) m.fuzz_color = V3<R>(c[0], c[1], c[2]);
+        if (fetch(TT_TEXTURE_GEOMETRY_NORMAL)) m.geometry_normal = octahedron_decode(c[0], c[1]);
+        if (fetch(TT_TEXTURE_BASE_METALNESS)) m.base_metalness = c[0];
+        if (fetch(TT_TEXTURE_BASE_DIFFUSE_ROUGHNESS)) m.base_diffuse_roughness = c[0];
+        if (fetch(TT_TEXTURE_SPECULAR_WEIGHT)) m.specular_weight = c[0];
+        if (fetch(TT_TEXTURE_SPECULAR_ROUGHNESS)) m.specular_roughness = c[0];
+        if (fetch(TT_TEXTURE_SPECULAR_ROUGHNESS_ANISOTROPY)) m.specular_roughness_anisotropy = c[0];
+        if (fetch(TT_TEXTURE_SPECULAR_IOR)) m.specular_ior = c[0];
+        if (fetch(TT_TEXTURE_SPECULAR_ROTATION)) m.specular_rotation = c[0];
+        if (fetch(TT_TEXTURE_TRANSMISSION_WEIGHT)) m.transmission_weight = c[0];
+        if (fetch(TT_TEXTURE_COAT_WEIGHT)) m.coat_weight = c[0];
+        if (fetch(TT_TEXTURE_COAT_DARKENING)) m.coat_darkening = c[0];
+        if (fetch(TT_TEXTURE_COAT_ROTATION)) m.coat_rotation = c[0];
+        if (fetch(TT_TEXTURE_FUZZ_WEIGHT)) m.fuzz_weight = c[0];
+        if (fetch(TT_TEXTURE_FUZZ_ROUGHNESS)) m.fuzz_roughness = c[0];
+        if (fetch(TT_TEXTURE_GEOMETRY_OPACITY)) m.geometry_opacity = c[0];
+        return m;
     }
 
     V3<R> vertex_position(uint32_t mesh, uint32_t prim, int corner) const {
@@ -563,10 +677,13 @@ struct RayHit {
     V3<R> world_position;
     Basis<R> tangent_space;
     uint32_t instance_index, material_index;
+    // Texture coordinates of the vertices and the hit, world vertices and
+    // barycentrics (for the first hit's ray differentials).
+    R u[3], v[3], tu, tv;
+    V3<R> p[3], bary;
 };
 
-// integrate.glsl:22-110 (texture coordinates and their differentials feed
-// only texture fetches, which scenes without textures never perform).
+// integrate.glsl:22-110.
 template <typename R>
 inline RayHit<R> get_ray_hit(const Scene<R>& s, const Hit& h) {
     RayHit<R> hit;
@@ -589,7 +706,33 @@ inline RayHit<R> get_ray_hit(const Scene<R>& s, const Hit& h) {
     R sign = rsign(f[0].bitangent_sign * bary.x + f[1].bitangent_sign * bary.y + f[2].bitangent_sign * bary.z);
     hit.tangent_space.bitangent = normalize(sign * cross(hit.tangent_space.normal, hit.tangent_space.tangent));
     hit.world_position = bary.x * p[0] + bary.y * p[1] + bary.z * p[2];
+    for (int i = 0; i < 3; i++) {
+        uint32_t uv = s.desc->vertices[s.desc->indices[base + i] + in.vertex_offset].texture_coordinates[0] |
+            uint32_t(s.desc->vertices[s.desc->indices[base + i] + in.vertex_offset].texture_coordinates[1]) << 16;
+        hit.u[i] = R(ttm::half_to_float(uint16_t(uv & 0xffffu)));
+        hit.v[i] = R(ttm::half_to_float(uint16_t(uv >> 16)));
+        hit.p[i] = p[i];
+    }
+    hit.tu = bary.x * hit.u[0] + bary.y * hit.u[1] + bary.z * hit.u[2];
+    hit.tv = bary.x * hit.v[0] + bary.y * hit.v[1] + bary.z * hit.v[2];
+    hit.bary = bary;
     return hit;
+}
+
+// math.glsl:113-125: barycentrics of a ray's crossing of the triangle plane.
+template <typename R>
+inline V3<R> triangle_intersection(const V3<R> p[3], V3<R> origin, V3<R> direction) {
+    V3<R> edge_to_origin = origin - p[0];
+    V3<R> edge_2 = p[2] - p[0];
+    V3<R> edge_1 = p[1] - p[0];
+    V3<R> r = cross(direction, edge_2);
+    V3<R> q = cross(edge_to_origin, edge_1);
+    R inverse_det = R(1) / dot(r, edge_1);
+    R v1 = dot(r, edge_to_origin);
+    R v2 = dot(q, direction);
+    R b = v1 * inverse_det;
+    R c = v2 * inverse_det;
+    return V3<R>(R(1) - b - c, b, c);
 }
 
 // integrate.glsl:195-201
@@ -644,12 +787,30 @@ inline V3<R> direct_light_contribution(const Scene<R>& s, V3<R> position, V3<R> 
 }
 
 // integrate.glsl:203-217 + 270-321, for one sample of one pixel.
+// calculate_mip_level (integrate.glsl:134-139) of the first hit from ray
+// differentials one pixel over in x and in y (integrate.glsl:89-107).
+template <typename R>
+inline R first_hit_lod(const Camera<R>& c, const RayHit<R>& hit, R ndc_x, R ndc_y) {
+    R w = R(float(c.width)), hgt = R(float(c.height));
+    R tsx = R(2) / w, tsy = R(2) / hgt;
+    V3<R> hx = triangle_intersection(hit.p, c.position, camera_ray_direction(c, ndc_x + tsx, ndc_y));
+    V3<R> hy = triangle_intersection(hit.p, c.position, camera_ray_direction(c, ndc_x, ndc_y + tsy));
+    V3<R> dx = hit.bary - hx, dy = hit.bary - hy;
+    R dxu = (dx.x * hit.u[0] + dx.y * hit.u[1] + dx.z * hit.u[2]) * w;
+    R dxv = (dx.x * hit.v[0] + dx.y * hit.v[1] + dx.z * hit.v[2]) * hgt;
+    R dyu = (dy.x * hit.u[0] + dy.y * hit.u[1] + dy.z * hit.u[2]) * w;
+    R dyv = (dy.x * hit.v[0] + dy.y * hit.v[1] + dy.z * hit.v[2]) * hgt;
+    R gx = dxu * dxu + dxv * dxv, gy = dyu * dyu + dyv * dyv;
+    return R(0.5) * Fn<R>::log2(gx < gy ? gy : gx);
+}
+
 template <typename R>
 inline V3<R> trace_path(const Scene<R>& s, const Camera<R>& c, uint32_t light_sampling, uint32_t bounce_count,
-                        V3<R> dir, Generator g, TraceCounts& counts) {
+                        V3<R> dir, Generator g, TraceCounts& counts, R ndc_x = R(0), R ndc_y = R(0)) {
     V3<R> accumulated;
     V3<R> throughput(R(1));
     V3<R> ray_dir = dir, ray_org = c.position;
+    R lod = R(1);  // PathState.mip_level (create_path_state)
     for (uint32_t bounce = 0; bounce < bounce_count; bounce++) {
         Hit h;
         counts.extension++;
@@ -658,7 +819,8 @@ inline V3<R> trace_path(const Scene<R>& s, const Camera<R>& c, uint32_t light_sa
             break;
         }
         RayHit<R> hit = get_ray_hit(s, h);
-        const Material<R>& m = s.materials[hit.material_index];
+        if (bounce == 0 && s.textured()) lod = first_hit_lod(c, hit, ndc_x, ndc_y);
+        const Material<R> m = s.material_at(hit.material_index, hit.tu, hit.tv, lod);
         Basis<R> basis = create_surface_basis(hit.tangent_space, m.geometry_normal);
         V3<R> wi = transform_from_basis(basis, -ray_dir);
         Lobes<R> lobes = bsdf_prepare(m, wi, g);
@@ -699,7 +861,7 @@ inline V3<R> render_pixel(const Scene<R>& s, const Camera<R>& c, const tt_consta
         R ndc_x = ((R(float(x)) + ox) / R(float(k.screen_size[0]))) * R(2) - R(1);
         R ndc_y = ((R(float(y)) + oy) / R(float(k.screen_size[1]))) * R(2) - R(1);
         V3<R> dir = camera_ray_direction(c, ndc_x, ndc_y);
-        accumulated = accumulated + trace_path(s, c, k.light_sampling, k.bounce_count, dir, g, counts);
+        accumulated = accumulated + trace_path(s, c, k.light_sampling, k.bounce_count, dir, g, counts, ndc_x, ndc_y);
     }
     return accumulated / R(float(k.sample_count));
 }

@@ -56,7 +56,25 @@ class SceneDesc(C.Structure):
                 ("meshes", C.c_void_p), ("mesh_count", C.c_uint32),
                 ("materials", C.c_void_p), ("material_count", C.c_uint32),
                 ("instances", C.c_void_p), ("instance_count", C.c_uint32),
-                ("emissive_triangles", C.c_void_p), ("emissive_triangle_count", C.c_uint32)]
+                ("emissive_triangles", C.c_void_p), ("emissive_triangle_count", C.c_uint32),
+                ("textures", C.c_void_p), ("texture_count", C.c_uint32),
+                ("texture_data", C.c_void_p), ("texture_data_bytes", C.c_uint64)]
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("channel", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("texels", C.c_void_p)]
+
+
+# MaterialTextures channels (tinytrace.h TT_TEXTURE_*), field order of asset::Material.
+TEXTURE_CHANNELS = ("base_color", "specular_color", "transmission_color", "coat_color", "fuzz_color", "emission_color",
+                    "geometry_normal", "base_weight", "base_metalness", "base_diffuse_roughness", "specular_weight",
+                    "specular_roughness", "specular_roughness_anisotropy", "specular_ior", "specular_rotation",
+                    "transmission_weight", "transmission_depth", "coat_weight", "coat_roughness",
+                    "coat_roughness_anisotropy", "coat_ior", "coat_darkening", "coat_rotation", "fuzz_weight",
+                    "fuzz_roughness", "emission_luminance", "geometry_opacity")
+TT_TEXTURE_FORMAT_RGBA8_SRGB, TT_TEXTURE_FORMAT_RG8_UNORM, TT_TEXTURE_FORMAT_R8_UNORM, TT_TEXTURE_FORMAT_R16_FLOAT = 0, 1, 2, 3
+TEXTURE_RECORD_DTYPE = np.dtype([("format", "<u4"), ("width", "<u4"), ("height", "<u4"), ("levels", "<u4"),
+                                 ("offsets", "<u4", 16)])
 
 
 class ContextDesc(C.Structure):
@@ -234,6 +252,8 @@ _PROTOTYPES = [
     ("tt_asset_add_model", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
     ("tt_asset_flatten", C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
     ("tt_asset_import_gltf", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("tt_asset_insert_material_textured", C.c_int,
+     [C.c_void_p, C.c_void_p, C.POINTER(TextureDesc), C.c_uint32, C.POINTER(C.c_uint32)]),
     ("tt_target_display", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Constants), C.c_void_p, C.c_size_t]),
 ]
 

@@ -215,10 +215,26 @@ class AssetScene:
         self._desc = None
         return index.value
 
-    def insert_material(self, material: Material) -> int:
+    def insert_material(self, material: Material, textures: Optional[dict] = None) -> int:
+        """Scene::insert_material; `textures` maps channel names
+        (_abi.TEXTURE_CHANNELS) to (H, W, 3) colour / normal or (H, W)
+        scalar float arrays (Param::Texture)."""
         index = C.c_uint32()
-        check(lib().tt_asset_insert_material(self._handle, C.byref(material._desc()), C.byref(index)), None,
-              "insert_material")
+        if not textures:
+            check(lib().tt_asset_insert_material(self._handle, C.byref(material._desc()), C.byref(index)), None,
+                  "insert_material")
+        else:
+            keep, descs = [], []
+            for name, texels in textures.items():
+                ch = _abi.TEXTURE_CHANNELS.index(name)
+                t = np.ascontiguousarray(texels, dtype=np.float32)
+                if t.ndim != (3 if ch <= 6 else 2) or (ch <= 6 and t.shape[2] != 3):
+                    raise ValueError(f"texture {name}: expected shape (H, W{', 3' if ch <= 6 else ''})")
+                keep.append(t)
+                descs.append(_abi.TextureDesc(ch, t.shape[1], t.shape[0], t.ctypes.data))
+            arr = (_abi.TextureDesc * len(descs))(*descs)
+            check(lib().tt_asset_insert_material_textured(self._handle, C.byref(material._desc()), arr, len(descs),
+                                                          C.byref(index)), None, "insert_material")
         self._desc = None
         return index.value
 
@@ -265,6 +281,8 @@ class AssetScene:
             "materials": view(d.materials, d.material_count, _abi.MATERIAL_DTYPE),
             "instances": view(d.instances, d.instance_count, _abi.INSTANCE_DTYPE),
             "emissive_triangles": view(d.emissive_triangles, d.emissive_triangle_count, _abi.EMISSIVE_DTYPE),
+            "textures": view(d.textures, d.texture_count, _abi.TEXTURE_RECORD_DTYPE),
+            "texture_data": view(d.texture_data, d.texture_data_bytes, np.uint8),
         }
 
 

@@ -19,6 +19,7 @@
 //   Renderer constants          src/lib.rs:125-142
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <new>
 #include <set>
 #include <string>
@@ -134,6 +135,14 @@ struct AssetScene {
     // Flattened output.
     std::vector<tt_instance> instances;
     std::vector<tt_emissive_triangle> emissive;
+    // Processed textures (scene.textures) and their flattened form.
+    struct Texture {
+        uint32_t format = 0, width = 0, height = 0;
+        std::vector<std::vector<uint8_t>> levels;
+    };
+    std::vector<Texture> textures;
+    std::vector<tt_texture_record> texture_records;
+    std::vector<uint8_t> texture_bytes;
     std::string error;
     AssetScene() { nodes.emplace_back(); }
 };
@@ -143,6 +152,83 @@ struct AssetScene {
 struct tt_asset_scene : tt::AssetScene {};
 
 using namespace tt;
+
+namespace {
+
+// generate_mip_chain (src/asset/texture.rs:121-143): each level averages the
+// 2 x 2 texels (2x, 2y), (2x, 2y + 1), (2x + 1, 2y), (2x + 1, 2y + 1) of
+// the one above, summed in that order, until a side is below 2.
+std::vector<std::vector<float>> mip_chain(const float* texels, uint32_t width, uint32_t height, int comps,
+                                          std::vector<std::pair<uint32_t, uint32_t>>& extents) {
+    std::vector<std::vector<float>> chain;
+    chain.emplace_back(texels, texels + size_t(width) * height * comps);
+    extents.assign(1, {width, height});
+    while (extents.back().first >= 2 && extents.back().second >= 2) {
+        uint32_t w = extents.back().first, h = extents.back().second;
+        uint32_t nw = w >> 1, nh = h >> 1;
+        const std::vector<float>& last = chain.back();
+        std::vector<float> next(size_t(nw) * nh * comps);
+        for (uint32_t y = 0; y < nh; y++)
+            for (uint32_t x = 0; x < nw; x++)
+                for (int c = 0; c < comps; c++) {
+                    auto at = [&](uint32_t xx, uint32_t yy) { return last[(size_t(yy) * w + xx) * comps + c]; };
+                    float sum = 0.0f;
+                    sum = sum + at(2 * x, 2 * y);
+                    sum = sum + at(2 * x, 2 * y + 1);
+                    sum = sum + at(2 * x + 1, 2 * y);
+                    sum = sum + at(2 * x + 1, 2 * y + 1);
+                    next[(size_t(y) * nw + x) * comps + c] = sum / 4.0f;
+                }
+        chain.push_back(std::move(next));
+        extents.push_back({nw, nh});
+    }
+    return chain;
+}
+
+// ProcessedTexture::from_* (texture.rs:57-119) without the block
+// compression: colour -> RGBA8 (alpha 1), normal -> octahedral RG8,
+// scalar -> R8 unorm or R16 float (material.rs:312-346 picks which).
+AssetScene::Texture process_texture(const tt_texture_desc& t) {
+    AssetScene::Texture out;
+    bool colour = t.channel < TT_TEXTURE_GEOMETRY_NORMAL;
+    bool normal = t.channel == TT_TEXTURE_GEOMETRY_NORMAL;
+    bool half = t.channel == TT_TEXTURE_SPECULAR_IOR || t.channel == TT_TEXTURE_SPECULAR_ROTATION ||
+        t.channel == TT_TEXTURE_TRANSMISSION_DEPTH || t.channel == TT_TEXTURE_COAT_IOR ||
+        t.channel == TT_TEXTURE_COAT_ROTATION || t.channel == TT_TEXTURE_EMISSION_LUMINANCE;
+    out.format = colour ? TT_TEXTURE_FORMAT_RGBA8_SRGB
+                        : (normal ? TT_TEXTURE_FORMAT_RG8_UNORM : (half ? TT_TEXTURE_FORMAT_R16_FLOAT : TT_TEXTURE_FORMAT_R8_UNORM));
+    out.width = t.width;
+    out.height = t.height;
+    int comps = colour || normal ? 3 : 1;
+    std::vector<std::pair<uint32_t, uint32_t>> extents;
+    std::vector<std::vector<float>> chain = mip_chain(t.texels, t.width, t.height, comps, extents);
+    for (const std::vector<float>& level : chain) {
+        size_t n = level.size() / comps;
+        std::vector<uint8_t> bytes;
+        for (size_t i = 0; i < n; i++) {
+            const float* v = &level[i * comps];
+            if (colour) {
+                for (int c = 0; c < 3; c++) bytes.push_back(uint8_t(quantize_unorm(v[c], 8)));
+                bytes.push_back(uint8_t(quantize_unorm(1.0f, 8)));
+            } else if (normal) {
+                float u, w;
+                encode_octahedron(Vec3(v[0], v[1], v[2]), u, w);
+                bytes.push_back(uint8_t(quantize_unorm(u, 8)));
+                bytes.push_back(uint8_t(quantize_unorm(w, 8)));
+            } else if (half) {
+                uint16_t h = float_to_half(v[0]);
+                bytes.push_back(uint8_t(h & 0xffu));
+                bytes.push_back(uint8_t(h >> 8));
+            } else {
+                bytes.push_back(uint8_t(quantize_unorm(v[0], 8)));
+            }
+        }
+        out.levels.push_back(std::move(bytes));
+    }
+    return out;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -374,7 +460,20 @@ tt_status tt_asset_insert_mesh(tt_asset_scene* scene, const tt_mesh_desc* mesh, 
 }
 
 tt_status tt_asset_insert_material(tt_asset_scene* scene, const tt_material_desc* d, uint32_t* index) {
-    if (!scene || !d) return TT_INVALID_ARGUMENT;
+    return tt_asset_insert_material_textured(scene, d, nullptr, 0, index);
+}
+
+tt_status tt_asset_insert_material_textured(tt_asset_scene* scene, const tt_material_desc* d,
+                                            const tt_texture_desc* textures, uint32_t texture_count, uint32_t* index) {
+    if (!scene || !d || (texture_count && !textures)) return TT_INVALID_ARGUMENT;
+    const tt_texture_desc* by_channel[TT_TEXTURE_CHANNELS] = {};
+    for (uint32_t k = 0; k < texture_count; k++) {
+        const tt_texture_desc& t = textures[k];
+        if (t.channel >= TT_TEXTURE_CHANNELS || by_channel[t.channel] || !t.texels || t.width == 0 || t.height == 0 ||
+            t.width > 32768 || t.height > 32768)
+            return TT_INVALID_ARGUMENT;
+        by_channel[t.channel] = &t;
+    }
     uint32_t i = uint32_t(scene->materials.size());
     if (d->emission_luminance > 0.0f) scene->emissive_materials.insert(i);
     tt_material m;
@@ -415,6 +514,35 @@ tt_status tt_asset_insert_material(tt_asset_scene* scene, const tt_material_desc
     c.geometry_opacity = d->geometry_opacity;
     for (int t = 0; t < 27; t++) m.textures[t] = TT_INVALID_INDEX;
     m.padding = 0;
+    // Textured channels: processed in the order insert_material visits them
+    // (material.rs:312-346), each index the next in scene.textures; the
+    // channel's constant is its type's default, zero (constant_or_default).
+    static const int order[TT_TEXTURE_CHANNELS] = {0,  1,  2,  3,  4,  5,  6,  7,  8,  9,  10, 11, 12, 15,
+                                                   17, 18, 19, 21, 23, 24, 26, 25, 20, 13, 16, 14, 22};
+    float* constants = reinterpret_cast<float*>(&m.constants);
+    for (int k = 0; k < TT_TEXTURE_CHANNELS; k++) {
+        int ch = order[k];
+        const tt_texture_desc* t = by_channel[ch];
+        if (!t) continue;
+        m.textures[ch] = uint32_t(scene->textures.size());
+        scene->textures.push_back(process_texture(*t));
+        if (ch < 7) {
+            for (int c = 0; c < 3; c++) constants[4 * ch + c] = 0.0f;
+        } else {
+            constants[28 + (ch - 7)] = 0.0f;
+        }
+    }
+    // is_emissive (material.rs:178-183): a positive constant luminance, or a
+    // luminance texture with any positive texel.
+    bool emissive = m.constants.emission_luminance > 0.0f;
+    if (const tt_texture_desc* t = by_channel[TT_TEXTURE_EMISSION_LUMINANCE]) {
+        emissive = false;
+        for (size_t k = 0; k < size_t(t->width) * t->height; k++) emissive = emissive || t->texels[k] > 0.0f;
+    }
+    if (emissive)
+        scene->emissive_materials.insert(i);
+    else
+        scene->emissive_materials.erase(i);
     scene->materials.push_back(m);
     if (index) *index = i;
     return TT_OK;
@@ -505,6 +633,28 @@ tt_status tt_asset_flatten(tt_asset_scene* scene, tt_scene_desc* out) {
     out->instance_count = uint32_t(scene->instances.size());
     out->emissive_triangles = scene->emissive.data();
     out->emissive_triangle_count = uint32_t(scene->emissive.size());
+    // Textures: every level at a 16-byte aligned offset.
+    scene->texture_records.clear();
+    scene->texture_bytes.clear();
+    for (const auto& t : scene->textures) {
+        tt_texture_record r;
+        std::memset(&r, 0, sizeof(r));
+        r.format = t.format;
+        r.width = t.width;
+        r.height = t.height;
+        r.levels = uint32_t(t.levels.size());
+        for (size_t k = 0; k < t.levels.size(); k++) {
+            scene->texture_bytes.resize((scene->texture_bytes.size() + 15) & ~size_t(15), 0);
+            r.offsets[k] = uint32_t(scene->texture_bytes.size());
+            scene->texture_bytes.insert(scene->texture_bytes.end(), t.levels[k].begin(), t.levels[k].end());
+        }
+        scene->texture_records.push_back(r);
+    }
+    scene->texture_bytes.resize((scene->texture_bytes.size() + 15) & ~size_t(15), 0);
+    out->textures = scene->texture_records.empty() ? nullptr : scene->texture_records.data();
+    out->texture_count = uint32_t(scene->texture_records.size());
+    out->texture_data = scene->texture_bytes.empty() ? nullptr : scene->texture_bytes.data();
+    out->texture_data_bytes = scene->texture_bytes.size();
     return TT_OK;
 }
 

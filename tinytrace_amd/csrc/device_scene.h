@@ -54,6 +54,10 @@ struct SceneView {
     const uint32_t* indices;
     const uint32_t* emissive; // tt_emissive_triangle, 9 x u32 (36 B)
     const float4* lights;     // light records, 5 x float4 per emissive triangle
+    const uint4* tex_records; // tt_texture_record, 5 x uint4
+    const uint8_t* tex_data;
+    const float* srgb_lut;
+    uint32_t texture_count;
 };
 
 struct HitRecord {

@@ -51,6 +51,10 @@ __device__ __host__ inline SceneView view(const tt::DeviceScene& d) {
     s.indices = static_cast<const uint32_t*>(d.indices);
     s.emissive = static_cast<const uint32_t*>(d.emissive);
     s.lights = static_cast<const float4*>(d.lights);
+    s.tex_records = static_cast<const uint4*>(d.tex_records);
+    s.tex_data = static_cast<const uint8_t*>(d.tex_data);
+    s.srgb_lut = static_cast<const float*>(d.srgb_lut);
+    s.texture_count = d.texture_count;
     return s;
 }
 
@@ -122,8 +126,150 @@ TT_DEV M4 load_m4(const float* p) {
     return r;
 }
 
-// The shading point of a closest hit: get_ray_hit without the texture terms
-// (no texture is ever bound) and create_surface_basis.
+// --------------------------------------------------------------- textures --
+// textureLod with the reference's sampler (scene.rs:121-128: linear
+// filtering, repeat addressing), evaluated in software: bilinear on the two
+// mip levels around the clamped level of detail, then linear between them.
+// The reference's 16x anisotropic filtering is not modelled.
+TT_DEV float4 texel(const SceneView& s, uint32_t format, const uint8_t* level, uint32_t i) {
+    if (format == TT_TEXTURE_FORMAT_RGBA8_SRGB) {
+        uint32_t v = *reinterpret_cast<const uint32_t*>(level + 4 * size_t(i));
+        return make_float4(s.srgb_lut[v & 0xffu], s.srgb_lut[(v >> 8) & 0xffu], s.srgb_lut[(v >> 16) & 0xffu],
+                           float(v >> 24) / 255.0f);
+    }
+    if (format == TT_TEXTURE_FORMAT_RG8_UNORM) {
+        uint32_t v = *reinterpret_cast<const uint16_t*>(level + 2 * size_t(i));
+        return make_float4(float(v & 0xffu) / 255.0f, float(v >> 8) / 255.0f, 0.0f, 1.0f);
+    }
+    if (format == TT_TEXTURE_FORMAT_R8_UNORM) return make_float4(float(level[i]) / 255.0f, 0.0f, 0.0f, 1.0f);
+    uint16_t h = *reinterpret_cast<const uint16_t*>(level + 2 * size_t(i));
+    return make_float4(ttm::half_to_float(h), 0.0f, 0.0f, 1.0f);
+}
+
+// Texel index along one axis of extent n: the repeat-wrapped coordinate's
+// lower neighbour i0, upper i1 and the weight of i1.
+TT_DEV void wrap_axis(float c, uint32_t n, uint32_t& i0, uint32_t& i1, float& a) {
+    float f = c - floorf(c);
+    if (!(f >= 0.0f && f < 1.0f)) f = 0.0f;
+    float x = f * float(n) - 0.5f;
+    float fx = floorf(x);
+    a = x - fx;
+    int i = int(fx);
+    i0 = i < 0 ? n - 1u : uint32_t(i);
+    i1 = i0 + 1u == n ? 0u : i0 + 1u;
+}
+
+TT_DEV float4 lerp4(float4 a, float4 b, float t) {
+    return make_float4(a.x * (1.0f - t) + b.x * t, a.y * (1.0f - t) + b.y * t, a.z * (1.0f - t) + b.z * t,
+                       a.w * (1.0f - t) + b.w * t);
+}
+
+TT_DEV float4 bilinear(const SceneView& s, uint32_t format, const uint8_t* level, uint32_t w, uint32_t h, float u,
+                       float v) {
+    uint32_t x0, x1, y0, y1;
+    float ax, ay;
+    wrap_axis(u, w, x0, x1, ax);
+    wrap_axis(v, h, y0, y1, ay);
+    float4 top = lerp4(texel(s, format, level, y0 * w + x0), texel(s, format, level, y0 * w + x1), ax);
+    float4 bottom = lerp4(texel(s, format, level, y1 * w + x0), texel(s, format, level, y1 * w + x1), ax);
+    return lerp4(top, bottom, ay);
+}
+
+TT_DEV float4 sample_texture(const SceneView& s, uint32_t index, float u, float v, float lod) {
+    const uint4* rec = s.tex_records + 5 * size_t(index);
+    uint4 r0 = rec[0];
+    uint32_t format = r0.x, w = r0.y, h = r0.z, levels = r0.w;
+    float top = float(levels - 1u);
+    float l = lod > 0.0f ? (lod < top ? lod : top) : 0.0f;
+    uint32_t l0 = uint32_t(floorf(l));
+    uint32_t l1 = l0 + 1u < levels ? l0 + 1u : levels - 1u;
+    float t = l - float(l0);
+    const uint32_t* offsets = reinterpret_cast<const uint32_t*>(rec + 1);
+    float4 a = bilinear(s, format, s.tex_data + offsets[l0], w >> l0, h >> l0, u, v);
+    float4 b = bilinear(s, format, s.tex_data + offsets[l1], w >> l1, h >> l1, u, v);
+    return lerp4(a, b, t);
+}
+
+// load_textures (integrate.glsl:219-267): the channels it overrides, in its
+// order (base weight, coat roughness and anisotropy, coat IOR, transmission
+// depth and emission are not read there).
+TT_DEV void load_textures(const SceneView& s, const float4* material, Mat& m, float u, float v, float lod) {
+    const uint32_t* t = reinterpret_cast<const uint32_t*>(material + 12);
+    auto fetch = [&](int ch, float4& out) {
+        if (t[ch] == 0xffffffffu) return false;
+        out = sample_texture(s, t[ch], u, v, lod);
+        return true;
+    };
+    float4 c;
+    if (fetch(TT_TEXTURE_BASE_COLOR, c)) m.base_color = f3(c.x, c.y, c.z);
+    if (fetch(TT_TEXTURE_SPECULAR_COLOR, c)) m.specular_color = f3(c.x, c.y, c.z);
+    if (fetch(TT_TEXTURE_TRANSMISSION_COLOR, c)) m.transmission_color = f3(c.x, c.y, c.z);
+    if (fetch(TT_TEXTURE_COAT_COLOR, c)) m.coat_color = f3(c.x, c.y, c.z);
+    if (fetch(TT_TEXTURE_FUZZ_COLOR, c)) m.fuzz_color = f3(c.x, c.y, c.z);
+    if (fetch(TT_TEXTURE_GEOMETRY_NORMAL, c)) m.geometry_normal = tt_octahedron_decode(c.x, c.y);
+    if (fetch(TT_TEXTURE_BASE_METALNESS, c)) m.base_metalness = c.x;
+    if (fetch(TT_TEXTURE_BASE_DIFFUSE_ROUGHNESS, c)) m.base_diffuse_roughness = c.x;
+    if (fetch(TT_TEXTURE_SPECULAR_WEIGHT, c)) m.specular_weight = c.x;
+    if (fetch(TT_TEXTURE_SPECULAR_ROUGHNESS, c)) m.specular_roughness = c.x;
+    if (fetch(TT_TEXTURE_SPECULAR_ROUGHNESS_ANISOTROPY, c)) m.specular_roughness_anisotropy = c.x;
+    if (fetch(TT_TEXTURE_SPECULAR_IOR, c)) m.specular_ior = c.x;
+    if (fetch(TT_TEXTURE_SPECULAR_ROTATION, c)) m.specular_rotation = c.x;
+    if (fetch(TT_TEXTURE_TRANSMISSION_WEIGHT, c)) m.transmission_weight = c.x;
+    if (fetch(TT_TEXTURE_COAT_WEIGHT, c)) m.coat_weight = c.x;
+    if (fetch(TT_TEXTURE_COAT_DARKENING, c)) m.coat_darkening = c.x;
+    if (fetch(TT_TEXTURE_COAT_ROTATION, c)) m.coat_rotation = c.x;
+    if (fetch(TT_TEXTURE_FUZZ_WEIGHT, c)) m.fuzz_weight = c.x;
+    if (fetch(TT_TEXTURE_FUZZ_ROUGHNESS, c)) m.fuzz_roughness = c.x;
+    if (fetch(TT_TEXTURE_GEOMETRY_OPACITY, c)) m.geometry_opacity = c.x;
+}
+
+// camera_ray_direction (integrate.glsl:17-20) of normalized device
+// coordinates.
+TT_DEV f3 camera_ray_direction(const float* inverse_view, const float* inverse_proj, float ndc_x, float ndc_y) {
+    const float* ip = inverse_proj;
+    f3 v(ip[0] * (-ndc_x) + ip[4] * (-ndc_y) + ip[8] * 1.0f + ip[12] * 1.0f,
+         ip[1] * (-ndc_x) + ip[5] * (-ndc_y) + ip[9] * 1.0f + ip[13] * 1.0f,
+         ip[2] * (-ndc_x) + ip[6] * (-ndc_y) + ip[10] * 1.0f + ip[14] * 1.0f);
+    return normalize(mul_point(inverse_view, v, 0.0f));
+}
+
+// triangle_intersection (math.glsl:113-125): barycentrics of a ray's
+// crossing of the triangle's plane.
+TT_DEV f3 triangle_intersection(const f3 p[3], f3 origin, f3 direction) {
+    f3 edge_to_origin = origin - p[0];
+    f3 edge_2 = p[2] - p[0];
+    f3 edge_1 = p[1] - p[0];
+    f3 r = cross(direction, edge_2);
+    f3 q = cross(edge_to_origin, edge_1);
+    float inverse_det = 1.0f / dot(r, edge_1);
+    float v1 = dot(r, edge_to_origin);
+    float v2 = dot(q, direction);
+    float b = v1 * inverse_det;
+    float c = v2 * inverse_det;
+    return f3(1.0f - b - c, b, c);
+}
+
+// The primary ray of a path, for the mip level of its first hit: camera
+// matrices, position, screen size and the sample's device coordinates.
+struct PrimaryRay {
+    const float* inverse_view;
+    const float* inverse_proj;
+    f3 origin;
+    float width, height;
+    float ndc_x, ndc_y;
+};
+
+// Texture terms of a hit: with `primary`, the level of detail from ray
+// differentials (integrate.glsl:89-107, calculate_mip_level :134-139) is
+// written to *lod; then, if the material has textures, load_textures at the
+// hit's texture coordinate and *lod.
+struct TextureHit {
+    const PrimaryRay* primary;
+    float* lod;
+};
+
+// The shading point of a closest hit: get_ray_hit (integrate.glsl:22-110)
+// and create_surface_basis.
 struct Surface {
     f3 position;
     Basis basis;  // surface basis (geometry normal applied)
@@ -132,7 +278,7 @@ struct Surface {
 };
 
 TT_DEV Surface surface_at_hit(const SceneView& s, uint32_t instance, uint32_t prim, float b1, float b2, f3 ray_dir,
-                              Mat& m) {
+                              Mat& m, TextureHit tex = TextureHit{nullptr, nullptr}) {
     const float* inst = s.instances + 52 * size_t(instance);
     M4 transform = load_m4(inst);
     M4 normal_transform = load_m4(inst + 32);
@@ -141,15 +287,46 @@ TT_DEV Surface surface_at_hit(const SceneView& s, uint32_t instance, uint32_t pr
     uint32_t base = 3 * prim + index_offset;
     f3 pos[3];
     TangentFrame fr[3];
+    uint32_t uv[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         uint32_t vi = s.indices[base + c] + vertex_offset;
         const int16_t* pp = s.positions + 3 * size_t(vi);
         pos[c] = mul_point1(transform.m, dequantize_snorm16(pp[0], pp[1], pp[2]));
-        fr[c] = decode_tangent_frame(s.vertices[vi].y);
+        uint2 vx = s.vertices[vi];
+        fr[c] = decode_tangent_frame(vx.y);
+        uv[c] = vx.x;
     }
     f3 bary(0.0f, b1, b2);
     bary.x = 1.0f - bary.y - bary.z;
+    float tu = 0.0f, tv = 0.0f;
+    const float4* material = s.materials + 19 * size_t(offs.z);
+    bool textured = s.texture_count > 0u && tex.lod != nullptr;
+    if (textured) {
+        float u[3], v[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            u[c] = ttm::half_to_float(uint16_t(uv[c] & 0xffffu));
+            v[c] = ttm::half_to_float(uint16_t(uv[c] >> 16));
+        }
+        tu = bary.x * u[0] + bary.y * u[1] + bary.z * u[2];
+        tv = bary.x * v[0] + bary.y * v[1] + bary.z * v[2];
+        if (tex.primary) {
+            const PrimaryRay& pr = *tex.primary;
+            float tsx = 2.0f / pr.width, tsy = 2.0f / pr.height;
+            f3 hx = triangle_intersection(
+                pos, pr.origin, camera_ray_direction(pr.inverse_view, pr.inverse_proj, pr.ndc_x + tsx, pr.ndc_y));
+            f3 hy = triangle_intersection(
+                pos, pr.origin, camera_ray_direction(pr.inverse_view, pr.inverse_proj, pr.ndc_x, pr.ndc_y + tsy));
+            f3 dx = bary - hx, dy = bary - hy;
+            float dxu = (dx.x * u[0] + dx.y * u[1] + dx.z * u[2]) * pr.width;
+            float dxv = (dx.x * v[0] + dx.y * v[1] + dx.z * v[2]) * pr.height;
+            float dyu = (dy.x * u[0] + dy.y * u[1] + dy.z * u[2]) * pr.width;
+            float dyv = (dy.x * v[0] + dy.y * v[1] + dy.z * v[2]) * pr.height;
+            float gx = dxu * dxu + dxv * dxv, gy = dyu * dyu + dyv * dyv;
+            *tex.lod = 0.5f * ttm::log2(gx < gy ? gy : gx);
+        }
+    }
     Basis ts;
     ts.normal = normalize(mul_dir(normal_transform.m, interpolate(bary, fr[0].normal, fr[1].normal, fr[2].normal)));
     ts.tangent = normalize(mul_dir(normal_transform.m, interpolate(bary, fr[0].tangent, fr[1].tangent, fr[2].tangent)));
@@ -158,7 +335,8 @@ TT_DEV Surface surface_at_hit(const SceneView& s, uint32_t instance, uint32_t pr
     Surface sf;
     sf.position = bary.x * pos[0] + bary.y * pos[1] + bary.z * pos[2];
     sf.material = offs.z;
-    m = load_material(s.materials + 19 * size_t(offs.z));
+    m = load_material(material);
+    if (textured) load_textures(s, material, m, tu, tv, *tex.lod);
     sf.basis.normal = transform_to_basis(ts, m.geometry_normal);
     sf.basis.tangent = normalize(ts.tangent - sf.basis.normal * dot(sf.basis.normal, ts.tangent));
     sf.basis.bitangent = normalize(cross(sf.basis.normal, sf.basis.tangent));

@@ -23,6 +23,12 @@ struct DeviceScene {
     const void* indices;
     const void* emissive;
     const void* lights;  // per emissive triangle, 5 x float4 (device_surface.h)
+    // Textures (none when texture_count == 0): tt_texture_record array,
+    // texel bytes and the 256-entry sRGB-to-linear table.
+    const void* tex_records;
+    const void* tex_data;
+    const void* srgb_lut;
+    uint32_t texture_count;
 };
 
 // Per-render camera/integrator constants passed by value.

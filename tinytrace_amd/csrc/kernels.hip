@@ -74,14 +74,13 @@ TT_DEV void generate_path(const tt::FrameArgs& f, const tt::PathBuffers& b, uint
     oy = oy - 0.5f;
     float ndc_x = ((float(x) + ox) / float(f.width)) * 2.0f - 1.0f;
     float ndc_y = ((float(y) + oy) / float(f.height)) * 2.0f - 1.0f;
-    const float* ip = f.inverse_proj;
-    f3 v(ip[0] * (-ndc_x) + ip[4] * (-ndc_y) + ip[8] * 1.0f + ip[12] * 1.0f,
-         ip[1] * (-ndc_x) + ip[5] * (-ndc_y) + ip[9] * 1.0f + ip[13] * 1.0f,
-         ip[2] * (-ndc_x) + ip[6] * (-ndc_y) + ip[10] * 1.0f + ip[14] * 1.0f);
-    f3 dir = normalize(mul_point(f.inverse_view, v, 0.0f));
+    f3 dir = camera_ray_direction(f.inverse_view, f.inverse_proj, ndc_x, ndc_y);
+    // The sample's device coordinates ride in the spare w components for the
+    // first hit's texture level of detail: x with the direction, y with the
+    // throughput (whose w then holds the level of detail).
     b.queue[0][0][p] = make_float4(f.camera_position[0], f.camera_position[1], f.camera_position[2], __uint_as_float(p));
-    b.queue[0][1][p] = make_float4(dir.x, dir.y, dir.z, 0.0f);
-    b.throughput[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    b.queue[0][1][p] = make_float4(dir.x, dir.y, dir.z, ndc_x);
+    b.throughput[p] = make_float4(1.0f, 1.0f, 1.0f, ndc_y);
     b.radiance[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     b.rng[p] = g.state;
     if (b.cache.train_slot) b.cache.train_slot[p] = 0xffffffffu;
@@ -149,6 +148,9 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
     f3 ray_dir(qd.x, qd.y, qd.z);
     float4 thr4 = b.throughput[p];
     float4 acc4 = b.radiance[p];
+    // Texture level of detail: the path's (create_path_state starts it at
+    // 1), set at the first hit from ray differentials.
+    float lod = bounce == 0 ? 1.0f : thr4.w;
     f3 throughput(thr4.x, thr4.y, thr4.z);
     f3 accumulated(acc4.x, acc4.y, acc4.z);
     Generator g{b.rng[p], f.rng};
@@ -176,7 +178,10 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
         added = f3(1.0f) * throughput;
         accumulated = accumulated + added;
     } else {
-        sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, m);
+        PrimaryRay primary{f.inverse_view, f.inverse_proj, f3(qo.x, qo.y, qo.z), float(f.width), float(f.height),
+                           qd.w, thr4.w};
+        sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, m,
+                            TextureHit{bounce == 0 ? &primary : nullptr, &lod});
         if (cache && bounce >= 1 && m.specular_roughness > 0.3f && tslot == 0xffffffffu) {
             // The cache vertex: a query ends the path here; a training path
             // records the vertex and runs on.
@@ -286,7 +291,7 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
     }
 
     b.radiance[p] = make_float4(accumulated.x, accumulated.y, accumulated.z, 0.0f);
-    b.throughput[p] = make_float4(throughput.x, throughput.y, throughput.z, 0.0f);
+    b.throughput[p] = make_float4(throughput.x, throughput.y, throughput.z, lod);
     b.rng[p] = g.state;
 
     uint32_t slot = wave_append(b.counters + C_EXT + bounce + 1, emit_ray);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <chrono>
 #include <cstring>
@@ -235,10 +236,28 @@ tt_status tt_scene_create_ex(tt_context* c, const tt_scene_desc* d, const tt_sce
             return fail(c, TT_INVALID_ARGUMENT, "instance " + std::to_string(i) + " refers to a missing mesh/material");
     }
     if (d->instance_count >= (1u << 24)) return fail(c, TT_INVALID_ARGUMENT, "more than 2^24 instances");
+    if (d->texture_count && (!d->textures || !d->texture_data))
+        return fail(c, TT_INVALID_ARGUMENT, "texture arrays missing");
+    for (uint32_t i = 0; i < d->texture_count; i++) {
+        const tt_texture_record& r = d->textures[i];
+        if (r.format > TT_TEXTURE_FORMAT_R16_FLOAT || r.width == 0 || r.height == 0 || r.levels == 0 || r.levels > 16)
+            return fail(c, TT_INVALID_ARGUMENT, "texture " + std::to_string(i) + " has a bad record");
+        uint64_t texel = r.format == TT_TEXTURE_FORMAT_RGBA8_SRGB ? 4 : (r.format == TT_TEXTURE_FORMAT_R8_UNORM ? 1 : 2);
+        for (uint32_t k = 0; k < r.levels; k++) {
+            uint64_t w = r.width >> k, h = r.height >> k;
+            if (w == 0 || h == 0 || r.offsets[k] % 4 != 0 || r.offsets[k] + w * h * texel > d->texture_data_bytes)
+                return fail(c, TT_INVALID_ARGUMENT, "texture " + std::to_string(i) + " level out of range");
+        }
+    }
     for (uint32_t i = 0; i < d->material_count; i++)
-        for (int t = 0; t < 27; t++)
-            if (d->materials[i].textures[t] != TT_INVALID_INDEX)
-                return fail(c, TT_UNSUPPORTED, "textured materials are not supported by this integrator");
+        for (int t = 0; t < 27; t++) {
+            uint32_t ti = d->materials[i].textures[t];
+            if (ti == TT_INVALID_INDEX) continue;
+            if (ti >= d->texture_count)
+                return fail(c, TT_INVALID_ARGUMENT, "material " + std::to_string(i) + " refers to a missing texture");
+            if (t == TT_TEXTURE_EMISSION_COLOR || t == TT_TEXTURE_EMISSION_LUMINANCE)
+                return fail(c, TT_UNSUPPORTED, "emission textures are not supported by this integrator");
+        }
     for (uint32_t e = 0; e < d->emissive_triangle_count; e++)
         if (d->emissive_triangles[e].instance_index >= d->instance_count)
             return fail(c, TT_INVALID_ARGUMENT, "emissive triangle refers to a missing instance");
@@ -271,6 +290,19 @@ tt_status tt_scene_create_ex(tt_context* c, const tt_scene_desc* d, const tt_sce
     up(d->vertices, size_t(d->vertex_count) * sizeof(tt_vertex), &s->view.vertices);
     up(d->indices, size_t(d->index_count) * sizeof(uint32_t), &s->view.indices);
     up(d->emissive_triangles, size_t(d->emissive_triangle_count) * sizeof(tt_emissive_triangle), &s->view.emissive);
+    if (d->texture_count) {
+        // Texture records, texel bytes and the sRGB decode table
+        // (c <= 0.04045 ? c / 12.92 : ((c + 0.055) / 1.055)^2.4, c = i / 255).
+        float lut[256];
+        for (int i = 0; i < 256; i++) {
+            double cv = double(i) / 255.0;
+            lut[i] = float(cv <= 0.04045 ? cv / 12.92 : std::pow((cv + 0.055) / 1.055, 2.4));
+        }
+        up(d->textures, size_t(d->texture_count) * sizeof(tt_texture_record), &s->view.tex_records);
+        up(d->texture_data, size_t(d->texture_data_bytes), &s->view.tex_data);
+        up(lut, sizeof(lut), &s->view.srgb_lut);
+        s->view.texture_count = d->texture_count;
+    }
     uint32_t node_count = 0;
     if (st == TT_OK && builder == TT_BVH_BUILDER_DEVICE_LBVH) {
         // Every BLAS on the device, then the TLAS on the host after them.
@@ -453,6 +485,8 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     const tt_render_settings& st = settings ? *settings : def;
     const bool restir = k->light_sampling == TT_LIGHT_SAMPLING_RESTIR;
     if (restir && k->sample_count != 1) return fail(c, TT_UNSUPPORTED, "ReSTIR DI renders one sample per pixel");
+    if (restir && s->view.texture_count > 0)
+        return fail(c, TT_UNSUPPORTED, "ReSTIR DI on textured scenes is not supported");
     if (restir && st.restir_candidates > 1024) return fail(c, TT_INVALID_ARGUMENT, "restir_candidates > 1024");
     if (st.rng > 1) return fail(c, TT_INVALID_ARGUMENT, "unknown generator");
     uint64_t paths64 = uint64_t(t->width) * t->height * k->sample_count;

@@ -111,6 +111,57 @@ def cornell_box(variant: int = 0) -> Tuple[AssetScene, Camera]:
     return scene, camera
 
 
+def textured_cornell_box() -> Tuple[AssetScene, Camera]:
+    """The Cornell box with textured floor and back wall: a 64 x 64 checker
+    base colour, roughness, IOR (a float channel), metalness stripes and a
+    bumpy normal map on the floor; a colour gradient on the back wall.  Each
+    face maps [0, 1]^2, so the floor repeats its texture through the
+    sampler's wrap when scaled (uv of the floor quad x 3)."""
+    scene = AssetScene()
+    base_only = dict(specular_weight=0.0, specular_roughness=1.0)
+    n = 64
+    y, x = np.mgrid[0:n, 0:n].astype(np.float32) / n
+    checker = ((np.floor(x * 8) + np.floor(y * 8)) % 2)[..., None]
+    floor_color = (0.15 + 0.6 * checker * np.array([0.9, 0.8, 0.6], np.float32) +
+                   0.1 * (1 - checker) * np.array([0.2, 0.5, 0.9], np.float32)).astype(np.float32)
+    roughness = (0.2 + 0.6 * np.abs(np.sin(6.0 * x) * np.cos(5.0 * y))).astype(np.float32)
+    ior = (1.3 + 0.6 * y).astype(np.float32)
+    metal = (np.floor(x * 4) % 2 * (y > 0.5)).astype(np.float32)
+    bump = np.stack([0.25 * np.sin(12.0 * x), 0.25 * np.cos(10.0 * y), np.ones_like(x)], axis=-1)
+    bump = (bump / np.linalg.norm(bump, axis=-1, keepdims=True)).astype(np.float32)
+    gradient = np.stack([0.2 + 0.6 * x, 0.3 + 0.4 * y, 0.7 - 0.5 * x], axis=-1).astype(np.float32)
+
+    light = scene.insert_material(Material(base_color=(0.73, 0.73, 0.73), emission_luminance=15.0,
+                                           emission_color=(1.0, 1.0, 1.0), **base_only))
+    white = scene.insert_material(Material(base_color=(0.73, 0.73, 0.73), **base_only))
+    red = scene.insert_material(Material(base_color=(0.65, 0.05, 0.05), **base_only))
+    green = scene.insert_material(Material(base_color=(0.12, 0.45, 0.15), **base_only))
+    floor = scene.insert_material(Material(specular_weight=1.0),
+                                  {"base_color": floor_color, "specular_roughness": roughness, "specular_ior": ior,
+                                   "base_metalness": metal, "geometry_normal": bump})
+    back = scene.insert_material(Material(**base_only), {"base_color": gradient})
+
+    def wall(center, normal, u, a, b, material, uv_scale=1.0):
+        p, nrm, t = _quad(center, normal, u, a, b)
+        uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], dtype=np.float32) * uv_scale
+        mesh = scene.insert_mesh(p, uv, _quad_indices(1), np.tile(nrm, (4, 1)), np.tile(np.append(t, 1.0), (4, 1)))
+        scene.add_model(scene.add_node(), mesh, material)
+
+    wall((0, 1.98, 0), (0, -1, 0), (1, 0, 0), 0.25, 0.25, light)
+    wall((0, 0, 0), (0, 1, 0), (1, 0, 0), 1, 1, floor, uv_scale=3.0)
+    wall((0, 2, 0), (0, -1, 0), (1, 0, 0), 1, 1, white)
+    wall((0, 1, -1), (0, 0, 1), (1, 0, 0), 1, 1, back)
+    wall((-1, 1, 0), (1, 0, 0), (0, 0, 1), 1, 1, red)
+    wall((1, 1, 0), (-1, 0, 0), (0, 0, 1), 1, 1, green)
+    for size, angle, center in (((0.6, 1.2, 0.6), math.radians(17.0), (-0.33, 0.6, -0.3)),
+                                ((0.6, 0.6, 0.6), math.radians(-17.0), (0.33, 0.3, 0.3))):
+        p, nrm, t = _box_faces(size, angle, center)
+        mesh = _insert_flat(scene, p, nrm, t, _quad_indices(6))
+        scene.add_model(scene.add_node(), mesh, white)
+    camera = Camera(position=(0.0, 1.0, 3.4), yaw=math.pi / 2, pitch=0.0, fov=0.7)
+    return scene, camera
+
+
 def icosphere(subdivisions: int = 4):
     """Unit icosphere: (positions, triangle indices), CCW seen from outside."""
     t = (1.0 + math.sqrt(5.0)) / 2.0
