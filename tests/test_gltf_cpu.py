@@ -2,7 +2,7 @@
 same files: the reference's own Cornell box scene (tests/golden/
 cornell_box.{gltf,bin}, copied from the reference's scenes/ folder as a
 fixture) read here with json + numpy, and small synthetic documents for TRS
-nodes, data-URI buffers and the refusals.
+nodes, data-URI buffers and the refusals (textures: test_gltf_texture_cpu).
 
 Tangents: the reference generates missing tangents with MikkTSpace
 (mikktspace 0.3.0), which is not restated here, so for the UV-less Cornell
@@ -252,11 +252,11 @@ def test_refusals(tmp_path):
     with open(glb, "wb") as f:
         f.write(b"glTF\x02\x00\x00\x00" + b"\x00" * 16)
     assert _status(lambda: AssetScene().import_gltf(glb)) == unsupported
-    # Textures.
+    # A texture reference with no textures array.
     shutil.copy(os.path.join(GOLDEN, "cornell_box.bin"), tmp)
     doc, _ = load(CORNELL)
     doc["materials"][0]["pbrMetallicRoughness"]["baseColorTexture"] = {"index": 0}
-    assert _status(lambda: AssetScene().import_gltf(_write(tmp, doc, "textured.gltf"))) == unsupported
+    assert _status(lambda: AssetScene().import_gltf(_write(tmp, doc, "textured.gltf"))) == invalid
     # Points instead of triangles.
     doc, _ = load(CORNELL)
     doc["meshes"][0]["primitives"][0]["mode"] = 0

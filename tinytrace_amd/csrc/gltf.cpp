@@ -11,19 +11,30 @@
 //                                             (zeros when absent), indices,
 //                                             optional normals and tangents,
 //                                             cached per (mesh, primitive)
-//   load_material            gltf.rs:242-381  pbrMetallicRoughness factors,
-//                                             emissive factor and
+//   load_material            gltf.rs:242-381  pbrMetallicRoughness factors
+//                                             and textures (base colour,
+//                                             metalness from B, roughness
+//                                             from G), normal and emissive
+//                                             textures, emissive factor,
 //                                             KHR_materials_emissive_strength,
-//                                             KHR_materials_specular, _ior,
-//                                             _transmission; cached per index
+//                                             KHR_materials_specular (weight
+//                                             from A, colour), _ior,
+//                                             _transmission (its texture
+//                                             lands on specular weight, as in
+//                                             the reference); cached per index
 //                                             (the default material once)
+//   image_data / load_png    gltf.rs:86-137   PNG images from buffer views,
+//                                             files or data URIs (png.cpp)
 //   load_nodes               gltf.rs:398-417  depth first: a node's models,
 //                                             then its children
 //   insert_in_scene          gltf.rs:419-452  default scene, else the first
 //
-// Textures are refused (TT_UNSUPPORTED): the library renders constant
-// materials only.  Mesh processing (bounding sphere, snorm16, normals and
-// tangents when absent) is tt_asset_insert_mesh's.
+// Textures become float texel arrays (factor x dequantized byte, as the
+// reference's Texture::transform closures) and go through
+// tt_asset_insert_material_textured.  The reference's placeholder 32 x 32
+// texture for scenes without any (gltf.rs:442-449) only fills an empty
+// descriptor array and is not added.  Mesh processing (bounding sphere,
+// snorm16, normals and tangents when absent) is tt_asset_insert_mesh's.
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -37,6 +48,11 @@
 #include <vector>
 
 #include "../../include/tinytrace.h"
+
+namespace tt {
+// png.cpp: PNG bytes -> RGBA8; an empty string on success.
+std::string decode_png(const uint8_t* data, size_t n, uint32_t& width, uint32_t& height, std::vector<uint8_t>& rgba);
+}  // namespace tt
 
 namespace {
 
@@ -311,8 +327,8 @@ bool base64(const std::string& s, size_t from, std::vector<uint8_t>& out) {
 
 class Importer {
   public:
-    Importer(const Json& doc, std::vector<std::vector<uint8_t>>& buffers, tt_asset_scene* scene)
-        : doc_(doc), buffers_(buffers), scene_(scene) {}
+    Importer(const Json& doc, std::vector<std::vector<uint8_t>>& buffers, tt_asset_scene* scene, std::string dir)
+        : doc_(doc), buffers_(buffers), scene_(scene), dir_(std::move(dir)) {}
 
     void run() {
         const Json* scenes = doc_.get("scenes");
@@ -332,6 +348,12 @@ class Importer {
     tt_asset_scene* scene_;
     std::map<std::pair<long long, long long>, uint32_t> meshes_;
     std::map<long long, uint32_t> materials_;
+    std::string dir_;
+    struct Image {
+        uint32_t width = 0, height = 0;
+        std::vector<uint8_t> rgba;
+    };
+    std::map<long long, Image> images_;
 
     const Json& element(const char* array, long long i) const {
         const Json* a = doc_.get(array);
@@ -443,7 +465,69 @@ class Importer {
         return index;
     }
 
-    static bool textured(const Json* info) { return info != nullptr; }
+    // image_data (gltf.rs:86-108) through load_png (gltf.rs:122-137): the
+    // source image of texture info `info` as RGBA8.  PNG only: a buffer view
+    // with mimeType image/png, a file beside the .gltf, or a base64 data URI.
+    // The texture's sampler and texCoord are not read (TEXCOORD_0, repeat
+    // addressing), as in the reference.
+    const Image& image(const Json& info) {
+        const Json& texture = element("textures", index_of(info.get("index")));
+        long long source = index_of(texture.get("source"));
+        if (source < 0) refuse(TT_UNSUPPORTED);
+        auto hit = images_.find(source);
+        if (hit != images_.end()) return hit->second;
+        const Json& img = element("images", source);
+        std::vector<uint8_t> bytes;
+        if (long long view_index = index_of(img.get("bufferView")); view_index >= 0) {
+            const Json* mime = img.get("mimeType");
+            if (!mime || mime->kind != Json::STRING || mime->text != "image/png") refuse(TT_UNSUPPORTED);
+            const Json& view = element("bufferViews", view_index);
+            long long buffer = index_of(view.get("buffer"));
+            if (buffer < 0 || size_t(buffer) >= buffers_.size()) refuse(TT_INVALID_ARGUMENT);
+            const std::vector<uint8_t>& data = buffers_[size_t(buffer)];
+            long long offset = index_of(view.get("byteOffset")), length = index_of(view.get("byteLength"));
+            size_t o = offset < 0 ? 0 : size_t(offset);
+            if (length < 0 || o + size_t(length) > data.size()) refuse(TT_INVALID_ARGUMENT);
+            bytes.assign(data.begin() + std::ptrdiff_t(o), data.begin() + std::ptrdiff_t(o + size_t(length)));
+        } else {
+            const Json* uri = img.get("uri");
+            if (!uri || uri->kind != Json::STRING) refuse(TT_INVALID_ARGUMENT);
+            const std::string& u = uri->text;
+            if (u.compare(0, 5, "data:") == 0) {
+                size_t comma = u.find(',');
+                if (comma == std::string::npos || u.compare(5, 9, "image/png") != 0 ||
+                    u.rfind(";base64", comma) == std::string::npos || !base64(u, comma + 1, bytes))
+                    refuse(TT_UNSUPPORTED);
+            } else {
+                size_t dot = u.find_last_of('.'), slash = u.find_last_of('/');
+                if (dot != std::string::npos && (slash == std::string::npos || dot > slash + 1) &&
+                    u.compare(dot + 1, std::string::npos, "png") != 0)
+                    refuse(TT_UNSUPPORTED);
+                if (!read_file(dir_ + u, bytes)) refuse(TT_INVALID_ARGUMENT);
+            }
+        }
+        Image im;
+        if (!tt::decode_png(bytes.data(), bytes.size(), im.width, im.height, im.rgba).empty()) refuse(TT_INVALID_ARGUMENT);
+        return images_.emplace(source, std::move(im)).first->second;
+    }
+
+    // A float texture the size of `im`, `comps` values a texel from f(rgba).
+    struct FloatTexture {
+        uint32_t width = 0, height = 0;
+        std::vector<float> texels;
+    };
+    template <typename F>
+    static FloatTexture transform(const Image& im, int comps, F f) {
+        FloatTexture t;
+        t.width = im.width;
+        t.height = im.height;
+        t.texels.resize(size_t(im.width) * im.height * size_t(comps));
+        for (size_t i = 0; i < size_t(im.width) * im.height; i++) f(&im.rgba[4 * i], &t.texels[i * size_t(comps)]);
+        return t;
+    }
+
+    // math::dequantize_unorm(v, 8).
+    static float unorm8(uint8_t v) { return float(v) / 255.0f; }
 
     // load_material (gltf.rs:242-381).
     uint32_t material(long long material_index) {
@@ -451,43 +535,88 @@ class Importer {
         if (hit != materials_.end()) return hit->second;
         tt_material_desc m;
         tt_material_desc_default(&m);
+        std::map<uint32_t, FloatTexture> textures;  // by TT_TEXTURE_* channel
         if (material_index >= 0) {
             const Json& g = element("materials", material_index);
             const Json* pbr = g.get("pbrMetallicRoughness");
             // glTF defaults: base colour (1, 1, 1, 1), metallic 1, roughness 1.
-            float base[3] = {1.0f, 1.0f, 1.0f};
+            float base[3];
             const Json* factor = pbr ? pbr->get("baseColorFactor") : nullptr;
             for (int c = 0; c < 3; c++) base[c] = float(number_or(factor ? factor->at(size_t(c)) : nullptr, 1.0));
-            if (pbr && (textured(pbr->get("baseColorTexture")) || textured(pbr->get("metallicRoughnessTexture"))))
-                refuse(TT_UNSUPPORTED);
             for (int c = 0; c < 3; c++) m.base_color[c] = base[c];
-            m.base_metalness = float(number_or(pbr ? pbr->get("metallicFactor") : nullptr, 1.0));
-            m.specular_roughness = float(number_or(pbr ? pbr->get("roughnessFactor") : nullptr, 1.0));
-            if (textured(g.get("normalTexture")) || textured(g.get("emissiveTexture"))) refuse(TT_UNSUPPORTED);
+            if (const Json* info = pbr ? pbr->get("baseColorTexture") : nullptr)
+                textures[TT_TEXTURE_BASE_COLOR] = transform(image(*info), 3, [&](const uint8_t* p, float* o) {
+                    for (int c = 0; c < 3; c++) o[c] = base[c] * unorm8(p[c]);
+                });
+            float metalness = float(number_or(pbr ? pbr->get("metallicFactor") : nullptr, 1.0));
+            float roughness = float(number_or(pbr ? pbr->get("roughnessFactor") : nullptr, 1.0));
+            m.base_metalness = metalness;
+            m.specular_roughness = roughness;
+            if (const Json* info = pbr ? pbr->get("metallicRoughnessTexture") : nullptr) {
+                const Image& im = image(*info);
+                textures[TT_TEXTURE_BASE_METALNESS] =
+                    transform(im, 1, [&](const uint8_t* p, float* o) { o[0] = unorm8(p[2]) * metalness; });
+                textures[TT_TEXTURE_SPECULAR_ROUGHNESS] =
+                    transform(im, 1, [&](const uint8_t* p, float* o) { o[0] = unorm8(p[1]) * roughness; });
+            }
+            if (const Json* info = g.get("normalTexture"))
+                textures[TT_TEXTURE_GEOMETRY_NORMAL] = transform(image(*info), 3, [](const uint8_t* p, float* o) {
+                    for (int c = 0; c < 3; c++) o[c] = unorm8(p[c]) * 2.0f - 1.0f;
+                });
             const Json* emissive = g.get("emissiveFactor");
-            for (int c = 0; c < 3; c++)
-                m.emission_color[c] = float(number_or(emissive ? emissive->at(size_t(c)) : nullptr, 0.0));
+            float emission[3];
+            for (int c = 0; c < 3; c++) emission[c] = float(number_or(emissive ? emissive->at(size_t(c)) : nullptr, 0.0));
+            for (int c = 0; c < 3; c++) m.emission_color[c] = emission[c];
+            if (const Json* info = g.get("emissiveTexture"))
+                textures[TT_TEXTURE_EMISSION_COLOR] = transform(image(*info), 3, [&](const uint8_t* p, float* o) {
+                    for (int c = 0; c < 3; c++) o[c] = emission[c] * unorm8(p[c]);
+                });
             const Json* ext = g.get("extensions");
             if (const Json* e = ext ? ext->get("KHR_materials_emissive_strength") : nullptr)
                 m.emission_luminance = float(number_or(e->get("emissiveStrength"), 1.0));
             if (const Json* e = ext ? ext->get("KHR_materials_specular") : nullptr) {
-                if (textured(e->get("specularTexture")) || textured(e->get("specularColorTexture")))
-                    refuse(TT_UNSUPPORTED);
-                m.specular_weight = float(number_or(e->get("specularFactor"), 1.0));
+                float weight = float(number_or(e->get("specularFactor"), 1.0));
                 const Json* color = e->get("specularColorFactor");
-                for (int c = 0; c < 3; c++)
-                    m.specular_color[c] = float(number_or(color ? color->at(size_t(c)) : nullptr, 1.0));
+                float specular[3];
+                for (int c = 0; c < 3; c++) specular[c] = float(number_or(color ? color->at(size_t(c)) : nullptr, 1.0));
+                m.specular_weight = weight;
+                if (const Json* info = e->get("specularTexture"))
+                    textures[TT_TEXTURE_SPECULAR_WEIGHT] =
+                        transform(image(*info), 1, [&](const uint8_t* p, float* o) { o[0] = unorm8(p[3]) * weight; });
+                for (int c = 0; c < 3; c++) m.specular_color[c] = specular[c];
+                if (const Json* info = e->get("specularColorTexture"))
+                    textures[TT_TEXTURE_SPECULAR_COLOR] = transform(image(*info), 3, [&](const uint8_t* p, float* o) {
+                        for (int c = 0; c < 3; c++) o[c] = specular[c] * unorm8(p[c]);
+                    });
             }
             if (const Json* e = ext ? ext->get("KHR_materials_ior") : nullptr)
                 m.specular_ior = float(number_or(e->get("ior"), 1.5));
             if (const Json* e = ext ? ext->get("KHR_materials_transmission") : nullptr) {
-                if (textured(e->get("transmissionTexture"))) refuse(TT_UNSUPPORTED);
-                m.transmission_weight = float(number_or(e->get("transmissionFactor"), 0.0));
+                float weight = float(number_or(e->get("transmissionFactor"), 0.0));
+                if (const Json* info = e->get("transmissionTexture")) {
+                    // gltf.rs:357-363 assigns the texture to specular.weight
+                    // and leaves transmission.weight at its default; kept.
+                    textures[TT_TEXTURE_SPECULAR_WEIGHT] =
+                        transform(image(*info), 1, [&](const uint8_t* p, float* o) { o[0] = weight * unorm8(p[0]); });
+                } else {
+                    m.transmission_weight = weight;
+                }
+                // transmission.color = 1 - base.color, texel by texel when
+                // the base colour is a texture.
                 for (int c = 0; c < 3; c++) m.transmission_color[c] = 1.0f - m.base_color[c];
+                auto b = textures.find(TT_TEXTURE_BASE_COLOR);
+                if (b != textures.end()) {
+                    FloatTexture t = b->second;
+                    for (float& v : t.texels) v = 1.0f - v;
+                    textures[TT_TEXTURE_TRANSMISSION_COLOR] = std::move(t);
+                }
             }
         }
+        std::vector<tt_texture_desc> descs;
+        for (const auto& [channel, t] : textures)
+            descs.push_back(tt_texture_desc{channel, t.width, t.height, t.texels.data()});
         uint32_t index = 0;
-        tt_status s = tt_asset_insert_material(scene_, &m, &index);
+        tt_status s = tt_asset_insert_material_textured(scene_, &m, descs.data(), uint32_t(descs.size()), &index);
         if (s != TT_OK) refuse(s);
         materials_[material_index] = index;
         return index;
@@ -579,7 +708,7 @@ extern "C" tt_status tt_asset_import_gltf(tt_asset_scene* scene, const char* pat
             long long length = index_of(list->at(i)->get("byteLength"));
             if (length < 0 || size_t(length) > buffers.back().size()) return TT_INVALID_ARGUMENT;
         }
-        Importer(doc, buffers, scene).run();
+        Importer(doc, buffers, scene, dir).run();
         return TT_OK;
     } catch (const Failure& f) {
         return f.status;
