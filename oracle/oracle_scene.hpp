@@ -530,6 +530,27 @@ struct Scene {
         return m;
     }
 
+    // material_emissive (integrate.glsl:148-154): emission colour x
+    // luminance, each from its texture where the material has one.
+    V3<R> emissive_at(uint32_t index, R u, R v, R lod) const {
+        const Material<R>& m = materials[index];
+        V3<R> color = m.emission_color;
+        R luminance = m.emission_luminance;
+        if (textured()) {
+            const uint32_t* t = desc->materials[index].textures;
+            R c[4];
+            if (t[TT_TEXTURE_EMISSION_COLOR] != TT_INVALID_INDEX) {
+                sample(t[TT_TEXTURE_EMISSION_COLOR], u, v, lod, c);
+                color = V3<R>(c[0], c[1], c[2]);
+            }
+            if (t[TT_TEXTURE_EMISSION_LUMINANCE] != TT_INVALID_INDEX) {
+                sample(t[TT_TEXTURE_EMISSION_LUMINANCE], u, v, lod, c);
+                luminance = c[0];
+            }
+        }
+        return color * luminance;
+    }
+
     V3<R> vertex_position(uint32_t mesh, uint32_t prim, int corner) const {
         const tt_mesh_record& m = desc->meshes[mesh];
         uint32_t index = desc->indices[m.index_offset + 3 * prim + corner];
@@ -779,8 +800,14 @@ inline V3<R> direct_light_contribution(const Scene<R>& s, V3<R> position, V3<R> 
     if (h.instance != tri.instance_index) return V3<R>();
     uint32_t triangle_index = h.primitive + d->instances[h.instance].index_offset / 3;
     if (triangle_index % 0xffffu != uint32_t(tri.hash)) return V3<R>();
-    const Material<R>& lm = s.materials[li.material_index];
-    V3<R> emissive = lm.emission_color * lm.emission_luminance;
+    // The sample's texture coordinate (scene.glsl:130-133) and
+    // material_emissive there at level 4 (integrate.glsl:189).
+    R tc[2];
+    for (int k = 0; k < 2; k++)
+        tc[k] = bary.x * R(ttm::half_to_float(tri.texture_coordinates[0][k])) +
+            bary.y * R(ttm::half_to_float(tri.texture_coordinates[1][k])) +
+            bary.z * R(ttm::half_to_float(tri.texture_coordinates[2][k]));
+    V3<R> emissive = s.emissive_at(li.material_index, tc[0], tc[1], R(4));
     R bsdf_density;
     V3<R> bsdf = bsdf_evaluate(m, lobes, wi, wo, bsdf_density);
     return bsdf * emissive * rabs(cos_theta(wo)) / rmax(density, K<R>::PDF_EPSILON);
@@ -831,7 +858,7 @@ inline V3<R> trace_path(const Scene<R>& s, const Camera<R>& c, uint32_t light_sa
             accumulated = accumulated + throughput * direct;
         }
         if (bounce == 0 || !nee) {
-            V3<R> emissive = m.emission_color * m.emission_luminance;
+            V3<R> emissive = s.emissive_at(hit.material_index, hit.tu, hit.tv, lod);
             accumulated = accumulated + emissive * throughput;
         }
         if (bounce + 1 == bounce_count) break;  // the continuation ray would never be traced

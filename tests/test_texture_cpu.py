@@ -162,3 +162,95 @@ def test_first_hit_level_of_detail_closed_form():
         lod = o.primary_lod(kc, 0.0, 0.0)
         want = math.log2(k * 2 * d * math.tan(fov / 2) * (w / h))
         assert abs(lod - want) < 2e-3, (d, k, fov, lod, want)
+
+
+def srgb_decode(code):
+    s = code / 255.0
+    return s / 12.92 if s <= 0.04045 else ((s + 0.055) / 1.055) ** 2.4
+
+
+def test_emission_texture_seen_directly():
+    """material_emissive at a first hit (integrate.glsl:305): a camera-filling
+    emitter whose colour texture is A on the left half and B on the right,
+    with a luminance texture L: one bounce shows A L and B L away from the
+    seam."""
+    w, h = 96, 64
+    a_col, b_col, lum = np.float32([0.2, 0.5, 0.9]), np.float32([0.7, 0.1, 0.4]), 3.0
+    tex = np.zeros((32, 64, 3), np.float32)
+    tex[:, :32] = a_col
+    tex[:, 32:] = b_col
+    scene = quad_scene({"emission_color": tex, "emission_luminance": np.full((32, 64), lum, np.float32)},
+                       Material(base_color=(0, 0, 0)), depth=2.0, half=2.0, uv_scale=0.25)
+    # uv = (x, y) / 4 + wrap: the quad x in [-2, 2] covers u in [-0.5, 0.5],
+    # i.e. the texture's right half (u < 0 wraps) on x < 0 and its left
+    # half on x > 0.
+    o = binding.OracleScene(scene, 32)
+    cam = Camera(position=(0.0, 0.0, 0.0), yaw=math.pi / 2, pitch=0.0, fov=math.pi / 2)
+    k = build_constants(cam, Config(bounce_count=1, sample_count=1), w, h, 0, 0)
+    img, _ = o.render(k, rng=_abi.TT_RNG_PCG, seed=1)
+    want_a = np.float32([srgb_decode(c) for c in quantize_unorm8(a_col)]) * lum
+    want_b = np.float32([srgb_decode(c) for c in quantize_unorm8(b_col)]) * lum
+    # The quad spans the full height and the middle 2/3 of the width (the
+    # field of view is vertical); screen x may grow either way, so find which
+    # side shows which half.
+    left, right = img[4:-4, 24:44, :3], img[4:-4, 52:72, :3]
+    if abs(left[0, 0, 0] - want_a[0]) > abs(left[0, 0, 0] - want_b[0]):
+        left, right = right, left
+    np.testing.assert_allclose(left.reshape(-1, 3), np.tile(want_a, (left.size // 3, 1)), rtol=1e-5)
+    np.testing.assert_allclose(right.reshape(-1, 3), np.tile(want_b, (right.size // 3, 1)), rtol=1e-5)
+
+
+def _wall_and_split_light(light_materials, light_uvs):
+    """A diffuse wall at z = -3 facing +z and a 1 x 1 emitter at z = -1.5
+    facing it, made of two one-triangle meshes (materials and constant
+    per-triangle texture coordinates given), outside a camera at the origin
+    looking down -z with a 90 degree field of view."""
+    s = AssetScene()
+    wall = np.array([[-3, -3, -3], [3, -3, -3], [3, 3, -3], [-3, 3, -3]], np.float32)
+    s.insert_mesh(wall, wall[:, :2] / 6.0, [0, 1, 2, 0, 2, 3], np.tile([0, 0, 1.0], (4, 1)), None)
+    quad = np.array([[1.6, 0.2, -1.5], [2.6, 0.2, -1.5], [2.6, 1.2, -1.5], [1.6, 1.2, -1.5]], np.float32)
+    for k, tri in enumerate(([0, 2, 1], [0, 3, 2])):  # wound to face -z
+        p = quad[tri]
+        uv = np.tile(np.float32(light_uvs[k]), (3, 1))
+        s.insert_mesh(p, uv, [0, 1, 2], np.tile([0, 0, -1.0], (3, 1)), None)
+    wall_m = s.insert_material(Material(base_color=(0.8, 0.8, 0.8), specular_roughness=0.5))
+    s.add_model(s.add_node(), 0, wall_m)
+    for k, (mat, tex) in enumerate(light_materials):
+        s.add_model(s.add_node(), 1 + k, s.insert_material(mat, tex))
+    return s
+
+
+def test_textured_light_samples_level_4_at_the_sample_coordinate():
+    """direct_light_contribution's material_emissive at the light sample's
+    interpolated texture coordinate and level 4 (integrate.glsl:189,
+    scene.glsl:130-133).  The emitter's texture is a 0/1 checkerboard on its
+    left half and 0.25 on its right; level 4 (4 x 4) is 0.5 | 0.25, while
+    level 0 at the chosen coordinates is a checker texel (0).  Each light
+    triangle has one constant coordinate, so NEE must match, to rounding, a
+    scene whose two light triangles have the constant emission
+    decode(quantize(0.5)) and decode(quantize(0.25)) x L."""
+    tex = np.zeros((64, 64, 3), np.float32)
+    yy, xx = np.mgrid[0:64, 0:32]
+    tex[:, :32] = ((xx + yy) % 2)[..., None]
+    tex[:, 32:] = 0.25
+    lum = 6.0
+    uv_a, uv_b = (16.5 / 64, 0.5 + 0.5 / 64), (48.5 / 64, 0.5 + 0.5 / 64)
+    textured = Material(base_color=(0, 0, 0), specular_roughness=0.5, emission_luminance=lum)
+    a = _wall_and_split_light([(textured, {"emission_color": tex})] * 2, [uv_a, uv_b])
+    ca = np.float32(srgb_decode(quantize_unorm8(0.5)))
+    cb = np.float32(srgb_decode(quantize_unorm8(0.25)))
+    const = lambda c: (Material(base_color=(0, 0, 0), specular_roughness=0.5, emission_luminance=lum,
+                                emission_color=(c, c, c)), None)
+    b = _wall_and_split_light([const(ca), const(cb)], [uv_a, uv_b])
+    assert len(a.arrays()["emissive_triangles"]) == 2
+    cam = Camera(position=(0.0, 0.0, 0.0), yaw=math.pi / 2, pitch=0.0, fov=math.pi / 2)
+    k = build_constants(cam, Config(bounce_count=3, sample_count=1), 64, 48, 0, 0)
+    img_a, _ = binding.OracleScene(a, 32).render(k, rng=_abi.TT_RNG_PCG, seed=2)
+    img_b, _ = binding.OracleScene(b, 32).render(k, rng=_abi.TT_RNG_PCG, seed=2)
+    assert img_b[..., :3].mean() > 0.01
+    np.testing.assert_allclose(img_a, img_b, rtol=1e-5, atol=1e-7)
+    # Emission from level 0 would be black on the first triangle: the two
+    # triangles' contributions differ, so a swapped coordinate shows too.
+    c = _wall_and_split_light([const(cb), const(ca)], [uv_a, uv_b])
+    img_c, _ = binding.OracleScene(c, 32).render(k, rng=_abi.TT_RNG_PCG, seed=2)
+    assert np.abs(img_c - img_b).max() > 1e-3

@@ -223,6 +223,18 @@ TT_DEV void load_textures(const SceneView& s, const float4* material, Mat& m, fl
     if (fetch(TT_TEXTURE_GEOMETRY_OPACITY, c)) m.geometry_opacity = c.x;
 }
 
+// material_emissive's fetches (integrate.glsl:148-154): the emission colour
+// (RGB) and luminance (.x) textures at (u, v) and level `lod`.
+TT_DEV void load_emission_textures(const SceneView& s, const float4* material, Mat& m, float u, float v, float lod) {
+    const uint32_t* t = reinterpret_cast<const uint32_t*>(material + 12);
+    if (t[TT_TEXTURE_EMISSION_COLOR] != 0xffffffffu) {
+        float4 c = sample_texture(s, t[TT_TEXTURE_EMISSION_COLOR], u, v, lod);
+        m.emission_color = f3(c.x, c.y, c.z);
+    }
+    if (t[TT_TEXTURE_EMISSION_LUMINANCE] != 0xffffffffu)
+        m.emission_luminance = sample_texture(s, t[TT_TEXTURE_EMISSION_LUMINANCE], u, v, lod).x;
+}
+
 // camera_ray_direction (integrate.glsl:17-20) of normalized device
 // coordinates.
 TT_DEV f3 camera_ray_direction(const float* inverse_view, const float* inverse_proj, float ndc_x, float ndc_y) {
@@ -336,7 +348,12 @@ TT_DEV Surface surface_at_hit(const SceneView& s, uint32_t instance, uint32_t pr
     sf.position = bary.x * pos[0] + bary.y * pos[1] + bary.z * pos[2];
     sf.material = offs.z;
     m = load_material(material);
-    if (textured) load_textures(s, material, m, tu, tv, *tex.lod);
+    if (textured) {
+        load_textures(s, material, m, tu, tv, *tex.lod);
+        // The hit's emission: material_emissive at the same coordinate and
+        // level (integrate.glsl:305).
+        load_emission_textures(s, material, m, tu, tv, *tex.lod);
+    }
     sf.basis.normal = transform_to_basis(ts, m.geometry_normal);
     sf.basis.tangent = normalize(ts.tangent - sf.basis.normal * dot(sf.basis.normal, ts.tangent));
     sf.basis.bitangent = normalize(cross(sf.basis.normal, sf.basis.tangent));
@@ -357,6 +374,7 @@ struct LightTriangle {
     f3 p[3], normal, emission;
     float area;
     uint32_t instance, hash;
+    uint32_t textured_material;  // material index if its emission is textured, else 0xffffffff
 };
 
 TT_DEV LightTriangle light_triangle(const SceneView& s, uint32_t li) {
@@ -375,20 +393,25 @@ TT_DEV LightTriangle light_triangle(const SceneView& s, uint32_t li) {
         t.p[c] = mul_point1(ltr.m, dequantize_snorm16(half(3 * c), half(3 * c + 1), half(3 * c + 2)));
     t.area = fmax_(0.00001f, 0.5f * length(cross(t.p[1] - t.p[0], t.p[2] - t.p[0])));
     t.normal = normalize(cross(t.p[1] - t.p[0], t.p[2] - t.p[0]));
-    Mat lm = load_material(s.materials + 19 * size_t(__float_as_uint(linst[50])));
+    uint32_t material = __float_as_uint(linst[50]);
+    Mat lm = load_material(s.materials + 19 * size_t(material));
     t.emission = lm.emission_color * lm.emission_luminance;
+    const uint32_t* textures = reinterpret_cast<const uint32_t*>(s.materials + 19 * size_t(material) + 12);
+    bool textured = s.texture_count > 0u && (textures[TT_TEXTURE_EMISSION_COLOR] != 0xffffffffu ||
+                                             textures[TT_TEXTURE_EMISSION_LUMINANCE] != 0xffffffffu);
+    t.textured_material = textured ? material : 0xffffffffu;
     return t;
 }
 
 // Light record (80 B, built once per scene from light_triangle):
 //   (p0, area) (p1, normal.x) (p2, normal.y) (emission, normal.z)
-//   (instance, hash, 0, 0)
+//   (instance, hash, textured material or 0xffffffff, 0)
 TT_DEV void store_light_record(float4* r, const LightTriangle& t) {
     r[0] = make_float4(t.p[0].x, t.p[0].y, t.p[0].z, t.area);
     r[1] = make_float4(t.p[1].x, t.p[1].y, t.p[1].z, t.normal.x);
     r[2] = make_float4(t.p[2].x, t.p[2].y, t.p[2].z, t.normal.y);
     r[3] = make_float4(t.emission.x, t.emission.y, t.emission.z, t.normal.z);
-    r[4] = make_float4(__uint_as_float(t.instance), __uint_as_float(t.hash), 0.0f, 0.0f);
+    r[4] = make_float4(__uint_as_float(t.instance), __uint_as_float(t.hash), __uint_as_float(t.textured_material), 0.0f);
 }
 
 // The point at barycentrics `lb` of emissive triangle `li`.
@@ -402,6 +425,26 @@ TT_DEV LightPoint light_point(const SceneView& s, uint32_t li, f3 lb) {
     lp.emission = f3(d.x, d.y, d.z);
     lp.instance = __float_as_uint(e.x);
     lp.hash = __float_as_uint(e.y);
+    uint32_t textured_material = __float_as_uint(e.z);
+    if (textured_material != 0xffffffffu) {
+        // material_emissive at the sample's texture coordinate, level 4
+        // (integrate.glsl:189; the coordinate as sample_random_light
+        // interpolates it, scene.glsl:130-133).
+        const uint32_t* et = s.emissive + 9 * size_t(li);
+        float u[3], v[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            uint32_t w = et[5 + c];
+            u[c] = ttm::half_to_float(uint16_t(w & 0xffffu));
+            v[c] = ttm::half_to_float(uint16_t(w >> 16));
+        }
+        float tu = lb.x * u[0] + lb.y * u[1] + lb.z * u[2];
+        float tv = lb.x * v[0] + lb.y * v[1] + lb.z * v[2];
+        const float4* material = s.materials + 19 * size_t(textured_material);
+        Mat lm = load_material(material);
+        load_emission_textures(s, material, lm, tu, tv, 4.0f);
+        lp.emission = lm.emission_color * lm.emission_luminance;
+    }
     return lp;
 }
 

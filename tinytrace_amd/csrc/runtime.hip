@@ -255,8 +255,6 @@ tt_status tt_scene_create_ex(tt_context* c, const tt_scene_desc* d, const tt_sce
             if (ti == TT_INVALID_INDEX) continue;
             if (ti >= d->texture_count)
                 return fail(c, TT_INVALID_ARGUMENT, "material " + std::to_string(i) + " refers to a missing texture");
-            if (t == TT_TEXTURE_EMISSION_COLOR || t == TT_TEXTURE_EMISSION_LUMINANCE)
-                return fail(c, TT_UNSUPPORTED, "emission textures are not supported by this integrator");
         }
     for (uint32_t e = 0; e < d->emissive_triangle_count; e++)
         if (d->emissive_triangles[e].instance_index >= d->instance_count)
