@@ -127,11 +127,21 @@ TT_DEV M4 load_m4(const float* p) {
 }
 
 // --------------------------------------------------------------- textures --
+// The scene's textures: records (tt_texture_record, 5 x uint4), texel bytes
+// and the sRGB decode table.
+struct TextureView {
+    const uint4* records;
+    const uint8_t* data;
+    const float* srgb_lut;
+};
+
+TT_DEV TextureView texture_view(const SceneView& s) { return TextureView{s.tex_records, s.tex_data, s.srgb_lut}; }
+
 // textureLod with the reference's sampler (scene.rs:121-128: linear
 // filtering, repeat addressing), evaluated in software: bilinear on the two
 // mip levels around the clamped level of detail, then linear between them.
 // The reference's 16x anisotropic filtering is not modelled.
-TT_DEV float4 texel(const SceneView& s, uint32_t format, const uint8_t* level, uint32_t i) {
+TT_DEV float4 texel(const TextureView& s, uint32_t format, const uint8_t* level, uint32_t i) {
     if (format == TT_TEXTURE_FORMAT_RGBA8_SRGB) {
         uint32_t v = *reinterpret_cast<const uint32_t*>(level + 4 * size_t(i));
         return make_float4(s.srgb_lut[v & 0xffu], s.srgb_lut[(v >> 8) & 0xffu], s.srgb_lut[(v >> 16) & 0xffu],
@@ -164,7 +174,7 @@ TT_DEV float4 lerp4(float4 a, float4 b, float t) {
                        a.w * (1.0f - t) + b.w * t);
 }
 
-TT_DEV float4 bilinear(const SceneView& s, uint32_t format, const uint8_t* level, uint32_t w, uint32_t h, float u,
+TT_DEV float4 bilinear(const TextureView& s, uint32_t format, const uint8_t* level, uint32_t w, uint32_t h, float u,
                        float v) {
     uint32_t x0, x1, y0, y1;
     float ax, ay;
@@ -175,8 +185,8 @@ TT_DEV float4 bilinear(const SceneView& s, uint32_t format, const uint8_t* level
     return lerp4(top, bottom, ay);
 }
 
-TT_DEV float4 sample_texture(const SceneView& s, uint32_t index, float u, float v, float lod) {
-    const uint4* rec = s.tex_records + 5 * size_t(index);
+TT_DEV float4 sample_texture(const TextureView& s, uint32_t index, float u, float v, float lod) {
+    const uint4* rec = s.records + 5 * size_t(index);
     uint4 r0 = rec[0];
     uint32_t format = r0.x, w = r0.y, h = r0.z, levels = r0.w;
     float top = float(levels - 1u);
@@ -185,15 +195,15 @@ TT_DEV float4 sample_texture(const SceneView& s, uint32_t index, float u, float 
     uint32_t l1 = l0 + 1u < levels ? l0 + 1u : levels - 1u;
     float t = l - float(l0);
     const uint32_t* offsets = reinterpret_cast<const uint32_t*>(rec + 1);
-    float4 a = bilinear(s, format, s.tex_data + offsets[l0], w >> l0, h >> l0, u, v);
-    float4 b = bilinear(s, format, s.tex_data + offsets[l1], w >> l1, h >> l1, u, v);
+    float4 a = bilinear(s, format, s.data + offsets[l0], w >> l0, h >> l0, u, v);
+    float4 b = bilinear(s, format, s.data + offsets[l1], w >> l1, h >> l1, u, v);
     return lerp4(a, b, t);
 }
 
 // load_textures (integrate.glsl:219-267): the channels it overrides, in its
 // order (base weight, coat roughness and anisotropy, coat IOR, transmission
 // depth and emission are not read there).
-TT_DEV void load_textures(const SceneView& s, const float4* material, Mat& m, float u, float v, float lod) {
+TT_DEV void load_textures(const TextureView& s, const float4* material, Mat& m, float u, float v, float lod) {
     const uint32_t* t = reinterpret_cast<const uint32_t*>(material + 12);
     auto fetch = [&](int ch, float4& out) {
         if (t[ch] == 0xffffffffu) return false;
@@ -225,7 +235,8 @@ TT_DEV void load_textures(const SceneView& s, const float4* material, Mat& m, fl
 
 // material_emissive's fetches (integrate.glsl:148-154): the emission colour
 // (RGB) and luminance (.x) textures at (u, v) and level `lod`.
-TT_DEV void load_emission_textures(const SceneView& s, const float4* material, Mat& m, float u, float v, float lod) {
+TT_DEV void load_emission_textures(const TextureView& s, const float4* material, Mat& m, float u, float v,
+                                   float lod) {
     const uint32_t* t = reinterpret_cast<const uint32_t*>(material + 12);
     if (t[TT_TEXTURE_EMISSION_COLOR] != 0xffffffffu) {
         float4 c = sample_texture(s, t[TT_TEXTURE_EMISSION_COLOR], u, v, lod);
@@ -233,6 +244,27 @@ TT_DEV void load_emission_textures(const SceneView& s, const float4* material, M
     }
     if (t[TT_TEXTURE_EMISSION_LUMINANCE] != 0xffffffffu)
         m.emission_luminance = sample_texture(s, t[TT_TEXTURE_EMISSION_LUMINANCE], u, v, lod).x;
+}
+
+// Texture code only appears in kernels instantiated for textured scenes:
+// a kernel's register count is set by its busiest point, taken or not, and
+// the sampler would otherwise cost untextured renders occupancy.
+
+// load_textures, then the hit's emission (material_emissive at the same
+// coordinate and level, integrate.glsl:305), on *m.
+TT_DEV void texture_hit(TextureView tv, const float4* material, Mat* m, float u, float v, float lod) {
+    Mat r = *m;
+    load_textures(tv, material, r, u, v, lod);
+    load_emission_textures(tv, material, r, u, v, lod);
+    *m = r;
+}
+
+// material_emissive (integrate.glsl:189) at level 4 at texture coordinate
+// (u, v) of a light sample.
+TT_DEV f3 textured_emission(TextureView tv, const float4* material, float u, float v) {
+    Mat m = load_material(material);
+    load_emission_textures(tv, material, m, u, v, 4.0f);
+    return m.emission_color * m.emission_luminance;
 }
 
 // camera_ray_direction (integrate.glsl:17-20) of normalized device
@@ -271,14 +303,34 @@ struct PrimaryRay {
     float ndc_x, ndc_y;
 };
 
-// Texture terms of a hit: with `primary`, the level of detail from ray
-// differentials (integrate.glsl:89-107, calculate_mip_level :134-139) is
-// written to *lod; then, if the material has textures, load_textures at the
-// hit's texture coordinate and *lod.
+// Texture terms of a hit.  With `primary`, the level of detail of the first
+// hit from ray differentials (integrate.glsl:89-107, calculate_mip_level
+// :134-139) replaces `lod`; then, if the scene has textures, load_textures
+// and the emission textures at the hit's texture coordinate and `lod`.
 struct TextureHit {
-    const PrimaryRay* primary;
-    float* lod;
+    bool enabled;
+    bool primary;
+    PrimaryRay ray;
+    float lod;  // in: the path's level of detail; out: the level used
 };
+
+// The first hit's level of detail from differentials one pixel over in x
+// and in y (integrate.glsl:89-107, 134-139).
+TT_DEV float first_hit_lod(PrimaryRay pr, f3 p0, f3 p1, f3 p2, f3 bary, f3 u, f3 v) {
+    const f3 pos[3] = {p0, p1, p2};
+    float tsx = 2.0f / pr.width, tsy = 2.0f / pr.height;
+    f3 hx = triangle_intersection(pos, pr.origin,
+                                  camera_ray_direction(pr.inverse_view, pr.inverse_proj, pr.ndc_x + tsx, pr.ndc_y));
+    f3 hy = triangle_intersection(pos, pr.origin,
+                                  camera_ray_direction(pr.inverse_view, pr.inverse_proj, pr.ndc_x, pr.ndc_y + tsy));
+    f3 dx = bary - hx, dy = bary - hy;
+    float dxu = (dx.x * u.x + dx.y * u.y + dx.z * u.z) * pr.width;
+    float dxv = (dx.x * v.x + dx.y * v.y + dx.z * v.z) * pr.height;
+    float dyu = (dy.x * u.x + dy.y * u.y + dy.z * u.z) * pr.width;
+    float dyv = (dy.x * v.x + dy.y * v.y + dy.z * v.z) * pr.height;
+    float gx = dxu * dxu + dxv * dxv, gy = dyu * dyu + dyv * dyv;
+    return 0.5f * ttm::log2(gx < gy ? gy : gx);
+}
 
 // The shading point of a closest hit: get_ray_hit (integrate.glsl:22-110)
 // and create_surface_basis.
@@ -290,7 +342,7 @@ struct Surface {
 };
 
 TT_DEV Surface surface_at_hit(const SceneView& s, uint32_t instance, uint32_t prim, float b1, float b2, f3 ray_dir,
-                              Mat& m, TextureHit tex = TextureHit{nullptr, nullptr}) {
+                              Mat& m, TextureHit& tex) {
     const float* inst = s.instances + 52 * size_t(instance);
     M4 transform = load_m4(inst);
     M4 normal_transform = load_m4(inst + 32);
@@ -313,31 +365,15 @@ TT_DEV Surface surface_at_hit(const SceneView& s, uint32_t instance, uint32_t pr
     bary.x = 1.0f - bary.y - bary.z;
     float tu = 0.0f, tv = 0.0f;
     const float4* material = s.materials + 19 * size_t(offs.z);
-    bool textured = s.texture_count > 0u && tex.lod != nullptr;
+    bool textured = s.texture_count > 0u && tex.enabled;
     if (textured) {
-        float u[3], v[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            u[c] = ttm::half_to_float(uint16_t(uv[c] & 0xffffu));
-            v[c] = ttm::half_to_float(uint16_t(uv[c] >> 16));
-        }
-        tu = bary.x * u[0] + bary.y * u[1] + bary.z * u[2];
-        tv = bary.x * v[0] + bary.y * v[1] + bary.z * v[2];
-        if (tex.primary) {
-            const PrimaryRay& pr = *tex.primary;
-            float tsx = 2.0f / pr.width, tsy = 2.0f / pr.height;
-            f3 hx = triangle_intersection(
-                pos, pr.origin, camera_ray_direction(pr.inverse_view, pr.inverse_proj, pr.ndc_x + tsx, pr.ndc_y));
-            f3 hy = triangle_intersection(
-                pos, pr.origin, camera_ray_direction(pr.inverse_view, pr.inverse_proj, pr.ndc_x, pr.ndc_y + tsy));
-            f3 dx = bary - hx, dy = bary - hy;
-            float dxu = (dx.x * u[0] + dx.y * u[1] + dx.z * u[2]) * pr.width;
-            float dxv = (dx.x * v[0] + dx.y * v[1] + dx.z * v[2]) * pr.height;
-            float dyu = (dy.x * u[0] + dy.y * u[1] + dy.z * u[2]) * pr.width;
-            float dyv = (dy.x * v[0] + dy.y * v[1] + dy.z * v[2]) * pr.height;
-            float gx = dxu * dxu + dxv * dxv, gy = dyu * dyu + dyv * dyv;
-            *tex.lod = 0.5f * ttm::log2(gx < gy ? gy : gx);
-        }
+        f3 u(ttm::half_to_float(uint16_t(uv[0] & 0xffffu)), ttm::half_to_float(uint16_t(uv[1] & 0xffffu)),
+             ttm::half_to_float(uint16_t(uv[2] & 0xffffu)));
+        f3 v(ttm::half_to_float(uint16_t(uv[0] >> 16)), ttm::half_to_float(uint16_t(uv[1] >> 16)),
+             ttm::half_to_float(uint16_t(uv[2] >> 16)));
+        tu = bary.x * u.x + bary.y * u.y + bary.z * u.z;
+        tv = bary.x * v.x + bary.y * v.y + bary.z * v.z;
+        if (tex.primary) tex.lod = first_hit_lod(tex.ray, pos[0], pos[1], pos[2], bary, u, v);
     }
     Basis ts;
     ts.normal = normalize(mul_dir(normal_transform.m, interpolate(bary, fr[0].normal, fr[1].normal, fr[2].normal)));
@@ -349,16 +385,22 @@ TT_DEV Surface surface_at_hit(const SceneView& s, uint32_t instance, uint32_t pr
     sf.material = offs.z;
     m = load_material(material);
     if (textured) {
-        load_textures(s, material, m, tu, tv, *tex.lod);
-        // The hit's emission: material_emissive at the same coordinate and
-        // level (integrate.glsl:305).
-        load_emission_textures(s, material, m, tu, tv, *tex.lod);
+        Mat r = m;
+        texture_hit(texture_view(s), material, &r, tu, tv, tex.lod);
+        m = r;
     }
     sf.basis.normal = transform_to_basis(ts, m.geometry_normal);
     sf.basis.tangent = normalize(ts.tangent - sf.basis.normal * dot(sf.basis.normal, ts.tangent));
     sf.basis.bitangent = normalize(cross(sf.basis.normal, sf.basis.tangent));
     sf.wi = transform_from_basis(sf.basis, -ray_dir);
     return sf;
+}
+
+// The same without textures.
+TT_DEV Surface surface_at_hit(const SceneView& s, uint32_t instance, uint32_t prim, float b1, float b2, f3 ray_dir,
+                              Mat& m) {
+    TextureHit none{};
+    return surface_at_hit(s, instance, prim, b1, b2, ray_dir, m, none);
 }
 
 // ---------------------------------------------------------------- lights --
@@ -414,7 +456,9 @@ TT_DEV void store_light_record(float4* r, const LightTriangle& t) {
     r[4] = make_float4(__uint_as_float(t.instance), __uint_as_float(t.hash), __uint_as_float(t.textured_material), 0.0f);
 }
 
-// The point at barycentrics `lb` of emissive triangle `li`.
+// The point at barycentrics `lb` of emissive triangle `li`, with the light
+// record's constant emission (untextured scenes, and ReSTIR, which refuses
+// textured ones).
 TT_DEV LightPoint light_point(const SceneView& s, uint32_t li, f3 lb) {
     const float4* r = s.lights + 5 * size_t(li);
     float4 a = r[0], b = r[1], c = r[2], d = r[3], e = r[4];
@@ -425,25 +469,24 @@ TT_DEV LightPoint light_point(const SceneView& s, uint32_t li, f3 lb) {
     lp.emission = f3(d.x, d.y, d.z);
     lp.instance = __float_as_uint(e.x);
     lp.hash = __float_as_uint(e.y);
-    uint32_t textured_material = __float_as_uint(e.z);
+    return lp;
+}
+
+// light_point for scenes with textures: a light whose material has emission
+// textures takes material_emissive at the sample's texture coordinate, level
+// 4 (integrate.glsl:189; the coordinate as sample_random_light interpolates
+// it, scene.glsl:130-133).
+TT_DEV LightPoint light_point_textured(const SceneView& s, uint32_t li, f3 lb) {
+    LightPoint lp = light_point(s, li, lb);
+    uint32_t textured_material = __float_as_uint(s.lights[5 * size_t(li) + 4].z);
     if (textured_material != 0xffffffffu) {
-        // material_emissive at the sample's texture coordinate, level 4
-        // (integrate.glsl:189; the coordinate as sample_random_light
-        // interpolates it, scene.glsl:130-133).
         const uint32_t* et = s.emissive + 9 * size_t(li);
-        float u[3], v[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            uint32_t w = et[5 + c];
-            u[c] = ttm::half_to_float(uint16_t(w & 0xffffu));
-            v[c] = ttm::half_to_float(uint16_t(w >> 16));
-        }
-        float tu = lb.x * u[0] + lb.y * u[1] + lb.z * u[2];
-        float tv = lb.x * v[0] + lb.y * v[1] + lb.z * v[2];
-        const float4* material = s.materials + 19 * size_t(textured_material);
-        Mat lm = load_material(material);
-        load_emission_textures(s, material, lm, tu, tv, 4.0f);
-        lp.emission = lm.emission_color * lm.emission_luminance;
+        uint32_t w0 = et[5], w1 = et[6], w2 = et[7];
+        float tu = lb.x * ttm::half_to_float(uint16_t(w0 & 0xffffu)) + lb.y * ttm::half_to_float(uint16_t(w1 & 0xffffu)) +
+            lb.z * ttm::half_to_float(uint16_t(w2 & 0xffffu));
+        float tv = lb.x * ttm::half_to_float(uint16_t(w0 >> 16)) + lb.y * ttm::half_to_float(uint16_t(w1 >> 16)) +
+            lb.z * ttm::half_to_float(uint16_t(w2 >> 16));
+        lp.emission = textured_emission(texture_view(s), s.materials + 19 * size_t(textured_material), tu, tv);
     }
     return lp;
 }

@@ -130,6 +130,10 @@ TT_DEV f3 cache_ratio(f3 a, f3 t) {
     return f3(t.x > 0.0f ? a.x / t.x : 0.0f, t.y > 0.0f ? a.y / t.y : 0.0f, t.z > 0.0f ? a.z / t.z : 0.0f);
 }
 
+// TEXTURED: the instantiation for scenes with textures (texture fetches at
+// hits and on textured light samples); untextured scenes run the other,
+// which carries no texture code.
+template <bool TEXTURED>
 __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::FrameArgs f, tt::PathBuffers b,
                                                     uint32_t bounce) {
     SceneView s = view(d);
@@ -178,10 +182,16 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
         added = f3(1.0f) * throughput;
         accumulated = accumulated + added;
     } else {
-        PrimaryRay primary{f.inverse_view, f.inverse_proj, f3(qo.x, qo.y, qo.z), float(f.width), float(f.height),
-                           qd.w, thr4.w};
-        sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, m,
-                            TextureHit{bounce == 0 ? &primary : nullptr, &lod});
+        if constexpr (TEXTURED) {
+            TextureHit tex{true, bounce == 0,
+                           PrimaryRay{f.inverse_view, f.inverse_proj, f3(qo.x, qo.y, qo.z), float(f.width),
+                                      float(f.height), qd.w, thr4.w},
+                           lod};
+            sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, m, tex);
+            lod = tex.lod;
+        } else {
+            sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, m);
+        }
         if (cache && bounce >= 1 && m.specular_roughness > 0.3f && tslot == 0xffffffffu) {
             // The cache vertex: a query ends the path here; a training path
             // records the vertex and runs on.
@@ -221,7 +231,8 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
             // part of direct_light_contribution (integrate.glsl:156-193).
             uint32_t li = random_uint(g, s.emissive_count);
             f2 r2 = random_vec2(g);
-            lp = light_point(s, li, sample_triangle(r2));
+            if constexpr (TEXTURED) lp = light_point_textured(s, li, sample_triangle(r2));
+            else lp = light_point(s, li, sample_triangle(r2));
             float light_probability = 1.0f / float(s.emissive_count);
             f3 to_light = lp.position - sf.position;
             f3 wo = transform_from_basis(sf.basis, to_light);
@@ -524,7 +535,10 @@ hipError_t launch_extend(hipStream_t s, const DeviceScene& scene, const PathBuff
 
 hipError_t launch_shade(hipStream_t s, const DeviceScene& scene, const FrameArgs& f, const PathBuffers& b,
                         uint32_t bounce, uint32_t paths) {
-    shade_kernel<<<blocks(paths, 256), 256, 0, s>>>(scene, f, b, bounce);
+    if (scene.texture_count > 0)
+        shade_kernel<true><<<blocks(paths, 256), 256, 0, s>>>(scene, f, b, bounce);
+    else
+        shade_kernel<false><<<blocks(paths, 256), 256, 0, s>>>(scene, f, b, bounce);
     return hipGetLastError();
 }
 
