@@ -159,6 +159,12 @@ tt_status tt_context_create(const tt_context_desc* desc, tt_context** out) {
     if (hipMalloc(&c->counters, tt::C_WORDS * sizeof(uint32_t)) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
     if (hipMalloc(&c->work, 256) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
     if (hipMalloc(&c->spill, tt::spill_words(c->grid) * sizeof(uint32_t)) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) return cleanup(TT_HIP_ERROR);
+    if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess)
+        return cleanup(TT_HIP_ERROR);
+    if (hipMalloc(&c->side_spill, tt::spill_words(c->grid) * sizeof(uint32_t)) != hipSuccess)
+        return cleanup(TT_OUT_OF_MEMORY);
     if (hipEventCreate(&c->integrate.begin) != hipSuccess || hipEventCreate(&c->integrate.end) != hipSuccess)
         return cleanup(TT_HIP_ERROR);
     *out = c;
@@ -173,6 +179,7 @@ void context_free(tt_context* c) {
     // Teardown is best effort: errors here have no caller to report to.
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->path_memory) (void)hipFree(c->path_memory);
     if (c->cache_memory) (void)hipFree(c->cache_memory);
     if (c->display_memory) (void)hipFree(c->display_memory);
@@ -181,6 +188,9 @@ void context_free(tt_context* c) {
     if (c->counters) (void)hipFree(c->counters);
     if (c->work) (void)hipFree(c->work);
     if (c->spill) (void)hipFree(c->spill);
+    if (c->side_spill) (void)hipFree(c->side_spill);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
     if (c->integrate.begin) (void)hipEventDestroy(c->integrate.begin);
     if (c->integrate.end) (void)hipEventDestroy(c->integrate.end);
     for (auto& e : c->trace_events) {
@@ -188,6 +198,7 @@ void context_free(tt_context* c) {
         (void)hipEventDestroy(e.end);
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->side) (void)hipStreamDestroy(c->side);
     delete c;
 }
 
@@ -549,13 +560,25 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     TT_HIP(c, tt::launch_zero(stream, c->counters, tt::C_WORDS));
     TT_HIP(c, tt::launch_generate(stream, f, b, paths));
     uint32_t ev = 0;
-    for (uint32_t bounce = 0; bounce < k->bounce_count; bounce++) {
+    // Bounce b's shadow rays (side stream) overlap bounce b + 1's extension
+    // (main stream): both only read what shade b wrote.  Shade b + 1 waits
+    // for them, since the shadow pass adds to the paths' radiance and its
+    // buffers are refilled by the next shade.
+    tt::PathBuffers side_buffers = b;
+    side_buffers.spill = c->side_spill;
+    const bool shadows = f.light_sampling != TT_LIGHT_SAMPLING_ON_HIT && s->view.emissive_count > 0;
+    auto extend = [&](uint32_t bounce) -> tt_status {
         if (stats) TT_HIP(c, hipEventRecord(c->trace_events[ev].begin, stream));
         TT_HIP(c, tt::launch_extend(stream, s->view, b, bounce, c->grid));
         if (stats) {
             c->trace_event_shadow[ev] = 0;
             TT_HIP(c, hipEventRecord(c->trace_events[ev++].end, stream));
         }
+        return TT_OK;
+    };
+    es = extend(0);
+    if (es != TT_OK) return es;
+    for (uint32_t bounce = 0; bounce < k->bounce_count; bounce++) {
         if (restir && bounce == 0 && s->view.emissive_count > 0) {
             if (stats) TT_HIP(c, hipEventRecord(c->trace_events[ev].begin, stream));
             TT_HIP(c, tt::launch_restir(stream, s->view, f, b, rb, c->grid));
@@ -564,16 +587,25 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
                 TT_HIP(c, hipEventRecord(c->trace_events[ev++].end, stream));
             }
         }
+        if (shadows && bounce > 0) TT_HIP(c, hipStreamWaitEvent(stream, c->join, 0));
         TT_HIP(c, tt::launch_shade(stream, s->view, f, b, bounce, paths));
-        if (f.light_sampling != TT_LIGHT_SAMPLING_ON_HIT && s->view.emissive_count > 0) {
-            if (stats) TT_HIP(c, hipEventRecord(c->trace_events[ev].begin, stream));
-            TT_HIP(c, tt::launch_shadow(stream, s->view, b, bounce, c->grid));
+        if (shadows) {
+            TT_HIP(c, hipEventRecord(c->fork, stream));
+            TT_HIP(c, hipStreamWaitEvent(c->side, c->fork, 0));
+            if (stats) TT_HIP(c, hipEventRecord(c->trace_events[ev].begin, c->side));
+            TT_HIP(c, tt::launch_shadow(c->side, s->view, side_buffers, bounce, c->grid));
             if (stats) {
                 c->trace_event_shadow[ev] = 1;
-                TT_HIP(c, hipEventRecord(c->trace_events[ev++].end, stream));
+                TT_HIP(c, hipEventRecord(c->trace_events[ev++].end, c->side));
             }
+            TT_HIP(c, hipEventRecord(c->join, c->side));
+        }
+        if (bounce + 1 < k->bounce_count) {
+            es = extend(bounce + 1);
+            if (es != TT_OK) return es;
         }
     }
+    if (shadows) TT_HIP(c, hipStreamWaitEvent(stream, c->join, 0));
     if (cache) {
         // Cached radiance for the paths that ended at a query.
         tt_status cs = tt::mlp_infer_device(cache, b.cache.query_rec, b.cache.query_scale,

@@ -31,6 +31,12 @@ struct tt_context {
     uint32_t* counters = nullptr;
     uint32_t* spill = nullptr;
     uint32_t* work = nullptr;
+    // Shadow rays of bounce b run on `side` while bounce b + 1 extends on
+    // `stream` (fork after shade b, join before shade b + 1), with their own
+    // traversal-stack spill area.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    uint32_t* side_spill = nullptr;
     // Timing.
     tt::EventPair integrate;
     std::vector<tt::EventPair> trace_events;
