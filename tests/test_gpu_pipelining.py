@@ -1,0 +1,96 @@
+"""Renders enqueued back to back overlap on the device (consecutive renders
+alternate between two lanes; each bounce's shadow rays overlap the next
+bounce's extension).  Their results must equal, bit for bit, renders that
+are waited for one by one: progressive accumulation order, ReSTIR temporal
+history across frames, two targets interleaved on one context, and API calls
+(download, clear) between renders."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tinytrace_amd import Config, LightSampling, Renderer, RenderSettings, Target, _abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+MODES = [LightSampling.NEXT_EVENT_ESTIMATION, LightSampling.RESTIR, LightSampling.ON_HIT]
+
+
+def make_renderer(mode, size=(320, 200)):
+    scene, camera = scenes.sphere_field(seed=1, spheres=60, lights=16, subdivisions=2)
+    r = Renderer(size, settings=RenderSettings(rng=_abi.TT_RNG_PCG, seed=7))
+    r.set_scene(scene)
+    r.camera = camera
+    r.set_config(Config(bounce_count=6, sample_count=1, light_sampling=mode))
+    return r
+
+
+def render_into(r, target):
+    """One render of r's scene, camera and frame counters into `target`."""
+    k = r.constants()
+    _abi.check(_abi.lib().tt_render_ex(r.context.handle, r.scene.handle, C.byref(k), C.byref(r.settings._desc()),
+                                       target.handle), r.context.handle, "tt_render_ex")
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_back_to_back_frames_equal_waited_frames(mode):
+    frames = 7
+    a = make_renderer(mode)
+    for _ in range(frames):
+        a.render_to_target()  # no wait in between
+    pipelined = a.target.download()
+    b = make_renderer(mode)
+    for _ in range(frames):
+        b.render_to_target()
+        b.target.download()  # waits for the render
+    waited = b.target.download()
+    assert np.isfinite(pipelined).all() and pipelined[..., :3].max() > 0
+    np.testing.assert_array_equal(pipelined, waited)
+
+
+@pytest.mark.parametrize("mode", [LightSampling.RESTIR, LightSampling.NEXT_EVENT_ESTIMATION])
+def test_two_targets_interleaved_on_one_context(mode):
+    """Frames alternate between two targets of one context; each target's
+    accumulation and ReSTIR history equal those of a renderer that renders it
+    alone."""
+    frames = 5
+    r = make_renderer(mode)
+    second = Target(r.context, *r.extent)
+    cam2 = r.camera
+    for _ in range(frames):
+        render_into(r, r.target)
+        render_into(r, second)  # same constants: same frame on both targets
+        r.accumulated_frame_count += 1
+        r.frame_index += 1
+    got_a, got_b = r.target.download(), second.download()
+    alone = make_renderer(mode)
+    alone.camera = cam2
+    for _ in range(frames):
+        alone.render_to_target()
+        alone.target.download()
+    want = alone.target.download()
+    np.testing.assert_array_equal(got_a, want)
+    np.testing.assert_array_equal(got_b, want)
+
+
+def test_api_calls_between_renders():
+    """A download and an accumulation reset between back-to-back renders see
+    the renders before them and not those after."""
+    r = make_renderer(LightSampling.NEXT_EVENT_ESTIMATION)
+    r.render_to_target()
+    r.render_to_target()
+    mid = r.target.download()
+    r.reset_accumulation()  # the next render overwrites the target
+    r.render_to_target()
+    r.render_to_target()
+    end = r.target.download()
+    w = make_renderer(LightSampling.NEXT_EVENT_ESTIMATION)
+    w.render_to_target()
+    w.target.download()
+    w.render_to_target()
+    np.testing.assert_array_equal(mid, w.target.download())
+    w.reset_accumulation()
+    w.render_to_target()
+    w.target.download()
+    w.render_to_target()
+    np.testing.assert_array_equal(end, w.target.download())

@@ -263,7 +263,7 @@ namespace tt {
 // count), triangles (Morton order) into tris; fills the per-mesh summaries.
 tt_status build_blas_device(tt_context* c, const tt_scene_desc& d, const int16_t* positions, const uint32_t* indices,
                             BvhNode* nodes, BvhTri* tris, std::vector<BlasSummary>& summary, uint32_t& node_count) {
-    hipStream_t st = c->stream;
+    hipStream_t st = tt::api_stream(c);
     const uint32_t meshes = d.mesh_count;
     std::vector<MeshRun> runs(std::max(meshes, 1u));
     uint64_t total = 0, internal = 0;

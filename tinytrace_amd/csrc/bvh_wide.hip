@@ -136,7 +136,7 @@ tt_status collapse_wide(tt_context* c, const BvhNode* nodes, uint32_t count, voi
                         uint32_t instance_count, uint32_t& tlas_root, BvhWide** out, uint32_t& wide_count) {
     *out = nullptr;
     wide_count = 0;
-    hipStream_t st = c->stream;
+    hipStream_t st = tt::api_stream(c);
     if (count == 0) {
         // No inner nodes: every root is a leaf (or the scene is empty).
         return TT_OK;

@@ -70,13 +70,13 @@ extern "C" tt_status tt_target_display(tt_context* c, const tt_target* t, const 
         TT_HIP(c, hipEventCreate(&c->post_process.begin));
         TT_HIP(c, hipEventCreate(&c->post_process.end));
     }
-    TT_HIP(c, hipEventRecord(c->post_process.begin, c->stream));
-    display_kernel<<<uint32_t((count + 255) / 256), 256, 0, c->stream>>>(
+    TT_HIP(c, hipEventRecord(c->post_process.begin, tt::api_stream(c)));
+    display_kernel<<<uint32_t((count + 255) / 256), 256, 0, tt::api_stream(c)>>>(
         t->pixels, uint32_t(count), constants->tonemap, static_cast<uint32_t*>(c->display_memory));
     TT_HIP(c, hipGetLastError());
-    TT_HIP(c, hipEventRecord(c->post_process.end, c->stream));
+    TT_HIP(c, hipEventRecord(c->post_process.end, tt::api_stream(c)));
     c->post_process_recorded = true;
-    TT_HIP(c, hipMemcpyAsync(rgba8, c->display_memory, count * 4, hipMemcpyDeviceToHost, c->stream));
-    TT_HIP(c, hipStreamSynchronize(c->stream));
+    TT_HIP(c, hipMemcpyAsync(rgba8, c->display_memory, count * 4, hipMemcpyDeviceToHost, tt::api_stream(c)));
+    TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
     return TT_OK;
 }

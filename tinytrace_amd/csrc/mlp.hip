@@ -416,7 +416,7 @@ using tt::fail;
 tt_status mlp_repack(tt_mlp* m) {
     tt_context* c = m->context;
     uint32_t total = MLP_FWD_ROWS * MLP_ROW + MLP_BWD_ROWS * MLP_ROW + 64 * MLP_BWD5_ROW;
-    mlp_pack_kernel<<<blocks_of(total, 256), 256, 0, c->stream>>>(m->master, m->wf, m->wb, m->wb5);
+    mlp_pack_kernel<<<blocks_of(total, 256), 256, 0, tt::api_stream(c)>>>(m->master, m->wf, m->wb, m->wb5);
     TT_HIP(c, hipGetLastError());
     return TT_OK;
 }
@@ -425,7 +425,7 @@ static tt_status ensure_train(tt_mlp* m, uint32_t nstride) {
     tt_context* c = m->context;
     if (nstride <= m->train_capacity) return TT_OK;
     if (m->train_memory) {
-        TT_HIP(c, hipStreamSynchronize(c->stream));
+        TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
         TT_HIP(c, hipFree(m->train_memory));
         m->train_memory = nullptr;
         m->train_capacity = 0;
@@ -458,7 +458,7 @@ tt_status mlp_set_scene_bounds(tt_mlp* m, const float lo[3], const float hi[3]) 
 tt_status mlp_infer_device(tt_mlp* m, const float4* records, const float4* scale_path, const uint32_t* count_dev,
                            uint32_t count_host, float* out_rgb, float4* radiance) {
     tt_context* c = m->context;
-    mlp_infer_kernel<<<c->cu_count * 2, MLP_BLOCK, 0, c->stream>>>(m->wf, m->bounds, records, scale_path, count_dev,
+    mlp_infer_kernel<<<c->cu_count * 2, MLP_BLOCK, 0, tt::api_stream(c)>>>(m->wf, m->bounds, records, scale_path, count_dev,
                                                                   count_host, out_rgb, radiance);
     TT_HIP(c, hipGetLastError());
     return TT_OK;
@@ -476,11 +476,11 @@ tt_status mlp_train_device(tt_mlp* m, const float4* records, const float4* targe
     float inv_norm = 1.0f / (3.0f * float(n));
     uint32_t tiles = nstride / 32;
     uint32_t grid = std::min<uint32_t>(blocks_of(tiles, MLP_BLOCK / 64), uint32_t(c->cu_count));
-    mlp_train_kernel<<<grid, MLP_BLOCK, 0, c->stream>>>(m->wf, m->wb, m->wb5, m->bounds, records, targets, n, nstride,
+    mlp_train_kernel<<<grid, MLP_BLOCK, 0, tt::api_stream(c)>>>(m->wf, m->wb, m->wb5, m->bounds, records, targets, n, nstride,
                                                        inv_norm, m->acts, m->deltas, m->loss_tiles);
     TT_HIP(c, hipGetLastError());
     uint32_t chunks = nstride / MLP_CHUNK;
-    mlp_wgrad_kernel<<<dim3(chunks, 6), 64, 0, c->stream>>>(m->acts, m->deltas, nstride, m->partial);
+    mlp_wgrad_kernel<<<dim3(chunks, 6), 64, 0, tt::api_stream(c)>>>(m->acts, m->deltas, nstride, m->partial);
     TT_HIP(c, hipGetLastError());
     AdamArgs a{m->lr, m->beta1, m->beta2, m->eps, 1.0f, 1.0f};
     if (apply) {
@@ -488,7 +488,7 @@ tt_status mlp_train_device(tt_mlp* m, const float4* records, const float4* targe
         a.c1 = float(1.0 / (1.0 - std::pow(double(m->beta1), double(m->step))));
         a.c2 = float(1.0 / (1.0 - std::pow(double(m->beta2), double(m->step))));
     }
-    mlp_adam_kernel<<<blocks_of(MLP_PARAMS, 256), 256, 0, c->stream>>>(m->master, m->m1, m->m2, m->partial, chunks, a,
+    mlp_adam_kernel<<<blocks_of(MLP_PARAMS, 256), 256, 0, tt::api_stream(c)>>>(m->master, m->m1, m->m2, m->partial, chunks, a,
                                                                       apply ? 1 : 0, grad_out);
     TT_HIP(c, hipGetLastError());
     if (apply) return mlp_repack(m);
@@ -499,8 +499,8 @@ tt_status mlp_last_loss(tt_mlp* m, uint32_t n, double* out) {
     tt_context* c = m->context;
     uint32_t tiles = (n + MLP_CHUNK - 1) / MLP_CHUNK * MLP_CHUNK / 32;
     std::vector<float> v(tiles);
-    TT_HIP(c, hipMemcpyAsync(v.data(), m->loss_tiles, tiles * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    TT_HIP(c, hipStreamSynchronize(c->stream));
+    TT_HIP(c, hipMemcpyAsync(v.data(), m->loss_tiles, tiles * sizeof(float), hipMemcpyDeviceToHost, tt::api_stream(c)));
+    TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
     double s = 0.0;
     for (float x : v) s += x;
     *out = s / (3.0 * n);
@@ -532,8 +532,8 @@ struct HostPcg {
 
 template <typename T>
 tt_status upload_host(tt_context* c, T* dst, const T* src, size_t count) {
-    TT_HIP(c, hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
-    TT_HIP(c, hipStreamSynchronize(c->stream));
+    TT_HIP(c, hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, tt::api_stream(c)));
+    TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
     return TT_OK;
 }
 
@@ -578,7 +578,7 @@ void tt_mlp_destroy(tt_mlp* m) {
     if (!m) return;
     tt_context* c = m->context;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(tt::api_stream(c));
     if (m->master) (void)hipFree(m->master);
     if (m->wf) (void)hipFree(m->wf);
     if (m->train_memory) (void)hipFree(m->train_memory);
@@ -597,15 +597,15 @@ tt_status tt_mlp_params(tt_mlp* m, float* get, const float* set, uint32_t count)
     if (count != uint32_t(MLP_PARAMS)) return fail(c, TT_INVALID_ARGUMENT, "parameter count is 20672");
     TT_HIP(c, hipSetDevice(c->device));
     if (get) {
-        TT_HIP(c, hipMemcpyAsync(get, m->master, MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-        TT_HIP(c, hipStreamSynchronize(c->stream));
+        TT_HIP(c, hipMemcpyAsync(get, m->master, MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost, tt::api_stream(c)));
+        TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
     }
     if (set) {
         tt_status st = upload_host(c, m->master, set, MLP_PARAMS);
         if (st != TT_OK) return st;
         st = tt::mlp_repack(m);
         if (st != TT_OK) return st;
-        TT_HIP(c, hipStreamSynchronize(c->stream));
+        TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
     }
     return TT_OK;
 }
@@ -629,12 +629,12 @@ tt_status tt_mlp_run(tt_mlp* m, int op, const float* records, const float* targe
     float4* dtgt = reinterpret_cast<float4*>(d + rec_bytes);
     float* dout = reinterpret_cast<float*>(d + rec_bytes + tgt_bytes);
     tt_status st = TT_OK;
-    hipError_t e = hipMemcpyAsync(drec, records, rec_bytes, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess && targets) e = hipMemcpyAsync(dtgt, targets, tgt_bytes, hipMemcpyHostToDevice, c->stream);
+    hipError_t e = hipMemcpyAsync(drec, records, rec_bytes, hipMemcpyHostToDevice, tt::api_stream(c));
+    if (e == hipSuccess && targets) e = hipMemcpyAsync(dtgt, targets, tgt_bytes, hipMemcpyHostToDevice, tt::api_stream(c));
     if (e != hipSuccess) st = tt::hip_fail(c, e, "tt_mlp_run upload");
     if (st == TT_OK) {
         if (op == 0) {
-            mlp_encode_kernel<<<blocks_of(n, 64), 64, 0, c->stream>>>(m->bounds, drec, n, dout);
+            mlp_encode_kernel<<<blocks_of(n, 64), 64, 0, tt::api_stream(c)>>>(m->bounds, drec, n, dout);
             e = hipGetLastError();
             if (e != hipSuccess) st = tt::hip_fail(c, e, "encode");
         } else if (op == 1) {
@@ -644,12 +644,12 @@ tt_status tt_mlp_run(tt_mlp* m, int op, const float* records, const float* targe
         }
     }
     if (st == TT_OK && out_floats) {
-        e = hipMemcpyAsync(out, dout, out_floats * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+        e = hipMemcpyAsync(out, dout, out_floats * sizeof(float), hipMemcpyDeviceToHost, tt::api_stream(c));
         if (e != hipSuccess) st = tt::hip_fail(c, e, "tt_mlp_run download");
     }
     if (st == TT_OK && op >= 2 && loss) st = tt::mlp_last_loss(m, n, loss);
     if (st == TT_OK) {
-        e = hipStreamSynchronize(c->stream);
+        e = hipStreamSynchronize(tt::api_stream(c));
         if (e != hipSuccess) st = tt::hip_fail(c, e, "tt_mlp_run");
     }
     (void)hipFree(d);

@@ -53,20 +53,21 @@ namespace tt {
 
 // Path buffers: throughput, radiance, 2x2 queue arrays, hits, 4 shadow
 // arrays (float4 each) and the generator words.
-tt_status ensure_paths(tt_context* c, size_t paths, PathBuffers& b) {
-    if (paths > c->path_capacity) {
-        if (c->path_memory) {
-            TT_HIP(c, hipStreamSynchronize(c->stream));
-            TT_HIP(c, hipFree(c->path_memory));
-            c->path_memory = nullptr;
-            c->path_capacity = 0;
+tt_status ensure_paths(tt_context* c, uint32_t lane, size_t paths, PathBuffers& b) {
+    Lane& L = c->lanes[lane];
+    if (paths > L.path_capacity) {
+        if (L.path_memory) {
+            TT_HIP(c, hipStreamSynchronize(api_stream(c)));
+            TT_HIP(c, hipFree(L.path_memory));
+            L.path_memory = nullptr;
+            L.path_capacity = 0;
         }
         size_t bytes = paths * PATH_BYTES;
-        TT_HIP(c, hipMalloc(&c->path_memory, bytes));
-        c->path_capacity = paths;
+        TT_HIP(c, hipMalloc(&L.path_memory, bytes));
+        L.path_capacity = paths;
     }
-    size_t n = c->path_capacity;
-    float4* f = static_cast<float4*>(c->path_memory);
+    size_t n = L.path_capacity;
+    float4* f = static_cast<float4*>(L.path_memory);
     b.throughput = f + 0 * n;
     b.radiance = f + 1 * n;
     b.queue[0][0] = f + 2 * n;
@@ -79,19 +80,30 @@ tt_status ensure_paths(tt_context* c, size_t paths, PathBuffers& b) {
     b.shadow[2] = f + 9 * n;
     b.shadow[3] = f + 10 * n;
     b.rng = reinterpret_cast<uint64_t*>(f + 11 * n);
-    b.counters = c->counters;
-    b.spill = c->spill;
+    b.counters = L.counters;
+    b.spill = L.spill;
     b.cache = CacheBuffers{};
     b.restir_final = nullptr;
     b.restir_initial = nullptr;
     return TT_OK;
 }
 
+hipStream_t api_stream(tt_context* c) {
+    for (Lane& L : c->lanes)
+        if (L.pending) {
+            // A failed wait leaves the order to the synchronisations that
+            // every reader of render results performs anyway.
+            (void)hipStreamWaitEvent(c->stream, L.done, 0);
+            L.pending = false;
+        }
+    return c->stream;
+}
+
 // Radiance-cache queues for `paths` paths (grown on demand).
 tt_status ensure_cache(tt_context* c, size_t paths, CacheBuffers& cb) {
     if (paths > c->cache_capacity) {
         if (c->cache_memory) {
-            TT_HIP(c, hipStreamSynchronize(c->stream));
+            TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
             TT_HIP(c, hipFree(c->cache_memory));
             c->cache_memory = nullptr;
             c->cache_capacity = 0;
@@ -156,17 +168,21 @@ tt_status tt_context_create(const tt_context_desc* desc, tt_context** out) {
     c->cu_count = prop.multiProcessorCount;
     c->grid = tt::trace_grid(c->cu_count);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(TT_HIP_ERROR);
-    if (hipMalloc(&c->counters, tt::C_WORDS * sizeof(uint32_t)) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
+    if (hipEventCreateWithFlags(&c->api_mark, hipEventDisableTiming) != hipSuccess) return cleanup(TT_HIP_ERROR);
     if (hipMalloc(&c->work, 256) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
-    if (hipMalloc(&c->spill, tt::spill_words(c->grid) * sizeof(uint32_t)) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
-    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) return cleanup(TT_HIP_ERROR);
-    if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess)
-        return cleanup(TT_HIP_ERROR);
-    if (hipMalloc(&c->side_spill, tt::spill_words(c->grid) * sizeof(uint32_t)) != hipSuccess)
-        return cleanup(TT_OUT_OF_MEMORY);
-    if (hipEventCreate(&c->integrate.begin) != hipSuccess || hipEventCreate(&c->integrate.end) != hipSuccess)
-        return cleanup(TT_HIP_ERROR);
+    const size_t spill_bytes = tt::spill_words(c->grid) * sizeof(uint32_t);
+    for (tt::Lane& L : c->lanes) {
+        if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking) != hipSuccess)
+            return cleanup(TT_HIP_ERROR);
+        for (hipEvent_t* e : {&L.fork, &L.join, &L.restir_read, &L.accumulated, &L.done})
+            if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return cleanup(TT_HIP_ERROR);
+        if (hipEventCreate(&L.integrate.begin) != hipSuccess || hipEventCreate(&L.integrate.end) != hipSuccess)
+            return cleanup(TT_HIP_ERROR);
+        if (hipMalloc(&L.counters, tt::C_WORDS * sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc(&L.spill, spill_bytes) != hipSuccess || hipMalloc(&L.side_spill, spill_bytes) != hipSuccess)
+            return cleanup(TT_OUT_OF_MEMORY);
+    }
     *out = c;
     return TT_OK;
 }
@@ -178,27 +194,32 @@ namespace {
 void context_free(tt_context* c) {
     // Teardown is best effort: errors here have no caller to report to.
     (void)hipSetDevice(c->device);
+    for (tt::Lane& L : c->lanes) {
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
+        if (L.side) (void)hipStreamSynchronize(L.side);
+    }
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->side) (void)hipStreamSynchronize(c->side);
-    if (c->path_memory) (void)hipFree(c->path_memory);
+    for (tt::Lane& L : c->lanes) {
+        if (L.path_memory) (void)hipFree(L.path_memory);
+        if (L.counters) (void)hipFree(L.counters);
+        if (L.spill) (void)hipFree(L.spill);
+        if (L.side_spill) (void)hipFree(L.side_spill);
+        for (hipEvent_t e : {L.fork, L.join, L.restir_read, L.accumulated, L.done, L.integrate.begin, L.integrate.end})
+            if (e) (void)hipEventDestroy(e);
+        if (L.stream) (void)hipStreamDestroy(L.stream);
+        if (L.side) (void)hipStreamDestroy(L.side);
+    }
+    if (c->api_mark) (void)hipEventDestroy(c->api_mark);
     if (c->cache_memory) (void)hipFree(c->cache_memory);
     if (c->display_memory) (void)hipFree(c->display_memory);
     if (c->post_process.begin) (void)hipEventDestroy(c->post_process.begin);
     if (c->post_process.end) (void)hipEventDestroy(c->post_process.end);
-    if (c->counters) (void)hipFree(c->counters);
     if (c->work) (void)hipFree(c->work);
-    if (c->spill) (void)hipFree(c->spill);
-    if (c->side_spill) (void)hipFree(c->side_spill);
-    if (c->fork) (void)hipEventDestroy(c->fork);
-    if (c->join) (void)hipEventDestroy(c->join);
-    if (c->integrate.begin) (void)hipEventDestroy(c->integrate.begin);
-    if (c->integrate.end) (void)hipEventDestroy(c->integrate.end);
     for (auto& e : c->trace_events) {
         (void)hipEventDestroy(e.begin);
         (void)hipEventDestroy(e.end);
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
-    if (c->side) (void)hipStreamDestroy(c->side);
     delete c;
 }
 
@@ -400,9 +421,9 @@ tt_status tt_scene_create_ex(tt_context* c, const tt_scene_desc* d, const tt_sce
         if (e == hipSuccess) {
             s->allocations.push_back(lights);
             s->info.device_bytes += bytes;
-            e = tt::launch_light_records(c->stream, s->view, lights);
+            e = tt::launch_light_records(tt::api_stream(c), s->view, lights);
         }
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(tt::api_stream(c));
         if (e != hipSuccess) {
             tt_scene_destroy(s);
             return hip_fail(c, e, "light records");
@@ -427,7 +448,7 @@ void tt_scene_destroy(tt_scene* s) {
     if (!s) return;
     if (s->context) {
         (void)hipSetDevice(s->context->device);
-        (void)hipStreamSynchronize(s->context->stream);
+        (void)hipStreamSynchronize(tt::api_stream(s->context));
     }
     for (void* p : s->allocations) (void)hipFree(p);
     tt_context* c = s->context;
@@ -452,7 +473,7 @@ tt_status tt_target_create(tt_context* c, uint32_t width, uint32_t height, tt_ta
     t->height = height;
     hipError_t e = hipSetDevice(c->device);
     if (e == hipSuccess) e = hipMalloc(&t->pixels, size_t(width) * height * sizeof(float4));
-    if (e == hipSuccess) e = hipMemsetAsync(t->pixels, 0, size_t(width) * height * sizeof(float4), c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(t->pixels, 0, size_t(width) * height * sizeof(float4), tt::api_stream(c));
     if (e != hipSuccess) {
         tt_target_destroy(t);
         return hip_fail(c, e, "target allocation");
@@ -465,7 +486,7 @@ void tt_target_destroy(tt_target* t) {
     if (!t) return;
     if (t->context) {
         (void)hipSetDevice(t->context->device);
-        (void)hipStreamSynchronize(t->context->stream);
+        (void)hipStreamSynchronize(tt::api_stream(t->context));
     }
     if (t->pixels) (void)hipFree(t->pixels);
     if (t->restir_memory) (void)hipFree(t->restir_memory);
@@ -502,11 +523,18 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     if (paths64 >= (1ull << 31)) return fail(c, TT_INVALID_ARGUMENT, "too many paths in one render");
     uint32_t paths = uint32_t(paths64);
     TT_HIP(c, hipSetDevice(c->device));
+    bool stats = (st.flags & TT_RENDER_COLLECT_STATS) != 0;
+    // Renders that collect statistics or use the radiance cache run on the
+    // API stream with lane 0's buffers (their results are read back at once,
+    // and the cache's own passes run there); the others take the next lane.
+    const bool on_api = stats || cache != nullptr;
+    const uint32_t lane = on_api ? 0u : c->next_lane;
+    tt::Lane& L = c->lanes[lane];
+    const tt::Lane& other = c->lanes[lane ^ 1u];
     tt::PathBuffers b;
-    tt_status es = tt::ensure_paths(c, paths, b);
+    tt_status es = tt::ensure_paths(c, lane, paths, b);
     if (es != TT_OK) return es;
     b.max_bounces = k->bounce_count;
-    bool stats = (st.flags & TT_RENDER_COLLECT_STATS) != 0;
     if (stats) {
         es = ensure_events(c, 2 * k->bounce_count + 1);
         if (es != TT_OK) return es;
@@ -555,9 +583,22 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
         if (cs != TT_OK) return cs;
     }
 
-    hipStream_t stream = c->stream;
-    TT_HIP(c, hipEventRecord(c->integrate.begin, stream));
-    TT_HIP(c, tt::launch_zero(stream, c->counters, tt::C_WORDS));
+    hipStream_t stream;
+    if (on_api) {
+        stream = tt::api_stream(c);
+    } else {
+        // After the API stream's work so far (uploads, target clears); the
+        // lane's previous render is ahead on its own stream.
+        TT_HIP(c, hipEventRecord(c->api_mark, c->stream));
+        stream = L.stream;
+        TT_HIP(c, hipStreamWaitEvent(stream, c->api_mark, 0));
+    }
+    // The previous render, if on the other lane, may still be running: this
+    // one reads and rewrites the target's ReSTIR buffers only after that
+    // render's last read of them, and accumulates after it.
+    const bool after_other = !on_api && other.used;
+    TT_HIP(c, hipEventRecord(L.integrate.begin, stream));
+    TT_HIP(c, tt::launch_zero(stream, L.counters, tt::C_WORDS));
     TT_HIP(c, tt::launch_generate(stream, f, b, paths));
     uint32_t ev = 0;
     // Bounce b's shadow rays (side stream) overlap bounce b + 1's extension
@@ -565,7 +606,7 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     // for them, since the shadow pass adds to the paths' radiance and its
     // buffers are refilled by the next shade.
     tt::PathBuffers side_buffers = b;
-    side_buffers.spill = c->side_spill;
+    side_buffers.spill = L.side_spill;
     const bool shadows = f.light_sampling != TT_LIGHT_SAMPLING_ON_HIT && s->view.emissive_count > 0;
     auto extend = [&](uint32_t bounce) -> tt_status {
         if (stats) TT_HIP(c, hipEventRecord(c->trace_events[ev].begin, stream));
@@ -580,6 +621,7 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     if (es != TT_OK) return es;
     for (uint32_t bounce = 0; bounce < k->bounce_count; bounce++) {
         if (restir && bounce == 0 && s->view.emissive_count > 0) {
+            if (after_other) TT_HIP(c, hipStreamWaitEvent(stream, other.restir_read, 0));
             if (stats) TT_HIP(c, hipEventRecord(c->trace_events[ev].begin, stream));
             TT_HIP(c, tt::launch_restir(stream, s->view, f, b, rb, c->grid));
             if (stats) {
@@ -587,38 +629,49 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
                 TT_HIP(c, hipEventRecord(c->trace_events[ev++].end, stream));
             }
         }
-        if (shadows && bounce > 0) TT_HIP(c, hipStreamWaitEvent(stream, c->join, 0));
+        if (shadows && bounce > 0) TT_HIP(c, hipStreamWaitEvent(stream, L.join, 0));
         TT_HIP(c, tt::launch_shade(stream, s->view, f, b, bounce, paths));
+        // Shade 0 is the last reader of the ReSTIR buffers.
+        if (bounce == 0) TT_HIP(c, hipEventRecord(L.restir_read, stream));
         if (shadows) {
-            TT_HIP(c, hipEventRecord(c->fork, stream));
-            TT_HIP(c, hipStreamWaitEvent(c->side, c->fork, 0));
-            if (stats) TT_HIP(c, hipEventRecord(c->trace_events[ev].begin, c->side));
-            TT_HIP(c, tt::launch_shadow(c->side, s->view, side_buffers, bounce, c->grid));
+            TT_HIP(c, hipEventRecord(L.fork, stream));
+            TT_HIP(c, hipStreamWaitEvent(L.side, L.fork, 0));
+            if (stats) TT_HIP(c, hipEventRecord(c->trace_events[ev].begin, L.side));
+            TT_HIP(c, tt::launch_shadow(L.side, s->view, side_buffers, bounce, c->grid));
             if (stats) {
                 c->trace_event_shadow[ev] = 1;
-                TT_HIP(c, hipEventRecord(c->trace_events[ev++].end, c->side));
+                TT_HIP(c, hipEventRecord(c->trace_events[ev++].end, L.side));
             }
-            TT_HIP(c, hipEventRecord(c->join, c->side));
+            TT_HIP(c, hipEventRecord(L.join, L.side));
         }
         if (bounce + 1 < k->bounce_count) {
             es = extend(bounce + 1);
             if (es != TT_OK) return es;
         }
     }
-    if (shadows) TT_HIP(c, hipStreamWaitEvent(stream, c->join, 0));
+    if (shadows) TT_HIP(c, hipStreamWaitEvent(stream, L.join, 0));
     if (cache) {
         // Cached radiance for the paths that ended at a query.
         tt_status cs = tt::mlp_infer_device(cache, b.cache.query_rec, b.cache.query_scale,
-                                            c->counters + tt::C_CACHE_QUERY, 0, nullptr, b.radiance);
+                                            L.counters + tt::C_CACHE_QUERY, 0, nullptr, b.radiance);
         if (cs != TT_OK) return cs;
     }
+    if (after_other) TT_HIP(c, hipStreamWaitEvent(stream, other.accumulated, 0));
     TT_HIP(c, tt::launch_accumulate(stream, f, b, t->pixels));
-    TT_HIP(c, hipEventRecord(c->integrate.end, stream));
+    TT_HIP(c, hipEventRecord(L.accumulated, stream));
+    TT_HIP(c, hipEventRecord(L.integrate.end, stream));
+    TT_HIP(c, hipEventRecord(L.done, stream));
+    L.used = true;
+    c->last_lane = lane;
+    if (!on_api) {
+        L.pending = true;
+        c->next_lane = lane ^ 1u;
+    }
     if (cache) {
         // Train on this frame's samples: train_steps Adam steps over
         // consecutive slices of the records.
         uint32_t counts[2];
-        TT_HIP(c, hipMemcpyAsync(counts, c->counters + tt::C_CACHE_QUERY, sizeof(counts), hipMemcpyDeviceToHost,
+        TT_HIP(c, hipMemcpyAsync(counts, L.counters + tt::C_CACHE_QUERY, sizeof(counts), hipMemcpyDeviceToHost,
                                  stream));
         TT_HIP(c, hipStreamSynchronize(stream));
         uint32_t n = counts[1];
@@ -681,17 +734,18 @@ tt_status tt_target_download(tt_context* c, const tt_target* t, float* rgba, siz
     size_t need = size_t(t->width) * t->height * sizeof(float4);
     if (bytes < need) return fail(c, TT_INVALID_ARGUMENT, "download buffer too small");
     TT_HIP(c, hipSetDevice(c->device));
-    TT_HIP(c, hipMemcpyAsync(rgba, t->pixels, need, hipMemcpyDeviceToHost, c->stream));
-    TT_HIP(c, hipStreamSynchronize(c->stream));
+    TT_HIP(c, hipMemcpyAsync(rgba, t->pixels, need, hipMemcpyDeviceToHost, tt::api_stream(c)));
+    TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
     return TT_OK;
 }
 
 tt_status tt_timing_ms(tt_context* c, const char* name, double* out) {
     if (!c || !name || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
-    TT_HIP(c, hipEventSynchronize(c->integrate.end));
+    const tt::EventPair& integrate = c->lanes[c->last_lane].integrate;
+    TT_HIP(c, hipEventSynchronize(integrate.end));
     if (std::strcmp(name, "integrate") == 0) {
         float ms = 0.0f;
-        TT_HIP(c, hipEventElapsedTime(&ms, c->integrate.begin, c->integrate.end));
+        TT_HIP(c, hipEventElapsedTime(&ms, integrate.begin, integrate.end));
         *out = ms;
         return TT_OK;
     }
@@ -721,9 +775,9 @@ tt_status tt_timing_ms(tt_context* c, const char* name, double* out) {
 tt_status tt_render_stats_get(tt_context* c, tt_render_stats* out) {
     if (!c || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     if (!c->stats_pending) return fail(c, TT_INVALID_ARGUMENT, "render without TT_RENDER_COLLECT_STATS");
-    TT_HIP(c, hipStreamSynchronize(c->stream));
+    TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
     uint32_t words[tt::C_WORDS];
-    TT_HIP(c, hipMemcpy(words, c->counters, sizeof(words), hipMemcpyDeviceToHost));
+    TT_HIP(c, hipMemcpy(words, c->lanes[c->last_lane].counters, sizeof(words), hipMemcpyDeviceToHost));
     *out = tt_render_stats{};
     out->paths = c->last_paths;
     for (uint32_t b = 0; b < c->last_bounces; b++) {
@@ -745,7 +799,7 @@ tt_status tt_render_stats_get(tt_context* c, tt_render_stats* out) {
         }
     }
     float ms = 0.0f;
-    TT_HIP(c, hipEventElapsedTime(&ms, c->integrate.begin, c->integrate.end));
+    TT_HIP(c, hipEventElapsedTime(&ms, c->lanes[c->last_lane].integrate.begin, c->lanes[c->last_lane].integrate.end));
     out->integrate_ms = ms;
     return TT_OK;
 }
@@ -756,9 +810,9 @@ tt_status tt_trace_closest_device(tt_context* c, const tt_scene* s, const tt_ray
     if (s->context != c) return fail(c, TT_STALE_HANDLE, "scene from another context");
     if (count == 0) return TT_OK;
     TT_HIP(c, hipSetDevice(c->device));
-    TT_HIP(c, tt::launch_zero(c->stream, c->work, 1));
-    TT_HIP(c, tt::launch_trace_rays(c->stream, s->view, rays, hits, count, (flags & TT_TRACE_CULL_BACK_FACES) != 0,
-                                    c->work, c->spill, c->grid));
+    TT_HIP(c, tt::launch_zero(tt::api_stream(c), c->work, 1));
+    TT_HIP(c, tt::launch_trace_rays(tt::api_stream(c), s->view, rays, hits, count, (flags & TT_TRACE_CULL_BACK_FACES) != 0,
+                                    c->work, c->lanes[0].spill, c->grid));
     return TT_OK;
 }
 
@@ -771,12 +825,12 @@ tt_status tt_trace_closest(tt_context* c, const tt_scene* s, const tt_ray* rays,
     tt_hit* dh = nullptr;
     TT_HIP(c, hipMalloc(&dr, size_t(count) * sizeof(tt_ray)));
     hipError_t e = hipMalloc(&dh, size_t(count) * sizeof(tt_hit));
-    if (e == hipSuccess) e = hipMemcpyAsync(dr, rays, size_t(count) * sizeof(tt_ray), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dr, rays, size_t(count) * sizeof(tt_ray), hipMemcpyHostToDevice, tt::api_stream(c));
     tt_status st = TT_OK;
     if (e == hipSuccess) st = tt_trace_closest_device(c, s, dr, dh, count, flags);
     if (e == hipSuccess && st == TT_OK)
-        e = hipMemcpyAsync(hits, dh, size_t(count) * sizeof(tt_hit), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess && st == TT_OK) e = hipStreamSynchronize(c->stream);
+        e = hipMemcpyAsync(hits, dh, size_t(count) * sizeof(tt_hit), hipMemcpyDeviceToHost, tt::api_stream(c));
+    if (e == hipSuccess && st == TT_OK) e = hipStreamSynchronize(tt::api_stream(c));
     (void)hipFree(dr);
     if (dh) (void)hipFree(dh);
     if (e != hipSuccess) return hip_fail(c, e, "tt_trace_closest");
@@ -792,11 +846,11 @@ tt_status tt_bsdf_evaluate(tt_context* c, const tt_material_constants* material,
     tt_bsdf_result* dr = nullptr;
     TT_HIP(c, hipMalloc(&dq, size_t(count) * sizeof(tt_bsdf_query)));
     hipError_t e = hipMalloc(&dr, size_t(count) * sizeof(tt_bsdf_result));
-    if (e == hipSuccess) e = hipMemcpyAsync(dq, queries, size_t(count) * sizeof(tt_bsdf_query), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = tt::launch_bsdf_evaluate(c->stream, material, dq, dr, count, rng_state);
+    if (e == hipSuccess) e = hipMemcpyAsync(dq, queries, size_t(count) * sizeof(tt_bsdf_query), hipMemcpyHostToDevice, tt::api_stream(c));
+    if (e == hipSuccess) e = tt::launch_bsdf_evaluate(tt::api_stream(c), material, dq, dr, count, rng_state);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(results, dr, size_t(count) * sizeof(tt_bsdf_result), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        e = hipMemcpyAsync(results, dr, size_t(count) * sizeof(tt_bsdf_result), hipMemcpyDeviceToHost, tt::api_stream(c));
+    if (e == hipSuccess) e = hipStreamSynchronize(tt::api_stream(c));
     (void)hipFree(dq);
     if (dr) (void)hipFree(dr);
     if (e != hipSuccess) return hip_fail(c, e, "tt_bsdf_evaluate");
@@ -809,11 +863,11 @@ tt_status tt_debug_math(tt_context* c, int op, const float* x, const float* y, f
     TT_HIP(c, hipSetDevice(c->device));
     float* d = nullptr;
     TT_HIP(c, hipMalloc(&d, size_t(count) * 3 * sizeof(float)));
-    hipError_t e = hipMemcpyAsync(d, x, count * sizeof(float), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(d + count, y, count * sizeof(float), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = tt::launch_math(c->stream, op, d, d + count, d + 2 * count, count);
-    if (e == hipSuccess) e = hipMemcpyAsync(out, d + 2 * count, count * sizeof(float), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipError_t e = hipMemcpyAsync(d, x, count * sizeof(float), hipMemcpyHostToDevice, tt::api_stream(c));
+    if (e == hipSuccess) e = hipMemcpyAsync(d + count, y, count * sizeof(float), hipMemcpyHostToDevice, tt::api_stream(c));
+    if (e == hipSuccess) e = tt::launch_math(tt::api_stream(c), op, d, d + count, d + 2 * count, count);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d + 2 * count, count * sizeof(float), hipMemcpyDeviceToHost, tt::api_stream(c));
+    if (e == hipSuccess) e = hipStreamSynchronize(tt::api_stream(c));
     (void)hipFree(d);
     if (e != hipSuccess) return hip_fail(c, e, "tt_debug_math");
     return TT_OK;

@@ -19,26 +19,52 @@ struct EventPair {
 
 }  // namespace tt
 
+namespace tt {
+
+// Where a render runs.  Consecutive renders alternate between two lanes, so
+// that one frame's last, latency-bound bounces overlap the next frame's
+// first ones; within a lane, bounce b's shadow rays run on `side` beside
+// bounce b + 1's extension on `stream` (fork after shade b, join before
+// shade b + 1).  Each lane has its own path buffers, counters and traversal
+// stack spill areas.
+struct Lane {
+    hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    // Cross-lane order: the next render on the other lane starts its ReSTIR
+    // passes after `restir_read` (this render's last read of the target's
+    // ReSTIR buffers) and accumulates after `accumulated`.
+    hipEvent_t restir_read = nullptr, accumulated = nullptr;
+    hipEvent_t done = nullptr;  // end of the lane's last render
+    EventPair integrate;
+    bool pending = false;       // a render the API stream has not waited for
+    bool used = false;          // restir_read / accumulated have been recorded
+    uint32_t* counters = nullptr;
+    uint32_t* spill = nullptr;
+    uint32_t* side_spill = nullptr;
+    size_t path_capacity = 0;
+    void* path_memory = nullptr;
+};
+
+}  // namespace tt
+
 struct tt_context {
     int device = 0;
+    // The API stream: uploads, downloads, target clears, display, serving,
+    // the radiance cache and renders that collect statistics or use the
+    // cache.  Other renders run on the lanes; code outside them takes the
+    // stream through tt::api_stream, which first orders it after every
+    // render enqueued so far.
     hipStream_t stream = nullptr;
     std::string error;
     int cu_count = 0;
     int grid = 0;
-    // Wavefront buffers, grown on demand.
-    size_t path_capacity = 0;
-    void* path_memory = nullptr;
-    uint32_t* counters = nullptr;
-    uint32_t* spill = nullptr;
+    tt::Lane lanes[2];
+    uint32_t next_lane = 0;
+    uint32_t last_lane = 0;  // lane of the last render (its counters and timing)
+    hipEvent_t api_mark = nullptr;
     uint32_t* work = nullptr;
-    // Shadow rays of bounce b run on `side` while bounce b + 1 extends on
-    // `stream` (fork after shade b, join before shade b + 1), with their own
-    // traversal-stack spill area.
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
-    uint32_t* side_spill = nullptr;
     // Timing.
-    tt::EventPair integrate;
     std::vector<tt::EventPair> trace_events;
     std::vector<uint8_t> trace_event_shadow;
     uint32_t trace_events_used = 0;
@@ -125,7 +151,9 @@ tt_status mlp_last_loss(tt_mlp* m, uint32_t n, double* out);
 
 // Path buffers of at least `paths` paths (grown on demand, kept in the
 // context); fills the pointers of `b`.
-tt_status ensure_paths(tt_context* c, size_t paths, PathBuffers& b);
+tt_status ensure_paths(tt_context* c, uint32_t lane, size_t paths, PathBuffers& b);
+// The API stream, after enqueueing waits for every render not yet waited for.
+hipStream_t api_stream(tt_context* c);
 // Bytes of path state per path slot.
 constexpr size_t PATH_BYTES = 11 * sizeof(float4) + sizeof(uint64_t);
 // Per-render arguments of the kernels from the reference Constants.

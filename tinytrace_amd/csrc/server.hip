@@ -118,7 +118,7 @@ tt_status admit(tt_server* sv) {
         TT_HIP(c, hipMalloc(&r->target, r->target_bytes()));
         sv->used += r->target_bytes();
         // Pass 0 computes (previous * 0 + new) / 1: start from zeros.
-        TT_HIP(c, tt::launch_zero(c->stream, reinterpret_cast<uint32_t*>(r->target), uint32_t(r->pixels() * 4)));
+        TT_HIP(c, tt::launch_zero(tt::api_stream(c), reinterpret_cast<uint32_t*>(r->target), uint32_t(r->pixels() * 4)));
         r->state = TT_REQUEST_ADMITTED;
         r->admitted_ns = sv->now_ns();
         sv->queued.pop_front();
@@ -152,7 +152,7 @@ tt_status tt_server_create(tt_context* c, const tt_server_desc* desc, tt_server*
 void tt_server_destroy(tt_server* sv) {
     if (!sv) return;
     (void)hipSetDevice(sv->context->device);
-    (void)hipStreamSynchronize(sv->context->stream);
+    (void)hipStreamSynchronize(tt::api_stream(sv->context));
     for (auto& kv : sv->requests)
         if (kv.second->target) (void)hipFree(kv.second->target);
     tt_context* c = sv->context;
@@ -223,7 +223,7 @@ tt_status tt_server_step(tt_server* sv, uint32_t* completed) {
     const uint32_t paths = base;
 
     tt::PathBuffers b;
-    st = tt::ensure_paths(c, sv->pool_paths, b);
+    st = tt::ensure_paths(c, 0, sv->pool_paths, b);
     if (st != TT_OK) return st;
     const tt_scene* scene = sv->scenes[lead->desc.scene];
     const tt_constants& lk = lead->desc.constants;
@@ -237,8 +237,8 @@ tt_status tt_server_step(tt_server* sv, uint32_t* completed) {
         return tt::frame_args(k, ch.request->width, ch.request->height, ch.request->desc.rng, ch.request->desc.seed);
     };
 
-    hipStream_t stream = c->stream;
-    TT_HIP(c, tt::launch_zero(stream, c->counters, tt::C_WORDS));
+    hipStream_t stream = tt::api_stream(c);
+    TT_HIP(c, tt::launch_zero(stream, c->lanes[0].counters, tt::C_WORDS));
     for (const Chunk& ch : chunks)
         TT_HIP(c, tt::launch_generate_chunk(stream, chunk_args(ch), b, ch.base, ch.pixel0, ch.count));
     tt::FrameArgs f = chunk_args(chunks.front());  // configuration fields are the group's
