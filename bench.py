@@ -16,6 +16,11 @@ import os
 import sys
 import time
 
+# Eight hardware queues (HIP's default is four) give each of the library's
+# two render lanes its own shadow-ray stream; read by the HIP runtime when it
+# starts, so set before anything initialises it.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -99,23 +104,18 @@ def main() -> int:
     ms_per_step = elapsed / args.steps * 1e3
     mpaths = paths * args.steps / elapsed / 1e6
 
-    # Ray counts and per-kernel durations of a few more passes (HIP events on
-    # the context's stream around every closest-hit launch).
+    # Ray counts and per-kernel durations of a few more passes, enqueued back
+    # to back like the timed ones (HIP events on the stream of every
+    # closest-hit launch; the library sums them over the passes).
     renderer.settings = dataclasses.replace(settings, collect_stats=True)
-    ext_rays = shadow_rays = restir_rays = 0
-    ext_ms = shadow_ms = restir_ms = 0.0
-    ext_launches = 0
     passes = 5
     for _ in range(passes):
         step()
-        s = renderer.stats()
-        ext_rays += s["extension_rays"]
-        shadow_rays += s["shadow_rays"]
-        ext_ms += s["extend_ms"]
-        shadow_ms += s["shadow_ms"]
-        restir_rays += s["restir_rays"]
-        restir_ms += s["restir_ms"]
-        ext_launches += s["extend_launches"]
+    s = renderer.stats()
+    ext_rays, shadow_rays, restir_rays = s["extension_rays"], s["shadow_rays"], s["restir_rays"]
+    ext_ms, shadow_ms, restir_ms = s["extend_ms"], s["shadow_ms"], s["restir_ms"]
+    ext_launches = s["extend_launches"]
+    assert s["paths"] == passes * paths
     rays_per_path = (ext_rays + shadow_rays + restir_rays) / (passes * paths)
     mrays = mpaths * rays_per_path
     bytes_per_ray = RAY_RECORD_BYTES + HIT_RECORD_BYTES + info["expected_bytes_per_ray"]

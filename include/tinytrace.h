@@ -285,7 +285,10 @@ typedef struct tt_render_settings {
 
 /* Integrate + accumulate into `target`, as `Integrator::integrate`
  * (src/integrate/mod.rs:39-61) dispatching integrate.glsl::main
- * (src/integrate/integrate.glsl:323-347).  Asynchronous. */
+ * (src/integrate/integrate.glsl:323-347).  Asynchronous: renders enqueued
+ * back to back overlap on the device (a frame's last bounces with the next
+ * frame's first) with the results of renders run one at a time; every other
+ * call on the context is ordered after the renders enqueued before it. */
 tt_status tt_render(tt_context* context, const tt_scene* scene, const tt_constants* constants,
                     tt_target* target);
 tt_status tt_render_ex(tt_context* context, const tt_scene* scene, const tt_constants* constants,
@@ -310,7 +313,7 @@ tt_status tt_target_display(tt_context* context, const tt_target* target, const 
  * (crates/tinytrace_backend/src/timing.rs:61-69).  Names: "integrate" (last
  * render), "post_process" (last tt_target_display); with
  * TT_RENDER_COLLECT_STATS also "trace" (sum over the last render's extension
- * and shadow closest-hit launches). */
+ * and shadow closest-hit launches, until tt_render_stats_get reads them). */
 tt_status tt_timing_ms(tt_context* context, const char* name, double* out);
 
 typedef struct tt_render_stats {
@@ -325,7 +328,10 @@ typedef struct tt_render_stats {
     uint64_t restir_rays;      /* ReSTIR visibility-reuse queries */
     double restir_ms;          /* ReSTIR passes (candidates, visibility, reuse) */
 } tt_render_stats;
-/* Statistics of the last render with TT_RENDER_COLLECT_STATS (synchronises). */
+/* Statistics summed over the renders with TT_RENDER_COLLECT_STATS since the
+ * last call (synchronises; the next call starts a new sum).  Renders enqueued
+ * back to back overlap on the device, so the summed durations of launches
+ * (and of whole renders, integrate_ms) can exceed the wall time. */
 tt_status tt_render_stats_get(tt_context* context, tt_render_stats* out);
 
 /* ---------------------------------------------------------------------- */

@@ -1,6 +1,6 @@
 """Renders enqueued back to back overlap on the device (consecutive renders
-alternate between two lanes; each bounce's shadow rays overlap the next
-bounce's extension).  Their results must equal, bit for bit, renders that
+alternate between two lanes; with enough hardware queues each bounce's
+shadow rays also overlap the next bounce's extension).  Their results must equal, bit for bit, renders that
 are waited for one by one: progressive accumulation order, ReSTIR temporal
 history across frames, two targets interleaved on one context, and API calls
 (download, clear) between renders."""
@@ -14,6 +14,14 @@ from tinytrace_amd import Config, LightSampling, Renderer, RenderSettings, Targe
 pytestmark = pytest.mark.gpu
 
 MODES = [LightSampling.NEXT_EVENT_ESTIMATION, LightSampling.RESTIR, LightSampling.ON_HIT]
+
+
+@pytest.fixture(autouse=True, params=[False, True], ids=["one-stream-lanes", "side-streams"])
+def side_streams(request, monkeypatch):
+    """Contexts created in the test use shadow side streams or not (the
+    library decides from GPU_MAX_HW_QUEUES at context creation)."""
+    monkeypatch.setenv("GPU_MAX_HW_QUEUES", "8" if request.param else "4")
+    return request.param
 
 
 def make_renderer(mode, size=(320, 200)):
@@ -94,3 +102,35 @@ def test_api_calls_between_renders():
     w.target.download()
     w.render_to_target()
     np.testing.assert_array_equal(end, w.target.download())
+
+
+def test_statistics_sum_over_back_to_back_renders():
+    """tt_render_stats_get sums the renders with collect_stats since the last
+    read: three renders enqueued back to back count the same rays and
+    launches as the same three renders read one at a time."""
+    import dataclasses
+
+    from tinytrace_amd import TinytraceError
+
+    frames, size = 3, (320, 200)
+    a = make_renderer(LightSampling.NEXT_EVENT_ESTIMATION, size)
+    a.settings = dataclasses.replace(a.settings, collect_stats=True)
+    for _ in range(frames):
+        a.render_to_target()
+    trace_ms = a.timings()["trace"]
+    s = a.stats()
+    b = make_renderer(LightSampling.NEXT_EVENT_ESTIMATION, size)
+    b.settings = dataclasses.replace(b.settings, collect_stats=True)
+    each = []
+    for _ in range(frames):
+        b.render_to_target()
+        each.append(b.stats())
+    paths = size[0] * size[1]
+    assert s["paths"] == frames * paths
+    for key in ("extension_rays", "shadow_rays", "extend_launches", "shadow_launches"):
+        assert s[key] == sum(e[key] for e in each), key
+    assert s["extension_rays"] >= frames * paths  # every path's primary ray
+    assert s["extend_launches"] == frames * 6 and s["shadow_launches"] == frames * 6
+    assert s["extend_ms"] > 0 and s["shadow_ms"] > 0 and s["integrate_ms"] > 0 and trace_ms > 0
+    with pytest.raises(TinytraceError):
+        a.stats()  # nothing rendered since the last read

@@ -455,6 +455,8 @@ class Renderer:
         return out
 
     def stats(self) -> dict:
+        """tt_render_stats_get: ray counts and launch timings summed over the
+        renders with collect_stats since the last call."""
         s = _abi.RenderStats()
         check(lib().tt_render_stats_get(self.context.handle, C.byref(s)), self.context.handle, "stats")
         return {f: getattr(s, f) for f, _ in _abi.RenderStats._fields_}

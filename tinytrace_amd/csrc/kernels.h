@@ -102,6 +102,7 @@ constexpr uint32_t C_CACHE_TRAIN = 131;  // radiance-cache training records
 constexpr uint32_t C_WORDS = 160;
 
 hipError_t launch_zero(hipStream_t s, uint32_t* p, uint32_t words);
+hipError_t launch_stats_add(hipStream_t s, const uint32_t* counters, unsigned long long* sum, uint32_t words);
 hipError_t launch_trace_rays(hipStream_t s, const DeviceScene& scene, const tt_ray* rays, tt_hit* hits, uint32_t count,
                              bool cull, uint32_t* work, uint32_t* spill, int grid);
 hipError_t launch_generate(hipStream_t s, const FrameArgs& f, const PathBuffers& b, uint32_t paths);

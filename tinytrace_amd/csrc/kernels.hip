@@ -436,6 +436,13 @@ __global__ void zero_kernel(uint32_t* p, uint32_t n) {
     if (i < n) p[i] = 0;
 }
 
+// sum[i] += counters[i]: a render's ray counters into the statistics sums
+// (renders on both lanes may add at once).
+__global__ void stats_add_kernel(const uint32_t* counters, unsigned long long* sum, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && counters[i]) atomicAdd(sum + i, (unsigned long long)counters[i]);
+}
+
 // BdsfSampler (src/material/test.rs:38-83): prepare from a fixed generator
 // state, then evaluate.
 __global__ void bsdf_evaluate_kernel(tt_material_constants mc, const tt_bsdf_query* q, tt_bsdf_result* out,
@@ -493,6 +500,11 @@ size_t spill_words(int grid) { return size_t(grid) * TRACE_BLOCK * TRACE_SPILL_S
 
 hipError_t launch_zero(hipStream_t s, uint32_t* p, uint32_t words) {
     zero_kernel<<<blocks(words, 256), 256, 0, s>>>(p, words);
+    return hipGetLastError();
+}
+
+hipError_t launch_stats_add(hipStream_t s, const uint32_t* counters, unsigned long long* sum, uint32_t words) {
+    stats_add_kernel<<<blocks(words, 256), 256, 0, s>>>(counters, sum, words);
     return hipGetLastError();
 }
 

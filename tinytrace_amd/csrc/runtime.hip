@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <atomic>
 #include <chrono>
 #include <cstring>
@@ -36,14 +37,28 @@ tt_status upload(tt_context* c, tt_scene* s, const void* src, size_t bytes, cons
     return TT_OK;
 }
 
-tt_status ensure_events(tt_context* c, uint32_t count) {
-    while (c->trace_events.size() < count) {
-        tt::EventPair e;
+// Records the start of a timed launch on `stream`; *index names the record
+// for stats_end.
+tt_status stats_begin(tt_context* c, tt::StatsKind kind, hipStream_t stream, size_t* index) {
+    tt::EventPair e;
+    if (c->stats_pool.empty()) {
         TT_HIP(c, hipEventCreate(&e.begin));
-        TT_HIP(c, hipEventCreate(&e.end));
-        c->trace_events.push_back(e);
-        c->trace_event_shadow.push_back(0);
+        if (hipEventCreate(&e.end) != hipSuccess) {
+            (void)hipEventDestroy(e.begin);
+            return fail(c, TT_HIP_ERROR, "hipEventCreate failed");
+        }
+    } else {
+        e = c->stats_pool.back();
+        c->stats_pool.pop_back();
     }
+    c->stats_launches.push_back(tt::StatsLaunch{e, kind});
+    *index = c->stats_launches.size() - 1;
+    TT_HIP(c, hipEventRecord(e.begin, stream));
+    return TT_OK;
+}
+
+tt_status stats_end(tt_context* c, size_t index, hipStream_t stream) {
+    TT_HIP(c, hipEventRecord(c->stats_launches[index].events.end, stream));
     return TT_OK;
 }
 
@@ -170,10 +185,18 @@ tt_status tt_context_create(const tt_context_desc* desc, tt_context** out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(TT_HIP_ERROR);
     if (hipEventCreateWithFlags(&c->api_mark, hipEventDisableTiming) != hipSuccess) return cleanup(TT_HIP_ERROR);
     if (hipMalloc(&c->work, 256) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
+    if (hipMalloc(&c->stats_sum, tt::C_WORDS * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(c->stats_sum, 0, tt::C_WORDS * sizeof(unsigned long long)) != hipSuccess)
+        return cleanup(TT_OUT_OF_MEMORY);
     const size_t spill_bytes = tt::spill_words(c->grid) * sizeof(uint32_t);
+    // Six queues: the null stream, the API stream and two per lane (HIP's
+    // default is four).
+    const char* queues = std::getenv("GPU_MAX_HW_QUEUES");
+    c->side_streams = queues && std::atoi(queues) >= 6;
     for (tt::Lane& L : c->lanes) {
-        if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking) != hipSuccess)
+        if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess)
+            return cleanup(TT_HIP_ERROR);
+        if (c->side_streams && hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking) != hipSuccess)
             return cleanup(TT_HIP_ERROR);
         for (hipEvent_t* e : {&L.fork, &L.join, &L.restir_read, &L.accumulated, &L.done})
             if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return cleanup(TT_HIP_ERROR);
@@ -215,10 +238,12 @@ void context_free(tt_context* c) {
     if (c->post_process.begin) (void)hipEventDestroy(c->post_process.begin);
     if (c->post_process.end) (void)hipEventDestroy(c->post_process.end);
     if (c->work) (void)hipFree(c->work);
-    for (auto& e : c->trace_events) {
+    for (auto& l : c->stats_launches) c->stats_pool.push_back(l.events);
+    for (auto& e : c->stats_pool) {
         (void)hipEventDestroy(e.begin);
         (void)hipEventDestroy(e.end);
     }
+    if (c->stats_sum) (void)hipFree(c->stats_sum);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -524,10 +549,10 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     uint32_t paths = uint32_t(paths64);
     TT_HIP(c, hipSetDevice(c->device));
     bool stats = (st.flags & TT_RENDER_COLLECT_STATS) != 0;
-    // Renders that collect statistics or use the radiance cache run on the
-    // API stream with lane 0's buffers (their results are read back at once,
-    // and the cache's own passes run there); the others take the next lane.
-    const bool on_api = stats || cache != nullptr;
+    // Renders with the radiance cache run on the API stream with lane 0's
+    // buffers (the cache's own passes run there and read back at once); the
+    // others take the next lane.
+    const bool on_api = cache != nullptr;
     const uint32_t lane = on_api ? 0u : c->next_lane;
     tt::Lane& L = c->lanes[lane];
     const tt::Lane& other = c->lanes[lane ^ 1u];
@@ -535,10 +560,6 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     tt_status es = tt::ensure_paths(c, lane, paths, b);
     if (es != TT_OK) return es;
     b.max_bounces = k->bounce_count;
-    if (stats) {
-        es = ensure_events(c, 2 * k->bounce_count + 1);
-        if (es != TT_OK) return es;
-    }
 
     tt::RestirBuffers rb{};
     b.restir_final = nullptr;
@@ -598,58 +619,72 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     // render's last read of them, and accumulates after it.
     const bool after_other = !on_api && other.used;
     TT_HIP(c, hipEventRecord(L.integrate.begin, stream));
+    size_t whole = 0;
+    if (stats) {
+        c->last_stats_first = c->stats_launches.size();
+        es = stats_begin(c, tt::STATS_RENDER, stream, &whole);
+        if (es != TT_OK) return es;
+    }
     TT_HIP(c, tt::launch_zero(stream, L.counters, tt::C_WORDS));
     TT_HIP(c, tt::launch_generate(stream, f, b, paths));
-    uint32_t ev = 0;
+    size_t ev = 0;
     // Bounce b's shadow rays (side stream) overlap bounce b + 1's extension
     // (main stream): both only read what shade b wrote.  Shade b + 1 waits
     // for them, since the shadow pass adds to the paths' radiance and its
     // buffers are refilled by the next shade.
     tt::PathBuffers side_buffers = b;
     side_buffers.spill = L.side_spill;
+    hipStream_t side = c->side_streams ? L.side : stream;
     const bool shadows = f.light_sampling != TT_LIGHT_SAMPLING_ON_HIT && s->view.emissive_count > 0;
     auto extend = [&](uint32_t bounce) -> tt_status {
-        if (stats) TT_HIP(c, hipEventRecord(c->trace_events[ev].begin, stream));
-        TT_HIP(c, tt::launch_extend(stream, s->view, b, bounce, c->grid));
         if (stats) {
-            c->trace_event_shadow[ev] = 0;
-            TT_HIP(c, hipEventRecord(c->trace_events[ev++].end, stream));
+            tt_status ss = stats_begin(c, tt::STATS_EXTEND, stream, &ev);
+            if (ss != TT_OK) return ss;
         }
-        return TT_OK;
+        TT_HIP(c, tt::launch_extend(stream, s->view, b, bounce, c->grid));
+        return stats ? stats_end(c, ev, stream) : TT_OK;
     };
     es = extend(0);
     if (es != TT_OK) return es;
     for (uint32_t bounce = 0; bounce < k->bounce_count; bounce++) {
         if (restir && bounce == 0 && s->view.emissive_count > 0) {
             if (after_other) TT_HIP(c, hipStreamWaitEvent(stream, other.restir_read, 0));
-            if (stats) TT_HIP(c, hipEventRecord(c->trace_events[ev].begin, stream));
+            if (stats) {
+                es = stats_begin(c, tt::STATS_RESTIR, stream, &ev);
+                if (es != TT_OK) return es;
+            }
             TT_HIP(c, tt::launch_restir(stream, s->view, f, b, rb, c->grid));
             if (stats) {
-                c->trace_event_shadow[ev] = 2;
-                TT_HIP(c, hipEventRecord(c->trace_events[ev++].end, stream));
+                es = stats_end(c, ev, stream);
+                if (es != TT_OK) return es;
             }
         }
-        if (shadows && bounce > 0) TT_HIP(c, hipStreamWaitEvent(stream, L.join, 0));
+        if (shadows && bounce > 0 && side != stream) TT_HIP(c, hipStreamWaitEvent(stream, L.join, 0));
         TT_HIP(c, tt::launch_shade(stream, s->view, f, b, bounce, paths));
         // Shade 0 is the last reader of the ReSTIR buffers.
         if (bounce == 0) TT_HIP(c, hipEventRecord(L.restir_read, stream));
         if (shadows) {
-            TT_HIP(c, hipEventRecord(L.fork, stream));
-            TT_HIP(c, hipStreamWaitEvent(L.side, L.fork, 0));
-            if (stats) TT_HIP(c, hipEventRecord(c->trace_events[ev].begin, L.side));
-            TT_HIP(c, tt::launch_shadow(L.side, s->view, side_buffers, bounce, c->grid));
-            if (stats) {
-                c->trace_event_shadow[ev] = 1;
-                TT_HIP(c, hipEventRecord(c->trace_events[ev++].end, L.side));
+            if (side != stream) {
+                TT_HIP(c, hipEventRecord(L.fork, stream));
+                TT_HIP(c, hipStreamWaitEvent(side, L.fork, 0));
             }
-            TT_HIP(c, hipEventRecord(L.join, L.side));
+            if (stats) {
+                es = stats_begin(c, tt::STATS_SHADOW, side, &ev);
+                if (es != TT_OK) return es;
+            }
+            TT_HIP(c, tt::launch_shadow(side, s->view, side_buffers, bounce, c->grid));
+            if (stats) {
+                es = stats_end(c, ev, side);
+                if (es != TT_OK) return es;
+            }
+            if (side != stream) TT_HIP(c, hipEventRecord(L.join, side));
         }
         if (bounce + 1 < k->bounce_count) {
             es = extend(bounce + 1);
             if (es != TT_OK) return es;
         }
     }
-    if (shadows) TT_HIP(c, hipStreamWaitEvent(stream, L.join, 0));
+    if (shadows && side != stream) TT_HIP(c, hipStreamWaitEvent(stream, L.join, 0));
     if (cache) {
         // Cached radiance for the paths that ended at a query.
         tt_status cs = tt::mlp_infer_device(cache, b.cache.query_rec, b.cache.query_scale,
@@ -659,6 +694,13 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     if (after_other) TT_HIP(c, hipStreamWaitEvent(stream, other.accumulated, 0));
     TT_HIP(c, tt::launch_accumulate(stream, f, b, t->pixels));
     TT_HIP(c, hipEventRecord(L.accumulated, stream));
+    if (stats) {
+        TT_HIP(c, tt::launch_stats_add(stream, L.counters, c->stats_sum, tt::C_WORDS));
+        es = stats_end(c, whole, stream);
+        if (es != TT_OK) return es;
+        c->stats_renders++;
+        c->stats_paths += paths;
+    }
     TT_HIP(c, hipEventRecord(L.integrate.end, stream));
     TT_HIP(c, hipEventRecord(L.done, stream));
     L.used = true;
@@ -693,10 +735,6 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
             cache_stats->loss = loss;
         }
     }
-    c->trace_events_used = stats ? ev : 0;
-    c->stats_pending = stats;
-    c->last_paths = paths;
-    c->last_bounces = k->bounce_count;
     c->last_integrate_ms = -1.0;
     if (restir && s->view.emissive_count > 0) {
         t->restir_cur ^= 1u;
@@ -758,12 +796,15 @@ tt_status tt_timing_ms(tt_context* c, const char* name, double* out) {
         return TT_OK;
     }
     if (std::strcmp(name, "trace") == 0) {
-        if (!c->stats_pending) return fail(c, TT_INVALID_ARGUMENT, "render without TT_RENDER_COLLECT_STATS");
+        if (c->stats_renders == 0)
+            return fail(c, TT_INVALID_ARGUMENT, "no render with TT_RENDER_COLLECT_STATS since the statistics were read");
+        TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
         double sum = 0.0;
-        for (uint32_t i = 0; i < c->trace_events_used; i++) {
-            if (c->trace_event_shadow[i] == 2) continue;  // ReSTIR passes
+        for (size_t i = c->last_stats_first; i < c->stats_launches.size(); i++) {
+            const tt::StatsLaunch& l = c->stats_launches[i];
+            if (l.kind != tt::STATS_EXTEND && l.kind != tt::STATS_SHADOW) continue;
             float ms = 0.0f;
-            TT_HIP(c, hipEventElapsedTime(&ms, c->trace_events[i].begin, c->trace_events[i].end));
+            TT_HIP(c, hipEventElapsedTime(&ms, l.events.begin, l.events.end));
             sum += ms;
         }
         *out = sum;
@@ -774,33 +815,45 @@ tt_status tt_timing_ms(tt_context* c, const char* name, double* out) {
 
 tt_status tt_render_stats_get(tt_context* c, tt_render_stats* out) {
     if (!c || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
-    if (!c->stats_pending) return fail(c, TT_INVALID_ARGUMENT, "render without TT_RENDER_COLLECT_STATS");
+    if (c->stats_renders == 0)
+        return fail(c, TT_INVALID_ARGUMENT, "no render with TT_RENDER_COLLECT_STATS since the statistics were read");
     TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
-    uint32_t words[tt::C_WORDS];
-    TT_HIP(c, hipMemcpy(words, c->lanes[c->last_lane].counters, sizeof(words), hipMemcpyDeviceToHost));
+    unsigned long long words[tt::C_WORDS];
+    TT_HIP(c, hipMemcpy(words, c->stats_sum, sizeof(words), hipMemcpyDeviceToHost));
     *out = tt_render_stats{};
-    out->paths = c->last_paths;
-    for (uint32_t b = 0; b < c->last_bounces; b++) {
-        out->extension_rays += words[0 + b];
-        out->shadow_rays += words[32 + b];
+    out->paths = c->stats_paths;
+    for (uint32_t b = 0; b < 32; b++) {
+        out->extension_rays += words[tt::C_EXT + b];
+        out->shadow_rays += words[tt::C_SHADOW + b];
     }
     out->restir_rays = words[tt::C_RESTIR_VIS];
-    for (uint32_t i = 0; i < c->trace_events_used; i++) {
+    for (const tt::StatsLaunch& l : c->stats_launches) {
         float ms = 0.0f;
-        TT_HIP(c, hipEventElapsedTime(&ms, c->trace_events[i].begin, c->trace_events[i].end));
-        if (c->trace_event_shadow[i] == 2) {
-            out->restir_ms += ms;
-        } else if (c->trace_event_shadow[i]) {
-            out->shadow_launches++;
-            out->shadow_ms += ms;
-        } else {
-            out->extend_launches++;
-            out->extend_ms += ms;
+        TT_HIP(c, hipEventElapsedTime(&ms, l.events.begin, l.events.end));
+        switch (l.kind) {
+            case tt::STATS_EXTEND:
+                out->extend_launches++;
+                out->extend_ms += ms;
+                break;
+            case tt::STATS_SHADOW:
+                out->shadow_launches++;
+                out->shadow_ms += ms;
+                break;
+            case tt::STATS_RESTIR:
+                out->restir_ms += ms;
+                break;
+            case tt::STATS_RENDER:
+                out->integrate_ms += ms;
+                break;
         }
     }
-    float ms = 0.0f;
-    TT_HIP(c, hipEventElapsedTime(&ms, c->lanes[c->last_lane].integrate.begin, c->lanes[c->last_lane].integrate.end));
-    out->integrate_ms = ms;
+    // Start the next sum.
+    TT_HIP(c, hipMemset(c->stats_sum, 0, sizeof(words)));
+    for (const tt::StatsLaunch& l : c->stats_launches) c->stats_pool.push_back(l.events);
+    c->stats_launches.clear();
+    c->last_stats_first = 0;
+    c->stats_renders = 0;
+    c->stats_paths = 0;
     return TT_OK;
 }
 

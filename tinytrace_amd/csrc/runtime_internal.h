@@ -23,10 +23,10 @@ namespace tt {
 
 // Where a render runs.  Consecutive renders alternate between two lanes, so
 // that one frame's last, latency-bound bounces overlap the next frame's
-// first ones; within a lane, bounce b's shadow rays run on `side` beside
-// bounce b + 1's extension on `stream` (fork after shade b, join before
-// shade b + 1).  Each lane has its own path buffers, counters and traversal
-// stack spill areas.
+// first ones; within a lane, when there are hardware queues for it, bounce
+// b's shadow rays run on `side` beside bounce b + 1's extension on `stream`
+// (fork after shade b, join before shade b + 1).  Each lane has its own path
+// buffers, counters and traversal stack spill areas.
 struct Lane {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;
@@ -46,6 +46,13 @@ struct Lane {
     void* path_memory = nullptr;
 };
 
+// A timed launch of a render that collects statistics.
+enum StatsKind : uint8_t { STATS_EXTEND = 0, STATS_SHADOW = 1, STATS_RESTIR = 2, STATS_RENDER = 3 };
+struct StatsLaunch {
+    EventPair events;
+    StatsKind kind;
+};
+
 }  // namespace tt
 
 struct tt_context {
@@ -60,17 +67,25 @@ struct tt_context {
     int cu_count = 0;
     int grid = 0;
     tt::Lane lanes[2];
+    // Whether the lanes have side streams: only when the HIP runtime gives
+    // the process enough hardware queues (GPU_MAX_HW_QUEUES) for the API
+    // stream, both lanes' streams and the null stream; streams that share a
+    // queue run one after the other, which would undo both overlaps.
+    bool side_streams = false;
     uint32_t next_lane = 0;
     uint32_t last_lane = 0;  // lane of the last render (its counters and timing)
     hipEvent_t api_mark = nullptr;
     uint32_t* work = nullptr;
     // Timing.
-    std::vector<tt::EventPair> trace_events;
-    std::vector<uint8_t> trace_event_shadow;
-    uint32_t trace_events_used = 0;
-    bool stats_pending = false;
-    uint64_t last_paths = 0;
-    uint32_t last_bounces = 0;
+    // Statistics (TT_RENDER_COLLECT_STATS), summed over the renders since the
+    // last tt_render_stats_get: their timed launches (event pairs drawn from
+    // a pool), path counts, and ray counters added up on the device.
+    std::vector<tt::EventPair> stats_pool;
+    std::vector<tt::StatsLaunch> stats_launches;
+    size_t last_stats_first = 0;  // the last stats render's first launch record
+    uint32_t stats_renders = 0;
+    uint64_t stats_paths = 0;
+    unsigned long long* stats_sum = nullptr;  // C_WORDS sums, device
     double last_integrate_ms = -1.0;
     // Scenes, targets and servers alive on this context.  Destroying the
     // context while any remain only marks it; the last of them frees it, so
