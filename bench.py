@@ -16,10 +16,12 @@ import os
 import sys
 import time
 
-# Eight hardware queues (HIP's default is four) give each of the library's
-# two render lanes its own shadow-ray stream; read by the HIP runtime when it
-# starts, so set before anything initialises it.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# Eight hardware queues (HIP's default is four) let the library overlap
+# three frames, each with a shadow-ray side stream, every stream on a queue
+# of its own.  The HIP runtime reads this when it starts, so it is set before
+# anything initialises it; a larger value in the environment is kept.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)

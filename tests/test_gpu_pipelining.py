@@ -1,5 +1,5 @@
 """Renders enqueued back to back overlap on the device (consecutive renders
-alternate between two lanes; with enough hardware queues each bounce's
+take two or three lanes in turn; with enough hardware queues each bounce's
 shadow rays also overlap the next bounce's extension).  Their results must equal, bit for bit, renders that
 are waited for one by one: progressive accumulation order, ReSTIR temporal
 history across frames, two targets interleaved on one context, and API calls
@@ -16,11 +16,13 @@ pytestmark = pytest.mark.gpu
 MODES = [LightSampling.NEXT_EVENT_ESTIMATION, LightSampling.RESTIR, LightSampling.ON_HIT]
 
 
-@pytest.fixture(autouse=True, params=[False, True], ids=["one-stream-lanes", "side-streams"])
-def side_streams(request, monkeypatch):
-    """Contexts created in the test use shadow side streams or not (the
-    library decides from GPU_MAX_HW_QUEUES at context creation)."""
-    monkeypatch.setenv("GPU_MAX_HW_QUEUES", "8" if request.param else "4")
+@pytest.fixture(autouse=True, params=["4", "5", "8"],
+                ids=["two-lanes", "three-lanes", "three-lanes-side-streams"])
+def hardware_queues(request, monkeypatch):
+    """Contexts created in the test take two or three lanes, with or without
+    shadow side streams: the library decides from GPU_MAX_HW_QUEUES at
+    context creation."""
+    monkeypatch.setenv("GPU_MAX_HW_QUEUES", request.param)
     return request.param
 
 

@@ -437,7 +437,7 @@ __global__ void zero_kernel(uint32_t* p, uint32_t n) {
 }
 
 // sum[i] += counters[i]: a render's ray counters into the statistics sums
-// (renders on both lanes may add at once).
+// (renders on several lanes may add at once).
 __global__ void stats_add_kernel(const uint32_t* counters, unsigned long long* sum, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && counters[i]) atomicAdd(sum + i, (unsigned long long)counters[i]);

@@ -189,11 +189,17 @@ tt_status tt_context_create(const tt_context_desc* desc, tt_context** out) {
         hipMemset(c->stats_sum, 0, tt::C_WORDS * sizeof(unsigned long long)) != hipSuccess)
         return cleanup(TT_OUT_OF_MEMORY);
     const size_t spill_bytes = tt::spill_words(c->grid) * sizeof(uint32_t);
-    // Six queues: the null stream, the API stream and two per lane (HIP's
-    // default is four).
-    const char* queues = std::getenv("GPU_MAX_HW_QUEUES");
-    c->side_streams = queues && std::atoi(queues) >= 6;
-    for (tt::Lane& L : c->lanes) {
+    // Hardware queues: the null stream and the API stream take two; each
+    // lane takes one, or two with its side stream (HIP's default is four).
+    // Measured on C2: three lanes with side streams (8 queues) 8.3 ms a
+    // frame, three without (5) 8.5, two with (6) 8.7, two without (4) 9.1;
+    // a fourth lane gains nothing.
+    const char* env = std::getenv("GPU_MAX_HW_QUEUES");
+    const int queues = env && std::atoi(env) > 0 ? std::atoi(env) : 4;
+    c->lane_count = queues >= 5 ? 3u : 2u;
+    c->side_streams = queues >= int(2 + 2 * c->lane_count);
+    for (uint32_t i = 0; i < c->lane_count; i++) {
+        tt::Lane& L = c->lanes[i];
         if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess)
             return cleanup(TT_HIP_ERROR);
         if (c->side_streams && hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking) != hipSuccess)
@@ -550,15 +556,26 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     TT_HIP(c, hipSetDevice(c->device));
     bool stats = (st.flags & TT_RENDER_COLLECT_STATS) != 0;
     // Renders with the radiance cache run on the API stream with lane 0's
-    // buffers (the cache's own passes run there and read back at once); the
-    // others take the next lane.
+    // buffers (the cache's own passes run there and read back at once).  The
+    // others take the next lane, unless every lane's path buffers at this
+    // size would pass the budget: such renders are long enough that
+    // overlapping their tails gains little, and they run on lane 0.
     const bool on_api = cache != nullptr;
-    const uint32_t lane = on_api ? 0u : c->next_lane;
+    const bool fits = uint64_t(paths) * tt::PATH_BYTES * c->lane_count <= tt::LANE_PATH_BUDGET;
+    const uint32_t lane = on_api || !fits ? 0u : c->next_lane;
     tt::Lane& L = c->lanes[lane];
-    const tt::Lane& other = c->lanes[lane ^ 1u];
     tt::PathBuffers b;
     tt_status es = tt::ensure_paths(c, lane, paths, b);
     if (es != TT_OK) return es;
+    if (!on_api && fits)
+        for (uint32_t i = 0; i < c->lane_count; i++) {
+            // Grow every lane together, so later renders allocate nothing.
+            tt::PathBuffers unused;
+            if (i != lane && c->lanes[i].path_capacity < paths) {
+                es = tt::ensure_paths(c, i, paths, unused);
+                if (es != TT_OK) return es;
+            }
+        }
     b.max_bounces = k->bounce_count;
 
     tt::RestirBuffers rb{};
@@ -614,10 +631,11 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
         stream = L.stream;
         TT_HIP(c, hipStreamWaitEvent(stream, c->api_mark, 0));
     }
-    // The previous render, if on the other lane, may still be running: this
+    // The previous render, if on another lane, may still be running: this
     // one reads and rewrites the target's ReSTIR buffers only after that
     // render's last read of them, and accumulates after it.
-    const bool after_other = !on_api && other.used;
+    const tt::Lane* other = c->prev_lane >= 0 && uint32_t(c->prev_lane) != lane ? &c->lanes[c->prev_lane] : nullptr;
+    const bool after_other = !on_api && other && other->used;
     TT_HIP(c, hipEventRecord(L.integrate.begin, stream));
     size_t whole = 0;
     if (stats) {
@@ -648,7 +666,7 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     if (es != TT_OK) return es;
     for (uint32_t bounce = 0; bounce < k->bounce_count; bounce++) {
         if (restir && bounce == 0 && s->view.emissive_count > 0) {
-            if (after_other) TT_HIP(c, hipStreamWaitEvent(stream, other.restir_read, 0));
+            if (after_other) TT_HIP(c, hipStreamWaitEvent(stream, other->restir_read, 0));
             if (stats) {
                 es = stats_begin(c, tt::STATS_RESTIR, stream, &ev);
                 if (es != TT_OK) return es;
@@ -691,7 +709,7 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
                                             L.counters + tt::C_CACHE_QUERY, 0, nullptr, b.radiance);
         if (cs != TT_OK) return cs;
     }
-    if (after_other) TT_HIP(c, hipStreamWaitEvent(stream, other.accumulated, 0));
+    if (after_other) TT_HIP(c, hipStreamWaitEvent(stream, other->accumulated, 0));
     TT_HIP(c, tt::launch_accumulate(stream, f, b, t->pixels));
     TT_HIP(c, hipEventRecord(L.accumulated, stream));
     if (stats) {
@@ -707,7 +725,8 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     c->last_lane = lane;
     if (!on_api) {
         L.pending = true;
-        c->next_lane = lane ^ 1u;
+        c->prev_lane = int(lane);
+        c->next_lane = (lane + 1u) % c->lane_count;
     }
     if (cache) {
         // Train on this frame's samples: train_steps Adam steps over

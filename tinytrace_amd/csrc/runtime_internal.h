@@ -21,9 +21,9 @@ struct EventPair {
 
 namespace tt {
 
-// Where a render runs.  Consecutive renders alternate between two lanes, so
-// that one frame's last, latency-bound bounces overlap the next frame's
-// first ones; within a lane, when there are hardware queues for it, bounce
+// Where a render runs.  Consecutive renders take the lanes in turn (two or
+// three), so that one frame's last, latency-bound bounces overlap the next
+// frames' first ones; within a lane, when there are hardware queues for it, bounce
 // b's shadow rays run on `side` beside bounce b + 1's extension on `stream`
 // (fork after shade b, join before shade b + 1).  Each lane has its own path
 // buffers, counters and traversal stack spill areas.
@@ -31,7 +31,7 @@ struct Lane {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
-    // Cross-lane order: the next render on the other lane starts its ReSTIR
+    // Cross-lane order: the next render (on the next lane) starts its ReSTIR
     // passes after `restir_read` (this render's last read of the target's
     // ReSTIR buffers) and accumulates after `accumulated`.
     hipEvent_t restir_read = nullptr, accumulated = nullptr;
@@ -66,13 +66,16 @@ struct tt_context {
     std::string error;
     int cu_count = 0;
     int grid = 0;
-    tt::Lane lanes[2];
-    // Whether the lanes have side streams: only when the HIP runtime gives
-    // the process enough hardware queues (GPU_MAX_HW_QUEUES) for the API
-    // stream, both lanes' streams and the null stream; streams that share a
-    // queue run one after the other, which would undo both overlaps.
+    static constexpr uint32_t MAX_LANES = 3;
+    tt::Lane lanes[MAX_LANES];
+    uint32_t lane_count = 2;
+    // Lanes in use and whether they have side streams, from the hardware
+    // queues the HIP runtime gives the process (GPU_MAX_HW_QUEUES): every
+    // stream gets a queue of its own, since streams that share a queue run
+    // one after the other, which would undo the overlaps.
     bool side_streams = false;
     uint32_t next_lane = 0;
+    int prev_lane = -1;      // lane of the last render that ran on a lane
     uint32_t last_lane = 0;  // lane of the last render (its counters and timing)
     hipEvent_t api_mark = nullptr;
     uint32_t* work = nullptr;
@@ -171,6 +174,9 @@ tt_status ensure_paths(tt_context* c, uint32_t lane, size_t paths, PathBuffers& 
 hipStream_t api_stream(tt_context* c);
 // Bytes of path state per path slot.
 constexpr size_t PATH_BYTES = 11 * sizeof(float4) + sizeof(uint64_t);
+// Path-buffer bytes all lanes together may hold before renders stop
+// overlapping (they then run on lane 0 alone).
+constexpr uint64_t LANE_PATH_BUDGET = 24ull << 30;
 // Per-render arguments of the kernels from the reference Constants.
 FrameArgs frame_args(const tt_constants& k, uint32_t width, uint32_t height, uint32_t rng, uint32_t seed);
 
