@@ -136,3 +136,20 @@ def test_statistics_sum_over_back_to_back_renders():
     assert s["extend_ms"] > 0 and s["shadow_ms"] > 0 and s["integrate_ms"] > 0 and trace_ms > 0
     with pytest.raises(TinytraceError):
         a.stats()  # nothing rendered since the last read
+
+
+def test_serving_batches_equal_progressive_render():
+    """Serving batches (with shadow side streams when the context has them)
+    give the progressive render's image bit for bit."""
+    from test_gpu_serving import NEE, progressive
+    from tinytrace_amd.serving import Server
+
+    scene, camera = scenes.cornell_box()
+    renderer = Renderer((64, 48))
+    renderer.set_scene(scene)
+    expected = progressive(renderer, camera, NEE, 64, 48, 4, seed=7)
+    server = Server(renderer.context, pool_paths=5000)  # several batches per pass
+    slot = server.add_scene(renderer.scene)
+    rid = server.submit(slot, camera, NEE, 64, 48, 4, seed=7)
+    server.run_until_done([rid])
+    np.testing.assert_array_equal(server.take_result(rid), expected)

@@ -237,17 +237,34 @@ tt_status tt_server_step(tt_server* sv, uint32_t* completed) {
         return tt::frame_args(k, ch.request->width, ch.request->height, ch.request->desc.rng, ch.request->desc.seed);
     };
 
+    // The batch runs on the API stream (after every render enqueued so far)
+    // with lane 0's buffers; with side streams, bounce b's shadow rays run on
+    // lane 0's side stream beside bounce b + 1's extension, as in a render.
     hipStream_t stream = tt::api_stream(c);
-    TT_HIP(c, tt::launch_zero(stream, c->lanes[0].counters, tt::C_WORDS));
+    tt::Lane& L = c->lanes[0];
+    hipStream_t side = c->side_streams ? L.side : stream;
+    tt::PathBuffers side_buffers = b;
+    side_buffers.spill = L.side_spill;
+    const bool shadows = lk.light_sampling != TT_LIGHT_SAMPLING_ON_HIT && scene->view.emissive_count > 0;
+    TT_HIP(c, tt::launch_zero(stream, L.counters, tt::C_WORDS));
     for (const Chunk& ch : chunks)
         TT_HIP(c, tt::launch_generate_chunk(stream, chunk_args(ch), b, ch.base, ch.pixel0, ch.count));
     tt::FrameArgs f = chunk_args(chunks.front());  // configuration fields are the group's
+    TT_HIP(c, tt::launch_extend(stream, scene->view, b, 0, c->grid));
     for (uint32_t bounce = 0; bounce < lk.bounce_count; bounce++) {
-        TT_HIP(c, tt::launch_extend(stream, scene->view, b, bounce, c->grid));
+        if (shadows && bounce > 0 && side != stream) TT_HIP(c, hipStreamWaitEvent(stream, L.join, 0));
         TT_HIP(c, tt::launch_shade(stream, scene->view, f, b, bounce, paths));
-        if (lk.light_sampling != TT_LIGHT_SAMPLING_ON_HIT && scene->view.emissive_count > 0)
-            TT_HIP(c, tt::launch_shadow(stream, scene->view, b, bounce, c->grid));
+        if (shadows) {
+            if (side != stream) {
+                TT_HIP(c, hipEventRecord(L.fork, stream));
+                TT_HIP(c, hipStreamWaitEvent(side, L.fork, 0));
+            }
+            TT_HIP(c, tt::launch_shadow(side, scene->view, side_buffers, bounce, c->grid));
+            if (side != stream) TT_HIP(c, hipEventRecord(L.join, side));
+        }
+        if (bounce + 1 < lk.bounce_count) TT_HIP(c, tt::launch_extend(stream, scene->view, b, bounce + 1, c->grid));
     }
+    if (shadows && side != stream) TT_HIP(c, hipStreamWaitEvent(stream, L.join, 0));
     for (const Chunk& ch : chunks)
         TT_HIP(c, tt::launch_accumulate_chunk(stream, chunk_args(ch), b, ch.base, ch.pixel0, ch.count,
                                               ch.request->target));
