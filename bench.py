@@ -94,9 +94,14 @@ def main() -> int:
         v = C.c_double()
         lib.tt_timing_ms(renderer.context.handle, b"integrate", C.byref(v))
 
+    # The timed steps also time every closest-hit launch with HIP events on
+    # its stream and count rays (collect_stats; the library sums both over
+    # the renders until they are read).
+    renderer.settings = dataclasses.replace(settings, collect_stats=True)
     for _ in range(args.warmup):
         step()
     sync()
+    renderer.stats()  # start the sums at the timed steps
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
@@ -105,23 +110,26 @@ def main() -> int:
     paths = WIDTH * HEIGHT
     ms_per_step = elapsed / args.steps * 1e3
     mpaths = paths * args.steps / elapsed / 1e6
-
-    # Ray counts and per-kernel durations of a few more passes, enqueued back
-    # to back like the timed ones (HIP events on the stream of every
-    # closest-hit launch; the library sums them over the passes).
-    renderer.settings = dataclasses.replace(settings, collect_stats=True)
-    passes = 5
-    for _ in range(passes):
-        step()
     s = renderer.stats()
     ext_rays, shadow_rays, restir_rays = s["extension_rays"], s["shadow_rays"], s["restir_rays"]
     ext_ms, shadow_ms, restir_ms = s["extend_ms"], s["shadow_ms"], s["restir_ms"]
     ext_launches = s["extend_launches"]
+    passes = args.steps
     assert s["paths"] == passes * paths
     rays_per_path = (ext_rays + shadow_rays + restir_rays) / (passes * paths)
     mrays = mpaths * rays_per_path
     bytes_per_ray = RAY_RECORD_BYTES + HIT_RECORD_BYTES + info["expected_bytes_per_ray"]
     achieved_gbs = ext_rays * bytes_per_ray / (ext_ms * 1e-3) / 1e9
+    # The same launches with one frame on the GPU at a time (each step waited
+    # for): consecutive frames then do not share the GPU.
+    isolated_passes = 3
+    for _ in range(isolated_passes):
+        step()
+        sync()
+    si = renderer.stats()
+    iso_ms = si["extend_ms"] / max(si["extend_launches"], 1)
+    iso_gbs = si["extension_rays"] * bytes_per_ray / (si["extend_ms"] * 1e-3) / 1e9
+    renderer.settings = settings
     # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
     # (scaled by a copy kernel of known size), committed under profiles/.
     traffic = None
@@ -159,7 +167,9 @@ def main() -> int:
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
                      "traffic": traffic, "bytes_per_ray": round(bytes_per_ray, 1),
                      "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4),
-                     "rays_per_launch": round(ext_rays / max(ext_launches, 1), 1)},
+                     "rays_per_launch": round(ext_rays / max(ext_launches, 1), 1),
+                     "isolated": {"avg_launch_ms": round(iso_ms, 4), "achieved": round(iso_gbs, 2),
+                                  "frac": round(iso_gbs / HBM_PEAK_GBS, 5)}},
     }
 
     if args.quality_frames > 0:
