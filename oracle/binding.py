@@ -112,7 +112,7 @@ class OracleScene:
         return out, rays
 
     def restir_state(self, width, height):
-        return {"gbuffer": np.zeros((2, height * width, 16), dtype=np.float32),
+        return {"gbuffer": np.zeros((2, height * width, 20), dtype=np.float32),
                 "reservoirs": np.zeros((2, height * width, 4), dtype=np.uint32), "cur": 0, "history": 0}
 
     def render_restir(self, constants, state, rng=0, seed=0, candidates=32, temporal=True, spatial=True,

@@ -173,7 +173,7 @@ int oracle_render_pixels(void* handle, const tt_constants* k, uint32_t rng, uint
     return 0;
 }
 
-// One ReSTIR DI frame (float oracle).  gbuffer: 2 x W*H x 16 floats,
+// One ReSTIR DI frame (float oracle).  gbuffer: 2 x W*H x RECORD floats,
 // reservoirs: 2 x W*H x 4 words (final reservoirs); [cur] is written, [cur^1]
 // is last frame's and is read when `history` != 0.  rgba as oracle_render.
 int oracle_render_restir(void* handle, const tt_constants* k, uint32_t rng, uint32_t seed, uint32_t candidates,
@@ -182,7 +182,6 @@ int oracle_render_restir(void* handle, const tt_constants* k, uint32_t rng, uint
     auto* os = static_cast<OracleScene*>(handle);
     if (os->precision != 32 || k->sample_count != 1) return 1;
     const Scene<float>& s = *os->f;
-    if (s.textured()) return 1;  // ReSTIR DI on textured scenes: not supported
     Camera<float> cam = make_camera<float>(*k);
     uint32_t w = k->screen_size[0], h = k->screen_size[1];
     uint32_t px = w * h;
@@ -193,8 +192,9 @@ int oracle_render_restir(void* handle, const tt_constants* k, uint32_t rng, uint
     std::vector<V3<float>> dirs(px);
     std::vector<Hit> hits(px);
     std::vector<uint8_t> found(px);
-    float* gcur = gbuffer + size_t(cur) * px * 16;
-    const float* gprev = gbuffer + size_t(cur ^ 1u) * px * 16;
+    std::vector<V2<float>> ndcs(px);
+    float* gcur = gbuffer + size_t(cur) * px * RECORD;
+    const float* gprev = gbuffer + size_t(cur ^ 1u) * px * RECORD;
     uint32_t* fcur = reservoirs + size_t(cur) * px * 4;
     const uint32_t* fprev = reservoirs + size_t(cur ^ 1u) * px * 4;
     std::vector<uint32_t> initial(size_t(px) * 4), temporal_out(size_t(px) * 4);
@@ -216,14 +216,18 @@ int oracle_render_restir(void* handle, const tt_constants* k, uint32_t rng, uint
         dirs[p] = dir;
         counts[tid].extension++;
         found[p] = s.closest_hit(cam.position, dir, true, 1.0e-3f, 1000.0f, hits[p]) ? 1 : 0;
-        float* g16 = gcur + size_t(p) * 16;
+        float* g16 = gcur + size_t(p) * RECORD;
+        ndcs[p] = V2<float>{ndc_x, ndc_y};
         if (!found[p]) {
             g16[4] = g16[5] = g16[6] = g16[7] = 0.0f;
             return;
         }
         RayHit<float> hit = get_ray_hit(s, hits[p]);
         ShadingF q;
-        q.m = s.materials[hit.material_index];
+        if (s.textured()) q.lod = first_hit_lod(cam, hit, ndc_x, ndc_y);
+        q.u = hit.tu;
+        q.v = hit.tv;
+        q.m = s.material_at(hit.material_index, q.u, q.v, q.lod);
         q.material = hit.material_index;
         q.position = hit.world_position;
         q.basis = create_surface_basis(hit.tangent_space, q.m.geometry_normal);
@@ -237,7 +241,7 @@ int oracle_render_restir(void* handle, const tt_constants* k, uint32_t rng, uint
         uint32_t* after_candidates = (temporal || spatial) ? initial.data() : fcur;
         parallel_for(px, 64, nthreads, [&](int tid, uint32_t p) {
             Res res;
-            const float* g16 = gcur + size_t(p) * 16;
+            const float* g16 = gcur + size_t(p) * RECORD;
             if (found[p] && g16[7] != 0.0f) {
                 ShadingF q = read_record(s, g16, p);
                 Generator gen = pass_generator(p, frame, seed, 1u);
@@ -277,9 +281,9 @@ int oracle_render_restir(void* handle, const tt_constants* k, uint32_t rng, uint
             uint32_t* dst = spatial ? temporal_out.data() : fcur;
             parallel_for(px, 64, nthreads, [&](int, uint32_t p) {
                 Res rq = unpack(current + size_t(p) * 4);
-                ShadingF q = read_record(s, gcur + size_t(p) * 16, p);
+                ShadingF q = read_record(s, gcur + size_t(p) * RECORD, p);
                 if (q.valid && history) {
-                    ShadingF o = read_record(s, gprev + size_t(p) * 16, p);
+                    ShadingF o = read_record(s, gprev + size_t(p) * RECORD, p);
                     if (similar(q, o)) {
                         Res ro = unpack(fprev + size_t(p) * 4);
                         ro.M = rmin(ro.M, 20.0f * float(candidates));
@@ -294,7 +298,7 @@ int oracle_render_restir(void* handle, const tt_constants* k, uint32_t rng, uint
         if (spatial) {
             parallel_for(px, 64, nthreads, [&](int, uint32_t p) {
                 Res rq = unpack(current + size_t(p) * 4);
-                ShadingF q = read_record(s, gcur + size_t(p) * 16, p);
+                ShadingF q = read_record(s, gcur + size_t(p) * RECORD, p);
                 if (q.valid) {
                     Generator gen = pass_generator(p, frame, seed, 3u);
                     float angle = 2.0f * K<float>::PI * random_float<float>(gen);
@@ -306,7 +310,7 @@ int oracle_render_restir(void* handle, const tt_constants* k, uint32_t rng, uint
                     ny = ny < 0 ? 0 : (ny >= int(h) ? int(h) - 1 : ny);
                     uint32_t n = uint32_t(ny) * w + uint32_t(nx);
                     if (n != p) {
-                        ShadingF o = read_record(s, gcur + size_t(n) * 16, n);
+                        ShadingF o = read_record(s, gcur + size_t(n) * RECORD, n);
                         if (similar(q, o)) rq = combine(s, q, rq, o, unpack(current + size_t(n) * 4), gen);
                     }
                 }
@@ -317,7 +321,7 @@ int oracle_render_restir(void* handle, const tt_constants* k, uint32_t rng, uint
     // Paths, with the final reservoir at bounce 0; progressive accumulation.
     parallel_for(px, 64, nthreads, [&](int tid, uint32_t p) {
         V3<float> c = trace_path_restir(s, cam, k->bounce_count, dirs[p], gens[p], found[p] != 0, hits[p],
-                                        fcur + size_t(p) * 4, p, frame, counts[tid]);
+                                        fcur + size_t(p) * 4, p, frame, counts[tid], ndcs[p].x, ndcs[p].y);
         c = c / float(k->sample_count);
         float* out = rgba + 4 * size_t(p);
         if (k->accumulated_frame_count != 0) {

@@ -44,8 +44,14 @@ inline LightPointF light_point(const Scene<float>& s, uint32_t li, V3<float> lb)
     lp.area = rmax(0.00001f, 0.5f * length(cross(p[1] - p[0], p[2] - p[0])));
     lp.normal = normalize(cross(p[1] - p[0], p[2] - p[0]));
     lp.position = interpolate(lb, p[0], p[1], p[2]);
-    const Material<float>& lm = s.materials[in.material_index];
-    lp.emission = lm.emission_color * lm.emission_luminance;
+    // material_emissive at the sample's texture coordinate, level 4
+    // (integrate.glsl:189; scene.glsl:130-133); the constants when untextured.
+    float tc[2];
+    for (int k = 0; k < 2; k++)
+        tc[k] = lb.x * ttm::half_to_float(tri.texture_coordinates[0][k]) +
+            lb.y * ttm::half_to_float(tri.texture_coordinates[1][k]) +
+            lb.z * ttm::half_to_float(tri.texture_coordinates[2][k]);
+    lp.emission = s.emissive_at(in.material_index, tc[0], tc[1], 4.0f);
     lp.instance = tri.instance_index;
     lp.hash = tri.hash;
     return lp;
@@ -95,6 +101,7 @@ struct ShadingF {
     Material<float> m;
     Lobes<float> lobes;
     float depth;
+    float u = 0.0f, v = 0.0f, lod = 1.0f;  // texture coordinate and level of detail of the hit
     bool valid = false;
 };
 
@@ -109,16 +116,19 @@ inline Generator pass_generator(uint32_t pixel, uint32_t frame, uint32_t seed, u
     return init_generator_from_index(pixel, frame, 1, seed ^ (salt * 0x9e3779b9u));
 }
 
-// G-buffer record: 16 floats (position, material bits | normal, valid |
-// tangent, depth | -ray direction, frame bits).
+// G-buffer record: RECORD floats (position, material bits | normal, valid |
+// tangent, depth | -ray direction, frame bits | texture coordinate, level of
+// detail, 0).
+constexpr size_t RECORD = 20;
+
 inline void write_record(float* g, const ShadingF& q, V3<float> wi_world, uint32_t frame) {
     uint32_t mb = q.material;
     float mf, ff;
     std::memcpy(&mf, &mb, 4);
     std::memcpy(&ff, &frame, 4);
-    float vals[16] = {q.position.x, q.position.y, q.position.z, mf, q.basis.normal.x, q.basis.normal.y,
-                      q.basis.normal.z, q.valid ? 1.0f : 0.0f, q.basis.tangent.x, q.basis.tangent.y,
-                      q.basis.tangent.z, q.depth, wi_world.x, wi_world.y, wi_world.z, ff};
+    float vals[RECORD] = {q.position.x, q.position.y, q.position.z, mf, q.basis.normal.x, q.basis.normal.y,
+                          q.basis.normal.z, q.valid ? 1.0f : 0.0f, q.basis.tangent.x, q.basis.tangent.y,
+                          q.basis.tangent.z, q.depth, wi_world.x, wi_world.y, wi_world.z, ff, q.u, q.v, q.lod, 0.0f};
     std::memcpy(g, vals, sizeof(vals));
 }
 
@@ -133,7 +143,10 @@ inline ShadingF read_record(const Scene<float>& s, const float* g, uint32_t pixe
     q.basis.bitangent = normalize(cross(q.basis.normal, q.basis.tangent));
     q.wi = transform_from_basis(q.basis, V3<float>(g[12], g[13], g[14]));
     q.depth = g[11];
-    q.m = s.materials[q.material];
+    q.u = g[16];
+    q.v = g[17];
+    q.lod = g[18];
+    q.m = s.material_at(q.material, q.u, q.v, q.lod);
     uint32_t frame;
     std::memcpy(&frame, &g[15], 4);
     Generator lg = lobe_generator(pixel, frame);
@@ -203,10 +216,11 @@ inline bool light_visible(const Scene<float>& s, V3<float> x, V3<float> dir, con
 // estimation afterwards (integrate.glsl:270-321 otherwise unchanged).
 inline V3<float> trace_path_restir(const Scene<float>& s, const Camera<float>& c, uint32_t bounce_count, V3<float> dir,
                                    Generator g, bool primary_found, const Hit& primary, const uint32_t* reservoir,
-                                   uint32_t pixel, uint32_t frame, TraceCounts& counts) {
+                                   uint32_t pixel, uint32_t frame, TraceCounts& counts, float ndc_x, float ndc_y) {
     V3<float> accumulated;
     V3<float> throughput(1.0f);
     V3<float> ray_dir = dir, ray_org = c.position;
+    float lod = 1.0f;  // PathState.mip_level
     for (uint32_t bounce = 0; bounce < bounce_count; bounce++) {
         Hit h;
         bool found;
@@ -222,7 +236,8 @@ inline V3<float> trace_path_restir(const Scene<float>& s, const Camera<float>& c
             break;
         }
         RayHit<float> hit = get_ray_hit(s, h);
-        const Material<float>& m = s.materials[hit.material_index];
+        if (bounce == 0 && s.textured()) lod = first_hit_lod(c, hit, ndc_x, ndc_y);
+        const Material<float> m = s.material_at(hit.material_index, hit.tu, hit.tv, lod);
         Basis<float> basis = create_surface_basis(hit.tangent_space, m.geometry_normal);
         V3<float> wi = transform_from_basis(basis, -ray_dir);
         Lobes<float> lobes;
@@ -233,7 +248,7 @@ inline V3<float> trace_path_restir(const Scene<float>& s, const Camera<float>& c
             lobes = bsdf_prepare(m, wi, g);
         }
         bool gate = s.desc->emissive_triangle_count > 0 && m.specular_roughness > 0.15f;
-        V3<float> emissive = m.emission_color * m.emission_luminance;
+        V3<float> emissive = s.emissive_at(hit.material_index, hit.tu, hit.tv, lod);
         bool sampled = false;
         V3<float> direct;
         if (gate && bounce > 0) {

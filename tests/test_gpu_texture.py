@@ -3,13 +3,15 @@
 sampling with wrap, first-hit level of detail from ray differentials),
 per pixel, for next-event estimation and hit-light-only, one and two samples
 per pixel; emission textures on lights (NEE at level 4 at the sample's
-coordinate) and on directly seen emitters; and the ReSTIR refusal."""
+coordinate) and on directly seen emitters; and ReSTIR DI on textured scenes
+(textured primary vertices, G-buffer records with the texture coordinate and
+level of detail, textured light emission), frame by frame."""
 import math
 
 import numpy as np
 import pytest
 
-from test_gpu_parity import assert_radiance_close, render_pair
+from test_gpu_parity import assert_radiance_close, render_pair, restir_pair
 from test_texture_cpu import _wall_and_split_light
 from tinytrace_amd import (AssetScene, Camera, Config, DeviceScene, LightSampling, Material, Renderer, RenderSettings,
                            TinytraceError, _abi, scenes)
@@ -27,15 +29,33 @@ def test_render_textured_cornell_matches_oracle(sampling, spp, rng):
     assert_radiance_close(image, reference)
 
 
-def test_textured_scene_refusals(context):
+@pytest.mark.parametrize("reuse", [(False, False), (True, True)])
+def test_render_restir_textured_cornell_matches_oracle(reuse):
     scene, camera = scenes.textured_cornell_box()
-    r = Renderer((32, 32))
-    r.set_scene(scene)
-    r.camera = camera
-    r.set_config(Config(light_sampling=LightSampling.RESTIR))
-    with pytest.raises(TinytraceError) as e:
-        r.render_to_target()
-    assert e.value.status == 8  # TT_UNSUPPORTED
+    config = Config(bounce_count=3, sample_count=1, light_sampling=LightSampling.RESTIR)
+    settings = RenderSettings(rng=_abi.TT_RNG_PCG, seed=5, restir_candidates=8, restir_temporal=reuse[0],
+                              restir_spatial=reuse[1])
+    for image, reference in restir_pair(scene, camera, config, 64, 48, 3, settings):
+        assert_radiance_close(image, reference)
+
+
+def test_render_restir_textured_lights_match_oracle():
+    """ReSTIR with textured emitters: the candidates' targets and the final
+    shading take each light sample's emission from its textures."""
+    rng = np.random.default_rng(9)
+    tex = rng.random((16, 16, 3)).astype(np.float32)
+    light = Material(base_color=(0, 0, 0), specular_roughness=0.5, emission_luminance=4.0)
+    scene = _wall_and_split_light([(light, {"emission_color": tex})] * 2, [(0.3, 0.6), (0.8, 0.2)])
+    camera = Camera(position=(0.0, 0.0, 0.0), yaw=math.pi / 2, pitch=0.0, fov=math.pi / 2)
+    config = Config(bounce_count=3, sample_count=1, light_sampling=LightSampling.RESTIR)
+    settings = RenderSettings(rng=_abi.TT_RNG_PCG, seed=2, restir_candidates=8, restir_temporal=True,
+                              restir_spatial=True)
+    for image, reference in restir_pair(scene, camera, config, 64, 48, 3, settings):
+        assert reference[..., :3].max() > 0.0
+        assert_radiance_close(image, reference)
+
+
+def test_textured_scene_loads(context):
     # Emission textures load.
     s = AssetScene()
     p = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)

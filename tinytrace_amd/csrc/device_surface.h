@@ -311,7 +311,8 @@ struct TextureHit {
     bool enabled;
     bool primary;
     PrimaryRay ray;
-    float lod;  // in: the path's level of detail; out: the level used
+    float lod;          // in: the path's level of detail; out: the level used
+    float u = 0.0f, v = 0.0f;  // out: the hit's texture coordinate
 };
 
 // The first hit's level of detail from differentials one pixel over in x
@@ -374,6 +375,8 @@ TT_DEV Surface surface_at_hit(const SceneView& s, uint32_t instance, uint32_t pr
         tu = bary.x * u.x + bary.y * u.y + bary.z * u.z;
         tv = bary.x * v.x + bary.y * v.y + bary.z * v.z;
         if (tex.primary) tex.lod = first_hit_lod(tex.ray, pos[0], pos[1], pos[2], bary, u, v);
+        tex.u = tu;
+        tex.v = tv;
     }
     Basis ts;
     ts.normal = normalize(mul_dir(normal_transform.m, interpolate(bary, fr[0].normal, fr[1].normal, fr[2].normal)));

@@ -75,7 +75,7 @@ struct PathBuffers {
 };
 
 // Per-target ReSTIR DI state (Bitterli et al. 2020): a G-buffer record
-// (4 x float4) and reservoirs (uint4, FP16 W and M) per pixel.
+// (5 x float4) and reservoirs (uint4, FP16 W and M) per pixel.
 struct RestirBuffers {
     float4* gbuffer[2];     // [cur] this frame, [cur ^ 1] the previous one
     uint4* initial;         // after candidate RIS + visibility

@@ -250,7 +250,8 @@ __global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::Frame
             // estimate the direct light as f(y) Le(y) G(y) W.
             Reservoir r = unpack_reservoir(b.restir_final[p]);
             if (r.light != 0xffffffffu && r.W > 0.0f) {
-                lp = light_point(s, r.light, reservoir_bary(r.bary));
+                if constexpr (TEXTURED) lp = light_point_textured(s, r.light, reservoir_bary(r.bary));
+                else lp = light_point(s, r.light, reservoir_bary(r.bary));
                 f3 to_light = lp.position - sf.position;
                 f3 wo = transform_from_basis(sf.basis, to_light);
                 float distance_squared = length_squared(to_light);

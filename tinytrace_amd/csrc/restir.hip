@@ -48,15 +48,21 @@ struct Shading {
     bool valid;
 };
 
-// Record: g0 = (position, material), g1 = (normal, valid), g2 = (tangent,
-// depth), g3 = (-ray direction, frame index).
-TT_DEV void write_record(float4* g, const Surface& sf, bool valid, float depth, f3 wi_world, uint32_t frame) {
+// Record (RECORD float4s): g0 = (position, material), g1 = (normal, valid),
+// g2 = (tangent, depth), g3 = (-ray direction, frame index), g4 = (texture
+// coordinate, level of detail, 0), read on textured scenes only.
+constexpr size_t RECORD = 5;
+
+TT_DEV void write_record(float4* g, const Surface& sf, bool valid, float depth, f3 wi_world, uint32_t frame, float u,
+                         float v, float lod) {
     g[0] = make_float4(sf.position.x, sf.position.y, sf.position.z, __uint_as_float(sf.material));
     g[1] = make_float4(sf.basis.normal.x, sf.basis.normal.y, sf.basis.normal.z, valid ? 1.0f : 0.0f);
     g[2] = make_float4(sf.basis.tangent.x, sf.basis.tangent.y, sf.basis.tangent.z, depth);
     g[3] = make_float4(wi_world.x, wi_world.y, wi_world.z, __uint_as_float(frame));
+    g[4] = make_float4(u, v, lod, 0.0f);
 }
 
+template <bool TEXTURED>
 TT_DEV Shading read_record(const SceneView& s, const float4* g, uint32_t pixel) {
     Shading q;
     float4 g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3];
@@ -69,14 +75,23 @@ TT_DEV Shading read_record(const SceneView& s, const float4* g, uint32_t pixel) 
     q.sf.basis.bitangent = normalize(cross(q.sf.basis.normal, q.sf.basis.tangent));
     q.sf.wi = transform_from_basis(q.sf.basis, f3(g3.x, g3.y, g3.z));
     q.depth = g2.w;
-    q.m = load_material(s.materials + 19 * size_t(q.sf.material));
+    const float4* material = s.materials + 19 * size_t(q.sf.material);
+    q.m = load_material(material);
+    if constexpr (TEXTURED) {
+        float4 g4 = g[4];
+        Mat t = q.m;
+        texture_hit(texture_view(s), material, &t, g4.x, g4.y, g4.z);
+        q.m = t;
+    }
     Generator lg = restir_lobe_generator(pixel, __float_as_uint(g3.w));
     q.lobes = bsdf_prepare(q.m, q.sf.wi, lg);
     return q;
 }
 
+template <bool TEXTURED>
 TT_DEV float target(const SceneView& s, const Shading& q, uint32_t light, uint32_t bary, float* area = nullptr) {
-    LightPoint lp = light_point(s, light, reservoir_bary(bary));
+    LightPoint lp = TEXTURED ? light_point_textured(s, light, reservoir_bary(bary))
+                             : light_point(s, light, reservoir_bary(bary));
     if (area) *area = lp.area;
     f3 to_light = lp.position - q.sf.position;
     f3 wo = transform_from_basis(q.sf.basis, to_light);
@@ -100,16 +115,17 @@ TT_DEV Reservoir empty_reservoir() { return Reservoir{0xffffffffu, 0u, 0.0f, 0.0
 
 // Generalized balance heuristic combination of the pixel's reservoir rq
 // (at q) with another pixel's ro (at o); one draw from g.
+template <bool TEXTURED>
 TT_DEV Reservoir combine(const SceneView& s, const Shading& q, Reservoir rq, const Shading& o, Reservoir ro,
                          Generator& g) {
     float pq_yq = 0.0f, po_yq = 0.0f, pq_yo = 0.0f, po_yo = 0.0f;
     if (rq.light != 0xffffffffu) {
-        pq_yq = target(s, q, rq.light, rq.bary);
-        po_yq = target(s, o, rq.light, rq.bary);
+        pq_yq = target<TEXTURED>(s, q, rq.light, rq.bary);
+        po_yq = target<TEXTURED>(s, o, rq.light, rq.bary);
     }
     if (ro.light != 0xffffffffu) {
-        pq_yo = target(s, q, ro.light, ro.bary);
-        po_yo = target(s, o, ro.light, ro.bary);
+        pq_yo = target<TEXTURED>(s, q, ro.light, ro.bary);
+        po_yo = target<TEXTURED>(s, o, ro.light, ro.bary);
     }
     float dq = rq.M * pq_yq + ro.M * po_yq;
     float d_o = rq.M * pq_yo + ro.M * po_yo;
@@ -131,12 +147,15 @@ TT_DEV Reservoir combine(const SceneView& s, const Shading& q, Reservoir rq, con
 }
 
 // ------------------------------------------------------------ candidates --
+// TEXTURED: the instantiation for scenes with textures (textured materials
+// at the primary vertex and in the records, textured light emission).
+template <bool TEXTURED>
 __global__ __launch_bounds__(256) void restir_candidates_kernel(tt::DeviceScene d, tt::FrameArgs f, tt::PathBuffers b,
                                                                 tt::RestirBuffers r, uint4* out) {
     SceneView s = view(d);
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= f.width * f.height) return;
-    float4* g = r.gbuffer[r.cur] + 4 * size_t(p);
+    float4* g = r.gbuffer[r.cur] + RECORD * size_t(p);
     float4 qd = b.queue[0][1][p];
     float4 hit = b.hits[p];
     uint32_t instance = __float_as_uint(hit.z);
@@ -150,11 +169,17 @@ __global__ __launch_bounds__(256) void restir_candidates_kernel(tt::DeviceScene 
     } else {
         f3 ray_dir(qd.x, qd.y, qd.z);
         Shading q;
-        q.sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, q.m);
-        q.valid = s.emissive_count > 0 && q.m.specular_roughness > 0.15f;
         f3 cam(f.camera_position[0], f.camera_position[1], f.camera_position[2]);
+        // The primary vertex as shade 0 builds it; the sample's device
+        // coordinates ride in the first ray's and the throughput's w.
+        TextureHit tex{TEXTURED, true,
+                       PrimaryRay{f.inverse_view, f.inverse_proj, cam, float(f.width), float(f.height), qd.w,
+                                  b.throughput[p].w},
+                       1.0f};
+        q.sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, q.m, tex);
+        q.valid = s.emissive_count > 0 && q.m.specular_roughness > 0.15f;
         q.depth = length(q.sf.position - cam);
-        write_record(g, q.sf, q.valid, q.depth, -ray_dir, f.frame_index);
+        write_record(g, q.sf, q.valid, q.depth, -ray_dir, f.frame_index, tex.u, tex.v, tex.lod);
         if (q.valid) {
             Generator lg = restir_lobe_generator(p, f.frame_index);
             q.lobes = bsdf_prepare(q.m, q.sf.wi, lg);
@@ -166,7 +191,7 @@ __global__ __launch_bounds__(256) void restir_candidates_kernel(tt::DeviceScene 
                 f2 r2 = random_vec2(gen);
                 uint32_t bary = quantize_bary(sample_triangle(r2));
                 float area;
-                float ph = target(s, q, li, bary, &area);
+                float ph = target<TEXTURED>(s, q, li, bary, &area);
                 float w = ph * (float(s.emissive_count) * area);
                 sum = sum + w;
                 float u = random_float(gen);
@@ -228,33 +253,35 @@ __global__ __launch_bounds__(TRACE_BLOCK) void restir_visibility_kernel(tt::Devi
 }
 
 // -------------------------------------------------------------- temporal --
+template <bool TEXTURED>
 __global__ __launch_bounds__(256) void restir_temporal_kernel(tt::DeviceScene d, tt::FrameArgs f, tt::RestirBuffers r,
                                                               const uint4* in, uint4* out) {
     SceneView s = view(d);
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= f.width * f.height) return;
     Reservoir rq = unpack_reservoir(in[p]);
-    Shading q = read_record(s, r.gbuffer[r.cur] + 4 * size_t(p), p);
+    Shading q = read_record<TEXTURED>(s, r.gbuffer[r.cur] + RECORD * size_t(p), p);
     if (q.valid && r.history_valid) {
-        Shading o = read_record(s, r.gbuffer[r.cur ^ 1u] + 4 * size_t(p), p);
+        Shading o = read_record<TEXTURED>(s, r.gbuffer[r.cur ^ 1u] + RECORD * size_t(p), p);
         if (similar(q, o)) {
             Reservoir ro = unpack_reservoir(r.final_[r.cur ^ 1u][p]);
             ro.M = fmin_(ro.M, RESTIR_MAX_HISTORY * float(r.candidates));
             Generator gen = pass_generator(f, p, 2u);
-            rq = combine(s, q, rq, o, ro, gen);
+            rq = combine<TEXTURED>(s, q, rq, o, ro, gen);
         }
     }
     out[p] = pack_reservoir(rq);
 }
 
 // --------------------------------------------------------------- spatial --
+template <bool TEXTURED>
 __global__ __launch_bounds__(256) void restir_spatial_kernel(tt::DeviceScene d, tt::FrameArgs f, tt::RestirBuffers r,
                                                              const uint4* in, uint4* out) {
     SceneView s = view(d);
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= f.width * f.height) return;
     Reservoir rq = unpack_reservoir(in[p]);
-    Shading q = read_record(s, r.gbuffer[r.cur] + 4 * size_t(p), p);
+    Shading q = read_record<TEXTURED>(s, r.gbuffer[r.cur] + RECORD * size_t(p), p);
     if (q.valid) {
         Generator gen = pass_generator(f, p, 3u);
         float angle = 2.0f * PI * random_float(gen);
@@ -266,8 +293,8 @@ __global__ __launch_bounds__(256) void restir_spatial_kernel(tt::DeviceScene d, 
         ny = ny < 0 ? 0 : (ny >= int(f.height) ? int(f.height) - 1 : ny);
         uint32_t n = uint32_t(ny) * f.width + uint32_t(nx);
         if (n != p) {
-            Shading o = read_record(s, r.gbuffer[r.cur] + 4 * size_t(n), n);
-            if (similar(q, o)) rq = combine(s, q, rq, o, unpack_reservoir(in[n]), gen);
+            Shading o = read_record<TEXTURED>(s, r.gbuffer[r.cur] + RECORD * size_t(n), n);
+            if (similar(q, o)) rq = combine<TEXTURED>(s, q, rq, o, unpack_reservoir(in[n]), gen);
         }
     }
     out[p] = pack_reservoir(rq);
@@ -275,26 +302,33 @@ __global__ __launch_bounds__(256) void restir_spatial_kernel(tt::DeviceScene d, 
 
 uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
+template <bool TEXTURED>
+hipError_t restir_passes(hipStream_t s, const tt::DeviceScene& scene, const tt::FrameArgs& f, const tt::PathBuffers& b,
+                         const tt::RestirBuffers& r, int grid) {
+    uint32_t pixels = f.width * f.height;
+    uint32_t nb = blocks(pixels, 256);
+    // initial -> (temporal) -> (spatial) -> final_[cur]
+    uint4* after_candidates = (r.temporal_reuse || r.spatial_reuse) ? r.initial : r.final_[r.cur];
+    restir_candidates_kernel<TEXTURED><<<nb, 256, 0, s>>>(scene, f, b, r, after_candidates);
+    restir_visibility_kernel<<<grid, TRACE_BLOCK, 0, s>>>(scene, b, after_candidates);
+    const uint4* current = after_candidates;
+    if (r.temporal_reuse) {
+        uint4* dst = r.spatial_reuse ? r.temporal : r.final_[r.cur];
+        restir_temporal_kernel<TEXTURED><<<nb, 256, 0, s>>>(scene, f, r, current, dst);
+        current = dst;
+    }
+    if (r.spatial_reuse) restir_spatial_kernel<TEXTURED><<<nb, 256, 0, s>>>(scene, f, r, current, r.final_[r.cur]);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 namespace tt {
 
 hipError_t launch_restir(hipStream_t s, const DeviceScene& scene, const FrameArgs& f, const PathBuffers& b,
                          const RestirBuffers& r, int grid) {
-    uint32_t pixels = f.width * f.height;
-    uint32_t nb = blocks(pixels, 256);
-    // initial -> (temporal) -> (spatial) -> final_[cur]
-    uint4* after_candidates = (r.temporal_reuse || r.spatial_reuse) ? r.initial : r.final_[r.cur];
-    restir_candidates_kernel<<<nb, 256, 0, s>>>(scene, f, b, r, after_candidates);
-    restir_visibility_kernel<<<grid, TRACE_BLOCK, 0, s>>>(scene, b, after_candidates);
-    const uint4* current = after_candidates;
-    if (r.temporal_reuse) {
-        uint4* dst = r.spatial_reuse ? r.temporal : r.final_[r.cur];
-        restir_temporal_kernel<<<nb, 256, 0, s>>>(scene, f, r, current, dst);
-        current = dst;
-    }
-    if (r.spatial_reuse) restir_spatial_kernel<<<nb, 256, 0, s>>>(scene, f, r, current, r.final_[r.cur]);
-    return hipGetLastError();
+    return scene.texture_count > 0 ? restir_passes<true>(s, scene, f, b, r, grid)
+                                   : restir_passes<false>(s, scene, f, b, r, grid);
 }
 
 }  // namespace tt

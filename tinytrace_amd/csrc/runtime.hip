@@ -546,8 +546,6 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     const tt_render_settings& st = settings ? *settings : def;
     const bool restir = k->light_sampling == TT_LIGHT_SAMPLING_RESTIR;
     if (restir && k->sample_count != 1) return fail(c, TT_UNSUPPORTED, "ReSTIR DI renders one sample per pixel");
-    if (restir && s->view.texture_count > 0)
-        return fail(c, TT_UNSUPPORTED, "ReSTIR DI on textured scenes is not supported");
     if (restir && st.restir_candidates > 1024) return fail(c, TT_INVALID_ARGUMENT, "restir_candidates > 1024");
     if (st.rng > 1) return fail(c, TT_INVALID_ARGUMENT, "unknown generator");
     uint64_t paths64 = uint64_t(t->width) * t->height * k->sample_count;
@@ -583,14 +581,15 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     if (restir) {
         size_t px = size_t(t->width) * t->height;
         if (!t->restir_memory) {
-            TT_HIP(c, hipMalloc(&t->restir_memory, px * (8 * sizeof(float4) + 4 * sizeof(uint4))));
+            // Two G-buffers of 5 float4 records, four reservoir arrays.
+            TT_HIP(c, hipMalloc(&t->restir_memory, px * (10 * sizeof(float4) + 4 * sizeof(uint4))));
             t->restir_cur = 0;
             t->restir_history = false;
         }
         float4* g = static_cast<float4*>(t->restir_memory);
-        uint4* r = reinterpret_cast<uint4*>(g + 8 * px);
+        uint4* r = reinterpret_cast<uint4*>(g + 10 * px);
         rb.gbuffer[0] = g;
-        rb.gbuffer[1] = g + 4 * px;
+        rb.gbuffer[1] = g + 5 * px;
         rb.initial = r;
         rb.temporal = r + px;
         rb.final_[0] = r + 2 * px;
