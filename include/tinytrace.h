@@ -651,11 +651,11 @@ tt_status tt_asset_add_model(tt_asset_scene* scene, uint32_t node, uint32_t mesh
 /* Replaces GltfImporter::new(path) + insert_in_scene (src/asset/gltf.rs:50-72,
  * 419-452): appends the default scene (else the first) of a .gltf file with
  * external or base64 data-URI buffers under the root node, as the reference
- * does (meshes per primitive, materials per index, node tree depth first).
- * TT_UNSUPPORTED for binary .glb, textures, non-triangle primitives and
- * accessor formats the reference rejects; TT_INVALID_ARGUMENT for unreadable
- * files and malformed documents.  On failure the scene may hold part of the
- * import. */
+ * does (meshes per primitive, materials per index with their PNG textures,
+ * node tree depth first).  TT_UNSUPPORTED for binary .glb, images other than
+ * PNG, non-triangle primitives and accessor formats the reference rejects;
+ * TT_INVALID_ARGUMENT for unreadable files, corrupt images and malformed
+ * documents.  On failure the scene may hold part of the import. */
 tt_status tt_asset_import_gltf(tt_asset_scene* scene, const char* path);
 
 /* Depth-first flatten into instances and emissive triangles, as
