@@ -460,8 +460,8 @@ TT_DEV void store_light_record(float4* r, const LightTriangle& t) {
 }
 
 // The point at barycentrics `lb` of emissive triangle `li`, with the light
-// record's constant emission (untextured scenes, and ReSTIR, which refuses
-// textured ones).
+// record's constant emission (kernels instantiated for untextured scenes,
+// and the visibility pass, which needs no emission).
 TT_DEV LightPoint light_point(const SceneView& s, uint32_t li, f3 lb) {
     const float4* r = s.lights + 5 * size_t(li);
     float4 a = r[0], b = r[1], c = r[2], d = r[3], e = r[4];
