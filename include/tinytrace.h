@@ -352,7 +352,9 @@ typedef struct tt_server tt_server;
 typedef struct tt_server_desc {
     uint64_t hbm_budget_bytes; /* path pool + resident targets (0: 64 GiB) */
     uint32_t pool_paths;       /* path slots per batch (0: 4 Mi) */
-    uint32_t reserved;
+    uint32_t batches_in_flight; /* each with its own pool (0: as many as the
+                                   context has render lanes, while their pools
+                                   take at most half the budget) */
 } tt_server_desc;
 
 typedef struct tt_server_request {
@@ -381,7 +383,7 @@ typedef struct tt_server_stats {
     uint64_t paths;          /* path samples rendered */
     uint64_t queued, active, done;
     uint64_t budget_used_bytes;
-    double busy_ms;          /* wall time spent inside tt_server_step batches */
+    double busy_ms;          /* wall time with at least one batch in flight */
 } tt_server_stats;
 
 tt_status tt_server_create(tt_context* context, const tt_server_desc* desc, tt_server** out);
@@ -390,8 +392,10 @@ void tt_server_destroy(tt_server* server);
 tt_status tt_server_add_scene(tt_server* server, const tt_scene* scene, uint32_t* slot);
 /* TT_OVER_BUDGET when the request's target can never fit next to the pool. */
 tt_status tt_server_submit(tt_server* server, const tt_server_request* request, uint64_t* id);
-/* Admits what fits, renders one batch (blocking) and downloads the requests
- * it completed; *completed = their number (0 and no work when idle). */
+/* Admits what fits and enqueues one batch; once batches_in_flight batches
+ * are in flight (or nothing is left to enqueue) waits for the oldest and
+ * downloads the requests it completed; *completed = their number (0 and no
+ * work when idle). */
 tt_status tt_server_step(tt_server* server, uint32_t* completed);
 tt_status tt_server_status(tt_server* server, uint64_t id, tt_request_status* out);
 /* Copies a done request's RGBA32F image (width*height*16 bytes) and forgets

@@ -138,7 +138,7 @@ TT_BVH_BUILDER_HOST_SAH, TT_BVH_BUILDER_DEVICE_LBVH = 0, 1
 
 
 class ServerDesc(C.Structure):
-    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("pool_paths", C.c_uint32), ("reserved", C.c_uint32)]
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("pool_paths", C.c_uint32), ("batches_in_flight", C.c_uint32)]
 
 
 class ServerRequest(C.Structure):

@@ -12,10 +12,16 @@
 //   3. give each of them an equal share of the pool's slots in arrival
 //      order, then hand what is left to the oldest first; each share is taken
 //      from the request's cursor as chunks (pass k, pixels [a, b));
-//   4. run the wavefront kernels once over the pool (generate per chunk,
+//   4. enqueue the wavefront kernels once over the pool (generate per chunk,
 //      extend / shade / shadow per bounce for all chunks together,
-//      accumulate per chunk in the order taken);
-//   5. download finished requests and release their targets.
+//      accumulate per chunk in the order taken) on the next render lane;
+//   5. once `depth` batches are in flight (or no batch could be formed),
+//      wait for the oldest, then download the requests it finished and
+//      release their targets.
+// Batches in flight overlap like renders on the context's lanes: a batch's
+// last, latency-bound bounces run beside the next batch's first; each
+// accumulates after the batch before it, so every pixel still takes its
+// passes in order.
 // A path's radiance depends only on (scene, camera, pixel, pass, seed,
 // configuration) and each pixel accumulates its passes in order with
 // (previous * k + new) / (k + 1), so a request's image is the same bits
@@ -57,6 +63,13 @@ struct Chunk {
     uint32_t pass, pixel0, count, base;
 };
 
+// A batch enqueued and not yet waited for.
+struct Batch {
+    uint32_t lane;
+    std::vector<Chunk> chunks;
+    uint32_t paths;
+};
+
 bool same_config(const Request& a, const Request& b) {
     return a.desc.scene == b.desc.scene && a.desc.rng == b.desc.rng &&
            a.desc.constants.bounce_count == b.desc.constants.bounce_count &&
@@ -75,6 +88,10 @@ struct tt_server {
     std::map<uint64_t, std::unique_ptr<Request>> requests;
     uint64_t next_id = 1;
     uint64_t used = 0;  // targets of admitted requests
+    uint32_t depth = 1;      // batches in flight, each with its own path pool (lanes 0 .. depth - 1)
+    uint32_t next_slot = 0;  // lane of the next batch
+    std::deque<Batch> inflight;
+    std::chrono::steady_clock::time_point busy_since;  // first batch of the current busy period
     std::chrono::steady_clock::time_point origin;
     tt_server_stats stats{};
 
@@ -82,7 +99,7 @@ struct tt_server {
         return uint64_t(std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - origin)
                             .count());
     }
-    uint64_t pool_bytes() const { return uint64_t(pool_paths) * tt::PATH_BYTES; }
+    uint64_t pool_bytes() const { return uint64_t(pool_paths) * tt::PATH_BYTES * depth; }
 };
 
 namespace {
@@ -140,6 +157,15 @@ tt_status tt_server_create(tt_context* c, const tt_server_desc* desc, tt_server*
     tt::context_retain(c);
     sv->budget = desc && desc->hbm_budget_bytes ? desc->hbm_budget_bytes : (64ull << 30);
     sv->pool_paths = desc && desc->pool_paths ? desc->pool_paths : (4u << 20);
+    // Batches in flight: as asked, else as many as the context has lanes
+    // when their pools take at most half the budget, else one.
+    const uint64_t pool = uint64_t(sv->pool_paths) * tt::PATH_BYTES;
+    if (desc && desc->batches_in_flight) {
+        sv->depth = std::min(desc->batches_in_flight, c->lane_count);
+    } else {
+        sv->depth = c->lane_count;
+        while (sv->depth > 1 && 2 * pool * sv->depth > sv->budget) sv->depth--;
+    }
     sv->origin = std::chrono::steady_clock::now();
     if (sv->pool_bytes() > sv->budget) {
         tt_server_destroy(sv);
@@ -152,7 +178,7 @@ tt_status tt_server_create(tt_context* c, const tt_server_desc* desc, tt_server*
 void tt_server_destroy(tt_server* sv) {
     if (!sv) return;
     (void)hipSetDevice(sv->context->device);
-    (void)hipStreamSynchronize(tt::api_stream(sv->context));
+    (void)hipStreamSynchronize(tt::api_stream(sv->context));  // also waits for batches in flight
     for (auto& kv : sv->requests)
         if (kv.second->target) (void)hipFree(kv.second->target);
     tt_context* c = sv->context;
@@ -194,14 +220,15 @@ tt_status tt_server_submit(tt_server* sv, const tt_server_request* q, uint64_t* 
     return TT_OK;
 }
 
-tt_status tt_server_step(tt_server* sv, uint32_t* completed) {
-    if (!sv) return TT_INVALID_ARGUMENT;
-    tt_context* c = sv->context;
-    if (completed) *completed = 0;
-    TT_HIP(c, hipSetDevice(c->device));
-    tt_status st = admit(sv);
-    if (st != TT_OK) return st;
+}  // extern "C"
 
+namespace {
+
+// Enqueues one batch over the pool on the server's next lane; false in
+// `formed` when no request has work left to schedule.
+tt_status enqueue_batch(tt_server* sv, bool& formed) {
+    formed = false;
+    tt_context* c = sv->context;
     // The oldest unfinished request sets the batch's configuration.
     Request* lead = nullptr;
     for (Request* r : sv->active)
@@ -215,6 +242,11 @@ tt_status tt_server_step(tt_server* sv, uint32_t* completed) {
     for (Request* r : sv->active)
         if (!r->scheduled_all() && same_config(*r, *lead)) group.push_back(r);
 
+    const uint32_t lane = sv->next_slot;
+    tt::PathBuffers b;
+    tt_status st = tt::ensure_paths(c, lane, sv->pool_paths, b);
+    if (st != TT_OK) return st;
+
     std::vector<Chunk> chunks;
     uint32_t base = 0;
     uint32_t share = std::max<uint32_t>(sv->pool_paths / uint32_t(group.size()), 1u);
@@ -222,9 +254,6 @@ tt_status tt_server_step(tt_server* sv, uint32_t* completed) {
     for (Request* r : group) take(r, sv->pool_paths - base, base, chunks);
     const uint32_t paths = base;
 
-    tt::PathBuffers b;
-    st = tt::ensure_paths(c, 0, sv->pool_paths, b);
-    if (st != TT_OK) return st;
     const tt_scene* scene = sv->scenes[lead->desc.scene];
     const tt_constants& lk = lead->desc.constants;
     b.max_bounces = lk.bounce_count;
@@ -237,11 +266,15 @@ tt_status tt_server_step(tt_server* sv, uint32_t* completed) {
         return tt::frame_args(k, ch.request->width, ch.request->height, ch.request->desc.rng, ch.request->desc.seed);
     };
 
-    // The batch runs on the API stream (after every render enqueued so far)
-    // with lane 0's buffers; with side streams, bounce b's shadow rays run on
-    // lane 0's side stream beside bounce b + 1's extension, as in a render.
-    hipStream_t stream = tt::api_stream(c);
-    tt::Lane& L = c->lanes[0];
+    // As a render: after the API stream's work so far (target clears), on
+    // the lane's stream, with shadow rays on its side stream when the
+    // context has side streams; accumulation after the previous batch's or
+    // render's.
+    tt::Lane& L = c->lanes[lane];
+    const tt::Lane* other = c->prev_lane >= 0 && uint32_t(c->prev_lane) != lane ? &c->lanes[c->prev_lane] : nullptr;
+    hipStream_t stream = L.stream;
+    TT_HIP(c, hipEventRecord(c->api_mark, c->stream));
+    TT_HIP(c, hipStreamWaitEvent(stream, c->api_mark, 0));
     hipStream_t side = c->side_streams ? L.side : stream;
     tt::PathBuffers side_buffers = b;
     side_buffers.spill = L.side_spill;
@@ -265,23 +298,46 @@ tt_status tt_server_step(tt_server* sv, uint32_t* completed) {
         if (bounce + 1 < lk.bounce_count) TT_HIP(c, tt::launch_extend(stream, scene->view, b, bounce + 1, c->grid));
     }
     if (shadows && side != stream) TT_HIP(c, hipStreamWaitEvent(stream, L.join, 0));
+    if (other && other->used) TT_HIP(c, hipStreamWaitEvent(stream, other->accumulated, 0));
     for (const Chunk& ch : chunks)
         TT_HIP(c, tt::launch_accumulate_chunk(stream, chunk_args(ch), b, ch.base, ch.pixel0, ch.count,
                                               ch.request->target));
-    TT_HIP(c, hipStreamSynchronize(stream));
+    TT_HIP(c, hipEventRecord(L.accumulated, stream));
+    TT_HIP(c, hipEventRecord(L.done, stream));
+    L.used = true;
+    L.pending = true;
+    c->prev_lane = int(lane);
+    c->next_lane = (lane + 1u) % c->lane_count;
+    sv->next_slot = (lane + 1u) % sv->depth;
+    if (sv->inflight.empty()) sv->busy_since = t0;
+    sv->inflight.push_back(Batch{lane, std::move(chunks), paths});
+    formed = true;
+    return TT_OK;
+}
+
+// Waits for the oldest batch in flight, then downloads the requests it
+// finished and releases their targets.
+tt_status finish_batch(tt_server* sv, uint32_t& done) {
+    tt_context* c = sv->context;
+    Batch batch = std::move(sv->inflight.front());
+    sv->inflight.pop_front();
+    TT_HIP(c, hipEventSynchronize(c->lanes[batch.lane].done));
     sv->stats.batches++;
-    sv->stats.paths += paths;
+    sv->stats.paths += batch.paths;
+    if (sv->inflight.empty())  // the GPU idles from here until the next batch
+        sv->stats.busy_ms +=
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - sv->busy_since).count();
 
     // Passes completed by this batch, then finished requests.
-    for (const Chunk& ch : chunks)
+    for (const Chunk& ch : batch.chunks)
         if (ch.pixel0 + ch.count == ch.request->pixels()) ch.request->passes_done = ch.pass + 1;
-    uint32_t done = 0;
     std::vector<Request*> still;
     for (Request* r : sv->active) {
         if (r->passes_done < r->desc.passes) {
             still.push_back(r);
             continue;
         }
+        // Its last chunk was in this batch: no batch in flight touches it.
         r->image.resize(size_t(r->pixels()) * 4);
         TT_HIP(c, hipMemcpy(r->image.data(), r->target, r->target_bytes(), hipMemcpyDeviceToHost));
         TT_HIP(c, hipFree(r->target));
@@ -292,7 +348,31 @@ tt_status tt_server_step(tt_server* sv, uint32_t* completed) {
         done++;
     }
     sv->active.swap(still);
-    sv->stats.busy_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return TT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+tt_status tt_server_step(tt_server* sv, uint32_t* completed) {
+    if (!sv) return TT_INVALID_ARGUMENT;
+    tt_context* c = sv->context;
+    if (completed) *completed = 0;
+    TT_HIP(c, hipSetDevice(c->device));
+    tt_status st = admit(sv);
+    if (st != TT_OK) return st;
+    bool formed = false;
+    st = enqueue_batch(sv, formed);
+    if (st != TT_OK) return st;
+    // Keep up to `depth` batches in flight; with nothing new to enqueue,
+    // finish the oldest.
+    uint32_t done = 0;
+    while (!sv->inflight.empty() && (sv->inflight.size() >= sv->depth || !formed)) {
+        st = finish_batch(sv, done);
+        if (st != TT_OK) return st;
+        if (!formed) break;
+    }
     if (completed) *completed = done;
     return TT_OK;
 }
