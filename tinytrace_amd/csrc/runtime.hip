@@ -630,6 +630,18 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
         stream = L.stream;
         TT_HIP(c, hipStreamWaitEvent(stream, c->api_mark, 0));
     }
+    // If an enqueue fails part way, still mark the lane's end after what was
+    // enqueued, so later API calls wait for it.
+    struct PartialGuard {
+        tt::Lane* lane;
+        hipStream_t stream;
+        bool armed;
+        ~PartialGuard() {
+            if (!armed) return;
+            (void)hipEventRecord(lane->done, stream);
+            lane->pending = true;
+        }
+    } partial{&L, stream, !on_api};
     // The previous render, if on another lane, may still be running: this
     // one reads and rewrites the target's ReSTIR buffers only after that
     // render's last read of them, and accumulates after it.
@@ -720,6 +732,7 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     }
     TT_HIP(c, hipEventRecord(L.integrate.end, stream));
     TT_HIP(c, hipEventRecord(L.done, stream));
+    partial.armed = false;
     L.used = true;
     c->last_lane = lane;
     if (!on_api) {
