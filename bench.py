@@ -353,16 +353,44 @@ def instanced(frames):
         stats.update(render_cached(renderer, cache))
 
     cached_ms, cached_rays = timed(cached_step)
+
+    # RMSE of 1 and 4 (...) independent frames, plain and cached, against the
+    # FP64 fixture (4096-spp double-precision oracle on a lattice).
+    import numpy as np
+
+    ref_path = os.path.join(ROOT, "tests", "golden", "fp64_instanced_2160p.npz")
+    with np.load(ref_path, allow_pickle=False) as f:
+        pixels, reference, ref_spp = f["pixels"], f["radiance"], int(f["spp"])
+
+    def rmse_curve(render):
+        samples = []
+        renderer.frame_index = 100000
+        for _ in range(frames):
+            renderer.reset_accumulation()
+            render()
+            samples.append(renderer.target.download().reshape(-1, 4)[pixels, :3].astype(np.float64))
+        samples = np.array(samples)
+        curve, n = {}, 1
+        while n <= frames:
+            err = samples[:n].mean(0) - reference
+            curve[str(n)] = round(float(np.sqrt((err * err).mean())), 5)
+            n *= 4
+        return curve
+
+    renderer.settings = dataclasses.replace(settings, collect_stats=False)
+    plain_rmse = rmse_curve(renderer.render_to_target)
+    cached_rmse = rmse_curve(lambda: render_cached(renderer, cache))
     cache.close()
     paths = width * height
     return {"workload": f"instanced field seed 0 ({info['unique_triangle_count']} unique / "
                         f"{info['triangle_count']} instanced triangles), {width}x{height}, 1 spp, depth {depth}, "
                         "ReSTIR DI (32 candidates, 1 temporal + 1 spatial reuse), PCG seed 0",
             "scene_build_s": round(build_s, 3), "bvh_nodes": info["node_count"],
+            "reference": f"FP64 oracle, {ref_spp} spp next-event estimation, {pixels.size}-pixel lattice (stride 64)",
             "plain": {"ms_per_frame": round(plain_ms, 3), "mpaths_per_s": round(paths / plain_ms / 1e3, 2),
-                      "mrays_per_s": round(plain_rays / plain_ms / 1e3, 2)},
+                      "mrays_per_s": round(plain_rays / plain_ms / 1e3, 2), "rmse": plain_rmse},
             "cached": {"ms_per_frame": round(cached_ms, 3), "mpaths_per_s": round(paths / cached_ms / 1e3, 2),
-                       "mrays_per_s": round(cached_rays / cached_ms / 1e3, 2),
+                       "mrays_per_s": round(cached_rays / cached_ms / 1e3, 2), "rmse": cached_rmse,
                        "queries_per_frame": stats.get("queries"), "training_samples_per_frame": stats.get("samples")}}
 
 

@@ -6,6 +6,8 @@ two benchmark workloads:
                               (32 x 32 = 1024 pixels)
   fp64_sphere_field_1080p.npz sphere field (seed 0), 1920x1080, depth 8,
                               lattice stride 32 (60 x 34 = 2040 pixels)
+  fp64_instanced_2160p.npz    instanced field (seed 0), 3840x2160, depth 12,
+                              lattice stride 64 (60 x 34 = 2040 pixels)
 
 Each file holds `pixels` (linear indices), `radiance` (float64, pixels x 3),
 `stderr` (per-pixel standard error of the mean, float64) and the settings.
@@ -34,6 +36,7 @@ BATCH = 16  # samples per oracle call (sample_count)
 WORKLOADS = {
     "fp64_cornell_512": dict(scene=lambda: scenes.cornell_box(), size=(512, 512), depth=4, stride=16),
     "fp64_sphere_field_1080p": dict(scene=lambda: scenes.sphere_field(seed=0), size=(1920, 1080), depth=8, stride=32),
+    "fp64_instanced_2160p": dict(scene=lambda: scenes.instanced_field(seed=0), size=(3840, 2160), depth=12, stride=64),
 }
 
 
