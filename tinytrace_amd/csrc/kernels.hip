@@ -104,7 +104,9 @@ __global__ __launch_bounds__(256) void generate_chunk_kernel(tt::FrameArgs f, tt
 }
 
 // ---------------------------------------------------------------- extend --
-__global__ __launch_bounds__(TRACE_BLOCK) void extend_kernel(tt::DeviceScene d, tt::PathBuffers b, uint32_t bounce) {
+// Five waves per SIMD (at most 102 VGPRs, no spills): with the LDS stack that
+// is the most a CU holds, and the persistent grid is sized to fill it.
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void extend_kernel(tt::DeviceScene d, tt::PathBuffers b, uint32_t bounce) {
     SceneView s = view(d);
     Stack st = make_stack(b.spill);
     const uint32_t count = b.counters[C_EXT + bounce];
@@ -132,9 +134,11 @@ TT_DEV f3 cache_ratio(f3 a, f3 t) {
 
 // TEXTURED: the instantiation for scenes with textures (texture fetches at
 // hits and on textured light samples); untextured scenes run the other,
-// which carries no texture code.
+// which carries no texture code.  The untextured one is held to four waves
+// per SIMD (128 VGPRs, a 32-B spill), which beats three waves without spills;
+// the textured one would spill 160 B and keeps the compiler's choice.
 template <bool TEXTURED>
-__global__ __launch_bounds__(256) void shade_kernel(tt::DeviceScene d, tt::FrameArgs f, tt::PathBuffers b,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 1 : 4))) void shade_kernel(tt::DeviceScene d, tt::FrameArgs f, tt::PathBuffers b,
                                                     uint32_t bounce) {
     SceneView s = view(d);
     const uint32_t count = b.counters[C_EXT + bounce];
@@ -493,8 +497,10 @@ uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
 namespace tt {
 
+// Ten 128-thread workgroups per CU: five waves per SIMD, the occupancy that
+// extend's register budget and the LDS stack allow.
 #ifndef TT_TRACE_GRID_PER_CU
-#define TT_TRACE_GRID_PER_CU 8
+#define TT_TRACE_GRID_PER_CU 10
 #endif
 int trace_grid(int cu_count) { return cu_count * TT_TRACE_GRID_PER_CU; }
 size_t spill_words(int grid) { return size_t(grid) * TRACE_BLOCK * TRACE_SPILL_STACK; }

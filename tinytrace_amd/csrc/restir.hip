@@ -148,9 +148,10 @@ TT_DEV Reservoir combine(const SceneView& s, const Shading& q, Reservoir rq, con
 
 // ------------------------------------------------------------ candidates --
 // TEXTURED: the instantiation for scenes with textures (textured materials
-// at the primary vertex and in the records, textured light emission).
+// at the primary vertex and in the records, textured light emission).  As
+// for shading, the untextured instantiation is held to four waves per SIMD.
 template <bool TEXTURED>
-__global__ __launch_bounds__(256) void restir_candidates_kernel(tt::DeviceScene d, tt::FrameArgs f, tt::PathBuffers b,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 1 : 4))) void restir_candidates_kernel(tt::DeviceScene d, tt::FrameArgs f, tt::PathBuffers b,
                                                                 tt::RestirBuffers r, uint4* out) {
     SceneView s = view(d);
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
