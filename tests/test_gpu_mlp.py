@@ -85,6 +85,28 @@ def test_adam_step_matches_oracle():
     np.testing.assert_allclose(c.params, want, rtol=1e-6, atol=1e-9)
 
 
+def test_weights_after_adam_steps_equal_weights_set_from_host():
+    """An Adam step writes the bf16 copies the kernels read along with the
+    fp32 parameters: after several steps, inference (which reads only the
+    copies) and a further gradient equal, bit for bit, those of a cache
+    whose parameters are set from the host to the same fp32 values."""
+    c = RadianceCache(Context(), seed=9, learning_rate=2e-3)
+    c.set_bounds(LO, HI)
+    r = _records(2000, 8)
+    t = np.random.default_rng(9).uniform(0, 3, (2000, 3))
+    for _ in range(5):
+        c.train_step(r, t)
+    fresh = RadianceCache(Context(), seed=1)
+    fresh.set_bounds(LO, HI)
+    fresh.params = c.params
+    assert not np.array_equal(c.params, RadianceCache(Context(), seed=9).params)
+    np.testing.assert_array_equal(c.infer(r), fresh.infer(r))
+    g, loss = c.gradient(r, t)
+    g_fresh, loss_fresh = fresh.gradient(r, t)
+    np.testing.assert_array_equal(g, g_fresh)
+    assert loss == loss_fresh
+
+
 def test_training_fits_a_smooth_target():
     c = RadianceCache(Context(), seed=7, learning_rate=3e-3)
     c.set_bounds(LO, HI)

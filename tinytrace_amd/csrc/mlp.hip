@@ -336,8 +336,11 @@ __device__ __forceinline__ void param_position(uint32_t idx, uint32_t& k, uint32
     i = local % 64u;
 }
 
+// With apply, the step also refreshes the bf16 copies of the updated
+// parameter (the positions mlp_pack_kernel gives it; the copies' padding is
+// written once, when the parameters are set).
 __global__ void mlp_adam_kernel(float* master, float* m1, float* m2, const float* partial, uint32_t chunks,
-                                AdamArgs a, int apply, float* grad_out) {
+                                AdamArgs a, int apply, float* grad_out, uint16_t* wf, uint16_t* wb, uint16_t* wb5) {
     uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= uint32_t(MLP_PARAMS)) return;
     uint32_t k, o, i;
@@ -350,7 +353,16 @@ __global__ void mlp_adam_kernel(float* master, float* m1, float* m2, const float
     float vv = a.beta2 * m2[idx] + (1.0f - a.beta2) * (g * g);
     m1[idx] = mm;
     m2[idx] = vv;
-    master[idx] = master[idx] - a.lr * (mm * a.c1) / (ttm::sqrt(vv * a.c2) + a.eps);
+    const float w = master[idx] - a.lr * (mm * a.c1) / (ttm::sqrt(vv * a.c2) + a.eps);
+    master[idx] = w;
+    const uint16_t h = float_to_bf16(w);
+    if (k < 5u) {
+        wf[(64u * k + o) * MLP_ROW + i] = h;
+        if (k >= 1u) wb[(64u * (k - 1u) + i) * MLP_ROW + o] = h;
+    } else {
+        wf[(320u + o) * MLP_ROW + i] = h;
+        wb5[i * MLP_BWD5_ROW + o] = h;
+    }
 }
 
 // bf16 copies: forward rows [out][in] (W_5 rows 3..31 zero), transposed
@@ -489,9 +501,8 @@ tt_status mlp_train_device(tt_mlp* m, const float4* records, const float4* targe
         a.c2 = float(1.0 / (1.0 - std::pow(double(m->beta2), double(m->step))));
     }
     mlp_adam_kernel<<<blocks_of(MLP_PARAMS, 256), 256, 0, tt::api_stream(c)>>>(m->master, m->m1, m->m2, m->partial, chunks, a,
-                                                                      apply ? 1 : 0, grad_out);
+                                                                      apply ? 1 : 0, grad_out, m->wf, m->wb, m->wb5);
     TT_HIP(c, hipGetLastError());
-    if (apply) return mlp_repack(m);
     return TT_OK;
 }
 
