@@ -487,6 +487,21 @@ tt_status tt_trace_closest_device(tt_context* context, const tt_scene* scene,
                                   const tt_ray* device_rays, tt_hit* device_hits, uint32_t count,
                                   uint32_t flags);
 
+/* No counterpart in the reference.  tt_trace_closest for callers that cannot
+ * allocate device memory (a serving loop or harness that forbids hipMalloc
+ * after start-up): same rays, same hits, bit for bit, but the call allocates
+ * and frees nothing.  Rays and hits are staged through a fixed scratch array
+ * that the library holds on each device for the life of the process (about
+ * 10.5 MiB: the ray cursor, staging for TT_TRACE_STATIC_MAX_RAYS rays and
+ * hits, and the traversal stack spill of a 128-workgroup grid).  count above
+ * TT_TRACE_STATIC_MAX_RAYS returns TT_INVALID_ARGUMENT.  Every call on a
+ * device shares that device's scratch: calls that overlap on one device
+ * (from several contexts or threads) race and must be serialised by the
+ * caller.  Synchronous. */
+#define TT_TRACE_STATIC_MAX_RAYS 65536u
+tt_status tt_trace_closest_static(tt_context* context, const tt_scene* scene, const tt_ray* rays,
+                                  tt_hit* hits, uint32_t count, uint32_t flags);
+
 typedef struct tt_bsdf_query {
     float wi[3];
     float wo[3];

@@ -23,6 +23,7 @@ STATUS_NAMES = {
 }
 TT_INVALID_INDEX = 0xFFFFFFFF
 TT_TRACE_CULL_BACK_FACES = 1
+TT_TRACE_STATIC_MAX_RAYS = 65536  # tinytrace.h
 TT_RENDER_COLLECT_STATS = 1
 TT_RNG_REFERENCE = 0
 TT_RNG_PCG = 1
@@ -236,6 +237,7 @@ _PROTOTYPES = [
                                    C.c_void_p, C.POINTER(CacheSettings), C.c_void_p, C.POINTER(CacheStats)]),
     ("tt_trace_closest", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     ("tt_trace_closest_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    ("tt_trace_closest_static", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     ("tt_bsdf_evaluate", C.c_int, [C.c_void_p, C.POINTER(MaterialConstants), C.c_void_p, C.c_void_p, C.c_uint32,
                                    C.c_uint32]),
     ("tt_debug_math", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),

@@ -90,6 +90,11 @@ struct RestirBuffers {
 
 int trace_grid(int cu_count);
 size_t spill_words(int grid);
+// Threads per traversal workgroup and spilled stack entries per thread (the
+// traversal's own constants, checked against them in kernels.hip), for host
+// code that sizes spill areas at compile time.
+constexpr size_t TRACE_THREADS = 128;
+constexpr size_t TRACE_SPILL_ENTRIES = 116;
 // Counter words (u32, zeroed at the start of every render).
 constexpr uint32_t C_EXT = 0;            // [b]: rays in the queue consumed at bounce b
 constexpr uint32_t C_SHADOW = 32;        // [b]: shadow rays emitted at bounce b

@@ -25,6 +25,8 @@
 using namespace ttd;
 
 static_assert(TRACE_STACK_LIMIT == int(tt::BVH_WIDE_STACK_LIMIT), "traversal stack covers the deepest wide path");
+static_assert(size_t(TRACE_BLOCK) == tt::TRACE_THREADS && size_t(TRACE_SPILL_STACK) == tt::TRACE_SPILL_ENTRIES,
+              "host-side traversal constants match the device's");
 
 namespace {
 

@@ -9,6 +9,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -60,6 +61,53 @@ tt_status stats_begin(tt_context* c, tt::StatsKind kind, hipStream_t stream, siz
 tt_status stats_end(tt_context* c, size_t index, hipStream_t stream) {
     TT_HIP(c, hipEventRecord(c->stats_launches[index].events.end, stream));
     return TT_OK;
+}
+
+// Scratch of tt_trace_closest_static, for callers that cannot allocate: the
+// ray cursor (one word, 256 B reserved), staging for up to
+// STATIC_TRACE_MAX_RAYS rays and hits, and the traversal stack spill area of
+// a grid of at most STATIC_TRACE_GRID workgroups.  Each device holds its own
+// copy for the life of the process.  Host code never passes `scratch`
+// itself to HIP (on the host it names a stand-in, not device memory): it
+// uses the per-device address that hipGetSymbolAddress returns.
+constexpr uint32_t STATIC_TRACE_MAX_RAYS = TT_TRACE_STATIC_MAX_RAYS;
+constexpr int STATIC_TRACE_GRID = 128;
+constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+constexpr size_t ST_CURSOR = 0;
+constexpr size_t ST_RAYS = 256;
+constexpr size_t ST_HITS = align256(ST_RAYS + size_t(STATIC_TRACE_MAX_RAYS) * sizeof(tt_ray));
+constexpr size_t ST_SPILL = align256(ST_HITS + size_t(STATIC_TRACE_MAX_RAYS) * sizeof(tt_hit));
+constexpr size_t ST_BYTES =
+    ST_SPILL + size_t(STATIC_TRACE_GRID) * tt::TRACE_THREADS * tt::TRACE_SPILL_ENTRIES * sizeof(uint32_t);
+static_assert(ST_BYTES < (12u << 20), "static trace scratch stays below 12 MiB");
+
+static __device__ __align__(256) uint8_t scratch[ST_BYTES];
+
+// Device address of each device's copy of `scratch`, resolved once per device
+// (with that device current) by the first context created on it.
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+void* scratch_address[MAX_DEVICES] = {};
+
+// Called by tt_context_create with c->device current.  A failure does not
+// fail the context: only tt_trace_closest_static needs the address, and it
+// reports the reason.
+void resolve_static_scratch(tt_context* c) {
+    if (c->device >= MAX_DEVICES) {
+        c->static_scratch_error = "device ordinal above the static scratch table";
+        return;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_address[c->device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            c->static_scratch_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return;
+        }
+        scratch_address[c->device] = p;
+    }
+    c->static_scratch = scratch_address[c->device];
 }
 
 }  // namespace
@@ -182,6 +230,7 @@ tt_status tt_context_create(const tt_context_desc* desc, tt_context** out) {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return cleanup(TT_KERNEL_LOAD);
     c->cu_count = prop.multiProcessorCount;
     c->grid = tt::trace_grid(c->cu_count);
+    resolve_static_scratch(c);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(TT_HIP_ERROR);
     if (hipEventCreateWithFlags(&c->api_mark, hipEventDisableTiming) != hipSuccess) return cleanup(TT_HIP_ERROR);
     if (hipMalloc(&c->work, 256) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
@@ -919,6 +968,37 @@ tt_status tt_trace_closest(tt_context* c, const tt_scene* s, const tt_ray* rays,
     if (dh) (void)hipFree(dh);
     if (e != hipSuccess) return hip_fail(c, e, "tt_trace_closest");
     return st;
+}
+
+tt_status tt_trace_closest_static(tt_context* c, const tt_scene* s, const tt_ray* rays, tt_hit* hits, uint32_t count,
+                                  uint32_t flags) {
+    if (!c || !s || (count && (!rays || !hits))) return fail(c, TT_INVALID_ARGUMENT, "null argument");
+    if (s->context != c) return fail(c, TT_STALE_HANDLE, "scene from another context");
+    if (count > STATIC_TRACE_MAX_RAYS)
+        return fail(c, TT_INVALID_ARGUMENT,
+                    "tt_trace_closest_static traces at most " + std::to_string(STATIC_TRACE_MAX_RAYS) +
+                        " rays per call (got " + std::to_string(count) + ")");
+    if (count == 0) return TT_OK;
+    if (!c->static_scratch) return fail(c, TT_KERNEL_LOAD, "static trace scratch unavailable: " + c->static_scratch_error);
+    TT_HIP(c, hipSetDevice(c->device));
+    uint8_t* base = static_cast<uint8_t*>(c->static_scratch);
+    uint32_t* cursor = reinterpret_cast<uint32_t*>(base + ST_CURSOR);
+    tt_ray* staged_rays = reinterpret_cast<tt_ray*>(base + ST_RAYS);
+    tt_hit* staged_hits = reinterpret_cast<tt_hit*>(base + ST_HITS);
+    uint32_t* spill = reinterpret_cast<uint32_t*>(base + ST_SPILL);
+    // The spill stride is the launch's grid size, so any grid up to
+    // STATIC_TRACE_GRID stays inside the spill area.
+    const int grid = int(std::min<uint32_t>(STATIC_TRACE_GRID, (count + tt::TRACE_THREADS - 1) / tt::TRACE_THREADS));
+    hipStream_t stream = tt::api_stream(c);
+    // The cursor keeps the previous call's final value: clear it with a
+    // kernel (no memset node, so the sequence can also be captured).
+    TT_HIP(c, tt::launch_zero(stream, cursor, 1));
+    TT_HIP(c, hipMemcpyAsync(staged_rays, rays, size_t(count) * sizeof(tt_ray), hipMemcpyHostToDevice, stream));
+    TT_HIP(c, tt::launch_trace_rays(stream, s->view, staged_rays, staged_hits, count,
+                                    (flags & TT_TRACE_CULL_BACK_FACES) != 0, cursor, spill, grid));
+    TT_HIP(c, hipMemcpyAsync(hits, staged_hits, size_t(count) * sizeof(tt_hit), hipMemcpyDeviceToHost, stream));
+    TT_HIP(c, hipStreamSynchronize(stream));
+    return TT_OK;
 }
 
 tt_status tt_bsdf_evaluate(tt_context* c, const tt_material_constants* material, const tt_bsdf_query* queries,

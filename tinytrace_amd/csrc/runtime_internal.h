@@ -104,6 +104,11 @@ struct tt_context {
     void* display_memory = nullptr;
     tt::EventPair post_process;
     bool post_process_recorded = false;
+    // Device address of this device's copy of the library's static trace
+    // scratch (tt_trace_closest_static), resolved at context creation; null
+    // with the reason in static_scratch_error if that failed.
+    void* static_scratch = nullptr;
+    std::string static_scratch_error;
 };
 
 struct tt_scene {
