@@ -20,8 +20,13 @@
  *  - Handles are owned by the context that created them.  They may be
  *    destroyed in any order: a destroyed context stays alive until its last
  *    scene, target and server is destroyed.  One context per host thread.
- *  - All device work of a context is ordered on one HIP stream;
- *    tt_target_download is the synchronisation point.
+ *  - Device work of a context runs on several HIP streams: an API stream
+ *    (uploads, downloads, clears, display, serving, the radiance cache) and
+ *    two or three render lanes, each with an optional side stream for
+ *    shadow rays, so that back-to-back renders overlap.  The library orders
+ *    them with events so that results equal those of calls run one at a
+ *    time: every call is ordered after the renders enqueued before it that
+ *    it depends on.  tt_target_download is the synchronisation point.
  *  - Structs marked "reference layout" are byte-identical to the reference's
  *    #[repr(C)] / std430 types and are checked with static asserts.
  */
@@ -48,7 +53,11 @@ typedef enum tt_status {
     TT_STALE_HANDLE = 5,     /* reference Handle epoch check panics */
     TT_OUT_OF_MEMORY = 6,
     TT_OVER_BUDGET = 7,      /* serving admission */
-    TT_UNSUPPORTED = 8       /* feature outside the implemented path */
+    TT_UNSUPPORTED = 8,      /* feature outside the implemented path */
+    TT_INTERNAL_ERROR = 9    /* a C++ exception inside the library other than a
+                                host allocation failure (a bug); the message is
+                                in tt_last_error.  Host allocation and thread
+                                failures return TT_OUT_OF_MEMORY. */
 } tt_status;
 
 /* LightSampling (src/config.rs:4-12).  TT_LIGHT_SAMPLING_RESTIR is new: ReSTIR

@@ -23,6 +23,70 @@ def test_library_exports_every_declared_symbol():
     assert declared == set(_abi.EXPORTED_SYMBOLS)
 
 
+# Exported functions whose bodies reach no throwing call (no containers,
+# strings or allocating new): plain field copies, arithmetic, forwarding to a
+# guarded function, or c_str().
+NO_THROW = {"tt_last_error", "tt_scene_info_get", "tt_scene_create", "tt_render", "tt_camera_default",
+            "tt_config_default", "tt_material_desc_default", "tt_constants_build", "tt_asset_scene_create"}
+
+
+def test_no_exception_crosses_the_abi():
+    """Every exported definition is a function-try-block (its handler maps
+    the exception to a status), or is on the short list above."""
+    csrc = os.path.join(ROOT, "tinytrace_amd", "csrc")
+    defined = {}
+    for name in sorted(os.listdir(csrc)):
+        if not name.endswith((".cpp", ".hip")):
+            continue
+        text = open(os.path.join(csrc, name)).read()
+        for m in re.finditer(r'^(?:extern "C" )?(?:tt_status|void|const char\*) (tt_\w+)\(', text, re.M):
+            depth, i = 1, m.end()
+            while depth:
+                depth += {"(": 1, ")": -1}.get(text[i], 0)
+                i += 1
+            head = text[i:text.index("{", i)].strip()
+            defined[m.group(1)] = (name, head)
+    assert set(defined) == set(_abi.EXPORTED_SYMBOLS)
+    unguarded = [f"{n} ({f})" for n, (f, head) in defined.items() if head != "try" and n not in NO_THROW]
+    assert not unguarded, unguarded
+
+
+def test_host_allocation_failure_returns_out_of_memory():
+    """A host allocation that fails inside the library (here: the mip chain of
+    a 4096^2 texture with 16 MiB of address space left) returns
+    TT_OUT_OF_MEMORY instead of terminating the caller."""
+    import subprocess
+    import sys
+
+    code = f"""
+import ctypes as C, resource, sys
+import numpy as np
+sys.path.insert(0, {ROOT!r})
+from tinytrace_amd import _abi
+lib = _abi.lib()
+scene = C.c_void_p()
+assert lib.tt_asset_scene_create(C.byref(scene)) == 0
+mat = _abi.MaterialDesc()
+lib.tt_material_desc_default(C.byref(mat))
+w = h = 4096
+texels = np.full(w * h * 3, 0.5, np.float32)
+tex = _abi.TextureDesc(0, w, h, texels.ctypes.data)
+index = C.c_uint32()
+vm = int(open("/proc/self/statm").read().split()[0]) * resource.getpagesize()
+resource.setrlimit(resource.RLIMIT_AS, (vm + (16 << 20), resource.RLIM_INFINITY))
+status = lib.tt_asset_insert_material_textured(scene, C.addressof(mat), C.byref(tex), 1, C.byref(index))
+resource.setrlimit(resource.RLIMIT_AS, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+small = np.full(4 * 4 * 3, 0.5, np.float32)
+tex = _abi.TextureDesc(0, 4, 4, small.ctypes.data)
+after = lib.tt_asset_insert_material_textured(scene, C.addressof(mat), C.byref(tex), 1, C.byref(index))
+lib.tt_asset_scene_destroy(scene)
+print("status", status, after)
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "status 6 0" in r.stdout, r.stdout
+
+
 def test_reference_struct_sizes():
     assert C.sizeof(_abi.Constants) == 368
     assert C.sizeof(_abi.MaterialConstants) == 192

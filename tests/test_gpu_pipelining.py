@@ -83,6 +83,43 @@ def test_two_targets_interleaved_on_one_context(mode):
     np.testing.assert_array_equal(got_b, want)
 
 
+def render_mode_into(r, target, mode):
+    """render_into with r's constants but another light-sampling mode."""
+    k = r.constants()
+    k.light_sampling = int(mode)
+    _abi.check(_abi.lib().tt_render_ex(r.context.handle, r.scene.handle, C.byref(k), C.byref(r.settings._desc()),
+                                       target.handle), r.context.handle, "tt_render_ex")
+
+
+def test_restir_frames_interleaved_with_other_renders():
+    """ReSTIR frames on one target alternate with NEE frames on a second
+    target of the same context, so consecutive ReSTIR frames run on lanes
+    with another render's lane between them.  Each ReSTIR frame's passes
+    must still run after the previous ReSTIR frame's last read of the
+    target's reservoirs: both targets equal frames waited for one by one."""
+    frames = 6
+
+    def run(wait):
+        r = make_renderer(LightSampling.RESTIR)
+        second = Target(r.context, *r.extent)
+        for _ in range(frames):
+            render_mode_into(r, r.target, LightSampling.RESTIR)
+            if wait:
+                r.target.download()
+            render_mode_into(r, second, LightSampling.NEXT_EVENT_ESTIMATION)
+            if wait:
+                second.download()
+            r.accumulated_frame_count += 1
+            r.frame_index += 1
+        return r.target.download(), second.download()
+
+    got_restir, got_nee = run(wait=False)
+    want_restir, want_nee = run(wait=True)
+    assert np.isfinite(got_restir).all() and got_restir[..., :3].max() > 0
+    np.testing.assert_array_equal(got_restir, want_restir)
+    np.testing.assert_array_equal(got_nee, want_nee)
+
+
 def test_api_calls_between_renders():
     """A download and an accumulation reset between back-to-back renders see
     the renders before them and not those after."""

@@ -20,6 +20,7 @@ TT_OK = 0
 STATUS_NAMES = {
     0: "TT_OK", 1: "TT_NO_DEVICE", 2: "TT_HIP_ERROR", 3: "TT_KERNEL_LOAD", 4: "TT_INVALID_ARGUMENT",
     5: "TT_STALE_HANDLE", 6: "TT_OUT_OF_MEMORY", 7: "TT_OVER_BUDGET", 8: "TT_UNSUPPORTED",
+    9: "TT_INTERNAL_ERROR",
 }
 TT_INVALID_INDEX = 0xFFFFFFFF
 TT_TRACE_CULL_BACK_FACES = 1

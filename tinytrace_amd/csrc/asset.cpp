@@ -25,6 +25,7 @@
 #include <string>
 #include <vector>
 
+#include "abi_guard.h"
 #include "../../include/tinytrace.h"
 #include "host_math.h"
 
@@ -331,9 +332,12 @@ tt_status tt_asset_scene_create(tt_asset_scene** out) {
     return *out ? TT_OK : TT_OUT_OF_MEMORY;
 }
 
-void tt_asset_scene_destroy(tt_asset_scene* scene) { delete scene; }
+void tt_asset_scene_destroy(tt_asset_scene* scene) try {
+    delete scene;
+} catch (...) {
+}
 
-tt_status tt_asset_insert_mesh(tt_asset_scene* scene, const tt_mesh_desc* mesh, uint32_t* index) {
+tt_status tt_asset_insert_mesh(tt_asset_scene* scene, const tt_mesh_desc* mesh, uint32_t* index) try {
     if (!scene || !mesh || !mesh->positions || !mesh->texture_coordinates) return TT_INVALID_ARGUMENT;
     uint32_t vc = mesh->vertex_count;
     if (vc == 0) return TT_INVALID_ARGUMENT;
@@ -457,14 +461,18 @@ tt_status tt_asset_insert_mesh(tt_asset_scene* scene, const tt_mesh_desc* mesh, 
     scene->meshes.push_back(rec);
     if (index) *index = mesh_index;
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(scene ? &scene->error : nullptr);
 }
 
-tt_status tt_asset_insert_material(tt_asset_scene* scene, const tt_material_desc* d, uint32_t* index) {
+tt_status tt_asset_insert_material(tt_asset_scene* scene, const tt_material_desc* d, uint32_t* index) try {
     return tt_asset_insert_material_textured(scene, d, nullptr, 0, index);
+} catch (...) {
+    return tt::exception_status(scene ? &scene->error : nullptr);
 }
 
 tt_status tt_asset_insert_material_textured(tt_asset_scene* scene, const tt_material_desc* d,
-                                            const tt_texture_desc* textures, uint32_t texture_count, uint32_t* index) {
+                                            const tt_texture_desc* textures, uint32_t texture_count, uint32_t* index) try {
     if (!scene || !d || (texture_count && !textures)) return TT_INVALID_ARGUMENT;
     const tt_texture_desc* by_channel[TT_TEXTURE_CHANNELS] = {};
     for (uint32_t k = 0; k < texture_count; k++) {
@@ -546,9 +554,11 @@ tt_status tt_asset_insert_material_textured(tt_asset_scene* scene, const tt_mate
     scene->materials.push_back(m);
     if (index) *index = i;
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(scene ? &scene->error : nullptr);
 }
 
-tt_status tt_asset_add_node(tt_asset_scene* scene, uint32_t parent, const float transform[16], uint32_t* index) {
+tt_status tt_asset_add_node(tt_asset_scene* scene, uint32_t parent, const float transform[16], uint32_t* index) try {
     if (!scene || !transform || parent >= scene->nodes.size()) return TT_INVALID_ARGUMENT;
     uint32_t i = uint32_t(scene->nodes.size());
     Node n;
@@ -557,16 +567,20 @@ tt_status tt_asset_add_node(tt_asset_scene* scene, uint32_t parent, const float 
     scene->nodes[parent].children.push_back(i);
     if (index) *index = i;
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(scene ? &scene->error : nullptr);
 }
 
-tt_status tt_asset_add_model(tt_asset_scene* scene, uint32_t node, uint32_t mesh, uint32_t material) {
+tt_status tt_asset_add_model(tt_asset_scene* scene, uint32_t node, uint32_t mesh, uint32_t material) try {
     if (!scene || node >= scene->nodes.size() || mesh >= scene->meshes.size() || material >= scene->materials.size())
         return TT_INVALID_ARGUMENT;
     scene->nodes[node].models.push_back(Model{mesh, material});
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(scene ? &scene->error : nullptr);
 }
 
-tt_status tt_asset_flatten(tt_asset_scene* scene, tt_scene_desc* out) {
+tt_status tt_asset_flatten(tt_asset_scene* scene, tt_scene_desc* out) try {
     if (!scene || !out) return TT_INVALID_ARGUMENT;
     scene->instances.clear();
     scene->emissive.clear();
@@ -656,6 +670,8 @@ tt_status tt_asset_flatten(tt_asset_scene* scene, tt_scene_desc* out) {
     out->texture_data = scene->texture_bytes.empty() ? nullptr : scene->texture_bytes.data();
     out->texture_data_bytes = scene->texture_bytes.size();
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(scene ? &scene->error : nullptr);
 }
 
 }  // extern "C"

@@ -7,6 +7,7 @@
 // Renderer::prepare_to_present brackets its dispatch (src/lib.rs:180-193).
 #include <hip/hip_runtime.h>
 
+#include "abi_guard.h"
 #include "device_math.h"
 #include "runtime_internal.h"
 
@@ -52,7 +53,7 @@ __global__ void display_kernel(const float4* target, uint32_t count, uint32_t to
 }  // namespace
 
 extern "C" tt_status tt_target_display(tt_context* c, const tt_target* t, const tt_constants* constants,
-                                       uint8_t* rgba8, size_t bytes) {
+                                       uint8_t* rgba8, size_t bytes) try {
     using tt::fail;
     if (!c || !t || !constants || !rgba8) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     if (t->context != c) return fail(c, TT_STALE_HANDLE, "target belongs to another context");
@@ -79,4 +80,6 @@ extern "C" tt_status tt_target_display(tt_context* c, const tt_target* t, const 
     TT_HIP(c, hipMemcpyAsync(rgba8, c->display_memory, count * 4, hipMemcpyDeviceToHost, tt::api_stream(c)));
     TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(c ? &c->error : nullptr);
 }

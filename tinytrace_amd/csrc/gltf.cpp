@@ -678,9 +678,9 @@ class Importer {
 
 }  // namespace
 
-extern "C" tt_status tt_asset_import_gltf(tt_asset_scene* scene, const char* path) {
+extern "C" tt_status tt_asset_import_gltf(tt_asset_scene* scene, const char* path) try {
     if (!scene || !path) return TT_INVALID_ARGUMENT;
-    try {
+    {
         std::vector<uint8_t> text;
         if (!read_file(path, text)) return TT_INVALID_ARGUMENT;
         if (text.size() >= 4 && std::memcmp(text.data(), "glTF", 4) == 0) return TT_UNSUPPORTED;  // .glb
@@ -710,11 +710,12 @@ extern "C" tt_status tt_asset_import_gltf(tt_asset_scene* scene, const char* pat
         }
         Importer(doc, buffers, scene, dir).run();
         return TT_OK;
-    } catch (const Failure& f) {
-        return f.status;
-    } catch (const std::bad_alloc&) {
-        return TT_OUT_OF_MEMORY;
-    } catch (...) {
-        return TT_INVALID_ARGUMENT;
     }
+} catch (const Failure& f) {
+    return f.status;
+} catch (const std::bad_alloc&) {
+    return TT_OUT_OF_MEMORY;
+} catch (...) {
+    // Anything else comes from a malformed document.
+    return TT_INVALID_ARGUMENT;
 }

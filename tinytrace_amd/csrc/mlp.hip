@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "mlp.h"
+#include "abi_guard.h"
 #include "runtime_internal.h"
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
@@ -552,7 +553,7 @@ tt_status upload_host(tt_context* c, T* dst, const T* src, size_t count) {
 
 extern "C" {
 
-tt_status tt_mlp_create(tt_context* c, const tt_mlp_desc* desc, tt_mlp** out) {
+tt_status tt_mlp_create(tt_context* c, const tt_mlp_desc* desc, tt_mlp** out) try {
     if (!c || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     tt_mlp* m = new (std::nothrow) tt_mlp();
@@ -583,9 +584,11 @@ tt_status tt_mlp_create(tt_context* c, const tt_mlp_desc* desc, tt_mlp** out) {
     if (st != TT_OK) return bail(st);
     *out = m;
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(c ? &c->error : nullptr);
 }
 
-void tt_mlp_destroy(tt_mlp* m) {
+void tt_mlp_destroy(tt_mlp* m) try {
     if (!m) return;
     tt_context* c = m->context;
     (void)hipSetDevice(c->device);
@@ -595,14 +598,17 @@ void tt_mlp_destroy(tt_mlp* m) {
     if (m->train_memory) (void)hipFree(m->train_memory);
     delete m;
     tt::context_release(c);
+} catch (...) {
 }
 
-tt_status tt_mlp_set_bounds(tt_mlp* m, const float lo[3], const float hi[3]) {
+tt_status tt_mlp_set_bounds(tt_mlp* m, const float lo[3], const float hi[3]) try {
     if (!m || !lo || !hi) return fail(m ? m->context : nullptr, TT_INVALID_ARGUMENT, "null argument");
     return tt::mlp_set_scene_bounds(m, lo, hi);
+} catch (...) {
+    return tt::exception_status(m && m->context ? &m->context->error : nullptr);
 }
 
-tt_status tt_mlp_params(tt_mlp* m, float* get, const float* set, uint32_t count) {
+tt_status tt_mlp_params(tt_mlp* m, float* get, const float* set, uint32_t count) try {
     if (!m) return TT_INVALID_ARGUMENT;
     tt_context* c = m->context;
     if (count != uint32_t(MLP_PARAMS)) return fail(c, TT_INVALID_ARGUMENT, "parameter count is 20672");
@@ -619,13 +625,15 @@ tt_status tt_mlp_params(tt_mlp* m, float* get, const float* set, uint32_t count)
         TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
     }
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(m && m->context ? &m->context->error : nullptr);
 }
 
 // Host-array entry points (tests): op 0 encode (out: 64 floats per record),
 // 1 infer (out: 3 floats), 2 gradient (targets: 4 floats per record; out:
 // MLP_PARAMS floats; loss), 3 one Adam step (targets; loss).
 tt_status tt_mlp_run(tt_mlp* m, int op, const float* records, const float* targets, uint32_t n, float* out,
-                     double* loss) {
+                     double* loss) try {
     if (!m || !records || n == 0) return fail(m ? m->context : nullptr, TT_INVALID_ARGUMENT, "null argument");
     tt_context* c = m->context;
     if ((op >= 2 && !targets) || (op <= 2 && !out) || op < 0 || op > 3)
@@ -665,6 +673,8 @@ tt_status tt_mlp_run(tt_mlp* m, int op, const float* records, const float* targe
     }
     (void)hipFree(d);
     return st;
+} catch (...) {
+    return tt::exception_status(m && m->context ? &m->context->error : nullptr);
 }
 
 }  // extern "C"

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "abi_guard.h"
 #include "../../include/tinytrace.h"
 #include "bvh.h"
 #include "kernels.h"
@@ -92,22 +93,26 @@ void* scratch_address[MAX_DEVICES] = {};
 // Called by tt_context_create with c->device current.  A failure does not
 // fail the context: only tt_trace_closest_static needs the address, and it
 // reports the reason.
-void resolve_static_scratch(tt_context* c) {
+void resolve_static_scratch(tt_context* c) noexcept {
     if (c->device >= MAX_DEVICES) {
-        c->static_scratch_error = "device ordinal above the static scratch table";
+        tt::set_error_nothrow(&c->static_scratch_error, "device ordinal above the static scratch table");
         return;
     }
-    std::lock_guard<std::mutex> lock(scratch_mutex);
-    if (!scratch_address[c->device]) {
-        void* p = nullptr;
-        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
-        if (e != hipSuccess || !p) {
-            c->static_scratch_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
-            return;
+    try {
+        std::lock_guard<std::mutex> lock(scratch_mutex);
+        if (!scratch_address[c->device]) {
+            void* p = nullptr;
+            hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                c->static_scratch_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+                return;
+            }
+            scratch_address[c->device] = p;
         }
-        scratch_address[c->device] = p;
+        c->static_scratch = scratch_address[c->device];
+    } catch (...) {
+        tt::set_error_nothrow(&c->static_scratch_error, "could not resolve the static scratch address");
     }
-    c->static_scratch = scratch_address[c->device];
 }
 
 }  // namespace
@@ -210,7 +215,7 @@ FrameArgs frame_args(const tt_constants& k, uint32_t width, uint32_t height, uin
 
 extern "C" {
 
-tt_status tt_context_create(const tt_context_desc* desc, tt_context** out) {
+tt_status tt_context_create(const tt_context_desc* desc, tt_context** out) try {
     if (!out) return TT_INVALID_ARGUMENT;
     *out = nullptr;
     int count = 0;
@@ -253,7 +258,7 @@ tt_status tt_context_create(const tt_context_desc* desc, tt_context** out) {
             return cleanup(TT_HIP_ERROR);
         if (c->side_streams && hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking) != hipSuccess)
             return cleanup(TT_HIP_ERROR);
-        for (hipEvent_t* e : {&L.fork, &L.join, &L.restir_read, &L.accumulated, &L.done})
+        for (hipEvent_t* e : {&L.fork, &L.join, &L.accumulated, &L.done})
             if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return cleanup(TT_HIP_ERROR);
         if (hipEventCreate(&L.integrate.begin) != hipSuccess || hipEventCreate(&L.integrate.end) != hipSuccess)
             return cleanup(TT_HIP_ERROR);
@@ -263,6 +268,8 @@ tt_status tt_context_create(const tt_context_desc* desc, tt_context** out) {
     }
     *out = c;
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(nullptr);
 }
 
 }  // extern "C"
@@ -282,7 +289,7 @@ void context_free(tt_context* c) {
         if (L.counters) (void)hipFree(L.counters);
         if (L.spill) (void)hipFree(L.spill);
         if (L.side_spill) (void)hipFree(L.side_spill);
-        for (hipEvent_t e : {L.fork, L.join, L.restir_read, L.accumulated, L.done, L.integrate.begin, L.integrate.end})
+        for (hipEvent_t e : {L.fork, L.join, L.accumulated, L.done, L.integrate.begin, L.integrate.end})
             if (e) (void)hipEventDestroy(e);
         if (L.stream) (void)hipStreamDestroy(L.stream);
         if (L.side) (void)hipStreamDestroy(L.side);
@@ -313,13 +320,14 @@ void tt::context_release(tt_context* c) {
 
 extern "C" {
 
-void tt_context_destroy(tt_context* c) {
+void tt_context_destroy(tt_context* c) try {
     if (!c) return;
     if (c->children > 0) {
         c->closing = true;
         return;
     }
     context_free(c);
+} catch (...) {
 }
 
 const char* tt_last_error(const tt_context* c) { return c ? c->error.c_str() : "no context"; }
@@ -328,7 +336,7 @@ tt_status tt_scene_create(tt_context* c, const tt_scene_desc* d, tt_scene** out)
     return tt_scene_create_ex(c, d, nullptr, out);
 }
 
-tt_status tt_scene_create_ex(tt_context* c, const tt_scene_desc* d, const tt_scene_options* options, tt_scene** out) {
+tt_status tt_scene_create_ex(tt_context* c, const tt_scene_desc* d, const tt_scene_options* options, tt_scene** out) try {
     if (!c || !d || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     auto t0 = std::chrono::steady_clock::now();
@@ -522,9 +530,11 @@ tt_status tt_scene_create_ex(tt_context* c, const tt_scene_desc* d, const tt_sce
     s->info.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = s;
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(c ? &c->error : nullptr);
 }
 
-void tt_scene_destroy(tt_scene* s) {
+void tt_scene_destroy(tt_scene* s) try {
     if (!s) return;
     if (s->context) {
         (void)hipSetDevice(s->context->device);
@@ -534,6 +544,7 @@ void tt_scene_destroy(tt_scene* s) {
     tt_context* c = s->context;
     delete s;
     if (c) tt::context_release(c);
+} catch (...) {
 }
 
 tt_status tt_scene_info_get(const tt_scene* s, tt_scene_info* out) {
@@ -542,7 +553,7 @@ tt_status tt_scene_info_get(const tt_scene* s, tt_scene_info* out) {
     return TT_OK;
 }
 
-tt_status tt_target_create(tt_context* c, uint32_t width, uint32_t height, tt_target** out) {
+tt_status tt_target_create(tt_context* c, uint32_t width, uint32_t height, tt_target** out) try {
     if (!c || !out || width == 0 || height == 0) return fail(c, TT_INVALID_ARGUMENT, "bad target size");
     *out = nullptr;
     tt_target* t = new (std::nothrow) tt_target();
@@ -560,9 +571,11 @@ tt_status tt_target_create(tt_context* c, uint32_t width, uint32_t height, tt_ta
     }
     *out = t;
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(c ? &c->error : nullptr);
 }
 
-void tt_target_destroy(tt_target* t) {
+void tt_target_destroy(tt_target* t) try {
     if (!t) return;
     if (t->context) {
         (void)hipSetDevice(t->context->device);
@@ -570,9 +583,11 @@ void tt_target_destroy(tt_target* t) {
     }
     if (t->pixels) (void)hipFree(t->pixels);
     if (t->restir_memory) (void)hipFree(t->restir_memory);
+    if (t->restir_done) (void)hipEventDestroy(t->restir_done);
     tt_context* c = t->context;
     delete t;
     if (c) tt::context_release(c);
+} catch (...) {
 }
 
 }  // extern "C"
@@ -629,6 +644,7 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     b.restir_final = nullptr;
     if (restir) {
         size_t px = size_t(t->width) * t->height;
+        if (!t->restir_done) TT_HIP(c, hipEventCreateWithFlags(&t->restir_done, hipEventDisableTiming));
         if (!t->restir_memory) {
             // Two G-buffers of 5 float4 records, four reservoir arrays.
             TT_HIP(c, hipMalloc(&t->restir_memory, px * (10 * sizeof(float4) + 4 * sizeof(uint4))));
@@ -679,21 +695,37 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
         stream = L.stream;
         TT_HIP(c, hipStreamWaitEvent(stream, c->api_mark, 0));
     }
-    // If an enqueue fails part way, still mark the lane's end after what was
-    // enqueued, so later API calls wait for it.
+    // If an enqueue fails part way, still order what was enqueued before
+    // what comes later: join the side stream back, mark the target's ReSTIR
+    // buffers as read once this render's passes end, and mark the lane's end,
+    // so later API calls and renders wait for it.
     struct PartialGuard {
         tt::Lane* lane;
         hipStream_t stream;
+        hipStream_t side;
+        tt_target* target;
+        bool on_lane;
+        bool restir_enqueued;
         bool armed;
         ~PartialGuard() {
             if (!armed) return;
-            (void)hipEventRecord(lane->done, stream);
-            lane->pending = true;
+            if (side && side != stream) {
+                (void)hipEventRecord(lane->join, side);
+                (void)hipStreamWaitEvent(stream, lane->join, 0);
+            }
+            if (restir_enqueued) {
+                (void)hipEventRecord(target->restir_done, stream);
+                target->restir_done_recorded = true;
+            }
+            if (on_lane) {
+                (void)hipEventRecord(lane->done, stream);
+                lane->pending = true;
+            }
         }
-    } partial{&L, stream, !on_api};
+    } partial{&L, stream, nullptr, t, !on_api, false, true};
     // The previous render, if on another lane, may still be running: this
-    // one reads and rewrites the target's ReSTIR buffers only after that
-    // render's last read of them, and accumulates after it.
+    // one accumulates after it (the accumulations of consecutive renders form
+    // a chain).  ReSTIR buffers are ordered by the target's own event.
     const tt::Lane* other = c->prev_lane >= 0 && uint32_t(c->prev_lane) != lane ? &c->lanes[c->prev_lane] : nullptr;
     const bool after_other = !on_api && other && other->used;
     TT_HIP(c, hipEventRecord(L.integrate.begin, stream));
@@ -713,6 +745,7 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     tt::PathBuffers side_buffers = b;
     side_buffers.spill = L.side_spill;
     hipStream_t side = c->side_streams ? L.side : stream;
+    partial.side = side;
     const bool shadows = f.light_sampling != TT_LIGHT_SAMPLING_ON_HIT && s->view.emissive_count > 0;
     auto extend = [&](uint32_t bounce) -> tt_status {
         if (stats) {
@@ -726,7 +759,10 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     if (es != TT_OK) return es;
     for (uint32_t bounce = 0; bounce < k->bounce_count; bounce++) {
         if (restir && bounce == 0 && s->view.emissive_count > 0) {
-            if (after_other) TT_HIP(c, hipStreamWaitEvent(stream, other->restir_read, 0));
+            // The previous ReSTIR render on this target, on whatever lane,
+            // has finished reading the buffers these passes rewrite.
+            if (t->restir_done_recorded) TT_HIP(c, hipStreamWaitEvent(stream, t->restir_done, 0));
+            partial.restir_enqueued = true;
             if (stats) {
                 es = stats_begin(c, tt::STATS_RESTIR, stream, &ev);
                 if (es != TT_OK) return es;
@@ -740,7 +776,11 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
         if (shadows && bounce > 0 && side != stream) TT_HIP(c, hipStreamWaitEvent(stream, L.join, 0));
         TT_HIP(c, tt::launch_shade(stream, s->view, f, b, bounce, paths));
         // Shade 0 is the last reader of the ReSTIR buffers.
-        if (bounce == 0) TT_HIP(c, hipEventRecord(L.restir_read, stream));
+        if (bounce == 0 && partial.restir_enqueued) {
+            TT_HIP(c, hipEventRecord(t->restir_done, stream));
+            t->restir_done_recorded = true;
+            partial.restir_enqueued = false;
+        }
         if (shadows) {
             if (side != stream) {
                 TT_HIP(c, hipEventRecord(L.fork, stream));
@@ -831,22 +871,26 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
 extern "C" {
 
 tt_status tt_render_ex(tt_context* c, const tt_scene* s, const tt_constants* k, const tt_render_settings* settings,
-                       tt_target* t) {
+                       tt_target* t) try {
     return render(c, s, k, settings, t, nullptr, nullptr, nullptr);
+} catch (...) {
+    return tt::exception_status(c ? &c->error : nullptr);
 }
 
 tt_status tt_render_cached(tt_context* c, const tt_scene* s, const tt_constants* k, const tt_render_settings* settings,
                            tt_mlp* cache, const tt_cache_settings* cache_settings, tt_target* t,
-                           tt_cache_stats* stats) {
+                           tt_cache_stats* stats) try {
     if (!cache) return fail(c, TT_INVALID_ARGUMENT, "null cache");
     return render(c, s, k, settings, t, cache, cache_settings, stats);
+} catch (...) {
+    return tt::exception_status(c ? &c->error : nullptr);
 }
 
 tt_status tt_render(tt_context* c, const tt_scene* s, const tt_constants* k, tt_target* t) {
     return tt_render_ex(c, s, k, nullptr, t);
 }
 
-tt_status tt_target_download(tt_context* c, const tt_target* t, float* rgba, size_t bytes) {
+tt_status tt_target_download(tt_context* c, const tt_target* t, float* rgba, size_t bytes) try {
     if (!c || !t || !rgba) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     if (t->context != c) return fail(c, TT_STALE_HANDLE, "target from another context");
     size_t need = size_t(t->width) * t->height * sizeof(float4);
@@ -855,9 +899,11 @@ tt_status tt_target_download(tt_context* c, const tt_target* t, float* rgba, siz
     TT_HIP(c, hipMemcpyAsync(rgba, t->pixels, need, hipMemcpyDeviceToHost, tt::api_stream(c)));
     TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(c ? &c->error : nullptr);
 }
 
-tt_status tt_timing_ms(tt_context* c, const char* name, double* out) {
+tt_status tt_timing_ms(tt_context* c, const char* name, double* out) try {
     if (!c || !name || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     const tt::EventPair& integrate = c->lanes[c->last_lane].integrate;
     TT_HIP(c, hipEventSynchronize(integrate.end));
@@ -891,9 +937,11 @@ tt_status tt_timing_ms(tt_context* c, const char* name, double* out) {
         return TT_OK;
     }
     return fail(c, TT_INVALID_ARGUMENT, std::string("unknown timing ") + name);
+} catch (...) {
+    return tt::exception_status(c ? &c->error : nullptr);
 }
 
-tt_status tt_render_stats_get(tt_context* c, tt_render_stats* out) {
+tt_status tt_render_stats_get(tt_context* c, tt_render_stats* out) try {
     if (!c || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     if (c->stats_renders == 0)
         return fail(c, TT_INVALID_ARGUMENT, "no render with TT_RENDER_COLLECT_STATS since the statistics were read");
@@ -935,10 +983,12 @@ tt_status tt_render_stats_get(tt_context* c, tt_render_stats* out) {
     c->stats_renders = 0;
     c->stats_paths = 0;
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(c ? &c->error : nullptr);
 }
 
 tt_status tt_trace_closest_device(tt_context* c, const tt_scene* s, const tt_ray* rays, tt_hit* hits, uint32_t count,
-                                  uint32_t flags) {
+                                  uint32_t flags) try {
     if (!c || !s || (count && (!rays || !hits))) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     if (s->context != c) return fail(c, TT_STALE_HANDLE, "scene from another context");
     if (count == 0) return TT_OK;
@@ -947,10 +997,12 @@ tt_status tt_trace_closest_device(tt_context* c, const tt_scene* s, const tt_ray
     TT_HIP(c, tt::launch_trace_rays(tt::api_stream(c), s->view, rays, hits, count, (flags & TT_TRACE_CULL_BACK_FACES) != 0,
                                     c->work, c->lanes[0].spill, c->grid));
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(c ? &c->error : nullptr);
 }
 
 tt_status tt_trace_closest(tt_context* c, const tt_scene* s, const tt_ray* rays, tt_hit* hits, uint32_t count,
-                           uint32_t flags) {
+                           uint32_t flags) try {
     if (!c || !s || (count && (!rays || !hits))) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     if (count == 0) return TT_OK;
     TT_HIP(c, hipSetDevice(c->device));
@@ -968,10 +1020,12 @@ tt_status tt_trace_closest(tt_context* c, const tt_scene* s, const tt_ray* rays,
     if (dh) (void)hipFree(dh);
     if (e != hipSuccess) return hip_fail(c, e, "tt_trace_closest");
     return st;
+} catch (...) {
+    return tt::exception_status(c ? &c->error : nullptr);
 }
 
 tt_status tt_trace_closest_static(tt_context* c, const tt_scene* s, const tt_ray* rays, tt_hit* hits, uint32_t count,
-                                  uint32_t flags) {
+                                  uint32_t flags) try {
     if (!c || !s || (count && (!rays || !hits))) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     if (s->context != c) return fail(c, TT_STALE_HANDLE, "scene from another context");
     if (count > STATIC_TRACE_MAX_RAYS)
@@ -999,10 +1053,12 @@ tt_status tt_trace_closest_static(tt_context* c, const tt_scene* s, const tt_ray
     TT_HIP(c, hipMemcpyAsync(hits, staged_hits, size_t(count) * sizeof(tt_hit), hipMemcpyDeviceToHost, stream));
     TT_HIP(c, hipStreamSynchronize(stream));
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(c ? &c->error : nullptr);
 }
 
 tt_status tt_bsdf_evaluate(tt_context* c, const tt_material_constants* material, const tt_bsdf_query* queries,
-                           tt_bsdf_result* results, uint32_t count, uint32_t rng_state) {
+                           tt_bsdf_result* results, uint32_t count, uint32_t rng_state) try {
     if (!c || !material || (count && (!queries || !results))) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     if (count == 0) return TT_OK;
     TT_HIP(c, hipSetDevice(c->device));
@@ -1019,9 +1075,11 @@ tt_status tt_bsdf_evaluate(tt_context* c, const tt_material_constants* material,
     if (dr) (void)hipFree(dr);
     if (e != hipSuccess) return hip_fail(c, e, "tt_bsdf_evaluate");
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(c ? &c->error : nullptr);
 }
 
-tt_status tt_debug_math(tt_context* c, int op, const float* x, const float* y, float* out, uint32_t count) {
+tt_status tt_debug_math(tt_context* c, int op, const float* x, const float* y, float* out, uint32_t count) try {
     if (!c || !x || !y || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     if (count == 0) return TT_OK;
     TT_HIP(c, hipSetDevice(c->device));
@@ -1035,6 +1093,8 @@ tt_status tt_debug_math(tt_context* c, int op, const float* x, const float* y, f
     (void)hipFree(d);
     if (e != hipSuccess) return hip_fail(c, e, "tt_debug_math");
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(c ? &c->error : nullptr);
 }
 
 }  // extern "C"

@@ -31,14 +31,14 @@ struct Lane {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
-    // Cross-lane order: the next render (on the next lane) starts its ReSTIR
-    // passes after `restir_read` (this render's last read of the target's
-    // ReSTIR buffers) and accumulates after `accumulated`.
-    hipEvent_t restir_read = nullptr, accumulated = nullptr;
+    // Cross-lane order: the next render (on the next lane) accumulates after
+    // `accumulated`.  (ReSTIR buffers are ordered per target instead:
+    // tt_target::restir_done.)
+    hipEvent_t accumulated = nullptr;
     hipEvent_t done = nullptr;  // end of the lane's last render
     EventPair integrate;
     bool pending = false;       // a render the API stream has not waited for
-    bool used = false;          // restir_read / accumulated have been recorded
+    bool used = false;          // accumulated has been recorded
     uint32_t* counters = nullptr;
     uint32_t* spill = nullptr;
     uint32_t* side_spill = nullptr;
@@ -130,6 +130,11 @@ struct tt_target {
     uint32_t restir_cur = 0;
     bool restir_history = false;
     uint64_t restir_scene = 0;  // serial of the scene the history belongs to
+    // Recorded after the last ReSTIR render's shade 0, the last reader of
+    // these buffers; the next ReSTIR render on this target, on whatever lane,
+    // waits for it before its ReSTIR passes rewrite them.
+    hipEvent_t restir_done = nullptr;
+    bool restir_done_recorded = false;
 };
 
 namespace tt {

@@ -37,6 +37,7 @@
 #include <new>
 #include <vector>
 
+#include "abi_guard.h"
 #include "runtime_internal.h"
 
 namespace {
@@ -148,7 +149,7 @@ tt_status admit(tt_server* sv) {
 
 extern "C" {
 
-tt_status tt_server_create(tt_context* c, const tt_server_desc* desc, tt_server** out) {
+tt_status tt_server_create(tt_context* c, const tt_server_desc* desc, tt_server** out) try {
     if (!c || !out) return fail(c, TT_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     tt_server* sv = new (std::nothrow) tt_server();
@@ -173,9 +174,11 @@ tt_status tt_server_create(tt_context* c, const tt_server_desc* desc, tt_server*
     }
     *out = sv;
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(c ? &c->error : nullptr);
 }
 
-void tt_server_destroy(tt_server* sv) {
+void tt_server_destroy(tt_server* sv) try {
     if (!sv) return;
     (void)hipSetDevice(sv->context->device);
     (void)hipStreamSynchronize(tt::api_stream(sv->context));  // also waits for batches in flight
@@ -184,17 +187,20 @@ void tt_server_destroy(tt_server* sv) {
     tt_context* c = sv->context;
     delete sv;
     tt::context_release(c);
+} catch (...) {
 }
 
-tt_status tt_server_add_scene(tt_server* sv, const tt_scene* s, uint32_t* slot) {
+tt_status tt_server_add_scene(tt_server* sv, const tt_scene* s, uint32_t* slot) try {
     if (!sv || !s || !slot) return fail(sv ? sv->context : nullptr, TT_INVALID_ARGUMENT, "null argument");
     if (s->context != sv->context) return fail(sv->context, TT_STALE_HANDLE, "scene from another context");
     *slot = uint32_t(sv->scenes.size());
     sv->scenes.push_back(s);
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(sv && sv->context ? &sv->context->error : nullptr);
 }
 
-tt_status tt_server_submit(tt_server* sv, const tt_server_request* q, uint64_t* id) {
+tt_status tt_server_submit(tt_server* sv, const tt_server_request* q, uint64_t* id) try {
     if (!sv || !q || !id) return fail(sv ? sv->context : nullptr, TT_INVALID_ARGUMENT, "null argument");
     tt_context* c = sv->context;
     const tt_constants& k = q->constants;
@@ -218,6 +224,8 @@ tt_status tt_server_submit(tt_server* sv, const tt_server_request* q, uint64_t* 
     *id = r->id;
     sv->requests[r->id] = std::move(r);
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(sv && sv->context ? &sv->context->error : nullptr);
 }
 
 }  // extern "C"
@@ -355,7 +363,7 @@ tt_status finish_batch(tt_server* sv, uint32_t& done) {
 
 extern "C" {
 
-tt_status tt_server_step(tt_server* sv, uint32_t* completed) {
+tt_status tt_server_step(tt_server* sv, uint32_t* completed) try {
     if (!sv) return TT_INVALID_ARGUMENT;
     tt_context* c = sv->context;
     if (completed) *completed = 0;
@@ -375,9 +383,11 @@ tt_status tt_server_step(tt_server* sv, uint32_t* completed) {
     }
     if (completed) *completed = done;
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(sv && sv->context ? &sv->context->error : nullptr);
 }
 
-tt_status tt_server_status(tt_server* sv, uint64_t id, tt_request_status* out) {
+tt_status tt_server_status(tt_server* sv, uint64_t id, tt_request_status* out) try {
     if (!sv || !out) return fail(sv ? sv->context : nullptr, TT_INVALID_ARGUMENT, "null argument");
     auto it = sv->requests.find(id);
     if (it == sv->requests.end()) return fail(sv->context, TT_STALE_HANDLE, "unknown request");
@@ -388,9 +398,11 @@ tt_status tt_server_status(tt_server* sv, uint64_t id, tt_request_status* out) {
     out->admitted_ns = r.admitted_ns;
     out->done_ns = r.done_ns;
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(sv && sv->context ? &sv->context->error : nullptr);
 }
 
-tt_status tt_server_take_result(tt_server* sv, uint64_t id, float* rgba, size_t bytes) {
+tt_status tt_server_take_result(tt_server* sv, uint64_t id, float* rgba, size_t bytes) try {
     if (!sv || !rgba) return fail(sv ? sv->context : nullptr, TT_INVALID_ARGUMENT, "null argument");
     auto it = sv->requests.find(id);
     if (it == sv->requests.end()) return fail(sv->context, TT_STALE_HANDLE, "unknown request");
@@ -400,9 +412,11 @@ tt_status tt_server_take_result(tt_server* sv, uint64_t id, float* rgba, size_t 
     std::memcpy(rgba, r.image.data(), r.target_bytes());
     sv->requests.erase(it);
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(sv && sv->context ? &sv->context->error : nullptr);
 }
 
-tt_status tt_server_stats_get(tt_server* sv, tt_server_stats* out) {
+tt_status tt_server_stats_get(tt_server* sv, tt_server_stats* out) try {
     if (!sv || !out) return fail(sv ? sv->context : nullptr, TT_INVALID_ARGUMENT, "null argument");
     *out = sv->stats;
     out->queued = sv->queued.size();
@@ -413,6 +427,8 @@ tt_status tt_server_stats_get(tt_server* sv, tt_server_stats* out) {
     out->done = done;
     out->budget_used_bytes = sv->used + sv->pool_bytes();
     return TT_OK;
+} catch (...) {
+    return tt::exception_status(sv && sv->context ? &sv->context->error : nullptr);
 }
 
 }  // extern "C"
