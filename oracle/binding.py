@@ -202,3 +202,103 @@ def display(rgba, tonemap=True):
     out = np.zeros(a.shape[:-1] + (4,), dtype=np.uint8)
     lib().oracle_display(a.ctypes.data, a.size // 4, 1 if tonemap else 0, out.ctypes.data)
     return out
+
+
+LOBES = {"fuzz": 0, "coat": 1, "metal": 2, "specular_brdf": 3, "specular_btdf": 4, "diffuse": 5}
+
+
+class BsdfHooks:
+    """Single BSDF functions of the oracle (bsdf_hooks.cpp), from liboracle.so
+    or from a library built from bsdf_hooks.cpp alone (`path`).  `precision`
+    0 runs the float instantiation, 1 the double one."""
+
+    def __init__(self, path: str | None = None, precision: int = 1):
+        self.L = lib() if path is None else C.CDLL(path)
+        self.p = precision
+        d, vp, u32, i = C.c_double, C.c_void_p, C.c_uint32, C.c_int
+        sig = {
+            "oracle_h_fresnel_metal": (None, [i, d, vp, vp, vp]),
+            "oracle_h_fresnel_dielectric": (d, [i, d, d]),
+            "oracle_h_average_dielectric_fresnel": (d, [i, d]),
+            "oracle_h_specular_ior": (d, [i, vp]),
+            "oracle_h_specular_ior_ratio": (d, [i, vp]),
+            "oracle_h_specular_fresnel_with_coat": (d, [i, vp, d, d]),
+            "oracle_h_oren_nayar": (d, [i, d, d]),
+            "oracle_h_energy_compensated_oren_nayar": (None, [i, vp, d, u32, vp, vp, vp]),
+            "oracle_h_fuzz_albedo": (d, [i, d, d]),
+            "oracle_h_ltc_inverse_coeffs": (None, [i, d, d, vp]),
+            "oracle_h_lobe_evaluate": (None, [i, vp, i, u32, vp, vp, vp]),
+            "oracle_h_lobe_sample": (i, [i, vp, i, vp, u32, vp, vp]),
+            "oracle_h_lobe_albedo": (None, [i, vp, i, vp, u32, vp, vp]),
+            "oracle_h_lobe_weights": (None, [i, vp, vp, u32, vp, vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(self.L, name)
+            fn.restype, fn.argtypes = res, args
+
+    @staticmethod
+    def _v(x, n=None):
+        a = np.ascontiguousarray(x, dtype=np.float64)
+        return a
+
+    def fresnel_metal(self, c, f0, tint):
+        out = np.zeros(3)
+        self.L.oracle_h_fresnel_metal(self.p, c, self._v(f0).ctypes.data, self._v(tint).ctypes.data, out.ctypes.data)
+        return out
+
+    def fresnel_dielectric(self, c, ior):
+        return self.L.oracle_h_fresnel_dielectric(self.p, c, ior)
+
+    def average_dielectric_fresnel(self, ior):
+        return self.L.oracle_h_average_dielectric_fresnel(self.p, ior)
+
+    def specular_ior(self, material):
+        return self.L.oracle_h_specular_ior(self.p, C.addressof(material))
+
+    def specular_ior_ratio(self, material):
+        return self.L.oracle_h_specular_ior_ratio(self.p, C.addressof(material))
+
+    def specular_fresnel_with_coat(self, material, c, ior):
+        return self.L.oracle_h_specular_fresnel_with_coat(self.p, C.addressof(material), c, ior)
+
+    def oren_nayar(self, c, s2):
+        return self.L.oracle_h_oren_nayar(self.p, c, s2)
+
+    def energy_compensated_oren_nayar(self, rho, sigma, wi, wo):
+        wi, wo = self._v(wi).reshape(-1, 3), self._v(wo).reshape(-1, 3)
+        wi = np.ascontiguousarray(np.broadcast_to(wi, wo.shape))
+        out = np.zeros(wo.shape)
+        self.L.oracle_h_energy_compensated_oren_nayar(self.p, self._v(rho).ctypes.data, sigma, wo.shape[0],
+                                                      wi.ctypes.data, wo.ctypes.data, out.ctypes.data)
+        return out
+
+    def fuzz_albedo(self, x, y):
+        return self.L.oracle_h_fuzz_albedo(self.p, x, y)
+
+    def ltc_inverse_coeffs(self, x, y):
+        out = np.zeros(2)
+        self.L.oracle_h_ltc_inverse_coeffs(self.p, x, y, out.ctypes.data)
+        return out
+
+    def lobe_evaluate(self, material, lobe, wi, wo):
+        """(n, 4): bsdf rgb and density of lobe `lobe` at n (wi, wo) pairs."""
+        wo = self._v(wo).reshape(-1, 3)
+        wi = np.ascontiguousarray(np.broadcast_to(self._v(wi).reshape(-1, 3), wo.shape))
+        out = np.zeros((wo.shape[0], 4))
+        self.L.oracle_h_lobe_evaluate(self.p, C.addressof(material), LOBES.get(lobe, lobe), wo.shape[0],
+                                      wi.ctypes.data, wo.ctypes.data, out.ctypes.data)
+        return out
+
+    def lobe_albedo(self, material, lobe, wi, rng_states):
+        states = np.ascontiguousarray(rng_states, dtype=np.uint32)
+        out = np.zeros((states.size, 3))
+        self.L.oracle_h_lobe_albedo(self.p, C.addressof(material), LOBES.get(lobe, lobe), self._v(wi).ctypes.data,
+                                    states.size, states.ctypes.data, out.ctypes.data)
+        return out
+
+    def lobe_weights(self, material, wi, rng_state=0x9E3779B9):
+        """bsdf_prepare: (weights (6, 3), albedos (6, 3), probabilities (6,))."""
+        w, a, p = np.zeros((6, 3)), np.zeros((6, 3)), np.zeros(6)
+        self.L.oracle_h_lobe_weights(self.p, C.addressof(material), self._v(wi).ctypes.data, rng_state,
+                                     w.ctypes.data, a.ctypes.data, p.ctypes.data)
+        return w, a, p

@@ -17,6 +17,16 @@
 //   * LobeDensities starts at zero (bsdf.glsl:157,190 leave it uninitialised).
 //   * A lobe sample that returns without writing `wo` reports failure
 //     (`ok = false`); bsdf_sample then fails and the path terminates.
+//
+// Pins (tests/test_bsdf_cpu.py, tests/test_bsdf_pins_cpu.py through
+// bsdf_hooks.cpp): closed forms, quadrature, published special cases and
+// the reference's fitted constants, each shown to fail on a planted slip.
+// Parity unpinned: specular_btdf_evaluate's density and Jacobian (the
+// reference's own, with normal-frame cosines where Walter 2007 has the
+// half-vector dots; its sample path is pinned through the smooth limit), and
+// fuzz_brdf_sample's density, which the reference evaluates at the LTC-space
+// direction (checked as that quirk only).  Both lobes are inactive in every
+// configuration's materials.
 #pragma once
 
 #include "oracle_math.hpp"
