@@ -4,7 +4,7 @@ benchmark configuration (512x512, 64 passes, depth 4)."""
 import numpy as np
 import pytest
 
-from test_gpu_parity import assert_hits_equal, assert_radiance_close, random_rays, render_pair, restir_pair
+from test_gpu_parity import assert_hits_equal, assert_radiance_close, axis_rays, random_rays, render_pair, restir_pair
 from tinytrace_amd import Config, DeviceScene, LightSampling, RenderSettings, _abi, scenes
 
 pytestmark = pytest.mark.gpu
@@ -29,7 +29,8 @@ def test_trace_instanced_bit_exact(context, instanced, cull):
     info = dev.info()
     assert info["unique_triangle_count"] == 3215512 and info["triangle_count"] == 596951286
     rng = np.random.default_rng(23)
-    rays = random_rays(rng, 200000, (100, 100, 100), 110.0)
+    rays = np.concatenate([random_rays(rng, 200000, (100, 100, 100), 110.0),
+                           axis_rays(rng, 100000, (100, 60, 100), 110.0)])
     assert_hits_equal(dev.trace(rays, cull), oracle.trace(rays, cull))
 
 

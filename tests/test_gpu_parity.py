@@ -69,6 +69,25 @@ def random_rays(rng, n, center, spread, tmin=1e-3, tmax=1e4):
     return rays
 
 
+def axis_rays(rng, n, center, spread, tmin=1e-3, tmax=1e4):
+    """Rays whose directions have exact zero components: along the axes and
+    in the coordinate planes (the box test must cull on such an axis by the
+    ray's constant coordinate, and stay conservative)."""
+    rays = random_rays(rng, n, center, spread, tmin, tmax)
+    d = rng.standard_normal((n, 3))
+    zero = rng.integers(0, 3, n)
+    d[np.arange(n), zero] = 0.0
+    axis = rng.random(n) < 0.3  # a third along one axis only
+    keep = rng.integers(0, 3, n)
+    d[axis] = 0.0
+    d[axis, keep[axis]] = rng.choice([-1.0, 1.0], axis.sum())
+    rays["direction"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    # Some origins on integer and zero coordinates (planes of the scenes' boxes).
+    snap = rng.random(n) < 0.2
+    rays["origin"][snap] = np.round(rays["origin"][snap])
+    return rays
+
+
 def edge_rays(arrays, rng, n_per_tri=4):
     """Rays from random points aimed exactly at triangle vertices and edge
     midpoints (world space), where a non-watertight test would leak."""
@@ -125,6 +144,17 @@ def test_trace_sphere_field_bit_exact(context, cull):
     rng = np.random.default_rng(11)
     rays = np.concatenate([random_rays(rng, 100000, (50, 50, 50), 60.0), edge_rays(scene.arrays(), rng, 1)])
     assert_hits_equal(dev.trace(rays, cull), oracle.trace(rays, cull))
+
+
+@pytest.mark.parametrize("cull", [True, False])
+def test_trace_axis_aligned_rays_bit_exact(context, cull):
+    scene, camera = scenes.sphere_field(seed=3, spheres=60, lights=16, subdivisions=3)
+    dev = DeviceScene(context, scene)
+    oracle = _oracle().OracleScene(scene, 32, use_bvh=True)
+    rays = axis_rays(np.random.default_rng(12 + int(cull)), 100000, (50, 50, 50), 60.0)
+    hits = dev.trace(rays, cull)
+    assert (hits["instance"] != 0xFFFFFFFF).mean() > 0.05
+    assert_hits_equal(hits, oracle.trace(rays, cull))
 
 
 MATERIALS = {

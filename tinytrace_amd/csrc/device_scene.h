@@ -77,10 +77,26 @@ struct TraceRay {
 
 TT_DEV float component(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
+// A direction component smaller in magnitude than tiny = e * 1e-6 (zero
+// included, which the instance transform produces for rays nearly
+// perpendicular to an object axis) is replaced by +-tiny for the box tests.
+// With 1 / 0 = inf the fused slab distance l * inv - (o + e) * inv would be
+// inf - inf = NaN for every box on the ray's side of the axis' origin, and
+// that axis, which should cull every box not straddling the ray's constant
+// coordinate, would cull nothing (one such ray walked half of an 81 920-
+// triangle BLAS).  With +-tiny the axis accepts exactly the boxes within e of
+// that coordinate, over a t range of +-e / tiny = +-1e6 (beyond every tmax),
+// and (o + e) / tiny <= 1.3e5 / 1e-6 stays finite; the shift e still covers
+// the rounding of (o + e) * inv (at most 8e3 there).  Triangle tests use the
+// true direction.
 TT_DEV void slab_setup(TraceRay& r, f3 o, f3 d) {
-    r.inv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     float mo = fmaxf(fmaxf(abs_(o.x), abs_(o.y)), abs_(o.z));
     float e = mo * (64.0f * 1.1920929e-7f) + 1e-30f;
+    float tiny = e * 1e-6f;
+    float dx = abs_(d.x) < tiny ? copysignf(tiny, d.x) : d.x;
+    float dy = abs_(d.y) < tiny ? copysignf(tiny, d.y) : d.y;
+    float dz = abs_(d.z) < tiny ? copysignf(tiny, d.z) : d.z;
+    r.inv = f3(1.0f / dx, 1.0f / dy, 1.0f / dz);
     r.olo_inv = f3((o.x + e) * r.inv.x, (o.y + e) * r.inv.y, (o.z + e) * r.inv.z);
     r.ohi_inv = f3((o.x - e) * r.inv.x, (o.y - e) * r.inv.y, (o.z - e) * r.inv.z);
 }
@@ -108,9 +124,9 @@ TT_DEV void woop_setup(TraceRay& r, f3 o, f3 d) {
 
 // Conservative slab interval of a box: plane distances as one fused
 // multiply-add from the shifted origins (the shift e covers the rounding of
-// origin * inv many times over), widened by a relative 1e-6.  A zero
-// direction component makes that axis' distances NaN, which min/max ignore:
-// the axis then does not cull, so the test stays conservative.
+// origin * inv many times over), widened by a relative 1e-6.  Directions
+// are never zero here (slab_setup); a NaN distance (a NaN ray) is ignored by
+// min/max, so that axis does not cull and the test stays conservative.
 TT_DEV void box_interval(const TraceRay& r, float lx, float hx, float ly, float hy, float lz, float hz, float& n,
                          float& f) {
     float t0x = __builtin_fmaf(lx, r.inv.x, -r.olo_inv.x), t1x = __builtin_fmaf(hx, r.inv.x, -r.ohi_inv.x);
