@@ -330,6 +330,10 @@ pub struct tt_server_request {
     pub seed: u32,
     pub passes: u32,
     pub constants: tt_constants,
+    pub restir_candidates: u32,
+    pub restir_temporal: u32,
+    pub restir_spatial: u32,
+    pub reserved: u32,
 }
 
 #[repr(C)]

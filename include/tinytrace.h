@@ -350,12 +350,15 @@ tt_status tt_render_stats_get(tt_context* context, tt_render_stats* out);
 
 /* A server keeps scenes resident and renders many requests concurrently:
  * every step runs one batch, a shared pool of path slots filled with
- * one-sample chunks (pass k, pixel range) of the admitted requests of one
- * scene and integrator configuration, and accumulates each chunk into its
- * request's target in pass order.  A request's image therefore depends only
- * on the request (keyed generators: pixel, pass, seed) and is bit-identical
- * whether it runs alone or in any mix.  Requests wait in arrival order until
- * their render target fits the HBM budget next to the path pool. */
+ * one-sample chunks (pass k, pixel range; whole passes for ReSTIR requests)
+ * of the admitted requests of one scene and integrator configuration, and
+ * accumulates each chunk into its request's target in pass order.  A
+ * request's image therefore depends only on the request (keyed generators:
+ * pixel, pass, seed) and is bit-identical whether it runs alone or in any
+ * mix.  Requests wait in arrival order until their target (and ReSTIR state)
+ * fits the HBM budget next to the path pool.  Scenes with textures are not
+ * served (TT_UNSUPPORTED): their level of detail needs each path's own
+ * camera. */
 typedef struct tt_server tt_server;
 
 typedef struct tt_server_desc {
@@ -372,9 +375,20 @@ typedef struct tt_server_request {
     uint32_t seed;
     uint32_t passes;         /* samples per pixel, one per pass */
     tt_constants constants;  /* camera, screen size, bounce_count,
-                                light_sampling (ON_HIT or NEE); frame_index
-                                and accumulated_frame_count are set per pass
-                                (k and k + 1: the result is the exact mean) */
+                                light_sampling (ON_HIT, NEE or RESTIR);
+                                frame_index and accumulated_frame_count are
+                                set per pass (k and k + 1: the result is the
+                                exact mean) */
+    /* ReSTIR DI (light_sampling RESTIR), as tt_render_settings: the request
+     * keeps its G-buffers and reservoirs (224 B per pixel, counted in the
+     * HBM budget) resident across its passes; pass k reuses pass k - 1's
+     * reservoirs.  Its passes run whole, so width * height must not exceed
+     * the server's pool_paths.  The image equals tt_render_ex's progressive
+     * render of the same passes on a fresh target, bit for bit. */
+    uint32_t restir_candidates;  /* 0: 32 */
+    uint32_t restir_temporal;
+    uint32_t restir_spatial;
+    uint32_t reserved;
 } tt_server_request;
 
 enum { TT_REQUEST_QUEUED = 0, TT_REQUEST_ADMITTED = 1, TT_REQUEST_DONE = 2 };

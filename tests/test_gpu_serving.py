@@ -99,3 +99,72 @@ def test_poisson_mix_completes():
                                                      light_sampling=LightSampling.NEXT_EVENT_ESTIMATION))
     assert out["requests"] == 24 and 0 < out["p50_ms"] <= out["p99_ms"]
     assert out["paths"] == sum(s.width * s.height * s.passes for s in specs)
+
+
+RESTIR = Config(bounce_count=3, sample_count=1, light_sampling=LightSampling.RESTIR)
+
+
+def _restir_scene():
+    return scenes.sphere_field(seed=2, spheres=40, lights=24, subdivisions=2)
+
+
+@pytest.mark.parametrize("pool,depth", [(20000, 0), (6144, 3), (6144, 1)],
+                         ids=["passes-share-a-batch", "one-pass-per-batch-three-in-flight", "one-in-flight"])
+def test_restir_request_equals_progressive_restir_render(pool, depth):
+    """A ReSTIR request keeps its reservoirs across its passes: its image
+    equals tt_render_ex's progressive ReSTIR render of the same passes on a
+    fresh target, whether its passes share batches or follow each other on
+    different lanes."""
+    scene, camera = _restir_scene()
+    renderer = Renderer((96, 64))
+    renderer.set_scene(scene)
+    expected = progressive(renderer, camera, RESTIR, 96, 64, 5, seed=3)
+    server = Server(renderer.context, pool_paths=pool, batches_in_flight=depth)
+    slot = server.add_scene(renderer.scene)
+    rid = server.submit(slot, camera, RESTIR, 96, 64, 5, seed=3)
+    server.run_until_done([rid])
+    got = server.take_result(rid)
+    assert np.isfinite(got).all() and got[..., :3].max() > 0
+    np.testing.assert_array_equal(got, expected)
+
+
+def test_restir_request_alone_equals_in_mix():
+    scene, camera = _restir_scene()
+    renderer = Renderer((8, 8))
+    ctx = renderer.context
+    restir_scene = DeviceScene(ctx, scene)
+    cornell = DeviceScene(ctx, scenes.cornell_box()[0])
+    rng = np.random.default_rng(9)
+
+    alone = Server(ctx, pool_paths=1 << 15)
+    s = alone.add_scene(restir_scene)
+    rid = alone.submit(s, camera, RESTIR, 80, 60, 4, seed=21)
+    alone.run_until_done([rid])
+    reference = alone.take_result(rid)
+
+    mixed = Server(ctx, pool_paths=12000)
+    slots = [mixed.add_scene(restir_scene), mixed.add_scene(cornell)]
+    others = [mixed.submit(slots[0], orbit_camera(rng, (50, 50, 50), 60.0), RESTIR, 64, 48, 3, seed=1),
+              mixed.submit(slots[1], orbit_camera(rng), NEE, 100, 50, 2, seed=2)]
+    target = mixed.submit(slots[0], camera, RESTIR, 80, 60, 4, seed=21)
+    mixed.step()
+    others.append(mixed.submit(slots[0], orbit_camera(rng, (50, 50, 50), 60.0), RESTIR, 40, 30, 6, seed=5,
+                               restir_temporal=False))
+    others.append(mixed.submit(slots[1], orbit_camera(rng), NEE, 64, 64, 3, seed=6))
+    mixed.run_until_done(others + [target])
+    np.testing.assert_array_equal(mixed.take_result(target), reference)
+
+
+def test_restir_request_refusals():
+    scene, camera = _restir_scene()
+    renderer = Renderer((8, 8))
+    server = Server(renderer.context, pool_paths=4096)
+    slot = server.add_scene(DeviceScene(renderer.context, scene))
+    with pytest.raises(_abi.TinytraceError) as e:
+        server.submit(slot, camera, RESTIR, 128, 64, 1)  # a whole pass does not fit the pool
+    assert e.value.status == 7  # TT_OVER_BUDGET
+    textured, tcam = scenes.textured_cornell_box()
+    tslot = server.add_scene(DeviceScene(renderer.context, textured))
+    with pytest.raises(_abi.TinytraceError) as e:
+        server.submit(tslot, tcam, NEE, 32, 32, 1)
+    assert e.value.status == 8  # TT_UNSUPPORTED

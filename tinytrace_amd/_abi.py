@@ -145,7 +145,8 @@ class ServerDesc(C.Structure):
 
 class ServerRequest(C.Structure):
     _fields_ = [("scene", C.c_uint32), ("rng", C.c_uint32), ("seed", C.c_uint32), ("passes", C.c_uint32),
-                ("constants", Constants)]
+                ("constants", Constants), ("restir_candidates", C.c_uint32), ("restir_temporal", C.c_uint32),
+                ("restir_spatial", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class RequestStatus(C.Structure):

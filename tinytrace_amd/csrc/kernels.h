@@ -68,8 +68,12 @@ struct PathBuffers {
     float4* shadow[4];    // origin, direction, direct, emission: [paths] each
     uint32_t* counters;   // see kernels.hip
     uint32_t* spill;      // traversal stack spill
-    const uint4* restir_final;  // per-pixel reservoirs used at the primary vertex
-    const uint4* restir_initial;  // the pixel's own candidate after the visibility pass
+    const uint4* restir_final;  // per-path reservoirs used at the primary vertex
+    const uint4* restir_initial;  // the path's own candidate after the visibility pass
+    // (pixel, frame index) of each path slot for the ReSTIR lobe generator;
+    // null when the slot is the pixel and the frame is the render's.
+    // Serving batches place passes of several requests in one pool.
+    const uint2* restir_key;
     uint32_t max_bounces;
     CacheBuffers cache;
 };
@@ -86,6 +90,10 @@ struct RestirBuffers {
     uint32_t candidates;
     uint32_t temporal_reuse;
     uint32_t spatial_reuse;
+    // Path slot of pixel 0: the passes read pixel p's primary path (queues,
+    // hit, throughput) at slot base + p; G-buffers and reservoirs are indexed
+    // by pixel.  0 for renders.
+    uint32_t base;
 };
 
 int trace_grid(int cu_count);
@@ -107,6 +115,8 @@ constexpr uint32_t C_CACHE_TRAIN = 131;  // radiance-cache training records
 constexpr uint32_t C_WORDS = 160;
 
 hipError_t launch_zero(hipStream_t s, uint32_t* p, uint32_t words);
+// key[i] = (first + i, frame) for i < count.
+hipError_t launch_slot_keys(hipStream_t s, uint2* key, uint32_t first, uint32_t frame, uint32_t count);
 hipError_t launch_stats_add(hipStream_t s, const uint32_t* counters, unsigned long long* sum, uint32_t words);
 hipError_t launch_trace_rays(hipStream_t s, const DeviceScene& scene, const tt_ray* rays, tt_hit* hits, uint32_t count,
                              bool cull, uint32_t* work, uint32_t* spill, int grid);

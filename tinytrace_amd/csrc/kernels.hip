@@ -220,7 +220,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 
         f3 wi = sf.wi;
         Lobes lobes;
         if (restir) {
-            Generator lg = restir_lobe_generator(p, f.frame_index);
+            uint2 key = b.restir_key ? b.restir_key[p] : make_uint2(p, f.frame_index);
+            Generator lg = restir_lobe_generator(key.x, key.y);
             lobes = bsdf_prepare(m, wi, lg);
         } else {
             lobes = bsdf_prepare(m, wi, g);
@@ -443,6 +444,11 @@ __global__ void zero_kernel(uint32_t* p, uint32_t n) {
     if (i < n) p[i] = 0;
 }
 
+__global__ void slot_keys_kernel(uint2* key, uint32_t first, uint32_t frame, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) key[i] = make_uint2(first + i, frame);
+}
+
 // sum[i] += counters[i]: a render's ray counters into the statistics sums
 // (renders on several lanes may add at once).
 __global__ void stats_add_kernel(const uint32_t* counters, unsigned long long* sum, uint32_t n) {
@@ -509,6 +515,12 @@ size_t spill_words(int grid) { return size_t(grid) * TRACE_BLOCK * TRACE_SPILL_S
 
 hipError_t launch_zero(hipStream_t s, uint32_t* p, uint32_t words) {
     zero_kernel<<<blocks(words, 256), 256, 0, s>>>(p, words);
+    return hipGetLastError();
+}
+
+hipError_t launch_slot_keys(hipStream_t s, uint2* key, uint32_t first, uint32_t frame, uint32_t count) {
+    if (count == 0) return hipSuccess;
+    slot_keys_kernel<<<blocks(count, 256), 256, 0, s>>>(key, first, frame, count);
     return hipGetLastError();
 }
 

@@ -157,8 +157,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= f.width * f.height) return;
     float4* g = r.gbuffer[r.cur] + RECORD * size_t(p);
-    float4 qd = b.queue[0][1][p];
-    float4 hit = b.hits[p];
+    const uint32_t path = r.base + p;  // the pixel's primary path
+    float4 qd = b.queue[0][1][path];
+    float4 hit = b.hits[path];
     uint32_t instance = __float_as_uint(hit.z);
     Reservoir res = empty_reservoir();
     bool emit = false;
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 
         // coordinates ride in the first ray's and the throughput's w.
         TextureHit tex{TEXTURED, true,
                        PrimaryRay{f.inverse_view, f.inverse_proj, cam, float(f.width), float(f.height), qd.w,
-                                  b.throughput[p].w},
+                                  b.throughput[path].w},
                        1.0f};
         q.sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, q.m, tex);
         q.valid = s.emissive_count > 0 && q.m.specular_roughness > 0.15f;

@@ -153,6 +153,7 @@ tt_status ensure_paths(tt_context* c, uint32_t lane, size_t paths, PathBuffers& 
     b.cache = CacheBuffers{};
     b.restir_final = nullptr;
     b.restir_initial = nullptr;
+    b.restir_key = nullptr;
     return TT_OK;
 }
 
@@ -286,6 +287,7 @@ void context_free(tt_context* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (tt::Lane& L : c->lanes) {
         if (L.path_memory) (void)hipFree(L.path_memory);
+        if (L.restir_slot_memory) (void)hipFree(L.restir_slot_memory);
         if (L.counters) (void)hipFree(L.counters);
         if (L.spill) (void)hipFree(L.spill);
         if (L.side_spill) (void)hipFree(L.side_spill);
@@ -666,6 +668,7 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
         rb.candidates = st.restir_candidates ? st.restir_candidates : 32u;
         rb.temporal_reuse = st.restir_temporal ? 1u : 0u;
         rb.spatial_reuse = st.restir_spatial ? 1u : 0u;
+        rb.base = 0;
         b.restir_final = rb.final_[rb.cur];
         b.restir_initial = (rb.temporal_reuse || rb.spatial_reuse) ? rb.initial : rb.final_[rb.cur];
     }

@@ -44,6 +44,11 @@ struct Lane {
     uint32_t* side_spill = nullptr;
     size_t path_capacity = 0;
     void* path_memory = nullptr;
+    // Serving batches with ReSTIR requests: per path slot the final and
+    // initial reservoirs and the (pixel, frame) key shade 0 reads (grown on
+    // demand).
+    size_t restir_slot_capacity = 0;
+    void* restir_slot_memory = nullptr;
 };
 
 // A timed launch of a render that collects statistics.

@@ -2,10 +2,13 @@
 // batching of render requests over one shared pool of path slots.
 //
 // A request of P passes over W x H pixels is a sequence of one-sample work
-// units (pass k, pixel range) in the order k = 0 .. P-1, pixels ascending.
-// Every step builds one batch:
+// units (pass k, pixel range) in the order k = 0 .. P-1, pixels ascending;
+// a ReSTIR DI request's units are whole passes, since its spatial reuse reads
+// neighbouring pixels of the same pass and its temporal reuse the previous
+// pass.  Every step builds one batch:
 //   1. admit queued requests in arrival order while their target
-//      (16 B/pixel) fits the HBM budget next to the path pool;
+//      (16 B/pixel) and ReSTIR state (224 B/pixel) fit the HBM budget next
+//      to the path pools;
 //   2. pick the integrator configuration (scene, bounce_count,
 //      light_sampling, generator) of the oldest unfinished request and gather
 //      the unfinished requests that share it;
@@ -13,15 +16,17 @@
 //      order, then hand what is left to the oldest first; each share is taken
 //      from the request's cursor as chunks (pass k, pixels [a, b));
 //   4. enqueue the wavefront kernels once over the pool (generate per chunk,
-//      extend / shade / shadow per bounce for all chunks together,
-//      accumulate per chunk in the order taken) on the next render lane;
+//      extend 0, the ReSTIR passes per chunk, then shade / shadow / extend per
+//      bounce for all chunks together, accumulate per chunk in the order
+//      taken) on the next render lane;
 //   5. once `depth` batches are in flight (or no batch could be formed),
-//      wait for the oldest, then download the requests it finished and
-//      release their targets.
+//      wait for the oldest, then download the requests it finished and keep
+//      their device memory for later requests.
 // Batches in flight overlap like renders on the context's lanes: a batch's
 // last, latency-bound bounces run beside the next batch's first; each
 // accumulates after the batch before it, so every pixel still takes its
-// passes in order.
+// passes in order, and a ReSTIR request's passes in one batch run after its
+// passes in the batch before (an event per request).
 // A path's radiance depends only on (scene, camera, pixel, pass, seed,
 // configuration) and each pixel accumulates its passes in order with
 // (previous * k + new) / (k + 1), so a request's image is the same bits
@@ -42,6 +47,19 @@
 
 namespace {
 
+// ReSTIR state per pixel: two G-buffer records of 5 float4 and four
+// reservoir arrays (initial, temporal, final of this and the last pass).
+constexpr uint64_t RESTIR_BYTES_PER_PIXEL = 10 * sizeof(float4) + 4 * sizeof(uint4);
+// Per path slot of a batch with ReSTIR requests: final and initial
+// reservoirs and the (pixel, pass) key.
+constexpr uint64_t RESTIR_SLOT_BYTES = 2 * sizeof(uint4) + sizeof(uint2);
+
+// A device allocation held by a request, or kept for the next one.
+struct Block {
+    void* ptr = nullptr;
+    uint64_t bytes = 0;
+};
+
 struct Request {
     uint64_t id = 0;
     tt_server_request desc{};
@@ -50,12 +68,19 @@ struct Request {
     // Scheduling cursor: next unit is (next_pass, next_pixel).
     uint32_t next_pass = 0, next_pixel = 0;
     uint32_t passes_done = 0;  // passes fully accumulated
-    float4* target = nullptr;
+    Block target;              // RGBA32F
+    Block restir;              // ReSTIR G-buffers and reservoirs
+    hipEvent_t restir_done = nullptr;  // after the last enqueued use of `restir`
+    bool restir_recorded = false;
+    bool cleared = false;      // the target has been zeroed by a batch
     std::vector<float> image;  // host copy once done
     uint64_t submitted_ns = 0, admitted_ns = 0, done_ns = 0;
 
     uint64_t pixels() const { return uint64_t(width) * height; }
+    bool is_restir() const { return desc.constants.light_sampling == TT_LIGHT_SAMPLING_RESTIR; }
     uint64_t target_bytes() const { return pixels() * sizeof(float4); }
+    uint64_t restir_bytes() const { return is_restir() ? pixels() * RESTIR_BYTES_PER_PIXEL : 0; }
+    uint64_t resident_bytes() const { return target_bytes() + restir_bytes(); }
     bool scheduled_all() const { return next_pass >= desc.passes; }
 };
 
@@ -70,6 +95,14 @@ struct Batch {
     std::vector<Chunk> chunks;
     uint32_t paths;
 };
+
+// The path buffers a lane's shadow pass reads: the batch's, with the side
+// stream's own traversal spill area.
+tt::PathBuffers side_buffers_of(const tt::PathBuffers& b, const tt::Lane& L) {
+    tt::PathBuffers s = b;
+    s.spill = L.side_spill;
+    return s;
+}
 
 bool same_config(const Request& a, const Request& b) {
     return a.desc.scene == b.desc.scene && a.desc.rng == b.desc.rng &&
@@ -88,7 +121,10 @@ struct tt_server {
     std::vector<Request*> active;  // arrival order
     std::map<uint64_t, std::unique_ptr<Request>> requests;
     uint64_t next_id = 1;
-    uint64_t used = 0;  // targets of admitted requests
+    uint64_t used = 0;          // blocks held by admitted requests
+    std::vector<Block> spare;   // blocks of finished requests
+    uint64_t spare_bytes = 0;
+    bool restir_slots = false;  // a ReSTIR request was submitted: pools carry slot reservoirs
     uint32_t depth = 1;      // batches in flight, each with its own path pool (lanes 0 .. depth - 1)
     uint32_t next_slot = 0;  // lane of the next batch
     std::deque<Batch> inflight;
@@ -100,16 +136,69 @@ struct tt_server {
         return uint64_t(std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - origin)
                             .count());
     }
-    uint64_t pool_bytes() const { return uint64_t(pool_paths) * tt::PATH_BYTES * depth; }
+    uint64_t pool_bytes(bool restir = false) const {
+        return uint64_t(pool_paths) * (tt::PATH_BYTES + (restir || restir_slots ? RESTIR_SLOT_BYTES : 0)) * depth;
+    }
 };
 
 namespace {
 
 using tt::fail;
 
-// Takes up to `want` path slots of work from r's cursor.
-void take(Request* r, uint32_t want, uint32_t& base, std::vector<Chunk>& out) {
+// A block of `bytes` for an admitted request: a kept block of the same size
+// class (up to 1.5x) when one fits the budget, else a new allocation, after
+// releasing kept blocks (hipFree waits for the device) only while the budget
+// needs their room.
+tt_status acquire(tt_server* sv, uint64_t bytes, Block& out) {
+    tt_context* c = sv->context;
+    int best = -1;
+    for (size_t i = 0; i < sv->spare.size(); i++) {
+        const Block& b = sv->spare[i];
+        if (b.bytes >= bytes && 2 * b.bytes <= 3 * bytes && (best < 0 || b.bytes < sv->spare[best].bytes))
+            best = int(i);
+    }
+    if (best >= 0 && sv->used + sv->spare[best].bytes + sv->pool_bytes() <= sv->budget) {
+        out = sv->spare[best];
+        sv->spare.erase(sv->spare.begin() + best);
+        sv->spare_bytes -= out.bytes;
+        sv->used += out.bytes;
+        return TT_OK;
+    }
+    while (!sv->spare.empty() && sv->used + sv->spare_bytes + bytes + sv->pool_bytes() > sv->budget) {
+        auto largest = std::max_element(sv->spare.begin(), sv->spare.end(),
+                                        [](const Block& a, const Block& b) { return a.bytes < b.bytes; });
+        TT_HIP(c, hipFree(largest->ptr));
+        sv->spare_bytes -= largest->bytes;
+        sv->spare.erase(largest);
+    }
+    out.bytes = bytes;
+    TT_HIP(c, hipMalloc(&out.ptr, bytes));
+    sv->used += bytes;
+    return TT_OK;
+}
+
+void release(tt_server* sv, Block& b) {
+    if (!b.ptr) return;
+    sv->used -= b.bytes;
+    sv->spare.push_back(b);
+    sv->spare_bytes += b.bytes;
+    b = Block{};
+}
+
+// Takes up to `want` path slots of work from r's cursor.  ReSTIR requests
+// take whole passes, at least one when it fits the rest of the pool.
+void take(tt_server* sv, Request* r, uint32_t want, uint32_t& base, std::vector<Chunk>& out) {
     const uint32_t px = uint32_t(r->pixels());
+    if (r->is_restir()) {
+        uint32_t taken = 0;
+        while (!r->scheduled_all() && base + px <= sv->pool_paths && (taken == 0 || taken + px <= want)) {
+            out.push_back(Chunk{r, r->next_pass, 0u, px, base});
+            base += px;
+            taken += px;
+            r->next_pass++;
+        }
+        return;
+    }
     while (want > 0 && !r->scheduled_all()) {
         uint32_t n = std::min<uint32_t>(want, px - r->next_pixel);
         out.push_back(Chunk{r, r->next_pass, r->next_pixel, n, base});
@@ -132,17 +221,58 @@ tt_status admit(tt_server* sv) {
     tt_context* c = sv->context;
     while (!sv->queued.empty()) {
         Request* r = sv->queued.front();
-        if (sv->used + r->target_bytes() + sv->pool_bytes() > sv->budget) break;
-        TT_HIP(c, hipMalloc(&r->target, r->target_bytes()));
-        sv->used += r->target_bytes();
-        // Pass 0 computes (previous * 0 + new) / 1: start from zeros.
-        TT_HIP(c, tt::launch_zero(tt::api_stream(c), reinterpret_cast<uint32_t*>(r->target), uint32_t(r->pixels() * 4)));
+        if (sv->used + r->resident_bytes() + sv->pool_bytes() > sv->budget) break;
+        // The target is zeroed by the first batch that uses it, on that
+        // batch's lane: nothing here waits for the batches in flight.
+        tt_status st = acquire(sv, r->target_bytes(), r->target);
+        if (st == TT_OK && r->is_restir()) {
+            st = acquire(sv, r->restir_bytes(), r->restir);
+            if (st == TT_OK) TT_HIP(c, hipEventCreateWithFlags(&r->restir_done, hipEventDisableTiming));
+        }
+        if (st != TT_OK) {
+            release(sv, r->target);
+            release(sv, r->restir);
+            return st;
+        }
         r->state = TT_REQUEST_ADMITTED;
         r->admitted_ns = sv->now_ns();
         sv->queued.pop_front();
         sv->active.push_back(r);
     }
     return TT_OK;
+}
+
+// ReSTIR buffers of request r for pass `pass`, its pixels at path slots
+// [base, base + pixels): the same state a render of the same passes on a
+// fresh target uses (current record and reservoirs alternate per pass,
+// history from the second pass on).
+tt::RestirBuffers restir_buffers(const Request& r, uint32_t pass, uint32_t base) {
+    const size_t px = size_t(r.pixels());
+    float4* g = static_cast<float4*>(r.restir.ptr);
+    uint4* res = reinterpret_cast<uint4*>(g + 10 * px);
+    tt::RestirBuffers rb{};
+    rb.gbuffer[0] = g;
+    rb.gbuffer[1] = g + 5 * px;
+    rb.initial = res;
+    rb.temporal = res + px;
+    rb.final_[0] = res + 2 * px;
+    rb.final_[1] = res + 3 * px;
+    rb.cur = pass & 1u;
+    rb.history_valid = pass > 0 ? 1u : 0u;
+    rb.candidates = r.desc.restir_candidates ? r.desc.restir_candidates : 32u;
+    rb.temporal_reuse = r.desc.restir_temporal ? 1u : 0u;
+    rb.spatial_reuse = r.desc.restir_spatial ? 1u : 0u;
+    rb.base = base;
+    return rb;
+}
+
+void request_free(Request& r) {
+    if (r.target.ptr) (void)hipFree(r.target.ptr);
+    if (r.restir.ptr) (void)hipFree(r.restir.ptr);
+    if (r.restir_done) (void)hipEventDestroy(r.restir_done);
+    r.target = Block{};
+    r.restir = Block{};
+    r.restir_done = nullptr;
 }
 
 }  // namespace
@@ -182,8 +312,8 @@ void tt_server_destroy(tt_server* sv) try {
     if (!sv) return;
     (void)hipSetDevice(sv->context->device);
     (void)hipStreamSynchronize(tt::api_stream(sv->context));  // also waits for batches in flight
-    for (auto& kv : sv->requests)
-        if (kv.second->target) (void)hipFree(kv.second->target);
+    for (auto& kv : sv->requests) request_free(*kv.second);
+    for (Block& b : sv->spare) (void)hipFree(b.ptr);
     tt_context* c = sv->context;
     delete sv;
     tt::context_release(c);
@@ -208,16 +338,23 @@ tt_status tt_server_submit(tt_server* sv, const tt_server_request* q, uint64_t* 
     if (q->passes == 0 || k.screen_size[0] == 0 || k.screen_size[1] == 0)
         return fail(c, TT_INVALID_ARGUMENT, "empty request");
     if (k.bounce_count == 0 || k.bounce_count > 31) return fail(c, TT_INVALID_ARGUMENT, "bounce_count in [1, 31]");
-    if (k.light_sampling != TT_LIGHT_SAMPLING_ON_HIT && k.light_sampling != TT_LIGHT_SAMPLING_NEXT_EVENT_ESTIMATION)
-        return fail(c, TT_UNSUPPORTED, "serving renders ON_HIT or NEXT_EVENT_ESTIMATION");
+    if (k.light_sampling != TT_LIGHT_SAMPLING_ON_HIT && k.light_sampling != TT_LIGHT_SAMPLING_NEXT_EVENT_ESTIMATION &&
+        k.light_sampling != TT_LIGHT_SAMPLING_RESTIR)
+        return fail(c, TT_UNSUPPORTED, "light sampling mode not supported");
     if (q->rng > 1) return fail(c, TT_INVALID_ARGUMENT, "unknown generator");
+    if (q->restir_candidates > 1024) return fail(c, TT_INVALID_ARGUMENT, "restir_candidates > 1024");
+    if (sv->scenes[q->scene]->view.texture_count > 0)
+        return fail(c, TT_UNSUPPORTED, "textured scenes are not served (their level of detail needs each path's camera)");
     auto r = std::make_unique<Request>();
     r->desc = *q;
     r->width = k.screen_size[0];
     r->height = k.screen_size[1];
     if (r->pixels() >= (1ull << 30)) return fail(c, TT_INVALID_ARGUMENT, "request too large");
-    if (r->target_bytes() + sv->pool_bytes() > sv->budget)
+    if (r->is_restir() && r->pixels() > sv->pool_paths)
+        return fail(c, TT_OVER_BUDGET, "a ReSTIR request renders whole passes, which must fit the path pool");
+    if (r->resident_bytes() + sv->pool_bytes(r->is_restir()) > sv->budget)
         return fail(c, TT_OVER_BUDGET, "the request's target does not fit the HBM budget next to the path pool");
+    if (r->is_restir()) sv->restir_slots = true;
     r->id = sv->next_id++;
     r->submitted_ns = sv->now_ns();
     sv->queued.push_back(r.get());
@@ -254,18 +391,29 @@ tt_status enqueue_batch(tt_server* sv, bool& formed) {
     tt::PathBuffers b;
     tt_status st = tt::ensure_paths(c, lane, sv->pool_paths, b);
     if (st != TT_OK) return st;
+    tt::Lane& L = c->lanes[lane];
+    const bool restir = lead->is_restir();
+    if (restir && L.restir_slot_capacity < sv->pool_paths) {
+        if (L.restir_slot_memory) {
+            TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
+            TT_HIP(c, hipFree(L.restir_slot_memory));
+            L.restir_slot_memory = nullptr;
+            L.restir_slot_capacity = 0;
+        }
+        TT_HIP(c, hipMalloc(&L.restir_slot_memory, size_t(sv->pool_paths) * RESTIR_SLOT_BYTES));
+        L.restir_slot_capacity = sv->pool_paths;
+    }
 
     std::vector<Chunk> chunks;
     uint32_t base = 0;
     uint32_t share = std::max<uint32_t>(sv->pool_paths / uint32_t(group.size()), 1u);
-    for (Request* r : group) take(r, uint32_t(std::min<uint64_t>(share, remaining_paths(r))), base, chunks);
-    for (Request* r : group) take(r, sv->pool_paths - base, base, chunks);
+    for (Request* r : group) take(sv, r, uint32_t(std::min<uint64_t>(share, remaining_paths(r))), base, chunks);
+    for (Request* r : group) take(sv, r, sv->pool_paths - base, base, chunks);
     const uint32_t paths = base;
 
     const tt_scene* scene = sv->scenes[lead->desc.scene];
     const tt_constants& lk = lead->desc.constants;
     b.max_bounces = lk.bounce_count;
-    b.restir_final = nullptr;
     auto chunk_args = [&](const Chunk& ch) {
         tt_constants k = ch.request->desc.constants;
         k.sample_count = 1;
@@ -274,24 +422,54 @@ tt_status enqueue_batch(tt_server* sv, bool& formed) {
         return tt::frame_args(k, ch.request->width, ch.request->height, ch.request->desc.rng, ch.request->desc.seed);
     };
 
-    // As a render: after the API stream's work so far (target clears), on
-    // the lane's stream, with shadow rays on its side stream when the
-    // context has side streams; accumulation after the previous batch's or
-    // render's.
-    tt::Lane& L = c->lanes[lane];
+    // As a render: after the API stream's work so far, on the lane's stream,
+    // with shadow rays on its side stream when the context has side streams;
+    // accumulation after the previous batch's or render's.
     const tt::Lane* other = c->prev_lane >= 0 && uint32_t(c->prev_lane) != lane ? &c->lanes[c->prev_lane] : nullptr;
     hipStream_t stream = L.stream;
     TT_HIP(c, hipEventRecord(c->api_mark, c->stream));
     TT_HIP(c, hipStreamWaitEvent(stream, c->api_mark, 0));
     hipStream_t side = c->side_streams ? L.side : stream;
-    tt::PathBuffers side_buffers = b;
-    side_buffers.spill = L.side_spill;
     const bool shadows = lk.light_sampling != TT_LIGHT_SAMPLING_ON_HIT && scene->view.emissive_count > 0;
+    // New targets start from zeros (pass 0 computes previous * 0 + new).
+    for (const Chunk& ch : chunks) {
+        Request* r = ch.request;
+        if (r->cleared) continue;
+        TT_HIP(c, tt::launch_zero(stream, static_cast<uint32_t*>(r->target.ptr), uint32_t(r->pixels() * 4)));
+        r->cleared = true;
+    }
     TT_HIP(c, tt::launch_zero(stream, L.counters, tt::C_WORDS));
     for (const Chunk& ch : chunks)
         TT_HIP(c, tt::launch_generate_chunk(stream, chunk_args(ch), b, ch.base, ch.pixel0, ch.count));
     tt::FrameArgs f = chunk_args(chunks.front());  // configuration fields are the group's
     TT_HIP(c, tt::launch_extend(stream, scene->view, b, 0, c->grid));
+    if (restir && scene->view.emissive_count > 0) {
+        // Each chunk is a whole pass: its ReSTIR passes over its request's
+        // state, then its reservoirs and slot keys at its slots for shade 0.
+        uint4* slot_final = static_cast<uint4*>(L.restir_slot_memory);
+        uint4* slot_initial = slot_final + L.restir_slot_capacity;
+        uint2* slot_key = reinterpret_cast<uint2*>(slot_initial + L.restir_slot_capacity);
+        for (const Chunk& ch : chunks) {
+            Request* r = ch.request;
+            // The request's passes in earlier batches (other lanes) are done
+            // with its state.
+            if (r->restir_recorded) TT_HIP(c, hipStreamWaitEvent(stream, r->restir_done, 0));
+            TT_HIP(c, tt::launch_zero(stream, L.counters + tt::C_RESTIR_VIS, 2));
+            tt::RestirBuffers rb = restir_buffers(*r, ch.pass, ch.base);
+            TT_HIP(c, tt::launch_restir(stream, scene->view, chunk_args(ch), b, rb, c->grid));
+            const uint4* initial = (rb.temporal_reuse || rb.spatial_reuse) ? rb.initial : rb.final_[rb.cur];
+            TT_HIP(c, hipMemcpyAsync(slot_final + ch.base, rb.final_[rb.cur], size_t(ch.count) * sizeof(uint4),
+                                     hipMemcpyDeviceToDevice, stream));
+            TT_HIP(c, hipMemcpyAsync(slot_initial + ch.base, initial, size_t(ch.count) * sizeof(uint4),
+                                     hipMemcpyDeviceToDevice, stream));
+            TT_HIP(c, tt::launch_slot_keys(stream, slot_key + ch.base, 0u, ch.pass, ch.count));
+            TT_HIP(c, hipEventRecord(r->restir_done, stream));
+            r->restir_recorded = true;
+        }
+        b.restir_final = slot_final;
+        b.restir_initial = slot_initial;
+        b.restir_key = slot_key;
+    }
     for (uint32_t bounce = 0; bounce < lk.bounce_count; bounce++) {
         if (shadows && bounce > 0 && side != stream) TT_HIP(c, hipStreamWaitEvent(stream, L.join, 0));
         TT_HIP(c, tt::launch_shade(stream, scene->view, f, b, bounce, paths));
@@ -300,7 +478,7 @@ tt_status enqueue_batch(tt_server* sv, bool& formed) {
                 TT_HIP(c, hipEventRecord(L.fork, stream));
                 TT_HIP(c, hipStreamWaitEvent(side, L.fork, 0));
             }
-            TT_HIP(c, tt::launch_shadow(side, scene->view, side_buffers, bounce, c->grid));
+            TT_HIP(c, tt::launch_shadow(side, scene->view, side_buffers_of(b, L), bounce, c->grid));
             if (side != stream) TT_HIP(c, hipEventRecord(L.join, side));
         }
         if (bounce + 1 < lk.bounce_count) TT_HIP(c, tt::launch_extend(stream, scene->view, b, bounce + 1, c->grid));
@@ -309,7 +487,7 @@ tt_status enqueue_batch(tt_server* sv, bool& formed) {
     if (other && other->used) TT_HIP(c, hipStreamWaitEvent(stream, other->accumulated, 0));
     for (const Chunk& ch : chunks)
         TT_HIP(c, tt::launch_accumulate_chunk(stream, chunk_args(ch), b, ch.base, ch.pixel0, ch.count,
-                                              ch.request->target));
+                                              static_cast<float4*>(ch.request->target.ptr)));
     TT_HIP(c, hipEventRecord(L.accumulated, stream));
     TT_HIP(c, hipEventRecord(L.done, stream));
     L.used = true;
@@ -324,7 +502,7 @@ tt_status enqueue_batch(tt_server* sv, bool& formed) {
 }
 
 // Waits for the oldest batch in flight, then downloads the requests it
-// finished and releases their targets.
+// finished and keeps their blocks for later requests.
 tt_status finish_batch(tt_server* sv, uint32_t& done) {
     tt_context* c = sv->context;
     Batch batch = std::move(sv->inflight.front());
@@ -347,10 +525,13 @@ tt_status finish_batch(tt_server* sv, uint32_t& done) {
         }
         // Its last chunk was in this batch: no batch in flight touches it.
         r->image.resize(size_t(r->pixels()) * 4);
-        TT_HIP(c, hipMemcpy(r->image.data(), r->target, r->target_bytes(), hipMemcpyDeviceToHost));
-        TT_HIP(c, hipFree(r->target));
-        r->target = nullptr;
-        sv->used -= r->target_bytes();
+        TT_HIP(c, hipMemcpy(r->image.data(), r->target.ptr, r->target_bytes(), hipMemcpyDeviceToHost));
+        release(sv, r->target);
+        release(sv, r->restir);
+        if (r->restir_done) {
+            (void)hipEventDestroy(r->restir_done);
+            r->restir_done = nullptr;
+        }
         r->state = TT_REQUEST_DONE;
         r->done_ns = sv->now_ns();
         done++;
@@ -425,7 +606,7 @@ tt_status tt_server_stats_get(tt_server* sv, tt_server_stats* out) try {
     for (auto& kv : sv->requests)
         if (kv.second->state == TT_REQUEST_DONE) done++;
     out->done = done;
-    out->budget_used_bytes = sv->used + sv->pool_bytes();
+    out->budget_used_bytes = sv->used + sv->spare_bytes + sv->pool_bytes();
     return TT_OK;
 } catch (...) {
     return tt::exception_status(sv && sv->context ? &sv->context->error : nullptr);

@@ -23,10 +23,10 @@ from .api import Camera, Config, Context, DeviceScene, build_constants
 class Server:
     """tt_server: resident scenes + request queue + batch scheduler."""
 
-    def __init__(self, context: Context, hbm_budget_bytes: int = 0, pool_paths: int = 0):
+    def __init__(self, context: Context, hbm_budget_bytes: int = 0, pool_paths: int = 0, batches_in_flight: int = 0):
         self.context = context
         h = C.c_void_p()
-        desc = _abi.ServerDesc(hbm_budget_bytes, pool_paths, 0)
+        desc = _abi.ServerDesc(hbm_budget_bytes, pool_paths, batches_in_flight)
         check(lib().tt_server_create(context.handle, C.byref(desc), C.byref(h)), context.handle, "tt_server_create")
         self.handle = h
         _abi.track(self, 0)
@@ -49,9 +49,14 @@ class Server:
         return slot.value
 
     def submit(self, scene: int, camera: Camera, config: Config, width: int, height: int, passes: int,
-               seed: int = 0, rng: int = _abi.TT_RNG_PCG) -> int:
+               seed: int = 0, rng: int = _abi.TT_RNG_PCG, restir_candidates: int = 32, restir_temporal: bool = True,
+               restir_spatial: bool = True) -> int:
+        """Queues a request; with config.light_sampling RESTIR the request
+        keeps its reservoirs resident across its passes (the reuse settings
+        as RenderSettings)."""
         k = build_constants(camera, dataclasses.replace(config, sample_count=1), width, height, 0, 0)
-        q = _abi.ServerRequest(scene, rng, seed, passes, k)
+        q = _abi.ServerRequest(scene, rng, seed, passes, k, restir_candidates, int(restir_temporal),
+                               int(restir_spatial), 0)
         rid = C.c_uint64()
         check(lib().tt_server_submit(self.handle, C.byref(q), C.byref(rid)), self.context.handle,
               "tt_server_submit")
