@@ -48,6 +48,12 @@ def main() -> int:
     p.add_argument("--serving-requests", type=int, default=40,
                    help="requests in the serving-latency leg (0 skips it)")
     p.add_argument("--serving-rate", type=float, default=5.0, help="Poisson arrival rate of that leg, requests/s")
+    p.add_argument("--serving-sweep", default="2,4,8,12,16,24,32",
+                   help="Poisson rates (requests/s) of the latency-vs-load sweep; empty skips it")
+    p.add_argument("--serving-sweep-requests", type=int, default=24, help="requests per rate of that sweep")
+    p.add_argument("--c5-serving-requests", type=int, default=12,
+                   help="requests of the served instanced-scene leg (ReSTIR, depth 12); 0 skips it")
+    p.add_argument("--c5-serving-rate", type=float, default=2.0, help="Poisson arrival rate of that leg")
     p.add_argument("--cache-frames", type=int, default=16,
                    help="frames of the radiance-cache leg on the Cornell box (0 skips it)")
     p.add_argument("--c5-frames", type=int, default=4,
@@ -179,12 +185,17 @@ def main() -> int:
         result["serving"] = serving(renderer.context, args.serving_requests, args.serving_rate)
         result["p50_ms"] = result["serving"]["p50_ms"]
         result["p99_ms"] = result["serving"]["p99_ms"]
+    if args.serving_sweep:
+        rates = [float(x) for x in args.serving_sweep.split(",") if x]
+        result["serving_sweep"] = serving_sweep(renderer.context, rates, args.serving_sweep_requests)
 
     if args.cache_frames > 0:
         result["radiance_cache"] = radiance_cache(args.cache_frames)
 
     if args.c5_frames > 0:
         result["instanced"] = instanced(args.c5_frames)
+    if args.c5_serving_requests > 0:
+        result["instanced_served"] = served_instanced(args.c5_serving_requests, args.c5_serving_rate)
 
     if args.copy_calibration_mib:
         # A device-to-device hipMemcpy of known size runs as the runtime's
@@ -231,6 +242,28 @@ def main() -> int:
                                   "sample": f"{frames} whole frames of the same workload (1920x1080, 1 spp, depth 8, "
                                             f"{args.light_sampling}, successive frame indices) on {threads} threads, {cpu_s:.1f} s",
                                   "mrays_per_s": round(cpu_rays / cpu_s / 1e6, 3)}
+        # One thread: the same scene and integrator at a sixteenth of the
+        # pixels (480x270), whole frames for at least 3 s.
+        from tinytrace_amd.api import build_constants
+
+        sw, sh = WIDTH // 4, HEIGHT // 4
+        state1 = oracle.restir_state(sw, sh) if restir else None
+        pixels1 = np.arange(sw * sh, dtype=np.uint32)
+        frames1 = 0
+        t0 = time.perf_counter()
+        while True:
+            k = build_constants(renderer.camera, renderer.config, sw, sh, 200000 + frames1, 0)
+            if restir:
+                oracle.render_restir(k, state1, rng=_abi.TT_RNG_PCG, seed=0, candidates=32, threads=1)
+            else:
+                oracle.render_pixels(k, pixels1, rng=_abi.TT_RNG_PCG, seed=0, threads=1)
+            frames1 += 1
+            s1 = time.perf_counter() - t0
+            if s1 >= 3.0:
+                break
+        result["cpu_baseline"]["single_thread"] = {
+            "value": round(frames1 * sw * sh / s1 / 1e6, 4), "unit": "Mpaths/s", "cores": 1,
+            "sample": f"{frames1} whole frames of the same scene and integrator at {sw}x{sh} on 1 thread, {s1:.1f} s"}
         result["gpu_over_cpu"] = round(result["value"] / result["cpu_baseline"]["value"], 1)
     print(json.dumps(result))
     return 0
@@ -400,18 +433,89 @@ def serving(context, requests, rate):
     arrivals, resolutions 256^2 / 512^2 / 1920x1080 / 3840x2160 with
     probabilities .4 / .3 / .2 / .1, 1 / 4 / 16 / 64 samples per pixel,
     cameras on a sphere of radius 3.5 around the box.  Latency runs from
-    arrival to the end of the image download."""
+    arrival to the end of the image download.  One more request, 512x512
+    with 64 passes from the Cornell fixture's camera, arrives a second in:
+    its image (served in the mix) against the FP64 fixture gives the served
+    requests' RMSE."""
+    import numpy as np
+
     from tinytrace_amd import Config, DeviceScene, LightSampling, scenes
-    from tinytrace_amd.serving import Server, request_mix, serve_poisson
+    from tinytrace_amd.serving import RequestSpec, Server, request_mix, serve_poisson
 
     server = Server(context, pool_paths=1 << 22)
     slots = [server.add_scene(DeviceScene(context, scenes.cornell_box(v)[0])) for v in range(4)]
     specs = request_mix(requests, rate_per_s=rate, scenes=4, seed=0)
+    _, c1_camera = scenes.cornell_box()
+    c1 = RequestSpec(1.0, 0, 512, 512, 64, c1_camera, seed=0)
+    specs = sorted(specs + [c1], key=lambda sp: sp.arrival_s)
+    c1_index = specs.index(c1)
     config = Config(bounce_count=4, sample_count=1, light_sampling=LightSampling.NEXT_EVENT_ESTIMATION)
-    out = serve_poisson(server, slots, specs, config)
+    out = serve_poisson(server, slots, specs, config, keep=[c1_index])
     server.close()
-    return {"workload": f"{requests} requests, Poisson {rate}/s, 4 resident Cornell scenes, depth 4, NEE, "
-                        "resolution mix 256^2/512^2/1080p/2160p (.4/.3/.2/.1), spp 1/4/16/64, pool 4 Mi paths",
+    with np.load(os.path.join(ROOT, "tests", "golden", "fp64_cornell_512.npz"), allow_pickle=False) as f:
+        pixels, reference = f["pixels"], f["radiance"]
+    img = out.pop("images")[c1_index].reshape(-1, 4)[pixels, :3].astype(np.float64)
+    latency = out.pop("latency_ms")
+    out["c1_request"] = {"workload": "512x512, 64 passes, Cornell fixture camera, served in the mix",
+                         "latency_ms": round(latency[c1_index], 3),
+                         "rmse_vs_fp64": round(float(np.sqrt(((img - reference) ** 2).mean())), 5)}
+    return {"workload": f"{requests} requests + the C1 request, Poisson {rate}/s, 4 resident Cornell scenes, depth 4, "
+                        "NEE, resolution mix 256^2/512^2/1080p/2160p (.4/.3/.2/.1), spp 1/4/16/64, pool 4 Mi paths",
+            **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in out.items()}}
+
+
+def serving_sweep(context, rates, requests):
+    """The serving mix at increasing Poisson rates (the same request mix,
+    re-drawn per rate): p50/p99 latency, throughput and the fraction of wall
+    time with a batch on the GPU, up to saturation."""
+    from tinytrace_amd import Config, DeviceScene, LightSampling, scenes
+    from tinytrace_amd.serving import Server, request_mix, serve_poisson
+
+    config = Config(bounce_count=4, sample_count=1, light_sampling=LightSampling.NEXT_EVENT_ESTIMATION)
+    resident = [DeviceScene(context, scenes.cornell_box(v)[0]) for v in range(4)]
+    points = []
+    for rate in rates:
+        server = Server(context, pool_paths=1 << 22)
+        slots = [server.add_scene(s) for s in resident]
+        specs = request_mix(requests, rate_per_s=rate, scenes=4, seed=1)
+        out = serve_poisson(server, slots, specs, config)
+        server.close()
+        points.append({"rate_per_s": rate, "p50_ms": round(out["p50_ms"], 3), "p99_ms": round(out["p99_ms"], 3),
+                       "mean_ms": round(out["mean_ms"], 3), "mpaths_per_s": round(out["mpaths_per_s"], 2),
+                       "busy_fraction": round(out["busy_fraction"], 3)})
+    return {"workload": f"{requests} requests per rate, the serving mix over 4 resident Cornell scenes, depth 4, NEE",
+            "points": points}
+
+
+def served_instanced(requests, rate):
+    """The instanced scene served: Poisson requests with the serving mix's
+    resolutions (256^2 .. 3840x2160) and passes (1 .. 64) over the resident
+    instanced scene, depth 12, ReSTIR DI (32 candidates, temporal + spatial
+    reuse) with each request's reservoirs resident across its passes,
+    cameras on a sphere of radius 180 around the scene's centre."""
+    import numpy as np
+
+    from tinytrace_amd import Config, Context, DeviceScene, LightSampling, scenes
+    from tinytrace_amd.serving import Server, orbit_camera, request_mix, serve_poisson
+
+    context = Context(0)
+    scene, _ = scenes.instanced_field(seed=0)
+    resident = DeviceScene(context, scene)
+    server = Server(context, pool_paths=3840 * 2160)
+    slot = server.add_scene(resident)
+    specs = request_mix(requests, rate_per_s=rate, scenes=1, seed=2)
+    rng = np.random.default_rng(3)
+    for sp in specs:
+        sp.camera = orbit_camera(rng, center=(100.0, 100.0, 100.0), radius=180.0, fov=1.0)
+    config = Config(bounce_count=12, sample_count=1, light_sampling=LightSampling.RESTIR)
+    out = serve_poisson(server, [slot], specs, config)
+    stats = server.stats()
+    server.close()
+    out.pop("latency_ms")
+    return {"workload": f"{requests} requests, Poisson {rate}/s, instanced field seed 0 resident (597 M instanced "
+                        "triangles), depth 12, ReSTIR DI with per-request reservoirs resident, resolution mix "
+                        "256^2/512^2/1080p/2160p (.4/.3/.2/.1), spp 1/4/16/64, pool 3840x2160 paths",
+            "budget_used_gib": round(stats["budget_used_bytes"] / 2 ** 30, 2),
             **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in out.items()}}
 
 

@@ -140,10 +140,12 @@ def request_mix(count: int, rate_per_s: float, scenes: int, seed: int = 0,
 
 
 def serve_poisson(server: Server, scene_slots: Sequence[int], specs: Sequence[RequestSpec], config: Config,
-                  keep_images: bool = False) -> dict:
+                  keep_images: bool = False, keep: Sequence[int] = ()) -> dict:
     """Replays `specs` against `server` in real time: a request is submitted
     once its arrival time has passed, the server steps while work is pending,
-    and latency runs from the arrival time to the end of the download."""
+    and latency runs from the arrival time to the end of the download.
+    Images are kept for every request (keep_images) or for the spec indices
+    in `keep`; per-request latencies come back in spec order."""
     t0 = time.perf_counter()
     ids: Dict[int, int] = {}
     arrival_ns: Dict[int, int] = {}
@@ -173,7 +175,7 @@ def serve_poisson(server: Server, scene_slots: Sequence[int], specs: Sequence[Re
                 if st["state"] == _abi.TT_REQUEST_DONE:
                     done[rid] = st["done_ns"]
                     image = server.take_result(rid)
-                    if keep_images:
+                    if keep_images or ids[rid] in keep:
                         images[ids[rid]] = image
         elif next_spec < len(specs) and not server.stats()["active"] and not server.stats()["queued"]:
             # Idle until the next arrival.
@@ -181,6 +183,9 @@ def serve_poisson(server: Server, scene_slots: Sequence[int], specs: Sequence[Re
             if wait > 0:
                 time.sleep(min(wait, 0.05))
     latency_ms = np.array([(done[r] - arrival_ns[r]) / 1e6 for r in ids])
+    by_spec = [0.0] * len(specs)
+    for r, i in ids.items():
+        by_spec[i] = (done[r] - arrival_ns[r]) / 1e6
     wall_s = time.perf_counter() - t0
     stats = server.stats()
     result = {
@@ -195,6 +200,7 @@ def serve_poisson(server: Server, scene_slots: Sequence[int], specs: Sequence[Re
         "busy_fraction": stats["busy_ms"] / 1e3 / wall_s,
         "batches": int(stats["batches"]),
     }
-    if keep_images:
+    result["latency_ms"] = by_spec
+    if keep_images or keep:
         result["images"] = images
     return result
