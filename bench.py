@@ -269,12 +269,43 @@ def main() -> int:
     return 0
 
 
+# Algorithmic MLP FLOPs (2 per multiply-add): inference of one query through
+# 96 -> 5 x 64 -> 3; a training sample is that forward pass, the backward
+# pass's input gradients of W_1 .. W_5, and the weight gradients of all six.
+MLP_PARAMS = 96 * 64 + 4 * 64 * 64 + 3 * 64
+MLP_INFER_FLOPS = 2 * MLP_PARAMS
+MLP_TRAIN_FLOPS = 2 * MLP_PARAMS + 2 * (MLP_PARAMS - 96 * 64) + 2 * MLP_PARAMS
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def mlp_roofline(stats):
+    """MFMA throughput of the cache's kernels from a frame's live HIP-event
+    timings (tt_cache_stats), against the dense bf16 peak."""
+    out = {"peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "infer_flops_per_query": MLP_INFER_FLOPS,
+           "train_flops_per_sample": MLP_TRAIN_FLOPS}
+    if stats["infer_ms"] > 0:
+        t = stats["queries"] * MLP_INFER_FLOPS / (stats["infer_ms"] * 1e-3) / 1e12
+        out["infer"] = {"ms": round(stats["infer_ms"], 4), "queries": stats["queries"], "tflops": round(t, 2),
+                        "frac": round(t / MFMA_BF16_PEAK_TFLOPS, 5)}
+    if stats["train_ms"] > 0:
+        t = stats["samples"] * MLP_TRAIN_FLOPS / (stats["train_ms"] * 1e-3) / 1e12
+        out["train"] = {"ms": round(stats["train_ms"], 4), "samples": stats["samples"], "tflops": round(t, 2),
+                        "frac": round(t / MFMA_BF16_PEAK_TFLOPS, 5)}
+    return out
+
+
+def judge(stats):
+    return {"samples": stats["judge_samples"], "traced": round(stats["judge_traced"], 5),
+            "cached": round(stats["judge_cached"], 5), "loss": round(stats["judge_loss"], 5)}
+
+
 def radiance_cache(frames):
     """Cornell box 512x512, depth 4, next-event estimation: the plain
-    integrator against the one with the MLP radiance cache (65536 training
-    paths and 4 Adam steps per frame, after 32 warm-up frames), each over
-    `frames` accumulated frames; RMSE against the FP64 Cornell fixture at
-    equal frames and at the plain renderer's wall time."""
+    integrator against the one with the MLP radiance cache (2^16 training
+    samples and 4 Adam steps per frame, 1% of the cached paths traced plainly
+    as the judge, after 32 warm-up frames), frames enqueued back to back in
+    both; RMSE against the FP64 Cornell fixture, and the plain frames (and
+    time) needed to reach the cached RMSE, from the measured plain curve."""
     import numpy as np
 
     from tinytrace_amd import Config, LightSampling, Renderer, RenderSettings, _abi, scenes
@@ -317,35 +348,54 @@ def radiance_cache(frames):
 
     plain_ms = ms_per_frame(renderer.render_to_target, 4 * frames)
     cache = RadianceCache(renderer.context, seed=0)
-    for _ in range(32):
-        render_cached(renderer, cache)
-    stats = {}
+    untrained = render_cached(renderer, cache)
+    for _ in range(31):
+        render_cached(renderer, cache, stats=False)
 
     def cached_step():
-        stats.update(render_cached(renderer, cache))
+        render_cached(renderer, cache, stats=False)
 
-    cached_ms = ms_per_frame(cached_step, frames)
+    cached_ms = ms_per_frame(cached_step, 4 * frames)
+    stats = render_cached(renderer, cache)  # one frame waited for: counts, judge, kernel times
     counts = [n for n in (1, 2, 4, 8, 16, 32, 64) if n <= frames]
     cached_rmse = curve(cached_step, counts)
-    equal_frames = max(1, int(frames * cached_ms / plain_ms))
-    plain_rmse = curve(renderer.render_to_target, counts + [equal_frames])
+    plain_counts = [n for n in (1, 2, 4, 8, 16, 32, 64, 128, 256) if n <= 16 * frames]
+    plain_rmse = curve(renderer.render_to_target, plain_counts)
     cache.close()
-    return {"workload": "Cornell box 512x512, depth 4, NEE, PCG seed 0; cache: 64-5x64-3 bf16 MLP, 65536 training "
-                        "paths + 4 Adam steps per frame, 32 warm-up frames",
+    # Plain frames to the cached RMSE after `frames` frames: log-log
+    # interpolation of the measured plain curve.
+    target = cached_rmse[str(frames)]
+    xs = np.log([float(n) for n in plain_counts])
+    ys = np.log([plain_rmse[str(n)] for n in plain_counts])
+    if target >= plain_rmse[str(plain_counts[0])]:
+        plain_needed = float(plain_counts[0])
+    elif target < plain_rmse[str(plain_counts[-1])]:
+        plain_needed = None
+    else:
+        plain_needed = float(np.exp(np.interp(-np.log(target), -ys, xs)))
+    equal_rmse = {"rmse": target, "cached_frames": frames, "cached_ms": round(frames * cached_ms, 3),
+                  "plain_frames": None if plain_needed is None else round(plain_needed, 2)}
+    if plain_needed is not None:
+        equal_rmse["plain_ms"] = round(plain_needed * plain_ms, 3)
+        equal_rmse["speedup"] = round(plain_needed * plain_ms / (frames * cached_ms), 3)
+    return {"workload": "Cornell box 512x512, depth 4, NEE, PCG seed 0; cache: 96-5x64-3 bf16 MLP (12-frequency "
+                        "position encoding), 2^16 training samples + 4 Adam steps per frame, 1% judge paths, "
+                        "32 warm-up frames",
             "frames": frames,
             "plain": {"ms_per_frame": round(plain_ms, 3), "rmse": plain_rmse},
             "cached": {"ms_per_frame": round(cached_ms, 3), "rmse": cached_rmse,
-                       "queries_per_frame": stats.get("queries"), "training_samples_per_frame": stats.get("samples"),
-                       "training_loss": round(stats.get("loss", 0.0), 5)},
-            "equal_time": {"ms": round(frames * cached_ms, 3), "rmse_cached": cached_rmse[str(frames)],
-                           "plain_frames": equal_frames, "rmse_plain": plain_rmse[str(equal_frames)]}}
+                       "queries_per_frame": stats["queries"], "training_samples_per_frame": stats["samples"],
+                       "training_loss": round(stats["loss"], 5)},
+            "judge": {"untrained": judge(untrained), "trained": judge(stats)},
+            "mlp": mlp_roofline(stats),
+            "equal_rmse": equal_rmse}
 
 
 def instanced(frames):
     """The instanced scene (64 displaced icosphere/torus meshes of 5k-200k
     triangles, 12 017 instances = 597 M instanced triangles, 4096 emissive)
     at 3840x2160, 1 spp, depth 12, ReSTIR DI: `frames` timed frames plain and
-    `frames` with the radiance cache (65536 training paths and 4 Adam steps
+    `frames` with the radiance cache (2^16 training samples and 4 Adam steps
     per frame, after 8 warm-up frames)."""
     from tinytrace_amd import Config, LightSampling, Renderer, RenderSettings, _abi, scenes
     from tinytrace_amd.cache import RadianceCache, render_cached
@@ -379,13 +429,13 @@ def instanced(frames):
     plain_ms, plain_rays = timed(renderer.render_to_target)
     cache = RadianceCache(renderer.context, seed=0)
     for _ in range(8):
-        render_cached(renderer, cache)
-    stats = {}
+        render_cached(renderer, cache, stats=False)
 
     def cached_step():
-        stats.update(render_cached(renderer, cache))
+        render_cached(renderer, cache, stats=False)
 
     cached_ms, cached_rays = timed(cached_step)
+    stats = render_cached(renderer, cache)
 
     # RMSE of 1 and 4 (...) independent frames, plain and cached, against the
     # FP64 fixture (4096-spp double-precision oracle on a lattice).
@@ -412,7 +462,7 @@ def instanced(frames):
 
     renderer.settings = dataclasses.replace(settings, collect_stats=False)
     plain_rmse = rmse_curve(renderer.render_to_target)
-    cached_rmse = rmse_curve(lambda: render_cached(renderer, cache))
+    cached_rmse = rmse_curve(cached_step)
     cache.close()
     paths = width * height
     return {"workload": f"instanced field seed 0 ({info['unique_triangle_count']} unique / "
@@ -424,7 +474,8 @@ def instanced(frames):
                       "mrays_per_s": round(plain_rays / plain_ms / 1e3, 2), "rmse": plain_rmse},
             "cached": {"ms_per_frame": round(cached_ms, 3), "mpaths_per_s": round(paths / cached_ms / 1e3, 2),
                        "mrays_per_s": round(cached_rays / cached_ms / 1e3, 2), "rmse": cached_rmse,
-                       "queries_per_frame": stats.get("queries"), "training_samples_per_frame": stats.get("samples")}}
+                       "queries_per_frame": stats["queries"], "training_samples_per_frame": stats["samples"],
+                       "judge": judge(stats), "mlp": mlp_roofline(stats)}}
 
 
 def serving(context, requests, rate):

@@ -369,8 +369,10 @@ pub struct tt_mlp_desc {
 #[repr(C)]
 #[derive(Clone, Copy, Debug, Default)]
 pub struct tt_cache_settings {
-    pub train_paths: u32,
+    pub train_samples: u32,
     pub train_steps: u32,
+    pub judge_fraction: f32,
+    pub records_per_path: u32,
 }
 
 #[repr(C)]
@@ -379,6 +381,13 @@ pub struct tt_cache_stats {
     pub queries: u32,
     pub samples: u32,
     pub loss: f64,
+    pub judge_samples: u32,
+    pub reserved: u32,
+    pub judge_traced: f64,
+    pub judge_cached: f64,
+    pub judge_loss: f64,
+    pub infer_ms: f64,
+    pub train_ms: f64,
 }
 
 #[repr(C)]

@@ -21,7 +21,8 @@
  *    destroyed in any order: a destroyed context stays alive until its last
  *    scene, target and server is destroyed.  One context per host thread.
  *  - Device work of a context runs on several HIP streams: an API stream
- *    (uploads, downloads, clears, display, serving, the radiance cache) and
+ *    (uploads, downloads, clears, display, serving, the radiance cache's
+ *    training and host operations) and
  *    two or three render lanes, each with an optional side stream for
  *    shadow rays, so that back-to-back renders overlap.  The library orders
  *    them with events so that results equal those of calls run one at a
@@ -427,16 +428,18 @@ tt_status tt_server_take_result(tt_server* server, uint64_t id, float* rgba, siz
 tt_status tt_server_stats_get(tt_server* server, tt_server_stats* out);
 
 /* ---------------------------------------------------------------------- */
-/* Radiance cache (new with this library): a 64-5x64-3 MLP trained online   */
+/* Radiance cache (new with this library): a 96-5x64-3 MLP trained online   */
 /* ---------------------------------------------------------------------- */
 
-/* Network: 64 encoded features (position frequencies, direction, normal,
- * base colour, roughness, metalness, bounce) -> 5 x (64, ReLU) -> 3
- * (softplus) = outgoing radiance.  20672 fp32 master parameters (no biases),
- * bf16 copies on the matrix cores, Adam.  A cached render ends each path at
- * its first vertex after the primary one whose specular roughness exceeds
- * 0.3 and adds throughput x cache; training paths run in full and teach the
- * cache the radiance their tails carry. */
+/* Network: 96 encoded features (12 sine/cosine frequencies of the position
+ * per axis, 2 of the direction, normal, base colour, roughness, metalness,
+ * bounce) -> 5 x (64, ReLU) -> 3 (softplus) = outgoing radiance.  22720 fp32
+ * master parameters (no biases), bf16 copies on the matrix cores, Adam.  A
+ * cached render ends each path at its cache vertex, its first vertex after
+ * the primary one whose specular roughness exceeds 0.3, and adds
+ * throughput x cache there.  Training paths run on plainly and teach the
+ * cache the radiance their tails carry; judge paths run on plainly and
+ * measure the cache's error. */
 typedef struct tt_mlp tt_mlp;
 
 typedef struct tt_mlp_desc {
@@ -449,31 +452,52 @@ tt_status tt_mlp_create(tt_context* context, const tt_mlp_desc* desc, tt_mlp** o
 void tt_mlp_destroy(tt_mlp* mlp);
 /* World box that positions are normalised to. */
 tt_status tt_mlp_set_bounds(tt_mlp* mlp, const float lo[3], const float hi[3]);
-/* Read (`get`) and/or replace (`set`) the 20672 fp32 parameters. */
+/* Read (`get`) and/or replace (`set`) the 22720 fp32 parameters. */
 tt_status tt_mlp_params(tt_mlp* mlp, float* get, const float* set, uint32_t count);
 /* Host-array operations on n raw records (16 floats each: position,
  * direction, normal, base colour, roughness, metalness, bounce, unused):
- *   op 0: encode -> out[n x 64];  op 1: infer -> out[n x 3];
- *   op 2: loss gradient for targets[n x 4] -> out[20672], *loss;
+ *   op 0: encode -> out[n x 96];  op 1: infer -> out[n x 3];
+ *   op 2: loss gradient for targets[n x 4] -> out[22720], *loss;
  *   op 3: one Adam step on targets -> *loss (before the step). */
 tt_status tt_mlp_run(tt_mlp* mlp, int op, const float* records, const float* targets, uint32_t n, float* out,
                      double* loss);
 
 typedef struct tt_cache_settings {
-    uint32_t train_paths;  /* training paths per frame (0: 65536) */
-    uint32_t train_steps;  /* Adam steps per frame over their samples (0: 4) */
+    /* Training samples per frame (0: 65536).  A training path records its
+     * cache vertex and up to records_per_path - 1 later rough vertices, so
+     * about train_samples / records_per_path paths train.  Samples from later
+     * vertices carry fewer remaining bounces than a query needs, and with
+     * more than one record per path the cache learns slower and settles
+     * lower (measured on the Cornell box: 2% low with one, 5% with two, 16%
+     * with three after 28 frames). */
+    uint32_t train_samples;
+    uint32_t train_steps;  /* Adam steps per frame over them (0: 4; at most 64) */
+    /* Share of the paths reaching a cache vertex that are traced plainly as
+     * the error judge (0: 0.01; negative: none). */
+    float judge_fraction;
+    uint32_t records_per_path;  /* training samples per training path (0: 1; 1 to 3) */
 } tt_cache_settings;
 
 typedef struct tt_cache_stats {
-    uint32_t queries;      /* paths ended at a cache query */
-    uint32_t samples;      /* training samples recorded */
-    double loss;           /* mean training loss of the frame's first step */
+    uint32_t queries;        /* paths ended at a cache query */
+    uint32_t samples;        /* training samples recorded and trained on */
+    double loss;             /* mean training loss of the frame's first Adam step */
+    uint32_t judge_samples;  /* judge paths, each traced plainly from its cache vertex */
+    uint32_t reserved;
+    double judge_traced;     /* their mean traced radiance from that vertex (per channel) */
+    double judge_cached;     /* the cache's mean prediction there, with the weights the frame's queries used */
+    double judge_loss;       /* mean (y - t)^2 / (y^2 + 0.01) of prediction y against traced t */
+    double infer_ms;         /* the frame's query inference (HIP events on its stream) */
+    double train_ms;         /* the frame's Adam steps (HIP events on the API stream) */
 } tt_cache_stats;
 
 /* tt_render_ex with the radiance cache: renders (any light sampling; with
  * ReSTIR the primary vertex uses the reservoirs and the cache ends paths
- * later), queries the cache for the paths it ends, then trains it on the
- * frame's training samples. */
+ * later), queries the cache for the paths it ends and for the judge
+ * records, then trains it on the frame's training samples.  Cached renders
+ * overlap on the render lanes like others; a frame's queries wait for the
+ * previous frame's training.  With `stats` the call waits for the frame and
+ * its training and fills them; without, it returns once enqueued. */
 tt_status tt_render_cached(tt_context* context, const tt_scene* scene, const tt_constants* constants,
                            const tt_render_settings* settings, tt_mlp* cache, const tt_cache_settings* cache_settings,
                            tt_target* target, tt_cache_stats* stats);

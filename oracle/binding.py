@@ -185,7 +185,8 @@ def pcg32_draws(initstate, initseq, n):
     return out
 
 
-MATH_OPS = {"sin": 0, "cos": 1, "exp": 2, "log": 3, "acos": 4, "asin": 5, "pow": 6, "sqrt": 7, "half": 8}
+MATH_OPS = {"sin": 0, "cos": 1, "exp": 2, "log": 3, "acos": 4, "asin": 5, "pow": 6, "sqrt": 7, "half": 8,
+            "sinpi": 9, "cospi": 10}
 
 
 def math_f32(op, x, y=None):

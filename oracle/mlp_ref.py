@@ -13,16 +13,19 @@ passes round to bfloat16 at exactly the points where the kernels do:
 
 Accumulation is exact (float64), where the kernels accumulate in fp32.
 
-Network: a_0 = encode(record) (64); a_{l+1} = relu(W_l a_l) for l = 0..4
-(W_l: 64 x 64); z = W_5 a_5 (3 x 64); y = softplus(z).  The loss is the mean
-over samples and channels of (y - t)^2 / (y^2 + 0.01), with the denominator
-held constant.
+Network: a_0 = encode(record) (96); a_1 = relu(W_0 a_0) (W_0: 64 x 96);
+a_{l+1} = relu(W_l a_l) for l = 1..4 (W_l: 64 x 64); z = W_5 a_5 (3 x 64);
+y = softplus(z).  The loss is the mean over samples and channels of
+(y - t)^2 / (y^2 + 0.01), with the denominator held constant.
 """
 from __future__ import annotations
 
 import numpy as np
 
-PARAMS = 5 * 64 * 64 + 3 * 64
+IN = 96
+W0 = 64 * IN
+W5 = W0 + 4 * 4096
+PARAMS = W5 + 3 * 64  # 22720
 LOSS_EPS = 0.01
 
 
@@ -36,11 +39,17 @@ def bf16(x):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+def layer_slices():
+    """(start, end) of W_0 .. W_5 in the parameter vector."""
+    return [(0, W0)] + [(W0 + 4096 * k, W0 + 4096 * (k + 1)) for k in range(4)] + [(W5, PARAMS)]
+
+
 def unpack(params):
-    """20672 parameters -> [W_0 .. W_4 (64 x 64), W_5 (3 x 64)] (float64)."""
+    """22720 parameters -> [W_0 (64 x 96), W_1 .. W_4 (64 x 64), W_5 (3 x 64)]."""
     p = np.asarray(params, dtype=np.float64)
-    ws = [p[k * 4096:(k + 1) * 4096].reshape(64, 64) for k in range(5)]
-    ws.append(p[20480:].reshape(3, 64))
+    ws = [p[0:W0].reshape(64, IN)]
+    ws += [p[W0 + 4096 * k: W0 + 4096 * (k + 1)].reshape(64, 64) for k in range(4)]
+    ws.append(p[W5:].reshape(3, 64))
     return ws
 
 
@@ -48,29 +57,45 @@ def pack(ws):
     return np.concatenate([w.reshape(-1) for w in ws])
 
 
+def unit_coordinates(records, lo, hi):
+    """u = (p - lo) * (1 / (hi - lo)) clamped to [0, 1], in float32 as the
+    kernels compute it (the high frequencies magnify any difference in u)."""
+    f = np.float32
+    p = np.asarray(records, dtype=np.float32)[:, :3]
+    lo32 = np.asarray(lo, dtype=np.float32)
+    ext = np.asarray(hi, dtype=np.float32) - lo32
+    with np.errstate(divide="ignore"):
+        inv = np.where(ext > 0, f(1.0) / np.where(ext > 0, ext, f(1.0)), f(0.0)).astype(np.float32)
+    u = (p - lo32) * inv
+    return np.clip(u, f(0.0), f(1.0)).astype(np.float64)
+
+
 def encode(records, lo, hi):
-    """Feature encoding of mlp.h (float64): records (n x 16) -> (n x 64)."""
-    r = np.asarray(records, dtype=np.float64)
+    """Feature encoding of mlp.h (sines and cosines in float64 of the same
+    float32 arguments): records (n x 16) -> (n x 96)."""
+    r = np.asarray(records, dtype=np.float32)
     n = r.shape[0]
-    f = np.zeros((n, 64))
-    ext = np.asarray(hi, dtype=np.float64) - np.asarray(lo, dtype=np.float64)
-    inv = np.where(ext > 0, 1.0 / np.where(ext > 0, ext, 1.0), 0.0)
+    f = np.zeros((n, IN))
+    u = unit_coordinates(r, lo, hi)
     for c in range(3):
-        u = np.clip((r[:, c] - lo[c]) * inv[c], 0.0, 1.0)
-        for i in range(6):
-            f[:, 12 * c + 2 * i] = np.sin(np.pi * 2 ** i * u)
-            f[:, 12 * c + 2 * i + 1] = np.cos(np.pi * 2 ** i * u)
-    f[:, 36:39] = r[:, 3:6]
-    f[:, 39:42] = r[:, 6:9]
-    f[:, 42:45] = r[:, 9:12]
-    f[:, 45] = r[:, 12]
-    f[:, 46] = r[:, 13]
-    f[:, 47] = r[:, 14] / 8.0
+        for i in range(12):
+            j = 12 * c + i
+            f[:, 2 * j] = np.sin(np.pi * 2.0 ** i * u[:, c])
+            f[:, 2 * j + 1] = np.cos(np.pi * 2.0 ** i * u[:, c])
+    r64 = r.astype(np.float64)
+    f[:, 72:75] = r64[:, 6:9]
+    f[:, 75:78] = r64[:, 9:12]
+    f[:, 78] = r64[:, 12]
+    f[:, 79] = r64[:, 13]
+    f[:, 80] = r64[:, 14] == 1.0
+    f[:, 81] = 1.0
+    f[:, 94] = r64[:, 14] == 2.0
+    f[:, 95] = r64[:, 14] >= 3.0
     for c in range(3):
-        d = r[:, 3 + c]
-        f[:, 48 + 4 * c: 52 + 4 * c] = np.stack([np.sin(np.pi * d), np.cos(np.pi * d), np.sin(2 * np.pi * d),
-                                                 np.cos(2 * np.pi * d)], axis=1)
-    f[:, 60] = 1.0
+        for o in range(2):
+            j = 41 + 2 * c + o
+            f[:, 2 * j] = np.sin(np.pi * 2.0 ** o * r64[:, 3 + c])
+            f[:, 2 * j + 1] = np.cos(np.pi * 2.0 ** o * r64[:, 3 + c])
     return f
 
 
@@ -109,7 +134,7 @@ def loss(params, features, targets, bf16_points=False):
 
 def gradient(params, features, targets, bf16_points=False):
     """Analytic gradient of `loss` (denominator held constant) w.r.t. the
-    20672 parameters, and the loss."""
+    22720 parameters, and the loss."""
     rnd = bf16 if bf16_points else (lambda v: np.asarray(v, dtype=np.float64))
     ws = unpack(params)
     if bf16_points:

@@ -461,7 +461,8 @@ void oracle_pcg32(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t* ou
 uint64_t oracle_splitmix64(uint64_t x) { return splitmix64(x); }
 float oracle_uint_to_unit_float(uint32_t v) { return uint_to_unit_float(v); }
 
-// op: 0 sin, 1 cos, 2 exp, 3 log, 4 acos, 5 asin, 6 pow(x, y), 7 sqrt
+// op: 0 sin, 1 cos, 2 exp, 3 log, 4 acos, 5 asin, 6 pow(x, y), 7 sqrt, 8 half round trip,
+// 9 sin(pi x), 10 cos(pi x)
 void oracle_math_f32(int op, const float* x, const float* y, float* out, uint32_t n) {
     for (uint32_t i = 0; i < n; i++) {
         float v = x[i];
@@ -474,6 +475,16 @@ void oracle_math_f32(int op, const float* x, const float* y, float* out, uint32_
             case 5: out[i] = ttm::asin(v); break;
             case 6: out[i] = ttm::pow(v, y[i]); break;
             case 8: out[i] = ttm::half_to_float(ttm::float_to_half(v)); break;
+            case 9: {
+                float c;
+                ttm::sincospi(v, out[i], c);
+                break;
+            }
+            case 10: {
+                float sn;
+                ttm::sincospi(v, sn, out[i]);
+                break;
+            }
             default: out[i] = ttm::sqrt(v); break;
         }
     }

@@ -100,8 +100,8 @@ def test_instanced_restir_with_cache_at_2160p():
 
     cache = RadianceCache(renderer.context, seed=0)
     for _ in range(8):
-        render_cached(renderer, cache)
-    cached, cached_s = frames(lambda: render_cached(renderer, cache), 4)
+        render_cached(renderer, cache, stats=False)
+    cached, cached_s = frames(lambda: render_cached(renderer, cache, stats=False), 4)
     plain, plain_s = frames(renderer.render_to_target, 2)
     cache.close()
     print(f"2160p: cached {cached_s * 1e3:.1f} ms/frame, plain {plain_s * 1e3:.1f} ms/frame", flush=True)

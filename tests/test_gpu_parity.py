@@ -59,6 +59,15 @@ def test_device_transcendentals_match_host(context, name, op, lo, hi):
     assert np.array_equal(host.view(np.uint32), dev.view(np.uint32))
 
 
+@pytest.mark.parametrize("name,op", [("sinpi", 11), ("cospi", 12)])
+def test_device_sincospi_matches_host(context, name, op):
+    rng = np.random.default_rng(op)
+    x = np.concatenate([rng.uniform(-2, 2, 50000), rng.uniform(0, 4096, 50000)]).astype(np.float32)
+    host = _oracle().math_f32(name, x)
+    dev = device_math(context, op, x)
+    assert np.array_equal(host.view(np.uint32), dev.view(np.uint32))
+
+
 def random_rays(rng, n, center, spread, tmin=1e-3, tmax=1e4):
     rays = np.zeros(n, dtype=_abi.RAY_DTYPE)
     rays["origin"] = np.asarray(center) + rng.uniform(-spread, spread, (n, 3))

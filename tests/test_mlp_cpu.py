@@ -20,8 +20,8 @@ def test_gradient_matches_finite_differences():
     params, feats, targets = _problem()
     grad, _ = mlp_ref.gradient(params, feats, targets)
     rng = np.random.default_rng(1)
-    # Parameters of every layer, including the output layer.
-    idx = np.concatenate([rng.integers(k * 4096, (k + 1) * 4096, 6) for k in range(5)] + [rng.integers(20480, 20672, 6)])
+    # Parameters of every layer, including the 96-input first and the output layer.
+    idx = np.concatenate([rng.integers(a, b, 6) for a, b in mlp_ref.layer_slices()])
     fd = mlp_ref.finite_difference(params, feats, targets, idx)
     np.testing.assert_allclose(grad[idx], fd, rtol=2e-5, atol=1e-9)
 
@@ -43,16 +43,51 @@ def test_bf16_round_to_nearest_even():
 
 def test_encoding_properties():
     rng = np.random.default_rng(2)
-    rec = rng.uniform(-3, 3, (100, 16))
+    rec = rng.uniform(-3, 3, (100, 16)).astype(np.float32)
     f = mlp_ref.encode(rec, (-2, -2, -2), (2, 2, 2))
+    assert f.shape == (100, 96)
+    # 12 frequencies of each position axis: unit (sin, cos) pairs, and each
+    # frequency the double angle of the one before it.
     for c in range(3):
-        for i in range(6):
-            s, co = f[:, 12 * c + 2 * i], f[:, 12 * c + 2 * i + 1]
+        for i in range(12):
+            s, co = f[:, 24 * c + 2 * i], f[:, 24 * c + 2 * i + 1]
             np.testing.assert_allclose(s * s + co * co, 1.0, atol=1e-12)
-    u = np.clip((rec[:, 0] + 2) / 4, 0, 1)
-    np.testing.assert_allclose(f[:, 0], np.sin(np.pi * u))
-    assert np.all(f[:, 60] == 1.0) and np.all(f[:, 61:] == 0.0)
-    np.testing.assert_array_equal(f[:, 36:39], rec[:, 3:6])
+            if i:
+                ps, pc = f[:, 24 * c + 2 * i - 2], f[:, 24 * c + 2 * i - 1]
+                np.testing.assert_allclose(s, 2 * ps * pc, atol=1e-9)
+                np.testing.assert_allclose(co, pc * pc - ps * ps, atol=1e-9)
+    u = np.clip((rec[:, 0].astype(np.float64) + 2) / 4, 0, 1)
+    np.testing.assert_allclose(f[:, 0], np.sin(np.pi * u), atol=1e-6)
+    np.testing.assert_allclose(f[:, 22], np.sin(np.pi * 2048 * u), atol=2e-3)
+    np.testing.assert_array_equal(f[:, 72:75], rec[:, 6:9])
+    np.testing.assert_array_equal(f[:, 75:78], rec[:, 9:12])
+    np.testing.assert_array_equal(f[:, 78:80], rec[:, 12:14])
+    assert np.all(f[:, 81] == 1.0)
+    # The bounce one-hot: 1, 2, >= 3.
+    b = np.array([0, 1, 2, 3, 7], dtype=np.float32)
+    rb = np.zeros((5, 16), dtype=np.float32)
+    rb[:, 14] = b
+    fb = mlp_ref.encode(rb, (-2, -2, -2), (2, 2, 2))
+    np.testing.assert_array_equal(fb[:, [80, 94, 95]], [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, 1]])
+    # Direction: octaves 0 and 1 of each axis.
+    for c in range(3):
+        d = rec[:, 3 + c].astype(np.float64)
+        np.testing.assert_allclose(f[:, 82 + 4 * c], np.sin(np.pi * d), atol=1e-12)
+        np.testing.assert_allclose(f[:, 85 + 4 * c], np.cos(2 * np.pi * d), atol=1e-12)
+
+
+def test_encoding_uses_the_kernels_float32_coordinate():
+    """u is (p - lo) * (1 / extent) in float32: at 2^11 pi u a float64 u
+    would differ by up to ~1e-3 from what the kernels encode."""
+    rng = np.random.default_rng(3)
+    rec = rng.uniform(-3, 3, (2000, 16)).astype(np.float32)
+    lo, hi = (-2.3, -2.3, -2.3), (1.7, 1.7, 1.7)
+    u32 = mlp_ref.unit_coordinates(rec, lo, hi)[:, 0]
+    f = mlp_ref.encode(rec, lo, hi)
+    np.testing.assert_array_equal(f[:, 22], np.sin(np.pi * 2048 * u32))
+    lo32, ext32 = np.float32(-2.3), np.float32(1.7) - np.float32(-2.3)
+    u_ref = np.clip((rec[:, 0] - lo32) * (np.float32(1) / ext32), 0, 1).astype(np.float64)
+    np.testing.assert_array_equal(u32, u_ref)
 
 
 def test_adam_first_step_is_signed_learning_rate():

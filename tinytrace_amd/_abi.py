@@ -167,14 +167,19 @@ class MlpDesc(C.Structure):
 
 
 class CacheSettings(C.Structure):
-    _fields_ = [("train_paths", C.c_uint32), ("train_steps", C.c_uint32)]
+    _fields_ = [("train_samples", C.c_uint32), ("train_steps", C.c_uint32), ("judge_fraction", C.c_float),
+                ("records_per_path", C.c_uint32)]
 
 
 class CacheStats(C.Structure):
-    _fields_ = [("queries", C.c_uint32), ("samples", C.c_uint32), ("loss", C.c_double)]
+    _fields_ = [("queries", C.c_uint32), ("samples", C.c_uint32), ("loss", C.c_double),
+                ("judge_samples", C.c_uint32), ("reserved", C.c_uint32), ("judge_traced", C.c_double),
+                ("judge_cached", C.c_double), ("judge_loss", C.c_double), ("infer_ms", C.c_double),
+                ("train_ms", C.c_double)]
 
 
-MLP_PARAMS = 20672
+MLP_PARAMS = 22720
+MLP_IN = 96
 
 
 # numpy dtypes of the reference byte layouts (little-endian, packed as C).

@@ -3,7 +3,8 @@
 `RadianceCache` wraps the network (parameters, encoding, inference,
 gradients, Adam steps on host arrays); `render_cached` renders one frame
 through a Renderer with the cache ending paths at their first rough vertex
-after the primary one and training on the frame's training paths.
+after the primary one, judging it against plainly traced paths and
+training it on the frame's training paths.
 """
 from __future__ import annotations
 
@@ -65,7 +66,7 @@ class RadianceCache:
 
     def encode(self, records) -> np.ndarray:
         n = np.asarray(records).reshape(-1, RECORD_FLOATS).shape[0]
-        return self._run(0, records, out_shape=(n, 64))[0]
+        return self._run(0, records, out_shape=(n, _abi.MLP_IN))[0]
 
     def infer(self, records) -> np.ndarray:
         n = np.asarray(records).reshape(-1, RECORD_FLOATS).shape[0]
@@ -80,17 +81,24 @@ class RadianceCache:
         return self._run(3, records, targets)[1]
 
 
-def render_cached(renderer: Renderer, cache: RadianceCache, train_paths: int = 65536, train_steps: int = 4,
-                  settings: Optional[RenderSettings] = None) -> dict:
+def render_cached(renderer: Renderer, cache: RadianceCache, train_samples: int = 65536, train_steps: int = 4,
+                  judge_fraction: float = 0.01, records_per_path: int = 1, settings: Optional[RenderSettings] = None,
+                  stats: bool = True) -> Optional[dict]:
     """One frame of `renderer` with the radiance cache (accumulates like
-    render_to_target); returns the frame's cache statistics."""
+    render_to_target).  With `stats` the call waits for the frame and its
+    training and returns the frame's cache statistics; without, it returns
+    None as soon as the frame is enqueued (frames then overlap like plain
+    ones).  judge_fraction < 0: no judge paths."""
     k = renderer.constants()
     st = settings or renderer.settings
-    cs = _abi.CacheSettings(train_paths, train_steps)
-    stats = _abi.CacheStats()
+    cs = _abi.CacheSettings(train_samples, train_steps, judge_fraction, records_per_path)
+    out = _abi.CacheStats() if stats else None
     check(lib().tt_render_cached(renderer.context.handle, renderer.scene.handle, C.byref(k), C.byref(st._desc()),
-                                 cache.handle, C.byref(cs), renderer.target.handle, C.byref(stats)),
+                                 cache.handle, C.byref(cs), renderer.target.handle,
+                                 C.byref(out) if stats else None),
           renderer.context.handle, "tt_render_cached")
     renderer.accumulated_frame_count += 1
     renderer.frame_index += 1
-    return {"queries": stats.queries, "samples": stats.samples, "loss": stats.loss}
+    if not stats:
+        return None
+    return {f: getattr(out, f) for f, _ in _abi.CacheStats._fields_ if f != "reserved"}

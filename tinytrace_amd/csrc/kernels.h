@@ -41,21 +41,38 @@ struct FrameArgs {
     uint32_t frame_index, accumulated_frame_count;
     uint32_t light_sampling;
     uint32_t rng, seed;
-    // Radiance cache: 0 off; else paths end at their first vertex after the
-    // primary one with specular roughness > 0.3 (a cache query), except
-    // training paths (hash of (path, frame) below train_threshold / 2^32).
+    // Radiance cache: 0 off; else a path's cache vertex is its first vertex
+    // after the primary one with specular roughness > 0.3.  A hash h of
+    // (path, frame) decides there: h < train_threshold makes a training path,
+    // h < judge_threshold (>= train_threshold) a judge path, and any other
+    // path ends at a cache query.  Training and judge paths run on plainly.
     uint32_t cache_mode;
     uint32_t train_threshold;
+    uint32_t judge_threshold;
+    uint32_t cache_records;  // training records a training path takes (1 .. CACHE_OPEN_RECORDS)
 };
 
-// Radiance-cache queues of one render (null when the cache is off).
+// Training records a training path can keep open: its cache vertex and the
+// next rough vertices, each the outgoing radiance at that vertex.
+constexpr uint32_t CACHE_OPEN_RECORDS = 3;
+constexpr uint32_t CACHE_NO_RECORD = 0xffffffffu;
+// open[p].w: number of open records (bits 0-1) and this flag for a path
+// past its cache vertex (a training or judge path).
+constexpr uint32_t CACHE_DECIDED = 4u;
+
+// Radiance-cache queues of one render (null when the cache is off).  Judge
+// records share the training arrays at slots [judge_base, judge_base +
+// judge_capacity); a record whose slot would pass its capacity is dropped.
 struct CacheBuffers {
-    uint32_t* train_slot;   // [paths] training record of the path or ~0
+    uint4* open;            // [paths] open record slots (x, y, z) and state (w)
     float4* query_rec;      // [paths x 4] raw query records (mlp.h)
     float4* query_scale;    // [paths] (throughput, path index)
-    float4* train_rec;      // [paths x 4] raw training records
-    float4* train_thr;      // [paths] (throughput at the record, path index)
-    float4* train_target;   // [paths] radiance the path carries from there on
+    float4* train_rec;      // [records x 4] raw training / judge records
+    float4* train_thr;      // [records] (throughput at the record, path index)
+    float4* train_target;   // [records] radiance the path carries from there on
+    uint32_t train_capacity;
+    uint32_t judge_base;
+    uint32_t judge_capacity;
 };
 
 // Wavefront buffers; `paths` = width * height * sample_count.
@@ -112,6 +129,7 @@ constexpr uint32_t C_RESTIR_VIS = 128;   // ReSTIR visibility queries
 constexpr uint32_t C_WORK_RESTIR = 129;  // their persistent cursor
 constexpr uint32_t C_CACHE_QUERY = 130;  // radiance-cache queries
 constexpr uint32_t C_CACHE_TRAIN = 131;  // radiance-cache training records
+constexpr uint32_t C_CACHE_JUDGE = 132;  // radiance-cache judge records
 constexpr uint32_t C_WORDS = 160;
 
 hipError_t launch_zero(hipStream_t s, uint32_t* p, uint32_t words);

@@ -85,7 +85,7 @@ TT_DEV void generate_path(const tt::FrameArgs& f, const tt::PathBuffers& b, uint
     b.throughput[p] = make_float4(1.0f, 1.0f, 1.0f, ndc_y);
     b.radiance[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     b.rng[p] = g.state;
-    if (b.cache.train_slot) b.cache.train_slot[p] = 0xffffffffu;
+    if (b.cache.open) b.cache.open[p] = make_uint4(tt::CACHE_NO_RECORD, tt::CACHE_NO_RECORD, tt::CACHE_NO_RECORD, 0u);
 }
 
 __global__ __launch_bounds__(256) void generate_kernel(tt::FrameArgs f, tt::PathBuffers b, uint32_t paths) {
@@ -134,6 +134,22 @@ TT_DEV f3 cache_ratio(f3 a, f3 t) {
     return f3(t.x > 0.0f ? a.x / t.x : 0.0f, t.y > 0.0f ? a.y / t.y : 0.0f, t.z > 0.0f ? a.z / t.z : 0.0f);
 }
 
+// open[p].w flag of a judge path (one record, at its cache vertex).
+constexpr uint32_t CACHE_JUDGE_PATH = 8u;
+
+// Adds radiance `a` (as the path's throughput scaled it) to the targets of
+// the path's open records, each divided by the throughput at its vertex.
+TT_DEV void add_to_records(const tt::CacheBuffers& cb, uint4 open, f3 a) {
+    const uint32_t n = open.w & 3u;
+    for (uint32_t j = 0; j < n; j++) {
+        uint32_t slot = j == 0 ? open.x : (j == 1 ? open.y : open.z);
+        float4 T = cb.train_thr[slot];
+        float4 t = cb.train_target[slot];
+        f3 d = cache_ratio(a, f3(T.x, T.y, T.z));
+        cb.train_target[slot] = make_float4(t.x + d.x, t.y + d.y, t.z + d.z, 0.0f);
+    }
+}
+
 // TEXTURED: the instantiation for scenes with textures (texture fetches at
 // hits and on textured light samples); untextured scenes run the other,
 // which carries no texture code.  The untextured one is held to four waves
@@ -166,12 +182,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 
     Generator g{b.rng[p], f.rng};
     uint32_t instance = __float_as_uint(hit.z);
 
-    // Radiance cache: the path's training record (if any), the radiance this
-    // vertex adds (for training targets) and a query / record at this vertex.
+    // Radiance cache: the path's open training / judge records, the radiance
+    // this vertex adds (for their targets) and a query or record here.
     const bool cache = f.cache_mode != 0u;
     const f3 thr_in = throughput;
-    const uint32_t tslot = cache ? b.cache.train_slot[p] : 0xffffffffu;
-    bool emit_query = false, emit_train = false;
+    uint4 open = cache ? b.cache.open[p] : make_uint4(0u, 0u, 0u, 0u);
+    bool emit_query = false, emit_train = false, emit_judge = false;
     f3 added;
     float rec[16];
 
@@ -198,22 +214,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 
         } else {
             sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, m);
         }
-        if (cache && bounce >= 1 && m.specular_roughness > 0.3f && tslot == 0xffffffffu) {
-            // The cache vertex: a query ends the path here; a training path
-            // records the vertex and runs on.
-            f3 v[5] = {sf.position, -ray_dir, sf.basis.normal, m.base_color, f3()};
-            for (int k = 0; k < 4; k++) {
-                rec[3 * k + 0] = v[k].x;
-                rec[3 * k + 1] = v[k].y;
-                rec[3 * k + 2] = v[k].z;
+        if (cache && bounce >= 1 && m.specular_roughness > 0.3f) {
+            if (!(open.w & tt::CACHE_DECIDED)) {
+                // The cache vertex: a query ends the path here; a training or
+                // judge path records the vertex and runs on.
+                uint32_t h = pcg32_output(splitmix64(pcg32_key(p, f.frame_index) ^ 0x6a09e667f3bcc909ull));
+                emit_train = h < f.train_threshold;
+                emit_judge = !emit_train && h < f.judge_threshold;
+                emit_query = !emit_train && !emit_judge;
+            } else if (!(open.w & CACHE_JUDGE_PATH) && (open.w & 3u) < f.cache_records) {
+                // A later rough vertex of a training path: one more sample.
+                emit_train = true;
             }
-            rec[12] = m.specular_roughness;
-            rec[13] = m.base_metalness;
-            rec[14] = float(bounce);
-            rec[15] = 0.0f;
-            uint32_t h = pcg32_output(splitmix64(pcg32_key(p, f.frame_index) ^ 0x6a09e667f3bcc909ull));
-            if (h < f.train_threshold) emit_train = true;
-            else emit_query = true;
+            if (emit_train || emit_judge || emit_query) {
+                f3 v[4] = {sf.position, -ray_dir, sf.basis.normal, m.base_color};
+                for (int k = 0; k < 4; k++) {
+                    rec[3 * k + 0] = v[k].x;
+                    rec[3 * k + 1] = v[k].y;
+                    rec[3 * k + 2] = v[k].z;
+                }
+                rec[12] = m.specular_roughness;
+                rec[13] = m.base_metalness;
+                rec[14] = float(bounce);
+                rec[15] = 0.0f;
+            }
         }
     }
     if (instance != 0xffffffffu && !emit_query) {
@@ -336,19 +360,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 
                 cb.query_rec[4 * size_t(qs) + k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
             cb.query_scale[qs] = make_float4(thr_in.x, thr_in.y, thr_in.z, __uint_as_float(p));
         }
+        // What this vertex adds reaches the records opened before it.
+        add_to_records(cb, open, added);
         uint32_t ts = wave_append(b.counters + tt::C_CACHE_TRAIN, emit_train);
-        if (emit_train) {
+        uint32_t js = wave_append(b.counters + tt::C_CACHE_JUDGE, emit_judge);
+        uint32_t slot = tt::CACHE_NO_RECORD;
+        if (emit_train && ts < cb.train_capacity) slot = ts;
+        if (emit_judge && js < cb.judge_capacity) slot = cb.judge_base + js;
+        if (slot != tt::CACHE_NO_RECORD) {
             for (int k = 0; k < 4; k++)
-                cb.train_rec[4 * size_t(ts) + k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
-            cb.train_thr[ts] = make_float4(thr_in.x, thr_in.y, thr_in.z, __uint_as_float(p));
+                cb.train_rec[4 * size_t(slot) + k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
+            cb.train_thr[slot] = make_float4(thr_in.x, thr_in.y, thr_in.z, __uint_as_float(p));
             f3 t = cache_ratio(added, thr_in);
-            cb.train_target[ts] = make_float4(t.x, t.y, t.z, 0.0f);
-            cb.train_slot[p] = ts;
-        } else if (tslot != 0xffffffffu) {
-            float4 T = cb.train_thr[tslot];
-            float4 t = cb.train_target[tslot];
-            f3 d = cache_ratio(added, f3(T.x, T.y, T.z));
-            cb.train_target[tslot] = make_float4(t.x + d.x, t.y + d.y, t.z + d.z, 0.0f);
+            cb.train_target[slot] = make_float4(t.x, t.y, t.z, 0.0f);
+            uint32_t n = open.w & 3u;
+            if (n == 0) open.x = slot;
+            else if (n == 1) open.y = slot;
+            else open.z = slot;
+            open.w += 1u;
+        }
+        if (emit_train || emit_judge) {
+            open.w |= tt::CACHE_DECIDED | (emit_judge ? CACHE_JUDGE_PATH : 0u);
+            cb.open[p] = open;
         }
     }
 }
@@ -375,16 +408,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void shadow_kernel(tt::DeviceScene d, 
             bool visible = light_visible(s, f3(o.x, o.y, o.z), f3(dd.x, dd.y, dd.z), __float_as_uint(dd.w),
                                          meta & 0xffffu, st);
             bool emission = (meta & 0x10000u) != 0;
-            if (visible && b.cache.train_slot) {
-                // Direct light reaching a training path after its record.
+            if (visible && b.cache.open) {
+                // Direct light reaching a training or judge path: it counts
+                // for the records at this vertex and before it.
                 uint32_t p = __float_as_uint(o.w);
-                uint32_t ts = b.cache.train_slot[p];
-                if (ts != 0xffffffffu) {
-                    float4 T = b.cache.train_thr[ts];
-                    float4 t = b.cache.train_target[ts];
-                    f3 dlt = cache_ratio(f3(direct.x, direct.y, direct.z), f3(T.x, T.y, T.z));
-                    b.cache.train_target[ts] = make_float4(t.x + dlt.x, t.y + dlt.y, t.z + dlt.z, 0.0f);
-                }
+                add_to_records(b.cache, b.cache.open[p], f3(direct.x, direct.y, direct.z));
             }
             if (visible || emission) {
                 uint32_t p = __float_as_uint(o.w);
@@ -494,6 +522,16 @@ __global__ void math_kernel(int op, const float* x, const float* y, float* out, 
         case 7: r = ttm::sqrt(v); break;
         case 8: r = v / y[i]; break;
         case 10: r = ttm::half_to_float(ttm::float_to_half(v)); break;
+        case 11: {
+            float c;
+            ttm::sincospi(v, r, c);
+            break;
+        }
+        case 12: {
+            float sn;
+            ttm::sincospi(v, sn, r);
+            break;
+        }
         default: r = 1.0f / v; break;
     }
     out[i] = r;

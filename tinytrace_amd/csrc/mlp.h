@@ -1,13 +1,14 @@
 // Radiance-cache MLP: layout, feature encoding and the device functions the
 // integrator and the MLP kernels share.
 //
-// Network (no biases; feature 60 is a constant 1):
-//   a0 = encode(record)              64 features
-//   a_l = relu(W_{l-1} a_{l-1})      l = 1..5, W_0..W_4: 64 x 64
+// Network (no biases; feature 81 is a constant 1):
+//   a0 = encode(record)              96 features
+//   a1 = relu(W_0 a0)                W_0: 64 x 96
+//   a_l = relu(W_{l-1} a_{l-1})      l = 2..5, W_1..W_4: 64 x 64
 //   L = softplus(W_5 a_5)            W_5: 3 x 64, L = radiance (RGB)
-// Parameters: 5 * 64 * 64 + 3 * 64 = 20672, stored fp32 (master, Adam state)
-// in layer order, each matrix row-major [out][in]; the kernels read bf16
-// copies and accumulate in fp32 with v_mfma_f32_32x32x8bf16_1k.
+// Parameters: 64 * 96 + 4 * 64 * 64 + 3 * 64 = 22720, stored fp32 (master,
+// Adam state) in layer order, each matrix row-major [out][in]; the kernels
+// read bf16 copies and accumulate in fp32 on the matrix cores.
 #pragma once
 
 #include <cstdint>
@@ -16,19 +17,29 @@
 
 namespace tt {
 
-constexpr int MLP_IN = 64;
+constexpr int MLP_IN = 96;
 constexpr int MLP_WIDTH = 64;
 constexpr int MLP_OUT = 3;
-constexpr int MLP_LAYERS = 6;                       // W_0 .. W_5
-constexpr int MLP_PARAMS = 5 * 64 * 64 + 3 * 64;     // 20672
-constexpr int MLP_RECORD_FLOATS = 16;                // raw features per query
-// bf16 forward weights in LDS/global: rows of 68 halves (64 + 4 pad, keeps
-// the 32 rows a wave reads on distinct LDS banks); W_5 padded to 32 rows.
+constexpr int MLP_LAYERS = 6;                           // W_0 .. W_5
+constexpr int MLP_W0 = 64 * MLP_IN;                     // 6144
+constexpr int MLP_W5_OFFSET = MLP_W0 + 4 * 64 * 64;     // 22528
+constexpr int MLP_PARAMS = MLP_W5_OFFSET + 3 * 64;      // 22720
+constexpr int MLP_RECORD_FLOATS = 16;                   // raw features per query
+// First parameter of W_k.
+constexpr int mlp_layer_offset(int k) { return k == 0 ? 0 : MLP_W0 + 4096 * (k - 1); }
+
+// bf16 forward weights in LDS/global: W_0 as 64 rows of MLP_ROW0 halves
+// (96 + 4 pad), then W_1..W_4 and W_5 (padded to 32 rows) as rows of MLP_ROW
+// halves (64 + 4 pad).  The pads keep the 32 rows a wave reads on distinct
+// LDS banks.
+constexpr int MLP_ROW0 = MLP_IN + 4;                     // 100
 constexpr int MLP_ROW = 68;
-constexpr int MLP_FWD_ROWS = 5 * 64 + 32;            // 352
+constexpr int MLP_FWD_W1 = 64 * MLP_ROW0;                // first half of W_1
+constexpr int MLP_FWD_W5 = MLP_FWD_W1 + 4 * 64 * MLP_ROW;  // first half of W_5
+constexpr int MLP_FWD_HALVES = MLP_FWD_W5 + 32 * MLP_ROW;  // 25984
 // bf16 transposed weights for the backward pass, W_l^T [in][out] for
-// l = 1..4 (64 rows of 68) and W_5^T [in][out padded to 8] (64 rows of 8).
-constexpr int MLP_BWD_ROWS = 4 * 64;                  // W_1^T .. W_4^T
+// l = 1..4 (64 rows of MLP_ROW) and W_5^T [in][out padded to 8] (64 rows of 8).
+constexpr int MLP_BWD_ROWS = 4 * 64;                     // W_1^T .. W_4^T
 constexpr int MLP_BWD5_ROW = 8;
 
 // Raw record (16 floats): position (3), direction toward the previous vertex
@@ -44,14 +55,17 @@ struct MlpBounds {
     float inv_extent[3];
 };
 
-// Encoded feature f of a record (the same arithmetic on host and device):
-//   0..35  position u = (p - lo) / extent clamped to [0, 1]: for axis c and
-//          octave i = 0..5: sin(2^i pi u_c), cos(2^i pi u_c) at 12c + 2i + {0,1}
-//   36..38 direction, 39..41 normal, 42..44 base colour
-//   45 roughness, 46 metalness, 47 bounce / 8
-//   48..59 direction: for axis c: sin(pi d), cos(pi d), sin(2 pi d),
-//          cos(2 pi d) at 48 + 4c + {0..3}
-//   60 constant 1, 61..63 zero
+// Encoded features, as 48 pairs (2j, 2j + 1):
+//   pairs 0..35 (features 0..71): position u = (p - lo) * inv_extent clamped
+//          to [0, 1]; pair 12c + i (axis c, frequency i = 0..11) is
+//          (sin(2^i pi u_c), cos(2^i pi u_c))
+//   72..74 shading normal, 75..77 base colour, 78 roughness, 79 metalness,
+//   80 [bounce == 1], 81 constant 1
+//   pairs 41..46 (features 82..93): direction d; pair 41 + 2c + o (axis c,
+//          octave o = 0, 1) is (sin(2^o pi d_c), cos(2^o pi d_c))
+//   94 [bounce == 2], 95 [bounce >= 3]
+// The bounce enters one-hot: a vertex's outgoing radiance depends on the
+// bounces left after it, and training samples come from several bounces.
 struct MlpFeatures {
     float f[MLP_IN];
 };
@@ -62,38 +76,45 @@ struct MlpFeatures {
 #define TT_MLP_HD inline
 #endif
 
+// Normalised coordinate of axis c (float arithmetic: subtract, multiply).
+TT_MLP_HD float mlp_unit(const MlpRecord& r, const MlpBounds& b, int c) {
+    float u = (r.v[c] - b.lo[c]) * b.inv_extent[c];
+    return u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
+}
+
+TT_MLP_HD float mlp_pick3(int c, float a, float b, float d) { return c == 0 ? a : (c == 1 ? b : d); }
+
+// 1 if the record's bounce index is b, else 0.
+TT_MLP_HD float mlp_bounce_is(const MlpRecord& r, int b) { return r.v[14] == float(b) ? 1.0f : 0.0f; }
+
+// Features 2j and 2j + 1.  The frequency 2^i multiplies exactly, and
+// sincospi reduces exactly, so the highest frequency carries no more
+// argument error than the lowest.  (Selects rather than indexing: on the
+// device j differs between the two halves of a wave.)
+TT_MLP_HD void mlp_encode_pair(const MlpRecord& r, const MlpBounds& b, int j, float& x0, float& x1) {
+    if (j < 36) {
+        int c = j / 12, i = j % 12;
+        float u = mlp_pick3(c, mlp_unit(r, b, 0), mlp_unit(r, b, 1), mlp_unit(r, b, 2));
+        ttm::sincospi(u * ttm::exp2i(i), x0, x1);
+    } else if (j < 41) {
+        // normal (x, y | z), base colour (r | g, b), roughness, metalness,
+        // [bounce == 1], constant 1
+        int k = j - 36;
+        x0 = k == 0 ? r.v[6] : (k == 1 ? r.v[8] : (k == 2 ? r.v[10] : (k == 3 ? r.v[12] : mlp_bounce_is(r, 1))));
+        x1 = k == 0 ? r.v[7] : (k == 1 ? r.v[9] : (k == 2 ? r.v[11] : (k == 3 ? r.v[13] : 1.0f)));
+    } else if (j < 47) {
+        int c = (j - 41) / 2, o = (j - 41) % 2;
+        float d = mlp_pick3(c, r.v[3], r.v[4], r.v[5]);
+        ttm::sincospi(o ? d * 2.0f : d, x0, x1);
+    } else {
+        x0 = mlp_bounce_is(r, 2);
+        x1 = r.v[14] >= 3.0f ? 1.0f : 0.0f;
+    }
+}
+
 TT_MLP_HD void mlp_encode(const MlpRecord& r, const MlpBounds& b, MlpFeatures& out) {
-    const float pi = 3.14159265358979323846f;
-    for (int c = 0; c < 3; c++) {
-        float u = (r.v[c] - b.lo[c]) * b.inv_extent[c];
-        u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
-        float scale = pi;
-        for (int i = 0; i < 6; i++) {
-            float x = scale * u;
-            out.f[12 * c + 2 * i] = ttm::sin(x);
-            out.f[12 * c + 2 * i + 1] = ttm::cos(x);
-            scale = scale * 2.0f;
-        }
-    }
-    for (int k = 0; k < 3; k++) {
-        out.f[36 + k] = r.v[3 + k];
-        out.f[39 + k] = r.v[6 + k];
-        out.f[42 + k] = r.v[9 + k];
-    }
-    out.f[45] = r.v[12];
-    out.f[46] = r.v[13];
-    out.f[47] = r.v[14] * 0.125f;
-    for (int c = 0; c < 3; c++) {
-        float d = r.v[3 + c];
-        out.f[48 + 4 * c + 0] = ttm::sin(pi * d);
-        out.f[48 + 4 * c + 1] = ttm::cos(pi * d);
-        out.f[48 + 4 * c + 2] = ttm::sin(2.0f * pi * d);
-        out.f[48 + 4 * c + 3] = ttm::cos(2.0f * pi * d);
-    }
-    out.f[60] = 1.0f;
-    out.f[61] = 0.0f;
-    out.f[62] = 0.0f;
-    out.f[63] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < MLP_IN / 2; j++) mlp_encode_pair(r, b, j, out.f[2 * j], out.f[2 * j + 1]);
 }
 
 // fp32 -> bf16, round to nearest even (NaN kept quiet).

@@ -165,29 +165,48 @@ hipStream_t api_stream(tt_context* c) {
             (void)hipStreamWaitEvent(c->stream, L.done, 0);
             L.pending = false;
         }
+    c->api_dirty = true;
     return c->stream;
 }
 
-// Radiance-cache queues for `paths` paths (grown on demand).
-tt_status ensure_cache(tt_context* c, size_t paths, CacheBuffers& cb) {
-    if (paths > c->cache_capacity) {
-        if (c->cache_memory) {
-            TT_HIP(c, hipStreamSynchronize(tt::api_stream(c)));
-            TT_HIP(c, hipFree(c->cache_memory));
-            c->cache_memory = nullptr;
-            c->cache_capacity = 0;
+// Radiance-cache queues of a lane (grown on demand; a lane's previous
+// render and training finish before its memory is replaced).
+tt_status ensure_cache(tt_context* c, uint32_t lane, size_t paths, uint32_t train, uint32_t judge, CacheBuffers& cb) {
+    Lane& L = c->lanes[lane];
+    if (paths > L.cache_paths || train > L.cache_train || judge > L.cache_judge) {
+        paths = std::max(paths, L.cache_paths);
+        train = std::max(train, L.cache_train);
+        judge = std::max(judge, L.cache_judge);
+        if (L.cache_memory) {
+            TT_HIP(c, hipStreamSynchronize(L.stream));
+            TT_HIP(c, hipStreamSynchronize(c->stream));
+            TT_HIP(c, hipFree(L.cache_memory));
+            L.cache_memory = nullptr;
+            L.cache_paths = 0;
+            L.cache_train = L.cache_judge = 0;
         }
-        TT_HIP(c, hipMalloc(&c->cache_memory, paths * (11 * sizeof(float4) + sizeof(uint32_t))));
-        c->cache_capacity = paths;
+        const size_t records = size_t(train) + judge;
+        const size_t bytes = paths * (sizeof(uint4) + 5 * sizeof(float4)) + records * 6 * sizeof(float4) +
+            size_t(judge) * 3 * sizeof(float) + 8 * sizeof(double);
+        TT_HIP(c, hipMalloc(&L.cache_memory, bytes));
+        L.cache_paths = paths;
+        L.cache_train = train;
+        L.cache_judge = judge;
     }
-    size_t n = c->cache_capacity;
-    float4* f = static_cast<float4*>(c->cache_memory);
+    const size_t n = L.cache_paths, records = size_t(L.cache_train) + L.cache_judge;
+    float4* f = static_cast<float4*>(L.cache_memory);
     cb.query_rec = f;
     cb.query_scale = f + 4 * n;
-    cb.train_rec = f + 5 * n;
-    cb.train_thr = f + 9 * n;
-    cb.train_target = f + 10 * n;
-    cb.train_slot = reinterpret_cast<uint32_t*>(f + 11 * n);
+    cb.open = reinterpret_cast<uint4*>(f + 5 * n);
+    float4* r = f + 6 * n;
+    cb.train_rec = r;
+    cb.train_thr = r + 4 * records;
+    cb.train_target = r + 5 * records;
+    L.judge_pred = reinterpret_cast<float*>(r + 6 * records);
+    L.judge_stats = reinterpret_cast<double*>(L.judge_pred + ((3 * size_t(L.cache_judge) + 1) & ~size_t(1)));
+    cb.train_capacity = train;
+    cb.judge_base = L.cache_train;
+    cb.judge_capacity = judge;
     return TT_OK;
 }
 
@@ -209,6 +228,8 @@ FrameArgs frame_args(const tt_constants& k, uint32_t width, uint32_t height, uin
     f.seed = seed;
     f.cache_mode = 0;
     f.train_threshold = 0;
+    f.judge_threshold = 0;
+    f.cache_records = 0;
     return f;
 }
 
@@ -288,16 +309,17 @@ void context_free(tt_context* c) {
     for (tt::Lane& L : c->lanes) {
         if (L.path_memory) (void)hipFree(L.path_memory);
         if (L.restir_slot_memory) (void)hipFree(L.restir_slot_memory);
+        if (L.cache_memory) (void)hipFree(L.cache_memory);
         if (L.counters) (void)hipFree(L.counters);
         if (L.spill) (void)hipFree(L.spill);
         if (L.side_spill) (void)hipFree(L.side_spill);
-        for (hipEvent_t e : {L.fork, L.join, L.accumulated, L.done, L.integrate.begin, L.integrate.end})
+        for (hipEvent_t e : {L.fork, L.join, L.accumulated, L.done, L.integrate.begin, L.integrate.end,
+                             L.cache_infer.begin, L.cache_infer.end})
             if (e) (void)hipEventDestroy(e);
         if (L.stream) (void)hipStreamDestroy(L.stream);
         if (L.side) (void)hipStreamDestroy(L.side);
     }
     if (c->api_mark) (void)hipEventDestroy(c->api_mark);
-    if (c->cache_memory) (void)hipFree(c->cache_memory);
     if (c->display_memory) (void)hipFree(c->display_memory);
     if (c->post_process.begin) (void)hipEventDestroy(c->post_process.begin);
     if (c->post_process.end) (void)hipEventDestroy(c->post_process.end);
@@ -619,19 +641,16 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     uint32_t paths = uint32_t(paths64);
     TT_HIP(c, hipSetDevice(c->device));
     bool stats = (st.flags & TT_RENDER_COLLECT_STATS) != 0;
-    // Renders with the radiance cache run on the API stream with lane 0's
-    // buffers (the cache's own passes run there and read back at once).  The
-    // others take the next lane, unless every lane's path buffers at this
+    // A render takes the next lane, unless every lane's path buffers at this
     // size would pass the budget: such renders are long enough that
     // overlapping their tails gains little, and they run on lane 0.
-    const bool on_api = cache != nullptr;
     const bool fits = uint64_t(paths) * tt::PATH_BYTES * c->lane_count <= tt::LANE_PATH_BUDGET;
-    const uint32_t lane = on_api || !fits ? 0u : c->next_lane;
+    const uint32_t lane = !fits ? 0u : c->next_lane;
     tt::Lane& L = c->lanes[lane];
     tt::PathBuffers b;
     tt_status es = tt::ensure_paths(c, lane, paths, b);
     if (es != TT_OK) return es;
-    if (!on_api && fits)
+    if (fits)
         for (uint32_t i = 0; i < c->lane_count; i++) {
             // Grow every lane together, so later renders allocate nothing.
             tt::PathBuffers unused;
@@ -674,30 +693,44 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     }
 
     tt::FrameArgs f = tt::frame_args(*k, t->width, t->height, st.rng, st.seed);
-    uint32_t train_steps = 0;
+    uint32_t train_steps = 0, train_capacity = 0, judge_capacity = 0;
     if (cache) {
         if (tt::mlp_context(cache) != c) return fail(c, TT_STALE_HANDLE, "cache from another context");
-        tt_status cs = tt::ensure_cache(c, paths, b.cache);
-        if (cs != TT_OK) return cs;
-        uint32_t train_paths = cache_settings && cache_settings->train_paths ? cache_settings->train_paths : 65536u;
-        train_steps = cache_settings && cache_settings->train_steps ? cache_settings->train_steps : 4u;
-        double frac = std::min(1.0, double(train_paths) / double(paths));
+        const tt_cache_settings* cs = cache_settings;
+        const uint32_t train_samples = cs && cs->train_samples ? cs->train_samples : 65536u;
+        train_steps = cs && cs->train_steps ? cs->train_steps : 4u;
+        if (train_steps > 64) return fail(c, TT_INVALID_ARGUMENT, "train_steps > 64");
+        const uint32_t records = cs && cs->records_per_path ? cs->records_per_path : 1u;
+        if (records > tt::CACHE_OPEN_RECORDS) return fail(c, TT_INVALID_ARGUMENT, "records_per_path > 3");
+        float judge = cs && cs->judge_fraction != 0.0f ? cs->judge_fraction : 0.01f;
+        if (!(judge > 0.0f)) judge = 0.0f;  // negative (or NaN): no judge paths
+        judge = std::min(judge, 1.0f);
+        // A training path records up to `records` samples.
+        const double train_frac = std::min(1.0, double(train_samples) / (double(records) * double(paths)));
+        // Record capacities: the expected counts with a margin (records past
+        // them are dropped).
+        train_capacity = uint32_t(std::min<uint64_t>(uint64_t(train_samples) + train_samples / 4 + 1024,
+                                                     uint64_t(records) * paths));
+        judge_capacity = judge > 0.0f ? uint32_t(std::min<double>(double(paths), 1.25 * judge * paths + 1024.0)) : 0u;
+        auto threshold = [](double x) { return x >= 1.0 ? 0xffffffffu : uint32_t(x * 4294967296.0); };
         f.cache_mode = 1;
-        f.train_threshold = frac >= 1.0 ? 0xffffffffu : uint32_t(frac * 4294967296.0);
-        cs = tt::mlp_set_scene_bounds(cache, s->bounds_lo, s->bounds_hi);
-        if (cs != TT_OK) return cs;
+        f.cache_records = records;
+        f.train_threshold = threshold(train_frac);
+        f.judge_threshold = threshold(train_frac + judge);
+        tt_status cst = tt::ensure_cache(c, lane, paths, train_capacity, judge_capacity, b.cache);
+        if (cst != TT_OK) return cst;
+        cst = tt::mlp_set_scene_bounds(cache, s->bounds_lo, s->bounds_hi);
+        if (cst != TT_OK) return cst;
     }
 
-    hipStream_t stream;
-    if (on_api) {
-        stream = tt::api_stream(c);
-    } else {
-        // After the API stream's work so far (uploads, target clears); the
-        // lane's previous render is ahead on its own stream.
+    // After the host calls' work on the API stream (uploads, target clears);
+    // the lane's previous render is ahead on its own stream.
+    if (c->api_dirty) {
         TT_HIP(c, hipEventRecord(c->api_mark, c->stream));
-        stream = L.stream;
-        TT_HIP(c, hipStreamWaitEvent(stream, c->api_mark, 0));
+        c->api_dirty = false;
     }
+    hipStream_t stream = L.stream;
+    TT_HIP(c, hipStreamWaitEvent(stream, c->api_mark, 0));
     // If an enqueue fails part way, still order what was enqueued before
     // what comes later: join the side stream back, mark the target's ReSTIR
     // buffers as read once this render's passes end, and mark the lane's end,
@@ -725,12 +758,12 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
                 lane->pending = true;
             }
         }
-    } partial{&L, stream, nullptr, t, !on_api, false, true};
+    } partial{&L, stream, nullptr, t, true, false, true};
     // The previous render, if on another lane, may still be running: this
     // one accumulates after it (the accumulations of consecutive renders form
     // a chain).  ReSTIR buffers are ordered by the target's own event.
     const tt::Lane* other = c->prev_lane >= 0 && uint32_t(c->prev_lane) != lane ? &c->lanes[c->prev_lane] : nullptr;
-    const bool after_other = !on_api && other && other->used;
+    const bool after_other = other && other->used;
     TT_HIP(c, hipEventRecord(L.integrate.begin, stream));
     size_t whole = 0;
     if (stats) {
@@ -807,10 +840,27 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     }
     if (shadows && side != stream) TT_HIP(c, hipStreamWaitEvent(stream, L.join, 0));
     if (cache) {
-        // Cached radiance for the paths that ended at a query.
-        tt_status cs = tt::mlp_infer_device(cache, b.cache.query_rec, b.cache.query_scale,
-                                            L.counters + tt::C_CACHE_QUERY, 0, nullptr, b.radiance);
+        // Cached radiance for the paths that ended at a query, and the
+        // cache's error at the judge records: both with the weights of the
+        // last training enqueued (the previous frame's).
+        if (cache_stats) {
+            if (!L.cache_infer.begin) TT_HIP(c, hipEventCreate(&L.cache_infer.begin));
+            if (!L.cache_infer.end) TT_HIP(c, hipEventCreate(&L.cache_infer.end));
+            // After the last training, so the pair times the inference alone.
+            tt_status ws = tt::mlp_wait_trained(cache, stream);
+            if (ws != TT_OK) return ws;
+            TT_HIP(c, hipEventRecord(L.cache_infer.begin, stream));
+        }
+        tt_status cs = tt::mlp_infer_device(cache, stream, b.cache.query_rec, b.cache.query_scale,
+                                            L.counters + tt::C_CACHE_QUERY, paths, nullptr, b.radiance);
         if (cs != TT_OK) return cs;
+        if (cache_stats) TT_HIP(c, hipEventRecord(L.cache_infer.end, stream));
+        if (judge_capacity) {
+            cs = tt::mlp_judge_device(cache, stream, b.cache.train_rec + 4 * size_t(b.cache.judge_base),
+                                      b.cache.train_target + b.cache.judge_base, L.counters + tt::C_CACHE_JUDGE,
+                                      judge_capacity, L.judge_pred, L.judge_stats);
+            if (cs != TT_OK) return cs;
+        }
     }
     if (after_other) TT_HIP(c, hipStreamWaitEvent(stream, other->accumulated, 0));
     TT_HIP(c, tt::launch_accumulate(stream, f, b, t->pixels));
@@ -822,41 +872,55 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
         c->stats_renders++;
         c->stats_paths += paths;
     }
+    if (cache) {
+        // Train on this frame's samples once its tracing is done: train_steps
+        // Adam steps over consecutive slices of the records, on the API
+        // stream.  The lane's queues are free again once training has read
+        // them, so the lane's end waits for it.
+        es = tt::mlp_train_device(cache, stream, b.cache.train_rec, b.cache.train_target,
+                                  L.counters + tt::C_CACHE_TRAIN, train_capacity, train_steps, true, nullptr,
+                                  cache_stats != nullptr);
+        if (es != TT_OK) return es;
+        es = tt::mlp_wait_trained(cache, stream);
+        if (es != TT_OK) return es;
+    }
     TT_HIP(c, hipEventRecord(L.integrate.end, stream));
     TT_HIP(c, hipEventRecord(L.done, stream));
     partial.armed = false;
     L.used = true;
     c->last_lane = lane;
-    if (!on_api) {
-        L.pending = true;
-        c->prev_lane = int(lane);
-        c->next_lane = (lane + 1u) % c->lane_count;
-    }
-    if (cache) {
-        // Train on this frame's samples: train_steps Adam steps over
-        // consecutive slices of the records.
-        uint32_t counts[2];
+    L.pending = true;
+    c->prev_lane = int(lane);
+    c->next_lane = (lane + 1u) % c->lane_count;
+    if (cache && cache_stats) {
+        // Waits for the frame and its training.
+        uint32_t counts[3] = {0u, 0u, 0u};
+        double js[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         TT_HIP(c, hipMemcpyAsync(counts, L.counters + tt::C_CACHE_QUERY, sizeof(counts), hipMemcpyDeviceToHost,
                                  stream));
+        if (judge_capacity)
+            TT_HIP(c, hipMemcpyAsync(js, L.judge_stats, sizeof(js), hipMemcpyDeviceToHost, stream));
         TT_HIP(c, hipStreamSynchronize(stream));
-        uint32_t n = counts[1];
+        const uint32_t samples = std::min(counts[1], train_capacity);
         double loss = 0.0;
-        uint32_t steps = std::min(train_steps, n);
-        for (uint32_t i = 0; i < steps; i++) {
-            uint32_t a = uint32_t(uint64_t(n) * i / steps), e = uint32_t(uint64_t(n) * (i + 1) / steps);
-            tt_status cs = tt::mlp_train_device(cache, b.cache.train_rec + 4 * size_t(a), b.cache.train_target + a,
-                                                e - a, true, nullptr);
-            if (cs != TT_OK) return cs;
-            if (i == 0 && cache_stats) {
-                cs = tt::mlp_last_loss(cache, e - a, &loss);
-                if (cs != TT_OK) return cs;
-            }
+        es = tt::mlp_first_step_loss(cache, samples, train_steps, &loss);
+        if (es != TT_OK) return es;
+        *cache_stats = tt_cache_stats{};
+        cache_stats->queries = counts[0];
+        cache_stats->samples = samples;
+        cache_stats->loss = loss;
+        const double n = js[0];
+        cache_stats->judge_samples = uint32_t(n);
+        if (n > 0.0) {
+            cache_stats->judge_cached = js[1] / (3.0 * n);
+            cache_stats->judge_traced = js[2] / (3.0 * n);
+            cache_stats->judge_loss = js[5] / (3.0 * n);
         }
-        if (cache_stats) {
-            cache_stats->queries = counts[0];
-            cache_stats->samples = n;
-            cache_stats->loss = loss;
-        }
+        float ms = 0.0f;
+        TT_HIP(c, hipEventElapsedTime(&ms, L.cache_infer.begin, L.cache_infer.end));
+        cache_stats->infer_ms = ms;
+        es = tt::mlp_train_ms(cache, &cache_stats->train_ms);
+        if (es != TT_OK) return es;
     }
     c->last_integrate_ms = -1.0;
     if (restir && s->view.emissive_count > 0) {

@@ -49,6 +49,16 @@ struct Lane {
     // demand).
     size_t restir_slot_capacity = 0;
     void* restir_slot_memory = nullptr;
+    // Radiance-cache queues (tt_render_cached), grown on demand: sizes the
+    // memory holds, and after the queues the judge predictions and the judge
+    // statistics (6 doubles).
+    size_t cache_paths = 0;
+    uint32_t cache_train = 0, cache_judge = 0;
+    void* cache_memory = nullptr;
+    float* judge_pred = nullptr;
+    double* judge_stats = nullptr;
+    // Timing of a cached render's query inference (created on first use).
+    EventPair cache_infer;
 };
 
 // A timed launch of a render that collects statistics.
@@ -62,9 +72,9 @@ struct StatsLaunch {
 
 struct tt_context {
     int device = 0;
-    // The API stream: uploads, downloads, target clears, display, serving,
-    // the radiance cache and renders that collect statistics or use the
-    // cache.  Other renders run on the lanes; code outside them takes the
+    // The API stream: uploads, downloads, target clears, display, serving
+    // and the cache's host operations.  Renders run on the lanes (cached
+    // ones train on the cache's own stream); code outside them takes the
     // stream through tt::api_stream, which first orders it after every
     // render enqueued so far.
     hipStream_t stream = nullptr;
@@ -82,7 +92,12 @@ struct tt_context {
     uint32_t next_lane = 0;
     int prev_lane = -1;      // lane of the last render that ran on a lane
     uint32_t last_lane = 0;  // lane of the last render (its counters and timing)
+    // Recorded on the API stream when a render starts after host calls put
+    // work there (api_dirty, set by tt::api_stream); renders wait for it.
+    // The cache's training, which renders enqueue on the API stream, does not
+    // set api_dirty: the next render's tracing need not wait for it.
     hipEvent_t api_mark = nullptr;
+    bool api_dirty = true;
     uint32_t* work = nullptr;
     // Timing.
     // Statistics (TT_RENDER_COLLECT_STATS), summed over the renders since the
@@ -100,9 +115,6 @@ struct tt_context {
     // handles may be destroyed in any order.
     uint32_t children = 0;
     bool closing = false;
-    // Radiance-cache queues, grown on demand.
-    size_t cache_capacity = 0;
-    void* cache_memory = nullptr;
     // Display pass (tt_target_display): RGBA8 staging, grown on demand, and
     // its timing events (created on first use).
     size_t display_capacity = 0;
@@ -157,7 +169,9 @@ inline tt_status hip_fail(tt_context* c, hipError_t e, const char* what) {
 void context_retain(tt_context* c);
 void context_release(tt_context* c);
 
-tt_status ensure_cache(tt_context* c, size_t paths, CacheBuffers& cb);
+// Radiance-cache queues of a lane for `paths` paths, `train` training and
+// `judge` judge records (grown on demand, kept in the lane).
+tt_status ensure_cache(tt_context* c, uint32_t lane, size_t paths, uint32_t train, uint32_t judge, CacheBuffers& cb);
 
 // Device LBVH build of every BLAS (bvh_gpu.hip).
 struct BvhNode;
@@ -173,14 +187,36 @@ struct BvhWide;
 tt_status collapse_wide(tt_context* c, const BvhNode* nodes, uint32_t count, void* instances,
                         uint32_t instance_count, uint32_t& tlas_root, BvhWide** out, uint32_t& wide_count);
 
-// Radiance-cache MLP (mlp.hip).
+// Radiance-cache MLP (mlp.hip).  Training runs on the cache's own stream;
+// its `trained` event orders later inference (and host access) after it.
 tt_context* mlp_context(const tt_mlp* m);
 tt_status mlp_set_scene_bounds(tt_mlp* m, const float lo[3], const float hi[3]);
-tt_status mlp_infer_device(tt_mlp* m, const float4* records, const float4* scale_path, const uint32_t* count_dev,
-                           uint32_t count_host, float* out_rgb, float4* radiance);
-tt_status mlp_train_device(tt_mlp* m, const float4* records, const float4* targets, uint32_t n, bool apply,
-                           float* grad_out);
-tt_status mlp_last_loss(tt_mlp* m, uint32_t n, double* out);
+// Makes `stream` wait for the last training enqueued.
+tt_status mlp_wait_trained(tt_mlp* m, hipStream_t stream);
+// Inference on `stream` of min(*count_dev, count) records (count records
+// when count_dev is null): out_rgb[3 q ..] = L and/or radiance[path] +=
+// throughput * L with (throughput, path) from scale_path.
+tt_status mlp_infer_device(tt_mlp* m, hipStream_t stream, const float4* records, const float4* scale_path,
+                           const uint32_t* count_dev, uint32_t count, float* out_rgb, float4* radiance);
+// The cache against the plainly traced judge records on `stream`:
+// predictions into pred (3 floats a record), then stats[0..5] = record count,
+// sum of predictions, sum of targets, sum of squared differences, sum of
+// squared targets, sum of the relative L2 loss terms (doubles).
+tt_status mlp_judge_device(tt_mlp* m, hipStream_t stream, const float4* records, const float4* targets,
+                           const uint32_t* count_dev, uint32_t capacity, float* pred, double* stats);
+// `steps` Adam steps over consecutive slices of min(*count_dev, capacity)
+// records (count_dev null: capacity records), on the cache's stream after the
+// work enqueued on `after` so far.  apply false: one gradient, no update,
+// into grad_out (device, MLP_PARAMS floats).  timed: HIP events around the
+// steps (mlp_train_ms).
+tt_status mlp_train_device(tt_mlp* m, hipStream_t after, const float4* records, const float4* targets,
+                           const uint32_t* count_dev, uint32_t capacity, uint32_t steps, bool apply, float* grad_out,
+                           bool timed = false);
+// Milliseconds of the last timed training (synchronises).
+tt_status mlp_train_ms(tt_mlp* m, double* out);
+// Mean loss of the first step of the last training, whose record count was
+// `count` over `steps` steps (synchronises).
+tt_status mlp_first_step_loss(tt_mlp* m, uint32_t count, uint32_t steps, double* out);
 
 // Path buffers of at least `paths` paths (grown on demand, kept in the
 // context); fills the pointers of `b`.

@@ -428,6 +428,7 @@ tt_status enqueue_batch(tt_server* sv, bool& formed) {
     const tt::Lane* other = c->prev_lane >= 0 && uint32_t(c->prev_lane) != lane ? &c->lanes[c->prev_lane] : nullptr;
     hipStream_t stream = L.stream;
     TT_HIP(c, hipEventRecord(c->api_mark, c->stream));
+    c->api_dirty = false;
     TT_HIP(c, hipStreamWaitEvent(stream, c->api_mark, 0));
     hipStream_t side = c->side_streams ? L.side : stream;
     const bool shadows = lk.light_sampling != TT_LIGHT_SAMPLING_ON_HIT && scene->view.emissive_count > 0;

@@ -96,6 +96,25 @@ TT_HD float cos_poly(float y) {
         0.5f * z + 1.0f;
 }
 
+// floor(x) for |x| < 2^31.
+TT_HD float floor_small(float x) {
+    float t = float(int(x));
+    return t > x ? t - 1.0f : t;
+}
+
+// sin(pi v) and cos(pi v) for |v| < 2^22.  v = h / 2 + r with h the nearest
+// integer to 2v and |r| <= 1/4; v - h / 2 is exact (Sterbenz), so only the
+// product pi r rounds, whatever the size of v.  Same polynomials as sin/cos.
+TT_HD void sincospi(float v, float& s, float& c) {
+    float h = floor_small(2.0f * v + 0.5f);
+    float r = v - 0.5f * h;
+    int q = int(h) & 3;
+    float y = r * 3.14159265358979323846f;
+    float sp = sin_poly(y), cp = cos_poly(y);
+    s = q == 0 ? sp : (q == 1 ? cp : (q == 2 ? -sp : -cp));
+    c = q == 0 ? cp : (q == 1 ? -sp : (q == 2 ? -cp : sp));
+}
+
 TT_HD float sin(float x) {
     if (isnan(x) || isinf(x)) return bits_float(0x7fc00000u);
     bool negative = x < 0.0f;
