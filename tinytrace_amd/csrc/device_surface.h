@@ -29,11 +29,48 @@ TT_DEV uint32_t wave_append(uint32_t* counter, bool pred) {
     return base + lanes_below(mask);
 }
 
-// Persistent-wave cursor: lane 0 takes the next 64 items.
-TT_DEV uint32_t next_batch(uint32_t* cursor) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(cursor, 64u);
-    return __shfl(base, 0);
+// Persistent-wave work fetch over n items in tt::WORK_PARTS contiguous
+// partitions, each with its own cursor on a line of its own.  A wave starts
+// on its workgroup's partition (blockIdx % WORK_PARTS: with workgroups
+// dispatched round-robin over the eight XCDs, a partition's waves share an
+// XCD and its L2), takes 64 items per atomic and, once that partition is
+// drained, moves through the others, skipping drained ones after a plain
+// load.  Every value here is wave-uniform.
+struct WorkFetch {
+    uint32_t* set;
+    uint32_t n;
+    uint32_t part;
+    uint32_t visited;
+};
+
+TT_DEV WorkFetch work_fetch(uint32_t* set, uint32_t n) {
+    return WorkFetch{set, n, blockIdx.x % tt::WORK_PARTS, 0u};
+}
+
+// The next batch: items [first, first + count), 1 <= count <= 64; false
+// once every partition is drained.
+TT_DEV bool next_batch(WorkFetch& w, uint32_t& first, uint32_t& count) {
+    while (w.visited < tt::WORK_PARTS) {
+        const uint32_t lo = uint32_t(uint64_t(w.n) * w.part / tt::WORK_PARTS);
+        const uint32_t len = uint32_t(uint64_t(w.n) * (w.part + 1u) / tt::WORK_PARTS) - lo;
+        uint32_t* cursor = w.set + w.part * tt::WORK_STRIDE;
+        bool drained = len == 0u;
+        if (!drained && w.visited > 0u)
+            drained = __hip_atomic_load(cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len;
+        if (!drained) {
+            uint32_t got = 0;
+            if (lane_id() == 0) got = atomicAdd(cursor, 64u);
+            got = __shfl(got, 0);
+            if (got < len) {
+                first = lo + got;
+                count = len - got < 64u ? len - got : 64u;
+                return true;
+            }
+        }
+        w.visited++;
+        w.part = w.part + 1u == tt::WORK_PARTS ? 0u : w.part + 1u;
+    }
+    return false;
 }
 
 __device__ __host__ inline SceneView view(const tt::DeviceScene& d) {

@@ -84,6 +84,7 @@ struct PathBuffers {
     float4* hits;         // [paths]
     float4* shadow[4];    // origin, direction, direct, emission: [paths] each
     uint32_t* counters;   // see kernels.hip
+    uint32_t* work;       // W_SETS cursor sets of the persistent traversal launches
     uint32_t* spill;      // traversal stack spill
     const uint4* restir_final;  // per-path reservoirs used at the primary vertex
     const uint4* restir_initial;  // the path's own candidate after the visibility pass
@@ -123,16 +124,28 @@ constexpr size_t TRACE_SPILL_ENTRIES = 116;
 // Counter words (u32, zeroed at the start of every render).
 constexpr uint32_t C_EXT = 0;            // [b]: rays in the queue consumed at bounce b
 constexpr uint32_t C_SHADOW = 32;        // [b]: shadow rays emitted at bounce b
-constexpr uint32_t C_WORK_EXT = 64;      // [b]: persistent cursor, extend b
-constexpr uint32_t C_WORK_SHADOW = 96;   // [b]: persistent cursor, shadow b
 constexpr uint32_t C_RESTIR_VIS = 128;   // ReSTIR visibility queries
-constexpr uint32_t C_WORK_RESTIR = 129;  // their persistent cursor
 constexpr uint32_t C_CACHE_QUERY = 130;  // radiance-cache queries
 constexpr uint32_t C_CACHE_TRAIN = 131;  // radiance-cache training records
 constexpr uint32_t C_CACHE_JUDGE = 132;  // radiance-cache judge records
 constexpr uint32_t C_WORDS = 160;
 
+// Work cursors of a persistent traversal launch: WORK_PARTS cursors, each
+// on a 128-B line of its own (atomics on one line serialise: one cursor for
+// a whole launch cost 0.43 ms of atomics for 2M rays on MI355X, 32 cursors
+// 26 us).  A render's launches take W_SETS sets: extend and shadow of each
+// bounce, and the ReSTIR visibility pass.
+constexpr uint32_t WORK_PARTS = 32;
+constexpr uint32_t WORK_STRIDE = 32;                      // words
+constexpr uint32_t WORK_SET = WORK_PARTS * WORK_STRIDE;   // words per launch
+constexpr uint32_t W_EXT = 0;       // [b]: extend of bounce b
+constexpr uint32_t W_SHADOW = 32;   // [b]: shadow rays of bounce b
+constexpr uint32_t W_RESTIR = 64;   // ReSTIR visibility
+constexpr uint32_t W_SETS = 65;
+
 hipError_t launch_zero(hipStream_t s, uint32_t* p, uint32_t words);
+// Clears the cursors of `sets` consecutive cursor sets.
+hipError_t launch_zero_work(hipStream_t s, uint32_t* work, uint32_t sets);
 // key[i] = (first + i, frame) for i < count.
 hipError_t launch_slot_keys(hipStream_t s, uint2* key, uint32_t first, uint32_t frame, uint32_t count);
 hipError_t launch_stats_add(hipStream_t s, const uint32_t* counters, unsigned long long* sum, uint32_t words);

@@ -32,8 +32,6 @@ namespace {
 
 using tt::C_EXT;
 using tt::C_SHADOW;
-using tt::C_WORK_EXT;
-using tt::C_WORK_SHADOW;
 
 // ------------------------------------------------------------ test rays --
 template <bool CULL>
@@ -41,11 +39,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void trace_rays_kernel(tt::DeviceScene
                                                                   uint32_t count, uint32_t* work, uint32_t* spill) {
     SceneView s = view(d);
     Stack st = make_stack(spill);
-    for (;;) {
-        uint32_t base = next_batch(work);
-        if (base >= count) break;
-        uint32_t i = base + lane_id();
-        if (i < count) {
+    WorkFetch wf = work_fetch(work, count);
+    uint32_t first, batch;
+    while (next_batch(wf, first, batch)) {
+        uint32_t i = first + lane_id();
+        if (lane_id() < batch) {
             tt_ray r = rays[i];
             HitRecord h;
             bool found = trace_closest<CULL>(s, f3(r.origin[0], r.origin[1], r.origin[2]),
@@ -114,11 +112,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(5))
     const uint32_t count = b.counters[C_EXT + bounce];
     const float4* qo = b.queue[bounce & 1][0];
     const float4* qd = b.queue[bounce & 1][1];
-    for (;;) {
-        uint32_t base = next_batch(b.counters + C_WORK_EXT + bounce);
-        if (base >= count) break;
-        uint32_t i = base + lane_id();
-        if (i < count) {
+    WorkFetch wf = work_fetch(b.work + (tt::W_EXT + bounce) * tt::WORK_SET, count);
+    uint32_t first, batch;
+    while (next_batch(wf, first, batch)) {
+        uint32_t i = first + lane_id();
+        if (lane_id() < batch) {
             float4 o = qo[i], dd = qd[i];
             HitRecord h;
             bool found = trace_closest<true>(s, f3(o.x, o.y, o.z), f3(dd.x, dd.y, dd.z), 1.0e-3f, 1000.0f, st, h);
@@ -396,11 +394,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void shadow_kernel(tt::DeviceScene d, 
     SceneView s = view(d);
     Stack st = make_stack(b.spill);
     const uint32_t count = b.counters[C_SHADOW + bounce];
-    for (;;) {
-        uint32_t base = next_batch(b.counters + C_WORK_SHADOW + bounce);
-        if (base >= count) break;
-        uint32_t i = base + lane_id();
-        if (i < count) {
+    WorkFetch wf = work_fetch(b.work + (tt::W_SHADOW + bounce) * tt::WORK_SET, count);
+    uint32_t first, batch;
+    while (next_batch(wf, first, batch)) {
+        uint32_t i = first + lane_id();
+        if (lane_id() < batch) {
             float4 o = b.shadow[0][i], dd = b.shadow[1][i];
             float4 direct = b.shadow[2][i];
             float4 e = b.shadow[3][i];
@@ -470,6 +468,12 @@ __global__ void light_records_kernel(tt::DeviceScene d, float4* out) {
 __global__ void zero_kernel(uint32_t* p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+// Cursor c of consecutive cursor sets sits at word c * WORK_STRIDE.
+__global__ void zero_work_kernel(uint32_t* work, uint32_t cursors) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cursors) work[size_t(i) * tt::WORK_STRIDE] = 0;
 }
 
 __global__ void slot_keys_kernel(uint2* key, uint32_t first, uint32_t frame, uint32_t n) {
@@ -553,6 +557,12 @@ size_t spill_words(int grid) { return size_t(grid) * TRACE_BLOCK * TRACE_SPILL_S
 
 hipError_t launch_zero(hipStream_t s, uint32_t* p, uint32_t words) {
     zero_kernel<<<blocks(words, 256), 256, 0, s>>>(p, words);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_work(hipStream_t s, uint32_t* work, uint32_t sets) {
+    const uint32_t cursors = sets * WORK_PARTS;
+    zero_work_kernel<<<blocks(cursors, 256), 256, 0, s>>>(work, cursors);
     return hipGetLastError();
 }
 

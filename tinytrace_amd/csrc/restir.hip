@@ -31,7 +31,6 @@ using namespace ttd;
 namespace {
 
 using tt::C_RESTIR_VIS;
-using tt::C_WORK_RESTIR;
 constexpr float RESTIR_RADIUS = 30.0f;
 constexpr float RESTIR_MAX_HISTORY = 20.0f;
 
@@ -236,11 +235,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void restir_visibility_kernel(tt::Devi
     SceneView s = view(d);
     Stack st = make_stack(b.spill);
     const uint32_t count = b.counters[C_RESTIR_VIS];
-    for (;;) {
-        uint32_t base = next_batch(b.counters + C_WORK_RESTIR);
-        if (base >= count) break;
-        uint32_t i = base + lane_id();
-        if (i < count) {
+    WorkFetch wf = work_fetch(b.work + tt::W_RESTIR * tt::WORK_SET, count);
+    uint32_t first, batch;
+    while (next_batch(wf, first, batch)) {
+        uint32_t i = first + lane_id();
+        if (lane_id() < batch) {
             float4 o = b.shadow[0][i], dd = b.shadow[1][i], meta4 = b.shadow[2][i];
             bool visible = light_visible(s, f3(o.x, o.y, o.z), f3(dd.x, dd.y, dd.z), __float_as_uint(dd.w),
                                          __float_as_uint(meta4.w) & 0xffffu, st);

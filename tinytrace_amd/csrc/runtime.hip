@@ -74,8 +74,8 @@ tt_status stats_end(tt_context* c, size_t index, hipStream_t stream) {
 constexpr uint32_t STATIC_TRACE_MAX_RAYS = TT_TRACE_STATIC_MAX_RAYS;
 constexpr int STATIC_TRACE_GRID = 128;
 constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
-constexpr size_t ST_CURSOR = 0;
-constexpr size_t ST_RAYS = 256;
+constexpr size_t ST_CURSOR = 0;  // one cursor set
+constexpr size_t ST_RAYS = align256(size_t(tt::WORK_SET) * sizeof(uint32_t));
 constexpr size_t ST_HITS = align256(ST_RAYS + size_t(STATIC_TRACE_MAX_RAYS) * sizeof(tt_ray));
 constexpr size_t ST_SPILL = align256(ST_HITS + size_t(STATIC_TRACE_MAX_RAYS) * sizeof(tt_hit));
 constexpr size_t ST_BYTES =
@@ -149,6 +149,7 @@ tt_status ensure_paths(tt_context* c, uint32_t lane, size_t paths, PathBuffers& 
     b.shadow[3] = f + 10 * n;
     b.rng = reinterpret_cast<uint64_t*>(f + 11 * n);
     b.counters = L.counters;
+    b.work = L.work;
     b.spill = L.spill;
     b.cache = CacheBuffers{};
     b.restir_final = nullptr;
@@ -260,7 +261,7 @@ tt_status tt_context_create(const tt_context_desc* desc, tt_context** out) try {
     resolve_static_scratch(c);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(TT_HIP_ERROR);
     if (hipEventCreateWithFlags(&c->api_mark, hipEventDisableTiming) != hipSuccess) return cleanup(TT_HIP_ERROR);
-    if (hipMalloc(&c->work, 256) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
+    if (hipMalloc(&c->work, tt::WORK_SET * sizeof(uint32_t)) != hipSuccess) return cleanup(TT_OUT_OF_MEMORY);
     if (hipMalloc(&c->stats_sum, tt::C_WORDS * sizeof(unsigned long long)) != hipSuccess ||
         hipMemset(c->stats_sum, 0, tt::C_WORDS * sizeof(unsigned long long)) != hipSuccess)
         return cleanup(TT_OUT_OF_MEMORY);
@@ -285,6 +286,7 @@ tt_status tt_context_create(const tt_context_desc* desc, tt_context** out) try {
         if (hipEventCreate(&L.integrate.begin) != hipSuccess || hipEventCreate(&L.integrate.end) != hipSuccess)
             return cleanup(TT_HIP_ERROR);
         if (hipMalloc(&L.counters, tt::C_WORDS * sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc(&L.work, size_t(tt::W_SETS) * tt::WORK_SET * sizeof(uint32_t)) != hipSuccess ||
             hipMalloc(&L.spill, spill_bytes) != hipSuccess || hipMalloc(&L.side_spill, spill_bytes) != hipSuccess)
             return cleanup(TT_OUT_OF_MEMORY);
     }
@@ -311,6 +313,7 @@ void context_free(tt_context* c) {
         if (L.restir_slot_memory) (void)hipFree(L.restir_slot_memory);
         if (L.cache_memory) (void)hipFree(L.cache_memory);
         if (L.counters) (void)hipFree(L.counters);
+        if (L.work) (void)hipFree(L.work);
         if (L.spill) (void)hipFree(L.spill);
         if (L.side_spill) (void)hipFree(L.side_spill);
         for (hipEvent_t e : {L.fork, L.join, L.accumulated, L.done, L.integrate.begin, L.integrate.end,
@@ -772,6 +775,7 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
         if (es != TT_OK) return es;
     }
     TT_HIP(c, tt::launch_zero(stream, L.counters, tt::C_WORDS));
+    TT_HIP(c, tt::launch_zero_work(stream, L.work, tt::W_SETS));
     TT_HIP(c, tt::launch_generate(stream, f, b, paths));
     size_t ev = 0;
     // Bounce b's shadow rays (side stream) overlap bounce b + 1's extension
@@ -1060,7 +1064,7 @@ tt_status tt_trace_closest_device(tt_context* c, const tt_scene* s, const tt_ray
     if (s->context != c) return fail(c, TT_STALE_HANDLE, "scene from another context");
     if (count == 0) return TT_OK;
     TT_HIP(c, hipSetDevice(c->device));
-    TT_HIP(c, tt::launch_zero(tt::api_stream(c), c->work, 1));
+    TT_HIP(c, tt::launch_zero_work(tt::api_stream(c), c->work, 1));
     TT_HIP(c, tt::launch_trace_rays(tt::api_stream(c), s->view, rays, hits, count, (flags & TT_TRACE_CULL_BACK_FACES) != 0,
                                     c->work, c->lanes[0].spill, c->grid));
     return TT_OK;
@@ -1111,9 +1115,9 @@ tt_status tt_trace_closest_static(tt_context* c, const tt_scene* s, const tt_ray
     // STATIC_TRACE_GRID stays inside the spill area.
     const int grid = int(std::min<uint32_t>(STATIC_TRACE_GRID, (count + tt::TRACE_THREADS - 1) / tt::TRACE_THREADS));
     hipStream_t stream = tt::api_stream(c);
-    // The cursor keeps the previous call's final value: clear it with a
+    // The cursors keep the previous call's final values: clear them with a
     // kernel (no memset node, so the sequence can also be captured).
-    TT_HIP(c, tt::launch_zero(stream, cursor, 1));
+    TT_HIP(c, tt::launch_zero_work(stream, cursor, 1));
     TT_HIP(c, hipMemcpyAsync(staged_rays, rays, size_t(count) * sizeof(tt_ray), hipMemcpyHostToDevice, stream));
     TT_HIP(c, tt::launch_trace_rays(stream, s->view, staged_rays, staged_hits, count,
                                     (flags & TT_TRACE_CULL_BACK_FACES) != 0, cursor, spill, grid));

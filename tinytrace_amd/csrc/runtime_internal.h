@@ -40,6 +40,7 @@ struct Lane {
     bool pending = false;       // a render the API stream has not waited for
     bool used = false;          // accumulated has been recorded
     uint32_t* counters = nullptr;
+    uint32_t* work = nullptr;  // W_SETS cursor sets (tt::WORK_SET words each)
     uint32_t* spill = nullptr;
     uint32_t* side_spill = nullptr;
     size_t path_capacity = 0;

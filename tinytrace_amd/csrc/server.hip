@@ -440,6 +440,7 @@ tt_status enqueue_batch(tt_server* sv, bool& formed) {
         r->cleared = true;
     }
     TT_HIP(c, tt::launch_zero(stream, L.counters, tt::C_WORDS));
+    TT_HIP(c, tt::launch_zero_work(stream, L.work, tt::W_SETS));
     for (const Chunk& ch : chunks)
         TT_HIP(c, tt::launch_generate_chunk(stream, chunk_args(ch), b, ch.base, ch.pixel0, ch.count));
     tt::FrameArgs f = chunk_args(chunks.front());  // configuration fields are the group's
@@ -455,7 +456,8 @@ tt_status enqueue_batch(tt_server* sv, bool& formed) {
             // The request's passes in earlier batches (other lanes) are done
             // with its state.
             if (r->restir_recorded) TT_HIP(c, hipStreamWaitEvent(stream, r->restir_done, 0));
-            TT_HIP(c, tt::launch_zero(stream, L.counters + tt::C_RESTIR_VIS, 2));
+            TT_HIP(c, tt::launch_zero(stream, L.counters + tt::C_RESTIR_VIS, 1));
+            TT_HIP(c, tt::launch_zero_work(stream, L.work + tt::W_RESTIR * tt::WORK_SET, 1));
             tt::RestirBuffers rb = restir_buffers(*r, ch.pass, ch.base);
             TT_HIP(c, tt::launch_restir(stream, scene->view, chunk_args(ch), b, rb, c->grid));
             const uint4* initial = (rb.temporal_reuse || rb.spatial_reuse) ? rb.initial : rb.final_[rb.cur];
