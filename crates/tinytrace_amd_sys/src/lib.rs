@@ -60,6 +60,8 @@ pub const TT_TEXTURE_FORMAT_RGBA8_SRGB: u32 = 0;
 pub const TT_TEXTURE_FORMAT_RG8_UNORM: u32 = 1;
 pub const TT_TEXTURE_FORMAT_R8_UNORM: u32 = 2;
 pub const TT_TEXTURE_FORMAT_R16_FLOAT: u32 = 3;
+pub const TT_TEXTURE_FORMAT_BC1_SRGB: u32 = 4;
+pub const TT_TEXTURE_FORMAT_BC5_UNORM: u32 = 5;
 // Texture channels: the fields of MaterialTextures in order
 // (src/asset/material.rs:203-231), i.e. the order of tt_material::textures.
 pub const TT_TEXTURE_BASE_COLOR: u32 = 0;
@@ -550,6 +552,7 @@ extern "C" {
 
     pub fn tt_asset_scene_create(out: *mut *mut tt_asset_scene) -> tt_status;
     pub fn tt_asset_scene_destroy(scene: *mut tt_asset_scene);
+    pub fn tt_asset_set_block_compression(scene: *mut tt_asset_scene, enable: u32) -> tt_status;
     pub fn tt_asset_insert_mesh(scene: *mut tt_asset_scene, mesh: *const tt_mesh_desc, index: *mut u32) -> tt_status;
     pub fn tt_material_desc_default(out: *mut tt_material_desc);
     pub fn tt_asset_insert_material(scene: *mut tt_asset_scene, material: *const tt_material_desc, index: *mut u32) -> tt_status;

@@ -196,19 +196,34 @@ typedef struct tt_scene_desc {
 } tt_scene_desc;
 
 /* Texel formats of processed textures (ProcessedTexture::from_*,
- * src/asset/texture.rs:57-119), stored uncompressed: colour as RGBA8 read
- * through the sRGB curve (the reference's BC1 sRGB), normal maps as RG8
- * octahedral (its BC5), scalars as R8 unorm or R16 float. */
+ * src/asset/texture.rs:57-119).  Asset processing stores colour as BC1 read
+ * through the sRGB curve (texture.rs:86-101) and normal maps as BC5 of the
+ * octahedral encoding (:103-117), both block-compressed by this library's
+ * own encoder, and scalars as R8 unorm or R16 float.  The sampler also reads
+ * the uncompressed RGBA8 sRGB and RG8 forms.
+ * Block layouts (4 x 4 texels, blocks row-major, ceil(w / 4) per row):
+ *  - BC1, 8 B: RGB565 endpoints c0, c1 (little-endian), then 2-bit indices,
+ *    texel (x, y) at bit 2 (4y + x).  Endpoints expand by bit replication;
+ *    c0 > c1: palette c0, c1, (2 c0 + c1 + 1) / 3, (c0 + 2 c1 + 1) / 3 per
+ *    8-bit channel; else c0, c1, (c0 + c1 + 1) / 2, black with alpha 0.
+ *  - BC5, 16 B: two BC4 blocks, red then green.  BC4: endpoints e0, e1
+ *    (bytes), then 3-bit indices, texel (x, y) at bit 3 (4y + x) of the
+ *    48-bit field; e0 > e1: palette e0, e1 and ((8 - k) e0 + (k - 1) e1 + 3)
+ *    / 7 for k = 2..7; else e0, e1, ((6 - k) e0 + (k - 1) e1 + 2) / 5 for
+ *    k = 2..5, then 0 and 255. */
 enum {
     TT_TEXTURE_FORMAT_RGBA8_SRGB = 0,
     TT_TEXTURE_FORMAT_RG8_UNORM = 1,
     TT_TEXTURE_FORMAT_R8_UNORM = 2,
-    TT_TEXTURE_FORMAT_R16_FLOAT = 3
+    TT_TEXTURE_FORMAT_R16_FLOAT = 3,
+    TT_TEXTURE_FORMAT_BC1_SRGB = 4,
+    TT_TEXTURE_FORMAT_BC5_UNORM = 5
 };
 
 /* One processed texture: its mip chain (generate_mip_chain,
  * texture.rs:121-143) in texture_data; level k is (width >> k) x
- * (height >> k) texels, rows from the top (v = 0). */
+ * (height >> k) texels, rows from the top (v = 0), each level at a 16-byte
+ * aligned offset. */
 typedef struct tt_texture_record {
     uint32_t format;
     uint32_t width, height;
@@ -697,6 +712,11 @@ typedef struct tt_texture_desc {
 
 tt_status tt_asset_scene_create(tt_asset_scene** out);
 void tt_asset_scene_destroy(tt_asset_scene* scene);
+/* Textures inserted from now on (materials and glTF imports): colour and
+ * normal maps block-compressed as BC1 / BC5 (enable 1, the default, as the
+ * reference's asset processing stores them) or uncompressed as RGBA8 / RG8
+ * (enable 0: the texels' exact quantized bytes). */
+tt_status tt_asset_set_block_compression(tt_asset_scene* scene, uint32_t enable);
 /* Scene::insert_mesh (src/asset/mod.rs:89-161). */
 tt_status tt_asset_insert_mesh(tt_asset_scene* scene, const tt_mesh_desc* mesh, uint32_t* index);
 /* Scene::insert_material (src/asset/material.rs:305-357). */

@@ -70,6 +70,8 @@ def lib():
         L.oracle_display.argtypes = [vp, u32, u32, vp]
         L.oracle_sample_texture.argtypes = [vp, u32, C.c_float, C.c_float, C.c_float, vp]
         L.oracle_primary_lod.argtypes = [vp, vp, C.c_float, C.c_float, vp]
+        L.oracle_block_texel.restype = None
+        L.oracle_block_texel.argtypes = [u32, vp, u32, u32, u32, vp]
         _lib = L
     return _lib
 
@@ -194,6 +196,15 @@ def math_f32(op, x, y=None):
     y = np.zeros_like(x) if y is None else np.ascontiguousarray(y, dtype=np.float32)
     out = np.zeros_like(x)
     lib().oracle_math_f32(MATH_OPS[op], x.ctypes.data, y.ctypes.data, out.ctypes.data, x.size)
+    return out
+
+
+def block_texel(format, data, width, x, y):
+    """8-bit channels of texel (x, y) of a BC1 / BC5 level `width` texels
+    wide stored in `data` (bytes), as the oracle's sampler decodes them."""
+    buf = np.frombuffer(bytes(data), dtype=np.uint8).copy()
+    out = np.zeros(4, dtype=np.uint8)
+    lib().oracle_block_texel(format, buf.ctypes.data, width, x, y, out.ctypes.data)
     return out
 
 

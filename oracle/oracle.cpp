@@ -567,6 +567,11 @@ int oracle_sample_texture(void* handle, uint32_t index, float u, float v, float 
     return 0;
 }
 
+// The 8-bit channels of texel (x, y) of a BC1 or BC5 level w texels wide.
+void oracle_block_texel(uint32_t format, const uint8_t* data, uint32_t w, uint32_t x, uint32_t y, uint8_t* out) {
+    Scene<float>::block_texel(format, data, w, x, y, out);
+}
+
 int oracle_primary_lod(void* handle, const tt_constants* k, float ndc_x, float ndc_y, float* lod) {
     auto* os = static_cast<OracleScene*>(handle);
     const Scene<float>& s = *os->f;

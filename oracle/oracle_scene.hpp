@@ -424,10 +424,18 @@ struct Scene {
 
     bool textured() const { return desc->texture_count > 0; }
 
-    // One texel of texture level `data` as the sampler returns it: RGBA8
-    // through the sRGB table, RG8 and R8 as unorm, R16 as half.
-    void texel(uint32_t format, const uint8_t* data, uint32_t i, R out[4]) const {
-        if (format == TT_TEXTURE_FORMAT_RGBA8_SRGB) {
+    // One texel (x, y) of texture level `data`, w texels wide, as the
+    // sampler returns it: BC1 and RGBA8 through the sRGB table, BC5, RG8 and
+    // R8 as unorm, R16 as half.
+    void texel(uint32_t format, const uint8_t* data, uint32_t w, uint32_t x, uint32_t y, R out[4]) const {
+        const size_t i = size_t(y) * w + x;
+        if (format == TT_TEXTURE_FORMAT_BC1_SRGB || format == TT_TEXTURE_FORMAT_BC5_UNORM) {
+            uint8_t c[4];
+            block_texel(format, data, w, x, y, c);
+            for (int k = 0; k < 3; k++)
+                out[k] = format == TT_TEXTURE_FORMAT_BC1_SRGB ? R(srgb[c[k]]) : R(float(c[k]) / 255.0f);
+            out[3] = R(float(c[3]) / 255.0f);
+        } else if (format == TT_TEXTURE_FORMAT_RGBA8_SRGB) {
             const uint8_t* t = data + 4 * size_t(i);
             out[0] = R(srgb[t[0]]);
             out[1] = R(srgb[t[1]]);
@@ -450,6 +458,53 @@ struct Scene {
         }
     }
 
+    // The 8-bit channels of texel (x, y) of a BC1 (RGB, alpha) or BC5 (R, G,
+    // 0, 255) level, decoded as the block formats define them.
+    static void block_texel(uint32_t format, const uint8_t* data, uint32_t w, uint32_t x, uint32_t y, uint8_t c[4]) {
+        const size_t bytes = format == TT_TEXTURE_FORMAT_BC1_SRGB ? 8 : 16;
+        const uint8_t* b = data + bytes * ((size_t(y) / 4) * ((w + 3) / 4) + x / 4);
+        const uint32_t t = (y % 4) * 4 + (x % 4);  // texel number in the block
+        if (format == TT_TEXTURE_FORMAT_BC1_SRGB) {
+            const uint32_t e[2] = {uint32_t(b[0]) | uint32_t(b[1]) << 8, uint32_t(b[2]) | uint32_t(b[3]) << 8};
+            const uint32_t index = (b[4 + t / 4] >> (2 * (t % 4))) & 3;
+            uint32_t rgb[2][3];
+            for (int j = 0; j < 2; j++) {
+                uint32_t r5 = e[j] >> 11, g6 = (e[j] >> 5) & 63, b5 = e[j] & 31;
+                rgb[j][0] = r5 * 8 + r5 / 4;
+                rgb[j][1] = g6 * 4 + g6 / 16;
+                rgb[j][2] = b5 * 8 + b5 / 4;
+            }
+            c[3] = 255;
+            for (int k = 0; k < 3; k++) {
+                uint32_t a = rgb[0][k], z = rgb[1][k];
+                uint32_t v;
+                if (index == 0) v = a;
+                else if (index == 1) v = z;
+                else if (e[0] > e[1]) v = index == 2 ? (2 * a + z + 1) / 3 : (a + 2 * z + 1) / 3;
+                else v = index == 2 ? (a + z + 1) / 2 : 0;
+                c[k] = uint8_t(v);
+            }
+            if (index == 3 && !(e[0] > e[1])) c[3] = 0;
+            return;
+        }
+        for (int ch = 0; ch < 2; ch++) {
+            const uint8_t* h = b + 8 * ch;
+            const uint32_t e0 = h[0], e1 = h[1];
+            uint64_t bits = 0;
+            for (int k = 0; k < 6; k++) bits |= uint64_t(h[2 + k]) << (8 * k);
+            const uint32_t index = uint32_t(bits >> (3 * t)) & 7;
+            uint32_t v;
+            if (index == 0) v = e0;
+            else if (index == 1) v = e1;
+            else if (e0 > e1) v = ((8 - index) * e0 + (index - 1) * e1 + 3) / 7;
+            else if (index < 6) v = ((6 - index) * e0 + (index - 1) * e1 + 2) / 5;
+            else v = index == 6 ? 0 : 255;
+            c[ch] = uint8_t(v);
+        }
+        c[2] = 0;
+        c[3] = 255;
+    }
+
     // Repeat addressing of one axis: lower and upper texel, weight of upper.
     static void wrap(R c, uint32_t n, uint32_t& i0, uint32_t& i1, R& a) {
         R f = c - std::floor(c);
@@ -468,10 +523,10 @@ struct Scene {
         wrap(u, w, x0, x1, ax);
         wrap(v, h, y0, y1, ay);
         R c00[4], c10[4], c01[4], c11[4];
-        texel(format, data, y0 * w + x0, c00);
-        texel(format, data, y0 * w + x1, c10);
-        texel(format, data, y1 * w + x0, c01);
-        texel(format, data, y1 * w + x1, c11);
+        texel(format, data, w, x0, y0, c00);
+        texel(format, data, w, x1, y0, c10);
+        texel(format, data, w, x0, y1, c01);
+        texel(format, data, w, x1, y1, c11);
         for (int k = 0; k < 4; k++) {
             R top = c00[k] * (R(1) - ax) + c10[k] * ax;
             R bottom = c01[k] * (R(1) - ax) + c11[k] * ax;

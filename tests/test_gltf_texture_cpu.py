@@ -196,14 +196,14 @@ def texture_level(arrays, index, level=0):
 
 def decoded_through_import(tmp, png):
     """RGBA8 of `png` as the importer decodes it: R, G, B from a base colour
-    texture (RGBA8 level 0, factor 1) and A from a specular weight texture
-    (R8 level 0, factor 1) sharing the image."""
+    texture (RGBA8 level 0, factor 1, imported uncompressed) and A from a
+    specular weight texture (R8 level 0, factor 1) sharing the image."""
     with open(os.path.join(tmp, "image.png"), "wb") as f:
         f.write(png)
     doc = quad_doc([{"pbrMetallicRoughness": {"baseColorTexture": {"index": 0}},
                      "extensions": {"KHR_materials_specular": {"specularTexture": {"index": 0}}}}],
                    [{"uri": "image.png"}])
-    a = GltfImporter(write_doc(tmp, doc)).new_scene().arrays()
+    a = GltfImporter(write_doc(tmp, doc)).new_scene(block_compression=False).arrays()
     assert a["textures"]["format"][0] == _abi.TT_TEXTURE_FORMAT_RGBA8_SRGB
     assert a["textures"]["format"][1] == _abi.TT_TEXTURE_FORMAT_R8_UNORM
     rgb = texture_level(a, 0)[..., :3]
@@ -312,7 +312,7 @@ def _import_one(tmp_path, png, source):
         doc["buffers"][0] = {"byteLength": len(data), "uri": _uri(data)}
         doc["bufferViews"].append({"buffer": 0, "byteOffset": len(geometry) + pad, "byteLength": len(png)})
         doc["images"][0] = {"bufferView": 3, "mimeType": "image/png"}
-    return GltfImporter(write_doc(tmp, doc, f"{source}.gltf")).new_scene().arrays()
+    return GltfImporter(write_doc(tmp, doc, f"{source}.gltf")).new_scene(block_compression=False).arrays()
 
 
 def _unorm(b):

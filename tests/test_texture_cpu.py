@@ -3,9 +3,11 @@ chains (src/asset/texture.rs, material.rs:305-357), the oracle's sampler
 (textureLod with linear filtering and repeat addressing) and its first-hit
 level of detail (integrate.glsl:89-107, 134-139), pinned by closed forms.
 
-Parity unpinned: the reference block-compresses colour (BC1) and normal
-(BC5) textures with texpresso and filters anisotropically in hardware; this
-library stores those texels uncompressed and filters trilinearly."""
+Colour and normal textures are block-compressed (BC1 / BC5) by this
+library's own encoder; tests of exact texel bytes build their scenes
+uncompressed.  Parity unpinned: the bytes of the reference's encoder
+(texpresso 2.0.1 ClusterFit, a dependency the reference does not vendor).
+The BC decode is pinned by known-answer blocks in test_bc_cpu.py."""
 import math
 
 import numpy as np
@@ -21,9 +23,10 @@ def quantize_unorm8(v):
     return np.floor(np.clip(np.asarray(v, np.float32), 0.0, 1.0) * np.float32(255.0) + np.float32(0.5)).astype(np.uint8)
 
 
-def quad_scene(textures, material=None, depth=2.0, half=50.0, uv_scale=0.5):
-    """A camera-facing quad (normal +z) at z = -depth with uv = uv_scale * (x, y)."""
-    s = AssetScene()
+def quad_scene(textures, material=None, depth=2.0, half=50.0, uv_scale=0.5, compress=False):
+    """A camera-facing quad (normal +z) at z = -depth with uv = uv_scale * (x, y);
+    colour and normal textures uncompressed unless `compress`."""
+    s = AssetScene(block_compression=compress)
     p = np.array([[-half, -half, -depth], [half, -half, -depth], [half, half, -depth], [-half, half, -depth]],
                  np.float32)
     uv = p[:, :2] * uv_scale
@@ -204,8 +207,9 @@ def _wall_and_split_light(light_materials, light_uvs):
     """A diffuse wall at z = -3 facing +z and a 1 x 1 emitter at z = -1.5
     facing it, made of two one-triangle meshes (materials and constant
     per-triangle texture coordinates given), outside a camera at the origin
-    looking down -z with a 90 degree field of view."""
-    s = AssetScene()
+    looking down -z with a 90 degree field of view (textures uncompressed,
+    so the closed form holds to rounding)."""
+    s = AssetScene(block_compression=False)
     wall = np.array([[-3, -3, -3], [3, -3, -3], [3, 3, -3], [-3, 3, -3]], np.float32)
     s.insert_mesh(wall, wall[:, :2] / 6.0, [0, 1, 2, 0, 2, 3], np.tile([0, 0, 1.0], (4, 1)), None)
     quad = np.array([[1.6, 0.2, -1.5], [2.6, 0.2, -1.5], [2.6, 1.2, -1.5], [1.6, 1.2, -1.5]], np.float32)

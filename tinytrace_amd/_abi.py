@@ -75,6 +75,7 @@ TEXTURE_CHANNELS = ("base_color", "specular_color", "transmission_color", "coat_
                     "coat_roughness_anisotropy", "coat_ior", "coat_darkening", "coat_rotation", "fuzz_weight",
                     "fuzz_roughness", "emission_luminance", "geometry_opacity")
 TT_TEXTURE_FORMAT_RGBA8_SRGB, TT_TEXTURE_FORMAT_RG8_UNORM, TT_TEXTURE_FORMAT_R8_UNORM, TT_TEXTURE_FORMAT_R16_FLOAT = 0, 1, 2, 3
+TT_TEXTURE_FORMAT_BC1_SRGB, TT_TEXTURE_FORMAT_BC5_UNORM = 4, 5
 TEXTURE_RECORD_DTYPE = np.dtype([("format", "<u4"), ("width", "<u4"), ("height", "<u4"), ("levels", "<u4"),
                                  ("offsets", "<u4", 16)])
 
@@ -255,6 +256,7 @@ _PROTOTYPES = [
     ("tt_material_desc_default", None, [C.POINTER(MaterialDesc)]),
     ("tt_asset_scene_create", C.c_int, [C.POINTER(C.c_void_p)]),
     ("tt_asset_scene_destroy", None, [C.c_void_p]),
+    ("tt_asset_set_block_compression", C.c_int, [C.c_void_p, C.c_uint32]),
     ("tt_asset_insert_mesh", C.c_int, [C.c_void_p, C.POINTER(MeshDesc), C.POINTER(C.c_uint32)]),
     ("tt_asset_insert_material", C.c_int, [C.c_void_p, C.POINTER(MaterialDesc), C.POINTER(C.c_uint32)]),
     ("tt_asset_add_node", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_uint32)]),

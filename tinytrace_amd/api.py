@@ -172,20 +172,25 @@ class GltfImporter:
     def insert_in_scene(self, scene: "AssetScene") -> None:
         scene.import_gltf(self.path)
 
-    def new_scene(self) -> "AssetScene":
-        scene = AssetScene()
+    def new_scene(self, block_compression: bool = True) -> "AssetScene":
+        scene = AssetScene(block_compression)
         self.insert_in_scene(scene)
         return scene
 
 
 class AssetScene:
-    """asset::Scene: processed meshes, materials and a node tree (node 0 = root)."""
+    """asset::Scene: processed meshes, materials and a node tree (node 0 = root).
+    Colour and normal textures are stored block-compressed (BC1 / BC5), as
+    the reference stores them; block_compression=False keeps their exact
+    quantized bytes (RGBA8 / RG8)."""
 
-    def __init__(self):
+    def __init__(self, block_compression: bool = True):
         h = C.c_void_p()
         check(lib().tt_asset_scene_create(C.byref(h)), None, "tt_asset_scene_create")
         self._handle = h
         self._desc: Optional[_abi.SceneDesc] = None
+        if not block_compression:
+            check(lib().tt_asset_set_block_compression(h, 0), None, "tt_asset_set_block_compression")
 
     def __del__(self):
         if getattr(self, "_handle", None):

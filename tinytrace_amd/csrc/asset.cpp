@@ -142,6 +142,7 @@ struct AssetScene {
         std::vector<std::vector<uint8_t>> levels;
     };
     std::vector<Texture> textures;
+    bool block_compression = true;  // colour / normal textures as BC1 / BC5
     std::vector<tt_texture_record> texture_records;
     std::vector<uint8_t> texture_bytes;
     std::string error;
@@ -186,31 +187,220 @@ std::vector<std::vector<float>> mip_chain(const float* texels, uint32_t width, u
     return chain;
 }
 
-// ProcessedTexture::from_* (texture.rs:57-119) without the block
-// compression: colour -> RGBA8 (alpha 1), normal -> octahedral RG8,
-// scalar -> R8 unorm or R16 float (material.rs:312-346 picks which).
-AssetScene::Texture process_texture(const tt_texture_desc& t) {
+// ---------------------------------------------------- block compression --
+// This library's BC1 / BC4 encoder (the reference calls texpresso's
+// ClusterFit, a third-party encoder).  Blocks decode as include/tinytrace.h
+// states; the encoder picks each texel's index against exactly that decoded
+// palette.
+
+// BC1 palette (8-bit RGB) of endpoints c0, c1; entries usable in the mode.
+int bc1_palette(uint32_t c0, uint32_t c1, int p[4][3]) {
+    const uint32_t e[2] = {c0, c1};
+    for (int j = 0; j < 2; j++) {
+        int r5 = int(e[j] >> 11), g6 = int((e[j] >> 5) & 63u), b5 = int(e[j] & 31u);
+        p[j][0] = (r5 << 3) | (r5 >> 2);
+        p[j][1] = (g6 << 2) | (g6 >> 4);
+        p[j][2] = (b5 << 3) | (b5 >> 2);
+    }
+    for (int k = 0; k < 3; k++) {
+        if (c0 > c1) {
+            p[2][k] = (2 * p[0][k] + p[1][k] + 1) / 3;
+            p[3][k] = (p[0][k] + 2 * p[1][k] + 1) / 3;
+        } else {
+            p[2][k] = (p[0][k] + p[1][k] + 1) / 2;
+            p[3][k] = 0;
+        }
+    }
+    return c0 > c1 ? 4 : 3;  // the three-colour mode's fourth entry is transparent black
+}
+
+uint32_t rgb565(const double e[3]) {
+    auto q = [](double v, int bits) {
+        const int top = (1 << bits) - 1;
+        long r = std::lround(std::min(255.0, std::max(0.0, v)) * top / 255.0);
+        return uint32_t(std::min<long>(top, std::max<long>(0, r)));
+    };
+    return (q(e[0], 5) << 11) | (q(e[1], 6) << 5) | q(e[2], 5);
+}
+
+// Endpoints (c0 >= c1, so four colours unless equal) and nearest-entry
+// indices; returns the squared error.
+int64_t bc1_fit(const int px[16][3], uint32_t c0, uint32_t c1, uint32_t& bits, uint32_t& e0, uint32_t& e1) {
+    if (c0 < c1) std::swap(c0, c1);
+    int p[4][3];
+    const int n = bc1_palette(c0, c1, p);
+    int64_t err = 0;
+    bits = 0;
+    for (int t = 0; t < 16; t++) {
+        int best = 0;
+        int64_t best_d = -1;
+        for (int k = 0; k < n; k++) {
+            int64_t d = 0;
+            for (int c = 0; c < 3; c++) d += int64_t(px[t][c] - p[k][c]) * (px[t][c] - p[k][c]);
+            if (best_d < 0 || d < best_d) {
+                best_d = d;
+                best = k;
+            }
+        }
+        bits |= uint32_t(best) << (2 * t);
+        err += best_d;
+    }
+    e0 = c0;
+    e1 = c1;
+    return err;
+}
+
+// One BC1 block of 16 texels (row-major within the block): endpoints from
+// the extent of the colours along their principal axis, then one
+// least-squares refit of the endpoints to the chosen indices.
+void bc1_block(const int px[16][3], uint8_t out[8]) {
+    double mean[3] = {0.0, 0.0, 0.0};
+    for (int t = 0; t < 16; t++)
+        for (int c = 0; c < 3; c++) mean[c] += px[t][c] / 16.0;
+    double cov[3][3] = {};
+    for (int t = 0; t < 16; t++)
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) cov[a][b] += (px[t][a] - mean[a]) * (px[t][b] - mean[b]);
+    double axis[3] = {1.0, 1.0, 1.0};
+    for (int it = 0; it < 8; it++) {
+        double v[3];
+        for (int a = 0; a < 3; a++) v[a] = cov[a][0] * axis[0] + cov[a][1] * axis[1] + cov[a][2] * axis[2];
+        double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (len < 1e-12) break;  // no spread: any axis
+        for (int a = 0; a < 3; a++) axis[a] = v[a] / len;
+    }
+    double lo = 0.0, hi = 0.0;
+    for (int t = 0; t < 16; t++) {
+        double s = 0.0;
+        for (int c = 0; c < 3; c++) s += (px[t][c] - mean[c]) * axis[c];
+        lo = std::min(lo, s);
+        hi = std::max(hi, s);
+    }
+    double a[3], b[3];
+    for (int c = 0; c < 3; c++) {
+        a[c] = mean[c] + axis[c] * hi;
+        b[c] = mean[c] + axis[c] * lo;
+    }
+    uint32_t bits, e0, e1;
+    int64_t err = bc1_fit(px, rgb565(a), rgb565(b), bits, e0, e1);
+    if (e0 > e1) {
+        // Least squares over the four-colour weights of the chosen indices.
+        const double wk[4] = {1.0, 0.0, 2.0 / 3.0, 1.0 / 3.0};
+        double A = 0.0, B = 0.0, Cc = 0.0, X[3] = {}, Y[3] = {};
+        for (int t = 0; t < 16; t++) {
+            double w = wk[(bits >> (2 * t)) & 3u];
+            A += w * w;
+            B += w * (1.0 - w);
+            Cc += (1.0 - w) * (1.0 - w);
+            for (int c = 0; c < 3; c++) {
+                X[c] += w * px[t][c];
+                Y[c] += (1.0 - w) * px[t][c];
+            }
+        }
+        const double det = A * Cc - B * B;
+        if (std::fabs(det) > 1e-9) {
+            for (int c = 0; c < 3; c++) {
+                a[c] = (Cc * X[c] - B * Y[c]) / det;
+                b[c] = (A * Y[c] - B * X[c]) / det;
+            }
+            uint32_t bits2, f0, f1;
+            int64_t err2 = bc1_fit(px, rgb565(a), rgb565(b), bits2, f0, f1);
+            if (err2 < err) {
+                bits = bits2;
+                e0 = f0;
+                e1 = f1;
+            }
+        }
+    }
+    out[0] = uint8_t(e0 & 0xffu);
+    out[1] = uint8_t(e0 >> 8);
+    out[2] = uint8_t(e1 & 0xffu);
+    out[3] = uint8_t(e1 >> 8);
+    for (int k = 0; k < 4; k++) out[4 + k] = uint8_t(bits >> (8 * k));
+}
+
+// One BC4 block: endpoints max > min (eight values), indices to the
+// nearest decoded value; a flat block is its value in both endpoints.
+void bc4_block(const int v[16], uint8_t out[8]) {
+    int e0 = 0, e1 = 255;
+    for (int t = 0; t < 16; t++) {
+        e0 = std::max(e0, v[t]);
+        e1 = std::min(e1, v[t]);
+    }
+    int p[8];
+    p[0] = e0;
+    p[1] = e1;
+    for (int k = 2; k < 8; k++) p[k] = ((8 - k) * e0 + (k - 1) * e1 + 3) / 7;
+    uint64_t bits = 0;
+    if (e0 > e1)
+        for (int t = 0; t < 16; t++) {
+            int best = 0;
+            for (int k = 1; k < 8; k++)
+                if (std::abs(v[t] - p[k]) < std::abs(v[t] - p[best])) best = k;
+            bits |= uint64_t(best) << (3 * t);
+        }
+    out[0] = uint8_t(e0);
+    out[1] = uint8_t(e1);
+    for (int k = 0; k < 6; k++) out[2 + k] = uint8_t(bits >> (8 * k));
+}
+
+// A level of `comps`-byte texels (w x h) as BC1 (comps 3) or BC5 (comps 2)
+// blocks; texels past the edge repeat the edge texel.
+std::vector<uint8_t> compress_level(const std::vector<uint8_t>& texels, uint32_t w, uint32_t h, int comps) {
+    const uint32_t bx = (w + 3) / 4, by = (h + 3) / 4;
+    const size_t block_bytes = comps == 3 ? 8 : 16;
+    std::vector<uint8_t> out(size_t(bx) * by * block_bytes);
+    for (uint32_t j = 0; j < by; j++)
+        for (uint32_t i = 0; i < bx; i++) {
+            int px[16][3];
+            for (int t = 0; t < 16; t++) {
+                uint32_t x = std::min(w - 1, 4 * i + uint32_t(t % 4)), y = std::min(h - 1, 4 * j + uint32_t(t / 4));
+                for (int c = 0; c < comps; c++) px[t][c] = texels[(size_t(y) * w + x) * comps + c];
+            }
+            uint8_t* dst = &out[(size_t(j) * bx + i) * block_bytes];
+            if (comps == 3) {
+                bc1_block(px, dst);
+            } else {
+                int r[16], g[16];
+                for (int t = 0; t < 16; t++) {
+                    r[t] = px[t][0];
+                    g[t] = px[t][1];
+                }
+                bc4_block(r, dst);
+                bc4_block(g, dst + 8);
+            }
+        }
+    return out;
+}
+
+// ProcessedTexture::from_* (texture.rs:57-119): colour -> BC1 of the
+// quantized RGB (the reference appends alpha 1 before compressing),
+// normal -> BC5 of the quantized octahedral encoding, scalar -> R8 unorm or
+// R16 float (material.rs:312-346 picks which).
+AssetScene::Texture process_texture(const tt_texture_desc& t, bool compress) {
     AssetScene::Texture out;
     bool colour = t.channel < TT_TEXTURE_GEOMETRY_NORMAL;
     bool normal = t.channel == TT_TEXTURE_GEOMETRY_NORMAL;
     bool half = t.channel == TT_TEXTURE_SPECULAR_IOR || t.channel == TT_TEXTURE_SPECULAR_ROTATION ||
         t.channel == TT_TEXTURE_TRANSMISSION_DEPTH || t.channel == TT_TEXTURE_COAT_IOR ||
         t.channel == TT_TEXTURE_COAT_ROTATION || t.channel == TT_TEXTURE_EMISSION_LUMINANCE;
-    out.format = colour ? TT_TEXTURE_FORMAT_RGBA8_SRGB
-                        : (normal ? TT_TEXTURE_FORMAT_RG8_UNORM : (half ? TT_TEXTURE_FORMAT_R16_FLOAT : TT_TEXTURE_FORMAT_R8_UNORM));
+    if (colour) out.format = compress ? TT_TEXTURE_FORMAT_BC1_SRGB : TT_TEXTURE_FORMAT_RGBA8_SRGB;
+    else if (normal) out.format = compress ? TT_TEXTURE_FORMAT_BC5_UNORM : TT_TEXTURE_FORMAT_RG8_UNORM;
+    else out.format = half ? TT_TEXTURE_FORMAT_R16_FLOAT : TT_TEXTURE_FORMAT_R8_UNORM;
     out.width = t.width;
     out.height = t.height;
     int comps = colour || normal ? 3 : 1;
     std::vector<std::pair<uint32_t, uint32_t>> extents;
     std::vector<std::vector<float>> chain = mip_chain(t.texels, t.width, t.height, comps, extents);
-    for (const std::vector<float>& level : chain) {
+    for (size_t li = 0; li < chain.size(); li++) {
+        const std::vector<float>& level = chain[li];
         size_t n = level.size() / comps;
         std::vector<uint8_t> bytes;
         for (size_t i = 0; i < n; i++) {
             const float* v = &level[i * comps];
             if (colour) {
                 for (int c = 0; c < 3; c++) bytes.push_back(uint8_t(quantize_unorm(v[c], 8)));
-                bytes.push_back(uint8_t(quantize_unorm(1.0f, 8)));
+                if (!compress) bytes.push_back(uint8_t(quantize_unorm(1.0f, 8)));
             } else if (normal) {
                 float u, w;
                 encode_octahedron(Vec3(v[0], v[1], v[2]), u, w);
@@ -224,6 +414,8 @@ AssetScene::Texture process_texture(const tt_texture_desc& t) {
                 bytes.push_back(uint8_t(quantize_unorm(v[0], 8)));
             }
         }
+        if (compress && (colour || normal))
+            bytes = compress_level(bytes, extents[li].first, extents[li].second, colour ? 3 : 2);
         out.levels.push_back(std::move(bytes));
     }
     return out;
@@ -471,6 +663,14 @@ tt_status tt_asset_insert_material(tt_asset_scene* scene, const tt_material_desc
     return tt::exception_status(scene ? &scene->error : nullptr);
 }
 
+tt_status tt_asset_set_block_compression(tt_asset_scene* scene, uint32_t enable) try {
+    if (!scene) return TT_INVALID_ARGUMENT;
+    scene->block_compression = enable != 0;
+    return TT_OK;
+} catch (...) {
+    return tt::exception_status(scene ? &scene->error : nullptr);
+}
+
 tt_status tt_asset_insert_material_textured(tt_asset_scene* scene, const tt_material_desc* d,
                                             const tt_texture_desc* textures, uint32_t texture_count, uint32_t* index) try {
     if (!scene || !d || (texture_count && !textures)) return TT_INVALID_ARGUMENT;
@@ -533,7 +733,7 @@ tt_status tt_asset_insert_material_textured(tt_asset_scene* scene, const tt_mate
         const tt_texture_desc* t = by_channel[ch];
         if (!t) continue;
         m.textures[ch] = uint32_t(scene->textures.size());
-        scene->textures.push_back(process_texture(*t));
+        scene->textures.push_back(process_texture(*t, scene->block_compression));
         if (ch < 7) {
             for (int c = 0; c < 3; c++) constants[4 * ch + c] = 0.0f;
         } else {

@@ -177,19 +177,74 @@ TT_DEV TextureView texture_view(const SceneView& s) { return TextureView{s.tex_r
 // textureLod with the reference's sampler (scene.rs:121-128: linear
 // filtering, repeat addressing), evaluated in software: bilinear on the two
 // mip levels around the clamped level of detail, then linear between them.
-// The reference's 16x anisotropic filtering is not modelled.
-TT_DEV float4 texel(const TextureView& s, uint32_t format, const uint8_t* level, uint32_t i) {
+// The sampler's max_anisotropy of 16 does not enter: textureLod passes an
+// explicit level of detail (integrate.glsl:144, the reference's only
+// texture read), which leaves no derivative footprint to filter over.
+
+// BC1 texel (x, y) as 8-bit sRGB-encoded RGB and alpha (public block
+// format: RGB565 endpoints expanded by bit replication, palette interpolated
+// in integers as the header states).
+TT_DEV uint4 bc1_texel(const uint8_t* level, uint32_t w, uint32_t x, uint32_t y) {
+    const uint32_t blocks_x = (w + 3u) >> 2;
+    const uint2 blk = *reinterpret_cast<const uint2*>(level + 8 * (size_t(y >> 2) * blocks_x + (x >> 2)));
+    const uint32_t c0 = blk.x & 0xffffu, c1 = blk.x >> 16;
+    const uint32_t k = (blk.y >> (2u * (((y & 3u) << 2) | (x & 3u)))) & 3u;
+    auto expand = [](uint32_t c, uint32_t& r, uint32_t& g, uint32_t& b) {
+        r = (c >> 11) & 31u;
+        g = (c >> 5) & 63u;
+        b = c & 31u;
+        r = (r << 3) | (r >> 2);
+        g = (g << 2) | (g >> 4);
+        b = (b << 3) | (b >> 2);
+    };
+    uint32_t r0, g0, b0, r1, g1, b1;
+    expand(c0, r0, g0, b0);
+    expand(c1, r1, g1, b1);
+    if (k == 0u) return make_uint4(r0, g0, b0, 255u);
+    if (k == 1u) return make_uint4(r1, g1, b1, 255u);
+    if (c0 > c1) {
+        if (k == 2u) return make_uint4((2u * r0 + r1 + 1u) / 3u, (2u * g0 + g1 + 1u) / 3u, (2u * b0 + b1 + 1u) / 3u, 255u);
+        return make_uint4((r0 + 2u * r1 + 1u) / 3u, (g0 + 2u * g1 + 1u) / 3u, (b0 + 2u * b1 + 1u) / 3u, 255u);
+    }
+    if (k == 2u) return make_uint4((r0 + r1 + 1u) / 2u, (g0 + g1 + 1u) / 2u, (b0 + b1 + 1u) / 2u, 255u);
+    return make_uint4(0u, 0u, 0u, 0u);
+}
+
+// BC4 value of texel (x, y) of one 8-byte block.
+TT_DEV uint32_t bc4_texel(uint2 blk, uint32_t x, uint32_t y) {
+    const uint32_t e0 = blk.x & 0xffu, e1 = (blk.x >> 8) & 0xffu;
+    const uint64_t bits = (uint64_t(blk.x) >> 16) | (uint64_t(blk.y) << 16);
+    const uint32_t k = uint32_t(bits >> (3u * (((y & 3u) << 2) | (x & 3u)))) & 7u;
+    if (k == 0u) return e0;
+    if (k == 1u) return e1;
+    if (e0 > e1) return ((8u - k) * e0 + (k - 1u) * e1 + 3u) / 7u;
+    if (k < 6u) return ((6u - k) * e0 + (k - 1u) * e1 + 2u) / 5u;
+    return k == 6u ? 0u : 255u;
+}
+
+// Texel (x, y) of a level w texels wide, as the sampler returns it.
+TT_DEV float4 texel(const TextureView& s, uint32_t format, const uint8_t* level, uint32_t w, uint32_t x, uint32_t y) {
+    const size_t i = size_t(y) * w + x;
+    if (format == TT_TEXTURE_FORMAT_BC1_SRGB) {
+        uint4 c = bc1_texel(level, w, x, y);
+        return make_float4(s.srgb_lut[c.x], s.srgb_lut[c.y], s.srgb_lut[c.z], float(c.w) / 255.0f);
+    }
+    if (format == TT_TEXTURE_FORMAT_BC5_UNORM) {
+        const uint4 blk = *reinterpret_cast<const uint4*>(level + 16 * (size_t(y >> 2) * ((w + 3u) >> 2) + (x >> 2)));
+        return make_float4(float(bc4_texel(make_uint2(blk.x, blk.y), x, y)) / 255.0f,
+                           float(bc4_texel(make_uint2(blk.z, blk.w), x, y)) / 255.0f, 0.0f, 1.0f);
+    }
     if (format == TT_TEXTURE_FORMAT_RGBA8_SRGB) {
-        uint32_t v = *reinterpret_cast<const uint32_t*>(level + 4 * size_t(i));
+        uint32_t v = *reinterpret_cast<const uint32_t*>(level + 4 * i);
         return make_float4(s.srgb_lut[v & 0xffu], s.srgb_lut[(v >> 8) & 0xffu], s.srgb_lut[(v >> 16) & 0xffu],
                            float(v >> 24) / 255.0f);
     }
     if (format == TT_TEXTURE_FORMAT_RG8_UNORM) {
-        uint32_t v = *reinterpret_cast<const uint16_t*>(level + 2 * size_t(i));
+        uint32_t v = *reinterpret_cast<const uint16_t*>(level + 2 * i);
         return make_float4(float(v & 0xffu) / 255.0f, float(v >> 8) / 255.0f, 0.0f, 1.0f);
     }
     if (format == TT_TEXTURE_FORMAT_R8_UNORM) return make_float4(float(level[i]) / 255.0f, 0.0f, 0.0f, 1.0f);
-    uint16_t h = *reinterpret_cast<const uint16_t*>(level + 2 * size_t(i));
+    uint16_t h = *reinterpret_cast<const uint16_t*>(level + 2 * i);
     return make_float4(ttm::half_to_float(h), 0.0f, 0.0f, 1.0f);
 }
 
@@ -217,8 +272,8 @@ TT_DEV float4 bilinear(const TextureView& s, uint32_t format, const uint8_t* lev
     float ax, ay;
     wrap_axis(u, w, x0, x1, ax);
     wrap_axis(v, h, y0, y1, ay);
-    float4 top = lerp4(texel(s, format, level, y0 * w + x0), texel(s, format, level, y0 * w + x1), ax);
-    float4 bottom = lerp4(texel(s, format, level, y1 * w + x0), texel(s, format, level, y1 * w + x1), ax);
+    float4 top = lerp4(texel(s, format, level, w, x0, y0), texel(s, format, level, w, x1, y0), ax);
+    float4 bottom = lerp4(texel(s, format, level, w, x0, y1), texel(s, format, level, w, x1, y1), ax);
     return lerp4(top, bottom, ay);
 }
 

@@ -387,12 +387,17 @@ tt_status tt_scene_create_ex(tt_context* c, const tt_scene_desc* d, const tt_sce
         return fail(c, TT_INVALID_ARGUMENT, "texture arrays missing");
     for (uint32_t i = 0; i < d->texture_count; i++) {
         const tt_texture_record& r = d->textures[i];
-        if (r.format > TT_TEXTURE_FORMAT_R16_FLOAT || r.width == 0 || r.height == 0 || r.levels == 0 || r.levels > 16)
+        if (r.format > TT_TEXTURE_FORMAT_BC5_UNORM || r.width == 0 || r.height == 0 || r.levels == 0 || r.levels > 16)
             return fail(c, TT_INVALID_ARGUMENT, "texture " + std::to_string(i) + " has a bad record");
-        uint64_t texel = r.format == TT_TEXTURE_FORMAT_RGBA8_SRGB ? 4 : (r.format == TT_TEXTURE_FORMAT_R8_UNORM ? 1 : 2);
+        // Bytes of a level and the alignment its texel or block loads need.
+        const bool blocks = r.format == TT_TEXTURE_FORMAT_BC1_SRGB || r.format == TT_TEXTURE_FORMAT_BC5_UNORM;
+        const uint64_t unit = r.format == TT_TEXTURE_FORMAT_BC1_SRGB ? 8 : r.format == TT_TEXTURE_FORMAT_BC5_UNORM ? 16
+            : r.format == TT_TEXTURE_FORMAT_RGBA8_SRGB ? 4 : (r.format == TT_TEXTURE_FORMAT_R8_UNORM ? 1 : 2);
+        const uint64_t align = blocks ? unit : 4;
         for (uint32_t k = 0; k < r.levels; k++) {
             uint64_t w = r.width >> k, h = r.height >> k;
-            if (w == 0 || h == 0 || r.offsets[k] % 4 != 0 || r.offsets[k] + w * h * texel > d->texture_data_bytes)
+            uint64_t bytes = blocks ? ((w + 3) / 4) * ((h + 3) / 4) * unit : w * h * unit;
+            if (w == 0 || h == 0 || r.offsets[k] % align != 0 || r.offsets[k] + bytes > d->texture_data_bytes)
                 return fail(c, TT_INVALID_ARGUMENT, "texture " + std::to_string(i) + " level out of range");
         }
     }
