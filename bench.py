@@ -61,11 +61,23 @@ def main() -> int:
                         "radiance cache); 0 skips it")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="also run a plain device copy of this many MiB (reference kernel for HBM counters)")
+    p.add_argument("--c1-passes", type=int, default=64,
+                   help="passes of the C1 leg (Cornell box 512x512, depth 4, NEE); 0 skips it")
+    p.add_argument("--texture-frames", type=int, default=32,
+                   help="frames of the texture leg (textured Cornell box, 1024^2 textures, BC against "
+                        "uncompressed); 0 skips it")
+    p.add_argument("--leg", choices=("c1", "cache", "texture"), default=None,
+                   help="run only this leg and print its JSON object (for profiling one workload)")
     args = p.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         return 2
+    if args.leg:
+        legs = {"c1": lambda: c1(args.c1_passes), "cache": lambda: radiance_cache(args.cache_frames),
+                "texture": lambda: texture(args.texture_frames)}
+        print(json.dumps({"leg": args.leg, **legs[args.leg]()}))
+        return 0
 
     import numpy as np
 
@@ -139,9 +151,20 @@ def main() -> int:
     # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
     # (scaled by a copy kernel of known size), committed under profiles/.
     traffic = None
+    rocprof = None
     traffic_path = os.path.join(ROOT, "profiles", "extend_kernel_traffic.json")
     if os.path.exists(traffic_path):
-        traffic = round(json.load(open(traffic_path))["bytes_per_launch"])
+        prof = json.load(open(traffic_path))
+        traffic = round(prof["bytes_per_launch"])
+        # The same algorithmic bytes over the launch durations rocprofv3
+        # recorded for this workload (committed with the traffic figures).
+        rays_per_launch = ext_rays / max(ext_launches, 1)
+        rocprof = {}
+        for phase in ("timed", "isolated"):
+            us = prof["rocprof_avg_us"][phase]
+            gbs = rays_per_launch * bytes_per_ray / (us * 1e-6) / 1e9
+            rocprof[phase] = {"avg_launch_ms": round(us / 1e3, 4), "achieved": round(gbs, 2),
+                              "frac": round(gbs / HBM_PEAK_GBS, 5)}
 
     light_desc = ("ReSTIR DI (32 candidates, 1 temporal + 1 spatial reuse)" if restir
                   else "next-event estimation")
@@ -175,7 +198,8 @@ def main() -> int:
                      "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4),
                      "rays_per_launch": round(ext_rays / max(ext_launches, 1), 1),
                      "isolated": {"avg_launch_ms": round(iso_ms, 4), "achieved": round(iso_gbs, 2),
-                                  "frac": round(iso_gbs / HBM_PEAK_GBS, 5)}},
+                                  "frac": round(iso_gbs / HBM_PEAK_GBS, 5)},
+                     "rocprof": rocprof},
     }
 
     if args.quality_frames > 0:
@@ -189,8 +213,12 @@ def main() -> int:
         rates = [float(x) for x in args.serving_sweep.split(",") if x]
         result["serving_sweep"] = serving_sweep(renderer.context, rates, args.serving_sweep_requests)
 
+    if args.c1_passes > 0:
+        result["c1"] = c1(args.c1_passes)
     if args.cache_frames > 0:
         result["radiance_cache"] = radiance_cache(args.cache_frames)
+    if args.texture_frames > 0:
+        result["texture"] = texture(args.texture_frames)
 
     if args.c5_frames > 0:
         result["instanced"] = instanced(args.c5_frames)
@@ -267,6 +295,78 @@ def main() -> int:
         result["gpu_over_cpu"] = round(result["value"] / result["cpu_baseline"]["value"], 1)
     print(json.dumps(result))
     return 0
+
+
+VALU_FP32_PEAK_TFLOPS = 157.3
+
+
+def c1(passes):
+    """C1: the Cornell box at 512x512, 1 spp a pass, depth 4, next-event
+    estimation, PCG seed 0: `passes` passes accumulated back to back after two
+    warm-up passes.  Its FP32 vector work per pass comes from the committed
+    rocprofv3 counters of this leg (SQ_INSTS_VALU_FLOPS_FP32 and its
+    transcendental part, summed over a pass's kernels), divided by the pass
+    time measured here; RMSE against the FP64 Cornell fixture after the
+    passes."""
+    import numpy as np
+
+    from tinytrace_amd import Config, LightSampling, Renderer, RenderSettings, _abi, scenes
+
+    scene, camera = scenes.cornell_box()
+    renderer = Renderer((512, 512), settings=RenderSettings(rng=_abi.TT_RNG_PCG, seed=0))
+    renderer.set_scene(scene)
+    renderer.camera = camera
+    renderer.set_config(Config(bounce_count=4, sample_count=1, light_sampling=LightSampling.NEXT_EVENT_ESTIMATION))
+    for _ in range(2):
+        renderer.render_to_target()
+    renderer.target.download()
+    renderer.reset_accumulation()
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        renderer.render_to_target()
+    img = renderer.target.download()
+    ms = (time.perf_counter() - t0) / passes * 1e3
+    with np.load(os.path.join(ROOT, "tests", "golden", "fp64_cornell_512.npz"), allow_pickle=False) as f:
+        pixels, reference = f["pixels"], f["radiance"]
+    got = img.reshape(-1, 4)[pixels, :3].astype(np.float64)
+    out = {"workload": "Cornell box 512x512, 1 spp per pass, depth 4, NEE, PCG seed 0, passes back to back",
+           "passes": passes, "ms_per_pass": round(ms, 4), "mpaths_per_s": round(512 * 512 / ms / 1e3, 2),
+           "rmse_vs_fp64": round(float(np.sqrt(((got - reference) ** 2).mean())), 5)}
+    valu_path = os.path.join(ROOT, "profiles", "c1_valu.json")
+    if os.path.exists(valu_path):
+        flops = json.load(open(valu_path))["fp32_flops_per_pass"]
+        t = flops / (ms * 1e-3) / 1e12
+        out["valu"] = {"fp32_flops_per_pass": flops, "tflops": round(t, 3), "peak": VALU_FP32_PEAK_TFLOPS,
+                       "unit": "TFLOP/s", "frac": round(t / VALU_FP32_PEAK_TFLOPS, 5)}
+    return out
+
+
+def texture(frames):
+    """The textured Cornell box with 1024x1024 textures (floor: base colour,
+    roughness, IOR, metalness, normal map; back wall: base colour) at
+    512x512, depth 4, NEE: frames back to back with colour and normal
+    textures block-compressed (BC1 / BC5) and uncompressed (RGBA8 / RG8),
+    with the texture bytes each way."""
+    from tinytrace_amd import Config, LightSampling, Renderer, RenderSettings, _abi, scenes
+
+    out = {"workload": "textured Cornell box, 1024^2 textures, 512x512, depth 4, NEE, PCG seed 0", "frames": frames}
+    for name, bc in (("bc", True), ("uncompressed", False)):
+        scene, camera = scenes.textured_cornell_box(size=1024, block_compression=bc)
+        renderer = Renderer((512, 512), settings=RenderSettings(rng=_abi.TT_RNG_PCG, seed=0))
+        renderer.set_scene(scene)
+        renderer.camera = camera
+        renderer.set_config(Config(bounce_count=4, sample_count=1,
+                                   light_sampling=LightSampling.NEXT_EVENT_ESTIMATION))
+        for _ in range(2):
+            renderer.render_to_target()
+        renderer.target.download()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            renderer.render_to_target()
+        renderer.target.download()
+        out[name] = {"ms_per_frame": round((time.perf_counter() - t0) / frames * 1e3, 4),
+                     "texture_bytes": int(scene.arrays()["texture_data"].size)}
+    return out
 
 
 # Algorithmic MLP FLOPs (2 per multiply-add): inference of one query through

@@ -56,11 +56,14 @@ TT_DEV bool next_batch(WorkFetch& w, uint32_t& first, uint32_t& count) {
         uint32_t* cursor = w.set + w.part * tt::WORK_STRIDE;
         bool drained = len == 0u;
         if (!drained && w.visited > 0u)
-            drained = __hip_atomic_load(cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len;
+            drained = __builtin_amdgcn_readfirstlane(
+                          __hip_atomic_load(cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= len;
         if (!drained) {
+            // readfirstlane keeps the wave-uniform cursor values in scalar
+            // registers (the whole wave runs this loop, lane 0 included).
             uint32_t got = 0;
             if (lane_id() == 0) got = atomicAdd(cursor, 64u);
-            got = __shfl(got, 0);
+            got = __builtin_amdgcn_readfirstlane(got);
             if (got < len) {
                 first = lo + got;
                 count = len - got < 64u ? len - got : 64u;

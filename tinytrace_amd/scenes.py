@@ -111,15 +111,16 @@ def cornell_box(variant: int = 0) -> Tuple[AssetScene, Camera]:
     return scene, camera
 
 
-def textured_cornell_box() -> Tuple[AssetScene, Camera]:
-    """The Cornell box with textured floor and back wall: a 64 x 64 checker
-    base colour, roughness, IOR (a float channel), metalness stripes and a
-    bumpy normal map on the floor; a colour gradient on the back wall.  Each
-    face maps [0, 1]^2, so the floor repeats its texture through the
-    sampler's wrap when scaled (uv of the floor quad x 3)."""
-    scene = AssetScene()
+def textured_cornell_box(size: int = 64, block_compression: bool = True) -> Tuple[AssetScene, Camera]:
+    """The Cornell box with textured floor and back wall: a size x size
+    checker base colour, roughness, IOR (a float channel), metalness stripes
+    and a bumpy normal map on the floor; a colour gradient on the back wall.
+    Each face maps [0, 1]^2, so the floor repeats its texture through the
+    sampler's wrap when scaled (uv of the floor quad x 3).  Colour and normal
+    textures are block-compressed unless block_compression is False."""
+    scene = AssetScene(block_compression=block_compression)
     base_only = dict(specular_weight=0.0, specular_roughness=1.0)
-    n = 64
+    n = size
     y, x = np.mgrid[0:n, 0:n].astype(np.float32) / n
     checker = ((np.floor(x * 8) + np.floor(y * 8)) % 2)[..., None]
     floor_color = (0.15 + 0.6 * checker * np.array([0.9, 0.8, 0.6], np.float32) +
