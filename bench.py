@@ -66,7 +66,7 @@ def main() -> int:
     p.add_argument("--texture-frames", type=int, default=32,
                    help="frames of the texture leg (textured Cornell box, 1024^2 textures, BC against "
                         "uncompressed); 0 skips it")
-    p.add_argument("--leg", choices=("c1", "cache", "texture"), default=None,
+    p.add_argument("--leg", choices=("c1", "cache", "texture", "instanced"), default=None,
                    help="run only this leg and print its JSON object (for profiling one workload)")
     args = p.parse_args()
     if args.gpus != 1:
@@ -75,7 +75,7 @@ def main() -> int:
         return 2
     if args.leg:
         legs = {"c1": lambda: c1(args.c1_passes), "cache": lambda: radiance_cache(args.cache_frames),
-                "texture": lambda: texture(args.texture_frames)}
+                "texture": lambda: texture(args.texture_frames), "instanced": lambda: instanced(args.c5_frames)}
         print(json.dumps({"leg": args.leg, **legs[args.leg]()}))
         return 0
 
