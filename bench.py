@@ -12,6 +12,7 @@ from __future__ import annotations
 import argparse
 import dataclasses
 import json
+import math
 import os
 import sys
 import time
@@ -56,7 +57,9 @@ def main() -> int:
     p.add_argument("--c5-serving-rate", type=float, default=2.0, help="Poisson arrival rate of that leg")
     p.add_argument("--cache-frames", type=int, default=16,
                    help="frames of the radiance-cache leg on the Cornell box (0 skips it)")
-    p.add_argument("--c5-frames", type=int, default=4,
+    p.add_argument("--c5-sequences", type=int, default=4,
+                   help="independent frame sequences whose lattice MSE the instanced leg's RMSE curves average")
+    p.add_argument("--c5-frames", type=int, default=16,
                    help="frames of the instanced-scene leg (3840x2160, depth 12, ReSTIR, with and without the "
                         "radiance cache); 0 skips it")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -66,7 +69,7 @@ def main() -> int:
     p.add_argument("--texture-frames", type=int, default=32,
                    help="frames of the texture leg (textured Cornell box, 1024^2 textures, BC against "
                         "uncompressed); 0 skips it")
-    p.add_argument("--leg", choices=("c1", "cache", "texture", "instanced"), default=None,
+    p.add_argument("--leg", choices=("c1", "cache", "texture", "instanced", "served"), default=None,
                    help="run only this leg and print its JSON object (for profiling one workload)")
     args = p.parse_args()
     if args.gpus != 1:
@@ -75,7 +78,8 @@ def main() -> int:
         return 2
     if args.leg:
         legs = {"c1": lambda: c1(args.c1_passes), "cache": lambda: radiance_cache(args.cache_frames),
-                "texture": lambda: texture(args.texture_frames), "instanced": lambda: instanced(args.c5_frames)}
+                "texture": lambda: texture(args.texture_frames), "instanced": lambda: instanced(args.c5_frames, args.c5_sequences),
+                "served": lambda: served_instanced(args.c5_serving_requests, args.c5_serving_rate)}
         print(json.dumps({"leg": args.leg, **legs[args.leg]()}))
         return 0
 
@@ -221,7 +225,7 @@ def main() -> int:
         result["texture"] = texture(args.texture_frames)
 
     if args.c5_frames > 0:
-        result["instanced"] = instanced(args.c5_frames)
+        result["instanced"] = instanced(args.c5_frames, args.c5_sequences)
     if args.c5_serving_requests > 0:
         result["instanced_served"] = served_instanced(args.c5_serving_requests, args.c5_serving_rate)
 
@@ -491,12 +495,41 @@ def radiance_cache(frames):
             "equal_rmse": equal_rmse}
 
 
-def instanced(frames):
+def equal_rmse(curve_a, ms_a, curve_b, ms_b, frames_b):
+    """Frames (and time) method a needs to reach method b's RMSE after
+    frames_b frames: log-log interpolation on a's curve, extrapolated from its
+    last two points past its end.  Returns (frames, ms, speedup of b over a)."""
+    import math
+
+    target = curve_b[str(frames_b)]
+    pts = sorted((int(n), v) for n, v in curve_a.items())
+    frames_a = None
+    for (n0, v0), (n1, v1) in zip(pts, pts[1:]):
+        if v0 >= target >= v1 and v0 > v1:
+            slope = (math.log(v1) - math.log(v0)) / (math.log(n1) - math.log(n0))
+            frames_a = n0 * math.exp((math.log(target) - math.log(v0)) / slope)
+            break
+    if frames_a is None and target < pts[-1][1]:
+        (n0, v0), (n1, v1) = pts[-2], pts[-1]
+        if v1 < v0:
+            slope = (math.log(v1) - math.log(v0)) / (math.log(n1) - math.log(n0))
+            frames_a = n1 * math.exp((math.log(target) - math.log(v1)) / slope)
+    if frames_a is None:
+        return None
+    return {"rmse": target, "frames": frames_b, "ms": round(frames_b * ms_b, 2), "plain_frames": round(frames_a, 2),
+            "plain_ms": round(frames_a * ms_a, 2), "speedup": round(frames_a * ms_a / (frames_b * ms_b), 3)}
+
+
+def instanced(frames, sequences):
     """The instanced scene (64 displaced icosphere/torus meshes of 5k-200k
     triangles, 12 017 instances = 597 M instanced triangles, 4096 emissive)
-    at 3840x2160, 1 spp, depth 12, ReSTIR DI: `frames` timed frames plain and
-    `frames` with the radiance cache (2^16 training samples and 4 Adam steps
-    per frame, after 8 warm-up frames)."""
+    at 3840x2160, 1 spp, depth 12, ReSTIR DI: frame time of `frames` frames
+    back to back plain and with the radiance cache (2^16 training samples and
+    4 Adam steps per frame, after 8 warm-up frames), and RMSE against the FP64
+    lattice fixture of the mean of the first n frames of a sequence, the
+    squared error averaged over `sequences` sequences, for n = 1, 2, 4 ...
+    `frames`: one sequence's RMSE is dominated by whichever rare bright
+    paths it happened to draw."""
     from tinytrace_amd import Config, LightSampling, Renderer, RenderSettings, _abi, scenes
     from tinytrace_amd.cache import RadianceCache, render_cached
 
@@ -515,15 +548,18 @@ def instanced(frames):
         renderer.target.download()
 
     def timed(step):
+        # Frames back to back (they overlap on the render lanes); the ray
+        # counts are summed over them and read once at the end.
         sync()
+        renderer.stats()
         t = time.perf_counter()
-        rays = 0
         for _ in range(frames):
             step()
-            st = renderer.stats()
-            rays += st["extension_rays"] + st["shadow_rays"] + st["restir_rays"]
         sync()
-        return (time.perf_counter() - t) / frames * 1e3, rays / frames
+        elapsed = time.perf_counter() - t
+        st = renderer.stats()
+        rays = st["extension_rays"] + st["shadow_rays"] + st["restir_rays"]
+        return elapsed / frames * 1e3, rays / frames
 
     renderer.render_to_target()
     plain_ms, plain_rays = timed(renderer.render_to_target)
@@ -537,45 +573,59 @@ def instanced(frames):
     cached_ms, cached_rays = timed(cached_step)
     stats = render_cached(renderer, cache)
 
-    # RMSE of 1 and 4 (...) independent frames, plain and cached, against the
-    # FP64 fixture (4096-spp double-precision oracle on a lattice).
     import numpy as np
 
     ref_path = os.path.join(ROOT, "tests", "golden", "fp64_instanced_2160p.npz")
     with np.load(ref_path, allow_pickle=False) as f:
         pixels, reference, ref_spp = f["pixels"], f["radiance"], int(f["spp"])
-
-    def rmse_curve(render):
-        samples = []
-        renderer.frame_index = 100000
-        for _ in range(frames):
-            renderer.reset_accumulation()
-            render()
-            samples.append(renderer.target.download().reshape(-1, 4)[pixels, :3].astype(np.float64))
-        samples = np.array(samples)
-        curve, n = {}, 1
-        while n <= frames:
-            err = samples[:n].mean(0) - reference
-            curve[str(n)] = round(float(np.sqrt((err * err).mean())), 5)
-            n *= 4
-        return curve
+    def rmse_curves(render, length):
+        """(RMSE from the MSE averaged over the sequences, median over the
+        sequences of their RMSE) of the mean of the first n frames."""
+        mse = {n: [] for n in (1, 2, 4, 8, 16, 32, 64, 128, 256) if n <= length}
+        for q in range(sequences):
+            renderer.frame_index = 100000 + 1000 * q
+            total = 0.0
+            for i in range(length):
+                renderer.reset_accumulation()
+                render()
+                total = total + renderer.target.download().reshape(-1, 4)[pixels, :3].astype(np.float64)
+                if i + 1 in mse:
+                    err = total / (i + 1) - reference
+                    mse[i + 1].append(float((err * err).mean()))
+        mean = {str(n): round(math.sqrt(sum(v) / len(v)), 5) for n, v in mse.items()}
+        median = {str(n): round(float(np.median(np.sqrt(v))), 5) for n, v in mse.items()}
+        return mean, median
 
     renderer.settings = dataclasses.replace(settings, collect_stats=False)
-    plain_rmse = rmse_curve(renderer.render_to_target)
-    cached_rmse = rmse_curve(cached_step)
+    # Plain sequences run 4x longer, so the frames plain needs to reach the
+    # cached RMSE are measured rather than extrapolated.
+    plain_rmse, plain_median = rmse_curves(renderer.render_to_target, 4 * frames)
+    cached_rmse, cached_median = rmse_curves(cached_step, frames)
     cache.close()
     paths = width * height
+    # Primary: the median curves (a rare path of 100x the pixel's radiance in
+    # one sequence sets the mean of a few sequences); the mean curves beside.
+    equal = {n: equal_rmse(plain_median, plain_ms, cached_median, cached_ms, int(n)) for n in cached_median
+             if int(n) >= 4}
+    equal_mean = {n: equal_rmse(plain_rmse, plain_ms, cached_rmse, cached_ms, int(n)) for n in cached_rmse
+                  if int(n) >= 4}
     return {"workload": f"instanced field seed 0 ({info['unique_triangle_count']} unique / "
                         f"{info['triangle_count']} instanced triangles), {width}x{height}, 1 spp, depth {depth}, "
                         "ReSTIR DI (32 candidates, 1 temporal + 1 spatial reuse), PCG seed 0",
             "scene_build_s": round(build_s, 3), "bvh_nodes": info["node_count"],
             "reference": f"FP64 oracle, {ref_spp} spp next-event estimation, {pixels.size}-pixel lattice (stride 64)",
+            "rmse_method": f"lattice RMSE of the mean of the first n frames of {sequences} sequences "
+                           f"({4 * frames} plain and {frames} cached frames each): rmse from the MSE averaged over "
+                           "the sequences, rmse_median the median of their RMSE; equal_rmse from the median curves",
             "plain": {"ms_per_frame": round(plain_ms, 3), "mpaths_per_s": round(paths / plain_ms / 1e3, 2),
-                      "mrays_per_s": round(plain_rays / plain_ms / 1e3, 2), "rmse": plain_rmse},
+                      "mrays_per_s": round(plain_rays / plain_ms / 1e3, 2), "rmse": plain_rmse,
+                      "rmse_median": plain_median},
             "cached": {"ms_per_frame": round(cached_ms, 3), "mpaths_per_s": round(paths / cached_ms / 1e3, 2),
                        "mrays_per_s": round(cached_rays / cached_ms / 1e3, 2), "rmse": cached_rmse,
+                       "rmse_median": cached_median,
                        "queries_per_frame": stats["queries"], "training_samples_per_frame": stats["samples"],
-                       "judge": judge(stats), "mlp": mlp_roofline(stats)}}
+                       "judge": judge(stats), "mlp": mlp_roofline(stats)},
+            "equal_rmse": equal, "equal_rmse_of_mean_curves": equal_mean}
 
 
 def serving(context, requests, rate):
@@ -643,31 +693,59 @@ def served_instanced(requests, rate):
     resolutions (256^2 .. 3840x2160) and passes (1 .. 64) over the resident
     instanced scene, depth 12, ReSTIR DI (32 candidates, temporal + spatial
     reuse) with each request's reservoirs resident across its passes,
-    cameras on a sphere of radius 180 around the scene's centre."""
+    cameras on a sphere of radius 180 around the scene's centre; the same
+    arrivals served plainly and with the radiance cache (trained online by
+    the batches, after four warm-up requests)."""
     import numpy as np
 
     from tinytrace_amd import Config, Context, DeviceScene, LightSampling, scenes
+    from tinytrace_amd.cache import RadianceCache
     from tinytrace_amd.serving import Server, orbit_camera, request_mix, serve_poisson
 
     context = Context(0)
     scene, _ = scenes.instanced_field(seed=0)
     resident = DeviceScene(context, scene)
-    server = Server(context, pool_paths=3840 * 2160)
-    slot = server.add_scene(resident)
-    specs = request_mix(requests, rate_per_s=rate, scenes=1, seed=2)
-    rng = np.random.default_rng(3)
-    for sp in specs:
-        sp.camera = orbit_camera(rng, center=(100.0, 100.0, 100.0), radius=180.0, fov=1.0)
     config = Config(bounce_count=12, sample_count=1, light_sampling=LightSampling.RESTIR)
-    out = serve_poisson(server, [slot], specs, config)
-    stats = server.stats()
-    server.close()
-    out.pop("latency_ms")
+
+    def run(cache):
+        server = Server(context, pool_paths=3840 * 2160)
+        slot = server.add_scene(resident)
+        if cache is not None:
+            server.set_cache(slot, cache)
+        specs = request_mix(requests, rate_per_s=rate, scenes=1, seed=2)
+        rng = np.random.default_rng(3)
+        for sp in specs:
+            sp.camera = orbit_camera(rng, center=(100.0, 100.0, 100.0), radius=180.0, fov=1.0)
+        out = serve_poisson(server, [slot], specs, config)
+        stats = server.stats()
+        server.close()
+        out.pop("latency_ms")
+        out = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in out.items()}
+        out["budget_used_gib"] = round(stats["budget_used_bytes"] / 2 ** 30, 2)
+        if cache is not None:
+            out["cache_queries"] = stats["cache_queries"]
+            out["cache_samples"] = stats["cache_samples"]
+            out["judge"] = {"samples": stats["judge_samples"], "traced": round(stats["judge_traced"], 5),
+                            "cached": round(stats["judge_cached"], 5)}
+        return out
+
+    plain = run(None)
+    cache = RadianceCache(context, seed=0)
+    warm = Server(context, pool_paths=3840 * 2160)
+    slot = warm.add_scene(resident)
+    warm.set_cache(slot, cache)
+    rng = np.random.default_rng(11)
+    ids = [warm.submit(slot, orbit_camera(rng, center=(100.0, 100.0, 100.0), radius=180.0, fov=1.0), config,
+                       1920, 1080, 4, seed=100 + i) for i in range(4)]
+    warm.run_until_done(ids)
+    warm.close()
+    cached = run(cache)
+    cache.close()
     return {"workload": f"{requests} requests, Poisson {rate}/s, instanced field seed 0 resident (597 M instanced "
                         "triangles), depth 12, ReSTIR DI with per-request reservoirs resident, resolution mix "
-                        "256^2/512^2/1080p/2160p (.4/.3/.2/.1), spp 1/4/16/64, pool 3840x2160 paths",
-            "budget_used_gib": round(stats["budget_used_bytes"] / 2 ** 30, 2),
-            **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in out.items()}}
+                        "256^2/512^2/1080p/2160p (.4/.3/.2/.1), spp 1/4/16/64, pool 3840x2160 paths; the same "
+                        "arrivals plainly and with the radiance cache (2^16 training samples and 4 Adam steps a batch)",
+            "plain": plain, "cached": cached}
 
 
 def quality(renderer, settings, restir, frames, step, sync, steps):

@@ -358,6 +358,11 @@ pub struct tt_server_stats {
     pub done: u64,
     pub budget_used_bytes: u64,
     pub busy_ms: f64,
+    pub cache_queries: u64,
+    pub cache_samples: u64,
+    pub judge_samples: u64,
+    pub judge_traced: f64,
+    pub judge_cached: f64,
 }
 
 #[repr(C)]
@@ -539,6 +544,7 @@ extern "C" {
     pub fn tt_mlp_params(mlp: *mut tt_mlp, get: *mut f32, set: *const f32, count: u32) -> tt_status;
     pub fn tt_mlp_run(mlp: *mut tt_mlp, op: c_int, records: *const f32, targets: *const f32, n: u32, out: *mut f32, loss: *mut f64) -> tt_status;
     pub fn tt_render_cached(context: *mut tt_context, scene: *const tt_scene, constants: *const tt_constants, settings: *const tt_render_settings, cache: *mut tt_mlp, cache_settings: *const tt_cache_settings, target: *mut tt_target, stats: *mut tt_cache_stats) -> tt_status;
+    pub fn tt_server_set_cache(server: *mut tt_server, scene_slot: u32, cache: *mut tt_mlp, settings: *const tt_cache_settings) -> tt_status;
 
     pub fn tt_trace_closest(context: *mut tt_context, scene: *const tt_scene, rays: *const tt_ray, hits: *mut tt_hit, count: u32, flags: u32) -> tt_status;
     pub fn tt_trace_closest_device(context: *mut tt_context, scene: *const tt_scene, device_rays: *const tt_ray, device_hits: *mut tt_hit, count: u32, flags: u32) -> tt_status;

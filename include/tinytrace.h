@@ -423,6 +423,15 @@ typedef struct tt_server_stats {
     uint64_t queued, active, done;
     uint64_t budget_used_bytes;
     double busy_ms;          /* wall time with at least one batch in flight */
+    /* Batches on scenes with a radiance cache (tt_server_set_cache): paths
+     * ended at a cache query, training samples trained on, judge paths and
+     * their mean traced radiance and the cache's mean prediction at their
+     * cache vertices (per channel), as tt_cache_stats, summed over batches. */
+    uint64_t cache_queries;
+    uint64_t cache_samples;
+    uint64_t judge_samples;
+    double judge_traced;
+    double judge_cached;
 } tt_server_stats;
 
 tt_status tt_server_create(tt_context* context, const tt_server_desc* desc, tt_server** out);
@@ -516,6 +525,18 @@ typedef struct tt_cache_stats {
 tt_status tt_render_cached(tt_context* context, const tt_scene* scene, const tt_constants* constants,
                            const tt_render_settings* settings, tt_mlp* cache, const tt_cache_settings* cache_settings,
                            tt_target* target, tt_cache_stats* stats);
+
+/* Serves the requests on `scene_slot` with the radiance cache, as
+ * tt_render_cached renders: every batch of them ends its paths at cache
+ * queries, infers them with the weights of the last training enqueued,
+ * then trains the cache on its training paths (`settings` per batch as per
+ * frame; null: the defaults); judge statistics go to tt_server_stats.  A
+ * null cache serves the slot plainly again.  The cache must belong to the
+ * server's context and stay alive while requests on the slot are served.
+ * A cached request's image depends on what the cache learnt from the batches
+ * before, so it is no longer the same bits alone and in a mix. */
+tt_status tt_server_set_cache(tt_server* server, uint32_t scene_slot, tt_mlp* cache,
+                              const tt_cache_settings* settings);
 
 /* ---------------------------------------------------------------------- */
 /* Test entry points                                                       */

@@ -168,3 +168,118 @@ def test_restir_request_refusals():
     with pytest.raises(_abi.TinytraceError) as e:
         server.submit(tslot, tcam, NEE, 32, 32, 1)
     assert e.value.status == 8  # TT_UNSUPPORTED
+
+
+# ------------------------------------------------------ radiance cache --
+def _trained_cornell_cache(renderer, camera, frames=12):
+    """A radiance cache trained by cached renders of the Cornell box."""
+    from tinytrace_amd.cache import RadianceCache, render_cached
+
+    renderer.camera = camera
+    renderer.set_config(Config(bounce_count=4, sample_count=1, light_sampling=LightSampling.NEXT_EVENT_ESTIMATION))
+    cache = RadianceCache(renderer.context, seed=3)
+    for _ in range(frames):
+        render_cached(renderer, cache, stats=False)
+    renderer.target.download()
+    return cache
+
+
+def test_served_with_cache_queries_trains_and_stays_close_to_plain():
+    """Requests on a scene slot with a radiance cache end their paths at cache
+    queries, train the cache batch by batch, report judge statistics, and
+    keep the image's mean radiance near the plainly served one's."""
+    scene, camera = scenes.cornell_box()
+    renderer = Renderer((256, 256), settings=RenderSettings(rng=_abi.TT_RNG_PCG, seed=1))
+    renderer.set_scene(scene)
+    cache = _trained_cornell_cache(renderer, camera)
+    before = cache.params
+    config = Config(bounce_count=4, sample_count=1, light_sampling=LightSampling.NEXT_EVENT_ESTIMATION)
+    rng = np.random.default_rng(2)
+    cams = [camera, orbit_camera(rng), orbit_camera(rng)]
+    sizes = [(256, 256, 6), (160, 120, 4), (96, 96, 8)]
+
+    def serve(with_cache):
+        server = Server(renderer.context, pool_paths=1 << 17)
+        slot = server.add_scene(renderer.scene)
+        if with_cache:
+            server.set_cache(slot, cache, train_samples=8192, train_steps=2)
+        ids = [server.submit(slot, cams[i], config, w, h, p, seed=5 + i) for i, (w, h, p) in enumerate(sizes)]
+        server.run_until_done(ids)
+        images = [server.take_result(i) for i in ids]
+        return images, server.stats()
+
+    plain, plain_stats = serve(False)
+    cached, stats = serve(True)
+    assert plain_stats["cache_queries"] == 0 and plain_stats["judge_samples"] == 0
+    assert stats["cache_queries"] > 0 and stats["cache_samples"] > 0 and stats["judge_samples"] > 0
+    assert abs(stats["judge_cached"] - stats["judge_traced"]) < 0.2 * stats["judge_traced"]
+    assert not np.array_equal(cache.params, before)  # the batches trained it
+    for (w, h, p), a, b in zip(sizes, cached, plain):
+        assert np.isfinite(a).all() and a.shape == (h, w, 4)
+        ma, mb = a[..., :3].mean(), b[..., :3].mean()
+        assert abs(ma - mb) < 0.1 * mb, (w, h, p, ma, mb)
+    cache.close()
+
+
+def test_served_restir_with_cache_on_instanced_scene():
+    """ReSTIR requests on the instanced field served with the cache (the
+    instanced workload as specified: reservoirs resident, paths ended at the
+    cache): finite images near the plainly served mean, queries made."""
+    from tinytrace_amd.cache import RadianceCache
+
+    scene, camera = scenes.instanced_field(seed=0)
+    renderer = Renderer((8, 8), settings=RenderSettings(rng=_abi.TT_RNG_PCG, seed=1, restir_candidates=32))
+    renderer.set_scene(scene)
+    config = Config(bounce_count=12, sample_count=1, light_sampling=LightSampling.RESTIR)
+    cache = RadianceCache(renderer.context, seed=0)
+
+    def serve(with_cache, passes):
+        server = Server(renderer.context, pool_paths=1 << 18)
+        slot = server.add_scene(renderer.scene)
+        if with_cache:
+            server.set_cache(slot, cache)
+        ids = [server.submit(slot, camera, config, 320, 180, passes, seed=9, restir_candidates=32),
+               server.submit(slot, camera, config, 256, 256, passes, seed=4, restir_candidates=32)]
+        server.run_until_done(ids)
+        return [server.take_result(i) for i in ids], server.stats()
+
+    serve(True, 24)  # trains the cache
+    cached, stats = serve(True, 8)
+    plain, _ = serve(False, 8)
+    assert stats["cache_queries"] > 0 and stats["judge_samples"] > 0
+    for a, b in zip(cached, plain):
+        assert np.isfinite(a).all()
+        # Robust statistics: a plainly traced pixel can carry a path whose
+        # BSDF value over its floored pdf is 1e10 times the pixel's radiance
+        # (near-specular lobes; the cache ends such paths).
+        la, lb = a[..., :3].sum(-1).ravel(), b[..., :3].sum(-1).ravel()
+        assert abs(np.median(la) - np.median(lb)) < 0.1 * np.median(lb)
+        ta = np.sort(la)[: int(0.999 * la.size)].mean()
+        tb = np.sort(lb)[: int(0.999 * lb.size)].mean()
+        assert abs(ta - tb) < 0.15 * tb, (ta, tb)
+    cache.close()
+
+
+def test_set_cache_refusals():
+    from tinytrace_amd import Context, TinytraceError
+    from tinytrace_amd.cache import RadianceCache
+
+    scene, _ = scenes.cornell_box()
+    renderer = Renderer((8, 8))
+    renderer.set_scene(scene)
+    server = Server(renderer.context)
+    slot = server.add_scene(renderer.scene)
+    cache = RadianceCache(renderer.context, seed=0)
+    with pytest.raises(TinytraceError):
+        server.set_cache(slot + 1, cache)  # unknown slot
+    with pytest.raises(TinytraceError):
+        server.set_cache(slot, cache, train_steps=65)
+    other = Context(0)
+    foreign = RadianceCache(other, seed=0)
+    with pytest.raises(TinytraceError):
+        server.set_cache(slot, foreign)
+    server.set_cache(slot, cache)
+    server.set_cache(slot, None)  # plain again
+    foreign.close()
+    other.close()
+    cache.close()

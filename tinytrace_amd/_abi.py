@@ -157,7 +157,9 @@ class RequestStatus(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("batches", C.c_uint64), ("paths", C.c_uint64), ("queued", C.c_uint64), ("active", C.c_uint64),
-                ("done", C.c_uint64), ("budget_used_bytes", C.c_uint64), ("busy_ms", C.c_double)]
+                ("done", C.c_uint64), ("budget_used_bytes", C.c_uint64), ("busy_ms", C.c_double),
+                ("cache_queries", C.c_uint64), ("cache_samples", C.c_uint64), ("judge_samples", C.c_uint64),
+                ("judge_traced", C.c_double), ("judge_cached", C.c_double)]
 
 
 TT_REQUEST_QUEUED, TT_REQUEST_ADMITTED, TT_REQUEST_DONE = 0, 1, 2
@@ -243,6 +245,7 @@ _PROTOTYPES = [
                              C.POINTER(C.c_double)]),
     ("tt_render_cached", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Constants), C.POINTER(RenderSettings),
                                    C.c_void_p, C.POINTER(CacheSettings), C.c_void_p, C.POINTER(CacheStats)]),
+    ("tt_server_set_cache", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(CacheSettings)]),
     ("tt_trace_closest", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     ("tt_trace_closest_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     ("tt_trace_closest_static", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),

@@ -211,6 +211,41 @@ tt_status ensure_cache(tt_context* c, uint32_t lane, size_t paths, uint32_t trai
     return TT_OK;
 }
 
+tt_status cache_plan(const tt_cache_settings* cs, uint32_t paths, CachePlan& out, const char** why) {
+    const uint32_t train_samples = cs && cs->train_samples ? cs->train_samples : 65536u;
+    out.train_steps = cs && cs->train_steps ? cs->train_steps : 4u;
+    if (out.train_steps > 64) {
+        *why = "train_steps > 64";
+        return TT_INVALID_ARGUMENT;
+    }
+    out.records = cs && cs->records_per_path ? cs->records_per_path : 1u;
+    if (out.records > CACHE_OPEN_RECORDS) {
+        *why = "records_per_path > 3";
+        return TT_INVALID_ARGUMENT;
+    }
+    float judge = cs && cs->judge_fraction != 0.0f ? cs->judge_fraction : 0.01f;
+    if (!(judge > 0.0f)) judge = 0.0f;  // negative (or NaN): no judge paths
+    judge = std::min(judge, 1.0f);
+    // A training path records up to `records` samples.
+    const double train_frac = std::min(1.0, double(train_samples) / (double(out.records) * double(paths)));
+    // Record capacities: the expected counts with a margin (records past
+    // them are dropped).
+    out.train_capacity = uint32_t(std::min<uint64_t>(uint64_t(train_samples) + train_samples / 4 + 1024,
+                                                      uint64_t(out.records) * paths));
+    out.judge_capacity = judge > 0.0f ? uint32_t(std::min<double>(double(paths), 1.25 * judge * paths + 1024.0)) : 0u;
+    auto threshold = [](double x) { return x >= 1.0 ? 0xffffffffu : uint32_t(x * 4294967296.0); };
+    out.train_threshold = threshold(train_frac);
+    out.judge_threshold = threshold(train_frac + judge);
+    return TT_OK;
+}
+
+void apply_cache_plan(const CachePlan& plan, FrameArgs& f) {
+    f.cache_mode = 1;
+    f.cache_records = plan.records;
+    f.train_threshold = plan.train_threshold;
+    f.judge_threshold = plan.judge_threshold;
+}
+
 FrameArgs frame_args(const tt_constants& k, uint32_t width, uint32_t height, uint32_t rng, uint32_t seed) {
     FrameArgs f;
     std::memcpy(f.inverse_view, k.inverse_view, sizeof(f.inverse_view));
@@ -704,27 +739,13 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     uint32_t train_steps = 0, train_capacity = 0, judge_capacity = 0;
     if (cache) {
         if (tt::mlp_context(cache) != c) return fail(c, TT_STALE_HANDLE, "cache from another context");
-        const tt_cache_settings* cs = cache_settings;
-        const uint32_t train_samples = cs && cs->train_samples ? cs->train_samples : 65536u;
-        train_steps = cs && cs->train_steps ? cs->train_steps : 4u;
-        if (train_steps > 64) return fail(c, TT_INVALID_ARGUMENT, "train_steps > 64");
-        const uint32_t records = cs && cs->records_per_path ? cs->records_per_path : 1u;
-        if (records > tt::CACHE_OPEN_RECORDS) return fail(c, TT_INVALID_ARGUMENT, "records_per_path > 3");
-        float judge = cs && cs->judge_fraction != 0.0f ? cs->judge_fraction : 0.01f;
-        if (!(judge > 0.0f)) judge = 0.0f;  // negative (or NaN): no judge paths
-        judge = std::min(judge, 1.0f);
-        // A training path records up to `records` samples.
-        const double train_frac = std::min(1.0, double(train_samples) / (double(records) * double(paths)));
-        // Record capacities: the expected counts with a margin (records past
-        // them are dropped).
-        train_capacity = uint32_t(std::min<uint64_t>(uint64_t(train_samples) + train_samples / 4 + 1024,
-                                                     uint64_t(records) * paths));
-        judge_capacity = judge > 0.0f ? uint32_t(std::min<double>(double(paths), 1.25 * judge * paths + 1024.0)) : 0u;
-        auto threshold = [](double x) { return x >= 1.0 ? 0xffffffffu : uint32_t(x * 4294967296.0); };
-        f.cache_mode = 1;
-        f.cache_records = records;
-        f.train_threshold = threshold(train_frac);
-        f.judge_threshold = threshold(train_frac + judge);
+        tt::CachePlan plan;
+        const char* why = nullptr;
+        if (tt::cache_plan(cache_settings, paths, plan, &why) != TT_OK) return fail(c, TT_INVALID_ARGUMENT, why);
+        train_steps = plan.train_steps;
+        train_capacity = plan.train_capacity;
+        judge_capacity = plan.judge_capacity;
+        tt::apply_cache_plan(plan, f);
         tt_status cst = tt::ensure_cache(c, lane, paths, train_capacity, judge_capacity, b.cache);
         if (cst != TT_OK) return cst;
         cst = tt::mlp_set_scene_bounds(cache, s->bounds_lo, s->bounds_hi);

@@ -170,6 +170,19 @@ inline tt_status hip_fail(tt_context* c, hipError_t e, const char* what) {
 void context_retain(tt_context* c);
 void context_release(tt_context* c);
 
+// A cached frame's shares over `paths` paths from tt_cache_settings (null
+// or zero fields: the defaults): training and judge thresholds on a path's
+// hash, record capacities with a margin, Adam steps.
+struct CachePlan {
+    uint32_t records;  // training records a training path takes
+    uint32_t train_steps;
+    uint32_t train_capacity, judge_capacity;
+    uint32_t train_threshold, judge_threshold;
+};
+// TT_INVALID_ARGUMENT with *why set for settings out of range.
+tt_status cache_plan(const tt_cache_settings* settings, uint32_t paths, CachePlan& out, const char** why);
+// Turns on the cache in f with the plan's shares.
+void apply_cache_plan(const CachePlan& plan, FrameArgs& f);
 // Radiance-cache queues of a lane for `paths` paths, `train` training and
 // `judge` judge records (grown on demand, kept in the lane).
 tt_status ensure_cache(tt_context* c, uint32_t lane, size_t paths, uint32_t train, uint32_t judge, CacheBuffers& cb);

@@ -94,6 +94,15 @@ struct Batch {
     uint32_t lane;
     std::vector<Chunk> chunks;
     uint32_t paths;
+    bool cached;              // ran with a radiance cache: its counts are read when it ends
+    uint32_t train_capacity;  // its training record slots
+    bool judged;              // with judge records
+};
+
+// The radiance cache serving a scene slot.
+struct CacheSlot {
+    tt_mlp* cache = nullptr;
+    tt_cache_settings settings{};
 };
 
 // The path buffers a lane's shadow pass reads: the batch's, with the side
@@ -128,9 +137,11 @@ struct tt_server {
     uint32_t depth = 1;      // batches in flight, each with its own path pool (lanes 0 .. depth - 1)
     uint32_t next_slot = 0;  // lane of the next batch
     std::deque<Batch> inflight;
+    std::map<uint32_t, CacheSlot> caches;  // scene slot -> its radiance cache
     std::chrono::steady_clock::time_point busy_since;  // first batch of the current busy period
     std::chrono::steady_clock::time_point origin;
     tt_server_stats stats{};
+    double judge_traced_sum = 0.0, judge_cached_sum = 0.0;  // over judge records, per channel
 
     uint64_t now_ns() const {
         return uint64_t(std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - origin)
@@ -330,6 +341,27 @@ tt_status tt_server_add_scene(tt_server* sv, const tt_scene* s, uint32_t* slot) 
     return tt::exception_status(sv && sv->context ? &sv->context->error : nullptr);
 }
 
+tt_status tt_server_set_cache(tt_server* sv, uint32_t slot, tt_mlp* cache, const tt_cache_settings* settings) try {
+    if (!sv) return TT_INVALID_ARGUMENT;
+    tt_context* c = sv->context;
+    if (slot >= sv->scenes.size()) return fail(c, TT_INVALID_ARGUMENT, "unknown scene slot");
+    if (!cache) {
+        sv->caches.erase(slot);
+        return TT_OK;
+    }
+    if (tt::mlp_context(cache) != c) return fail(c, TT_STALE_HANDLE, "cache from another context");
+    CacheSlot cs;
+    cs.cache = cache;
+    if (settings) cs.settings = *settings;
+    tt::CachePlan plan;
+    const char* why = nullptr;
+    if (tt::cache_plan(&cs.settings, sv->pool_paths, plan, &why) != TT_OK) return fail(c, TT_INVALID_ARGUMENT, why);
+    sv->caches[slot] = cs;
+    return TT_OK;
+} catch (...) {
+    return tt::exception_status(sv && sv->context ? &sv->context->error : nullptr);
+}
+
 tt_status tt_server_submit(tt_server* sv, const tt_server_request* q, uint64_t* id) try {
     if (!sv || !q || !id) return fail(sv ? sv->context : nullptr, TT_INVALID_ARGUMENT, "null argument");
     tt_context* c = sv->context;
@@ -392,6 +424,21 @@ tt_status enqueue_batch(tt_server* sv, bool& formed) {
     tt_status st = tt::ensure_paths(c, lane, sv->pool_paths, b);
     if (st != TT_OK) return st;
     tt::Lane& L = c->lanes[lane];
+    // The radiance cache of the group's scene: queues sized for a full pool
+    // before any request's cursor moves.
+    const tt_scene* scene = sv->scenes[lead->desc.scene];
+    auto cache_it = sv->caches.find(lead->desc.scene);
+    tt_mlp* cache = cache_it != sv->caches.end() ? cache_it->second.cache : nullptr;
+    const tt_cache_settings* cache_settings = cache ? &cache_it->second.settings : nullptr;
+    if (cache) {
+        tt::CachePlan full;
+        const char* why = nullptr;
+        if (tt::cache_plan(cache_settings, sv->pool_paths, full, &why) != TT_OK) return fail(c, TT_INVALID_ARGUMENT, why);
+        st = tt::ensure_cache(c, lane, sv->pool_paths, full.train_capacity, full.judge_capacity, b.cache);
+        if (st != TT_OK) return st;
+        st = tt::mlp_set_scene_bounds(cache, scene->bounds_lo, scene->bounds_hi);
+        if (st != TT_OK) return st;
+    }
     const bool restir = lead->is_restir();
     if (restir && L.restir_slot_capacity < sv->pool_paths) {
         if (L.restir_slot_memory) {
@@ -410,8 +457,13 @@ tt_status enqueue_batch(tt_server* sv, bool& formed) {
     for (Request* r : group) take(sv, r, uint32_t(std::min<uint64_t>(share, remaining_paths(r))), base, chunks);
     for (Request* r : group) take(sv, r, sv->pool_paths - base, base, chunks);
     const uint32_t paths = base;
+    // This batch's shares: capacities no larger than the full pool's.
+    tt::CachePlan plan{};
+    if (cache) {
+        const char* why = nullptr;
+        (void)tt::cache_plan(cache_settings, paths, plan, &why);
+    }
 
-    const tt_scene* scene = sv->scenes[lead->desc.scene];
     const tt_constants& lk = lead->desc.constants;
     b.max_bounces = lk.bounce_count;
     auto chunk_args = [&](const Chunk& ch) {
@@ -444,6 +496,7 @@ tt_status enqueue_batch(tt_server* sv, bool& formed) {
     for (const Chunk& ch : chunks)
         TT_HIP(c, tt::launch_generate_chunk(stream, chunk_args(ch), b, ch.base, ch.pixel0, ch.count));
     tt::FrameArgs f = chunk_args(chunks.front());  // configuration fields are the group's
+    if (cache) tt::apply_cache_plan(plan, f);
     TT_HIP(c, tt::launch_extend(stream, scene->view, b, 0, c->grid));
     if (restir && scene->view.emissive_count > 0) {
         // Each chunk is a whole pass: its ReSTIR passes over its request's
@@ -487,11 +540,33 @@ tt_status enqueue_batch(tt_server* sv, bool& formed) {
         if (bounce + 1 < lk.bounce_count) TT_HIP(c, tt::launch_extend(stream, scene->view, b, bounce + 1, c->grid));
     }
     if (shadows && side != stream) TT_HIP(c, hipStreamWaitEvent(stream, L.join, 0));
+    if (cache) {
+        // As a cached render: the queries' radiance, and the cache against
+        // the judge records, with the weights of the last training enqueued.
+        st = tt::mlp_infer_device(cache, stream, b.cache.query_rec, b.cache.query_scale, L.counters + tt::C_CACHE_QUERY,
+                                  paths, nullptr, b.radiance);
+        if (st != TT_OK) return st;
+        if (plan.judge_capacity) {
+            st = tt::mlp_judge_device(cache, stream, b.cache.train_rec + 4 * size_t(b.cache.judge_base),
+                                      b.cache.train_target + b.cache.judge_base, L.counters + tt::C_CACHE_JUDGE,
+                                      plan.judge_capacity, L.judge_pred, L.judge_stats);
+            if (st != TT_OK) return st;
+        }
+    }
     if (other && other->used) TT_HIP(c, hipStreamWaitEvent(stream, other->accumulated, 0));
     for (const Chunk& ch : chunks)
         TT_HIP(c, tt::launch_accumulate_chunk(stream, chunk_args(ch), b, ch.base, ch.pixel0, ch.count,
                                               static_cast<float4*>(ch.request->target.ptr)));
     TT_HIP(c, hipEventRecord(L.accumulated, stream));
+    if (cache) {
+        // Train on the batch's records; the lane's queues are free again once
+        // training has read them, so the lane's end waits for it.
+        st = tt::mlp_train_device(cache, stream, b.cache.train_rec, b.cache.train_target, L.counters + tt::C_CACHE_TRAIN,
+                                  plan.train_capacity, plan.train_steps, true, nullptr);
+        if (st != TT_OK) return st;
+        st = tt::mlp_wait_trained(cache, stream);
+        if (st != TT_OK) return st;
+    }
     TT_HIP(c, hipEventRecord(L.done, stream));
     L.used = true;
     L.pending = true;
@@ -499,7 +574,8 @@ tt_status enqueue_batch(tt_server* sv, bool& formed) {
     c->next_lane = (lane + 1u) % c->lane_count;
     sv->next_slot = (lane + 1u) % sv->depth;
     if (sv->inflight.empty()) sv->busy_since = t0;
-    sv->inflight.push_back(Batch{lane, std::move(chunks), paths});
+    sv->inflight.push_back(Batch{lane, std::move(chunks), paths, cache != nullptr, plan.train_capacity,
+                                 cache != nullptr && plan.judge_capacity > 0});
     formed = true;
     return TT_OK;
 }
@@ -513,6 +589,21 @@ tt_status finish_batch(tt_server* sv, uint32_t& done) {
     TT_HIP(c, hipEventSynchronize(c->lanes[batch.lane].done));
     sv->stats.batches++;
     sv->stats.paths += batch.paths;
+    if (batch.cached) {
+        // The lane is not reused before its batch is finished here.
+        const tt::Lane& L = c->lanes[batch.lane];
+        uint32_t counts[3] = {0u, 0u, 0u};
+        TT_HIP(c, hipMemcpy(counts, L.counters + tt::C_CACHE_QUERY, sizeof(counts), hipMemcpyDeviceToHost));
+        sv->stats.cache_queries += counts[0];
+        sv->stats.cache_samples += std::min(counts[1], batch.train_capacity);
+        if (batch.judged) {
+            double js[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            TT_HIP(c, hipMemcpy(js, L.judge_stats, sizeof(js), hipMemcpyDeviceToHost));
+            sv->stats.judge_samples += uint64_t(js[0]);
+            sv->judge_cached_sum += js[1] / 3.0;
+            sv->judge_traced_sum += js[2] / 3.0;
+        }
+    }
     if (sv->inflight.empty())  // the GPU idles from here until the next batch
         sv->stats.busy_ms +=
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - sv->busy_since).count();
@@ -610,6 +701,10 @@ tt_status tt_server_stats_get(tt_server* sv, tt_server_stats* out) try {
         if (kv.second->state == TT_REQUEST_DONE) done++;
     out->done = done;
     out->budget_used_bytes = sv->used + sv->spare_bytes + sv->pool_bytes();
+    if (sv->stats.judge_samples > 0) {
+        out->judge_traced = sv->judge_traced_sum / double(sv->stats.judge_samples);
+        out->judge_cached = sv->judge_cached_sum / double(sv->stats.judge_samples);
+    }
     return TT_OK;
 } catch (...) {
     return tt::exception_status(sv && sv->context ? &sv->context->error : nullptr);

@@ -32,6 +32,7 @@ class Server:
         _abi.track(self, 0)
         self._scenes: List[DeviceScene] = []
         self._sizes: Dict[int, tuple] = {}
+        self._caches: Dict[int, object] = {}  # caches serving scene slots, kept alive with the server
 
     def close(self):
         if getattr(self, "handle", None):
@@ -47,6 +48,20 @@ class Server:
               "tt_server_add_scene")
         self._scenes.append(scene)  # keeps the scene alive while the server is
         return slot.value
+
+    def set_cache(self, scene: int, cache, train_samples: int = 0, train_steps: int = 0, judge_fraction: float = 0.0,
+                  records_per_path: int = 0) -> None:
+        """tt_server_set_cache: requests on scene slot `scene` are served
+        with the radiance cache `cache` (a cache.RadianceCache; None: plainly
+        again), which every batch of them trains (settings per batch as
+        render_cached's per frame; 0: the defaults)."""
+        settings = _abi.CacheSettings(train_samples, train_steps, judge_fraction, records_per_path)
+        check(lib().tt_server_set_cache(self.handle, scene, None if cache is None else cache.handle,
+                                        C.byref(settings)), self.context.handle, "tt_server_set_cache")
+        if cache is None:
+            self._caches.pop(scene, None)
+        else:
+            self._caches[scene] = cache
 
     def submit(self, scene: int, camera: Camera, config: Config, width: int, height: int, passes: int,
                seed: int = 0, rng: int = _abi.TT_RNG_PCG, restir_candidates: int = 32, restir_temporal: bool = True,
