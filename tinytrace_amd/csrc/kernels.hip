@@ -307,13 +307,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 
             accumulated = accumulated + emissive * throughput;
         } else if (sampled) {
             direct = throughput * contribution;
-            shadow_org = sf.position;
-            light_instance = lp.instance;
-            light_hash = lp.hash;
-            emit_shadow = true;
-            if (bounce == 0) {
-                emission_after = emissive * throughput;
-                has_emission_after = true;
+            if (direct.x == 0.0f && direct.y == 0.0f && direct.z == 0.0f) {
+                // Visibility only scales the contribution, so a zero one (the
+                // light's back faces the point, or the direction lies below
+                // the surface) needs no shadow ray: the result is the
+                // occluded one, and at the primary vertex that adds the
+                // deferred emission, as the shadow kernel would.
+                if (bounce == 0) accumulated = accumulated + emissive * throughput;
+            } else {
+                shadow_org = sf.position;
+                light_instance = lp.instance;
+                light_hash = lp.hash;
+                emit_shadow = true;
+                if (bounce == 0) {
+                    emission_after = emissive * throughput;
+                    has_emission_after = true;
+                }
             }
         } else if (bounce == 0 || !nee) {
             added = emissive * throughput;
