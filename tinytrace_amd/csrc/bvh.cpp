@@ -297,6 +297,66 @@ void build_blas(const tt_scene_desc& d, uint32_t mesh, Blas& out) {
         out.stats);
 }
 
+// Object-space boxes that together bound mesh m: its triangles binned by
+// centroid into a PART_GRID^3 grid over the mesh's box, one box per
+// non-empty cell around the vertices of the cell's triangles.  Every triangle
+// lies inside its own cell's box, so the union of the boxes' world images
+// bounds the instance; for a rotated mesh that union is much tighter than the
+// world box of the whole mesh's box (the axis-aligned box of a rotated cube
+// is up to sqrt(3) times wider on each axis).  Vertices are dequantized as the
+// BLAS does (v / 32767 in float).
+constexpr uint32_t PART_GRID = 4;
+
+std::vector<Box> part_boxes(const tt_scene_desc& d, uint32_t mesh) {
+    const tt_mesh_record& m = d.meshes[mesh];
+    const uint32_t tri_count = m.index_count / 3;
+    auto vertex = [&](uint32_t t, int c, float* v) {
+        uint32_t vi = m.vertex_offset + d.indices[m.index_offset + 3 * t + c];
+        for (int a = 0; a < 3; a++) v[a] = float(d.positions[3 * size_t(vi) + a]) / 32767.0f;
+    };
+    Box whole;
+    for (uint32_t t = 0; t < tri_count; t++)
+        for (int c = 0; c < 3; c++) {
+            float v[3];
+            vertex(t, c, v);
+            whole.grow(v);
+        }
+    std::vector<Box> cells(PART_GRID * PART_GRID * PART_GRID);
+    for (uint32_t t = 0; t < tri_count; t++) {
+        float v[3][3];
+        uint32_t cell = 0;
+        for (int c = 0; c < 3; c++) vertex(t, c, v[c]);
+        for (int a = 0; a < 3; a++) {
+            float lo = std::min(v[0][a], std::min(v[1][a], v[2][a]));
+            float hi = std::max(v[0][a], std::max(v[1][a], v[2][a]));
+            float ext = whole.hi[a] - whole.lo[a];
+            float u = ext > 0.0f ? (0.5f * (lo + hi) - whole.lo[a]) / ext : 0.0f;
+            uint32_t k = uint32_t(std::min(std::max(u * float(PART_GRID), 0.0f), float(PART_GRID - 1)));
+            cell = cell * PART_GRID + k;
+        }
+        for (int c = 0; c < 3; c++) cells[cell].grow(v[c]);
+    }
+    std::vector<Box> out;
+    for (const Box& b : cells)
+        if (b.valid()) out.push_back(b);
+    return out;
+}
+
+// Bounds of the image of object-space box b under the instance transform
+// (column-major 4x4), in double: centre M c + t, half-extent |M| h.
+void grow_world(const float* T, const Box& b, double lo[3], double hi[3]) {
+    for (int r = 0; r < 3; r++) {
+        double c = double(T[12 + r]), h = 0.0;
+        for (int k = 0; k < 3; k++) {
+            double m = double(T[k * 4 + r]);
+            c += m * 0.5 * (double(b.lo[k]) + double(b.hi[k]));
+            h += std::fabs(m) * 0.5 * (double(b.hi[k]) - double(b.lo[k]));
+        }
+        lo[r] = std::min(lo[r], c - h);
+        hi[r] = std::max(hi[r], c + h);
+    }
+}
+
 }  // namespace
 
 // TLAS over the instances, instance records, world boxes and the two-level
@@ -304,8 +364,27 @@ void build_blas(const tt_scene_desc& d, uint32_t mesh, Blas& out) {
 // TLAS nodes are appended to out.nodes and numbered from node_base on.
 bool build_tlas(const tt_scene_desc& d, const std::vector<BlasSummary>& blas, uint32_t node_base,
                 BvhBuildResult& out) {
-    // Instance world boxes: 8 transformed corners of the object box in
-    // double, padded by 1e-5 of (|lo| + |hi| + extent).
+    // Instance world boxes: the union of the world images of the mesh's part
+    // boxes (in double), padded by 1e-5 of (|lo| + |hi| + extent) of the
+    // image of the whole mesh's box, which contains the union.
+    std::vector<std::vector<Box>> parts(d.mesh_count);
+    {
+        std::vector<uint8_t> used(d.mesh_count, 0);
+        for (uint32_t i = 0; i < d.instance_count; i++)
+            if (d.instances[i].mesh_index < d.mesh_count) used[d.instances[i].mesh_index] = 1;
+        std::atomic<uint32_t> next{0};
+        unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (unsigned t = 0; t < nt; t++)
+            pool.emplace_back([&] {
+                for (;;) {
+                    uint32_t m = next.fetch_add(1);
+                    if (m >= d.mesh_count) break;
+                    if (used[m] && !blas[m].empty) parts[m] = part_boxes(d, m);
+                }
+            });
+        for (auto& t : pool) t.join();
+    }
     std::vector<Prim> prims;
     std::vector<Box> world(d.instance_count);
     for (uint32_t i = 0; i < d.instance_count; i++) {
@@ -317,21 +396,28 @@ bool build_tlas(const tt_scene_desc& d, const std::vector<BlasSummary>& blas, ui
         const BlasSummary& b = blas[in.mesh_index];
         if (b.empty) continue;
         double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (int c = 0; c < 8; c++) {
-            double p[3] = {(c & 1) ? b.hi[0] : b.lo[0], (c & 2) ? b.hi[1] : b.lo[1], (c & 4) ? b.hi[2] : b.lo[2]};
+        Box whole;
+        for (int a = 0; a < 3; a++) {
+            whole.lo[a] = b.lo[a];
+            whole.hi[a] = b.hi[a];
+        }
+        grow_world(in.transform, whole, lo, hi);
+        double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+        double pad[3];
+        for (int r = 0; r < 3; r++) pad[r] = (std::fabs(lo[r]) + std::fabs(hi[r]) + ext) * 1e-5;
+        const std::vector<Box>& pb = parts[in.mesh_index];
+        if (!pb.empty()) {
+            double plo[3] = {INFINITY, INFINITY, INFINITY}, phi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (const Box& q : pb) grow_world(in.transform, q, plo, phi);
             for (int r = 0; r < 3; r++) {
-                double w = double(in.transform[12 + r]);
-                for (int k = 0; k < 3; k++) w += double(in.transform[k * 4 + r]) * p[k];
-                lo[r] = std::min(lo[r], w);
-                hi[r] = std::max(hi[r], w);
+                lo[r] = std::max(lo[r], plo[r]);
+                hi[r] = std::min(hi[r], phi[r]);
             }
         }
-        double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
         Prim p;
         for (int r = 0; r < 3; r++) {
-            double pad = (std::fabs(lo[r]) + std::fabs(hi[r]) + ext) * 1e-5;
-            p.box.lo[r] = float(lo[r] - pad);
-            p.box.hi[r] = float(hi[r] + pad);
+            p.box.lo[r] = float(lo[r] - pad[r]);
+            p.box.hi[r] = float(hi[r] + pad[r]);
             p.centroid[r] = float(0.5 * (lo[r] + hi[r]));
         }
         p.id = i;
