@@ -12,6 +12,9 @@
 #ifndef TT_SAH_TRAVERSAL_COST
 #define TT_SAH_TRAVERSAL_COST 1.0
 #endif
+#ifndef TT_SAH_BINS
+#define TT_SAH_BINS 16
+#endif
 #ifndef TT_BLAS_MAX_LEAF
 #define TT_BLAS_MAX_LEAF 4
 #endif
@@ -86,7 +89,7 @@ struct Builder {
         uint32_t mid = 0;
         double leaf_cost = double(count);
         if (depth < BVH_MEDIAN_DEPTH) {
-            const int BINS = 16;
+            const int BINS = TT_SAH_BINS;
             double best = INFINITY;
             int best_axis = -1, best_bin = 0;
             for (int a = 0; a < 3; a++) {
@@ -298,14 +301,17 @@ void build_blas(const tt_scene_desc& d, uint32_t mesh, Blas& out) {
 }
 
 // Object-space boxes that together bound mesh m: its triangles binned by
-// centroid into a PART_GRID^3 grid over the mesh's box, one box per
+// centroid into a PART_GRID^3 grid (8^3) over the mesh's box, one box per
 // non-empty cell around the vertices of the cell's triangles.  Every triangle
 // lies inside its own cell's box, so the union of the boxes' world images
 // bounds the instance; for a rotated mesh that union is much tighter than the
 // world box of the whole mesh's box (the axis-aligned box of a rotated cube
 // is up to sqrt(3) times wider on each axis).  Vertices are dequantized as the
 // BLAS does (v / 32767 in float).
-constexpr uint32_t PART_GRID = 4;
+#ifndef TT_PART_GRID
+#define TT_PART_GRID 8
+#endif
+constexpr uint32_t PART_GRID = TT_PART_GRID;
 
 std::vector<Box> part_boxes(const tt_scene_desc& d, uint32_t mesh) {
     const tt_mesh_record& m = d.meshes[mesh];
