@@ -50,7 +50,22 @@ struct FrameArgs {
     uint32_t train_threshold;
     uint32_t judge_threshold;
     uint32_t cache_records;  // training records a training path takes (1 .. CACHE_OPEN_RECORDS)
+    // Primary rays in tile order (whole-frame renders): pixel (x, y)'s rays
+    // take the queue slots of primary_slot(), so a wave's 64 slots are an
+    // 8 x 8 pixel tile.  0: slot = path (serving chunks).
+    uint32_t primary_tiles;
 };
+
+// Queue slot of pixel (x, y)'s first primary ray when primary rays are in
+// tile order: the image is cut into bands of 8 rows (the last band may be
+// lower); a band's pixels follow column by column, so 64 consecutive slots
+// cover 8 columns of one full band, an 8 x 8 tile.  A bijection of
+// [0, width * height).
+__host__ __device__ inline uint32_t primary_slot(uint32_t x, uint32_t y, uint32_t width, uint32_t height) {
+    const uint32_t band = y / 8u, row0 = band * 8u;
+    const uint32_t rows = height - row0 < 8u ? height - row0 : 8u;
+    return row0 * width + x * rows + (y - row0);
+}
 
 // Training records a training path can keep open: its cache vertex and the
 // next rough vertices, each the outgoing radiance at that vertex.

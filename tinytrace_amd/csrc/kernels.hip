@@ -62,8 +62,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void trace_rays_kernel(tt::DeviceScene
 // -------------------------------------------------------------- generate --
 // integrate.glsl:129-132 (pixel_ndc), :17-20 (camera_ray_direction),
 // :323-333 (seed, per-sample generator copy).  Path p = pixel * spp + sample.
-// Path p of pixel `pixel`, sample `sample` of this pass.
-TT_DEV void generate_path(const tt::FrameArgs& f, const tt::PathBuffers& b, uint32_t p, uint32_t pixel,
+// Path p of pixel `pixel`, sample `sample` of this pass, its ray in queue
+// slot `slot`.
+TT_DEV void generate_path(const tt::FrameArgs& f, const tt::PathBuffers& b, uint32_t p, uint32_t slot, uint32_t pixel,
                           uint32_t sample) {
     uint32_t x = pixel % f.width, y = pixel / f.width;
     Generator g = init_generator(pixel, f.frame_index, f.rng, f.seed);
@@ -78,8 +79,8 @@ TT_DEV void generate_path(const tt::FrameArgs& f, const tt::PathBuffers& b, uint
     // The sample's device coordinates ride in the spare w components for the
     // first hit's texture level of detail: x with the direction, y with the
     // throughput (whose w then holds the level of detail).
-    b.queue[0][0][p] = make_float4(f.camera_position[0], f.camera_position[1], f.camera_position[2], __uint_as_float(p));
-    b.queue[0][1][p] = make_float4(dir.x, dir.y, dir.z, ndc_x);
+    b.queue[0][0][slot] = make_float4(f.camera_position[0], f.camera_position[1], f.camera_position[2], __uint_as_float(p));
+    b.queue[0][1][slot] = make_float4(dir.x, dir.y, dir.z, ndc_x);
     b.throughput[p] = make_float4(1.0f, 1.0f, 1.0f, ndc_y);
     b.radiance[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     b.rng[p] = g.state;
@@ -89,7 +90,11 @@ TT_DEV void generate_path(const tt::FrameArgs& f, const tt::PathBuffers& b, uint
 __global__ __launch_bounds__(256) void generate_kernel(tt::FrameArgs f, tt::PathBuffers b, uint32_t paths) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= paths) return;
-    generate_path(f, b, p, p / f.sample_count, p % f.sample_count);
+    const uint32_t pixel = p / f.sample_count, sample = p % f.sample_count;
+    uint32_t slot = p;
+    if (f.primary_tiles)
+        slot = tt::primary_slot(pixel % f.width, pixel / f.width, f.width, f.height) * f.sample_count + sample;
+    generate_path(f, b, p, slot, pixel, sample);
     if (p == 0) b.counters[C_EXT + 0] = paths;
 }
 
@@ -99,7 +104,7 @@ __global__ __launch_bounds__(256) void generate_chunk_kernel(tt::FrameArgs f, tt
                                                              uint32_t pixel0, uint32_t count) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
-    generate_path(f, b, base + i, pixel0 + i, 0u);
+    generate_path(f, b, base + i, base + i, pixel0 + i, 0u);
     if (i == 0) atomicAdd(b.counters + C_EXT + 0, count);
 }
 

@@ -157,8 +157,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 
     if (p >= f.width * f.height) return;
     float4* g = r.gbuffer[r.cur] + RECORD * size_t(p);
     const uint32_t path = r.base + p;  // the pixel's primary path
-    float4 qd = b.queue[0][1][path];
-    float4 hit = b.hits[path];
+    const uint32_t qslot = f.primary_tiles ? tt::primary_slot(p % f.width, p / f.width, f.width, f.height) : path;
+    float4 qd = b.queue[0][1][qslot];
+    float4 hit = b.hits[qslot];
     uint32_t instance = __float_as_uint(hit.z);
     Reservoir res = empty_reservoir();
     bool emit = false;

@@ -266,6 +266,7 @@ FrameArgs frame_args(const tt_constants& k, uint32_t width, uint32_t height, uin
     f.train_threshold = 0;
     f.judge_threshold = 0;
     f.cache_records = 0;
+    f.primary_tiles = 0;
     return f;
 }
 
@@ -736,6 +737,7 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
     }
 
     tt::FrameArgs f = tt::frame_args(*k, t->width, t->height, st.rng, st.seed);
+    f.primary_tiles = 1;
     uint32_t train_steps = 0, train_capacity = 0, judge_capacity = 0;
     if (cache) {
         if (tt::mlp_context(cache) != c) return fail(c, TT_STALE_HANDLE, "cache from another context");
