@@ -67,6 +67,14 @@ __host__ __device__ inline uint32_t primary_slot(uint32_t x, uint32_t y, uint32_
     return row0 * width + x * rows + (y - row0);
 }
 
+// The pixel whose primary_slot() is `slot`.
+__host__ __device__ inline uint32_t slot_pixel(uint32_t slot, uint32_t width, uint32_t height) {
+    const uint32_t row0 = (slot / (8u * width)) * 8u;  // a band starts at slot row0 * width
+    const uint32_t rows = height - row0 < 8u ? height - row0 : 8u;
+    const uint32_t off = slot - row0 * width;
+    return (row0 + off % rows) * width + off / rows;
+}
+
 // Training records a training path can keep open: its cache vertex and the
 // next rough vertices, each the outgoing radiance at that vertex.
 constexpr uint32_t CACHE_OPEN_RECORDS = 3;

@@ -153,11 +153,14 @@ template <bool TEXTURED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 1 : 4))) void restir_candidates_kernel(tt::DeviceScene d, tt::FrameArgs f, tt::PathBuffers b,
                                                                 tt::RestirBuffers r, uint4* out) {
     SceneView s = view(d);
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= f.width * f.height) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= f.width * f.height) return;
+    // Whole-frame renders take the pixels in the primary rays' tile order
+    // (tt::primary_slot), so a wave's lanes are an 8 x 8 pixel tile.
+    const uint32_t p = f.primary_tiles ? tt::slot_pixel(t, f.width, f.height) : t;
     float4* g = r.gbuffer[r.cur] + RECORD * size_t(p);
     const uint32_t path = r.base + p;  // the pixel's primary path
-    const uint32_t qslot = f.primary_tiles ? tt::primary_slot(p % f.width, p / f.width, f.width, f.height) : path;
+    const uint32_t qslot = f.primary_tiles ? t : path;
     float4 qd = b.queue[0][1][qslot];
     float4 hit = b.hits[qslot];
     uint32_t instance = __float_as_uint(hit.z);
@@ -259,8 +262,9 @@ template <bool TEXTURED>
 __global__ __launch_bounds__(256) void restir_temporal_kernel(tt::DeviceScene d, tt::FrameArgs f, tt::RestirBuffers r,
                                                               const uint4* in, uint4* out) {
     SceneView s = view(d);
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= f.width * f.height) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= f.width * f.height) return;
+    const uint32_t p = f.primary_tiles ? tt::slot_pixel(t, f.width, f.height) : t;  // tile order, as candidates
     Reservoir rq = unpack_reservoir(in[p]);
     Shading q = read_record<TEXTURED>(s, r.gbuffer[r.cur] + RECORD * size_t(p), p);
     if (q.valid && r.history_valid) {
@@ -280,8 +284,9 @@ template <bool TEXTURED>
 __global__ __launch_bounds__(256) void restir_spatial_kernel(tt::DeviceScene d, tt::FrameArgs f, tt::RestirBuffers r,
                                                              const uint4* in, uint4* out) {
     SceneView s = view(d);
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= f.width * f.height) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= f.width * f.height) return;
+    const uint32_t p = f.primary_tiles ? tt::slot_pixel(t, f.width, f.height) : t;  // tile order, as candidates
     Reservoir rq = unpack_reservoir(in[p]);
     Shading q = read_record<TEXTURED>(s, r.gbuffer[r.cur] + RECORD * size_t(p), p);
     if (q.valid) {
