@@ -163,238 +163,269 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 
                                                     uint32_t bounce) {
     SceneView s = view(d);
     const uint32_t count = b.counters[C_EXT + bounce];
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;  // whole waves exit together except the last one
-    const bool last = bounce + 1 >= f.bounce_count;
-    const bool restir = f.light_sampling == TT_LIGHT_SAMPLING_RESTIR && bounce == 0;
-    const bool nee_mode = f.light_sampling == TT_LIGHT_SAMPLING_NEXT_EVENT_ESTIMATION ||
-        (f.light_sampling == TT_LIGHT_SAMPLING_RESTIR && bounce > 0);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x * blockDim.x >= count) return;  // the whole workgroup
+    // The continuation ray, appended after the workgroup has binned them.
+    bool cont_out = false;
+    f3 cont_org(0.0f), cont_dir(0.0f);
+    uint32_t cont_path = 0;
+    if (i < count) {
+        const bool last = bounce + 1 >= f.bounce_count;
+        const bool restir = f.light_sampling == TT_LIGHT_SAMPLING_RESTIR && bounce == 0;
+        const bool nee_mode = f.light_sampling == TT_LIGHT_SAMPLING_NEXT_EVENT_ESTIMATION ||
+            (f.light_sampling == TT_LIGHT_SAMPLING_RESTIR && bounce > 0);
 
-    float4 qd = b.queue[bounce & 1][1][i];
-    float4 qo = b.queue[bounce & 1][0][i];
-    float4 hit = b.hits[i];
-    uint32_t p = __float_as_uint(qo.w);
-    f3 ray_dir(qd.x, qd.y, qd.z);
-    float4 thr4 = b.throughput[p];
-    float4 acc4 = b.radiance[p];
-    // Texture level of detail: the path's (create_path_state starts it at
-    // 1), set at the first hit from ray differentials.
-    float lod = bounce == 0 ? 1.0f : thr4.w;
-    f3 throughput(thr4.x, thr4.y, thr4.z);
-    f3 accumulated(acc4.x, acc4.y, acc4.z);
-    Generator g{b.rng[p], f.rng};
-    uint32_t instance = __float_as_uint(hit.z);
+        float4 qd = b.queue[bounce & 1][1][i];
+        float4 qo = b.queue[bounce & 1][0][i];
+        float4 hit = b.hits[i];
+        uint32_t p = __float_as_uint(qo.w);
+        f3 ray_dir(qd.x, qd.y, qd.z);
+        float4 thr4 = b.throughput[p];
+        float4 acc4 = b.radiance[p];
+        // Texture level of detail: the path's (create_path_state starts it at
+        // 1), set at the first hit from ray differentials.
+        float lod = bounce == 0 ? 1.0f : thr4.w;
+        f3 throughput(thr4.x, thr4.y, thr4.z);
+        f3 accumulated(acc4.x, acc4.y, acc4.z);
+        Generator g{b.rng[p], f.rng};
+        uint32_t instance = __float_as_uint(hit.z);
 
-    // Radiance cache: the path's open training / judge records, the radiance
-    // this vertex adds (for their targets) and a query or record here.
-    const bool cache = f.cache_mode != 0u;
-    const f3 thr_in = throughput;
-    uint4 open = cache ? b.cache.open[p] : make_uint4(0u, 0u, 0u, 0u);
-    bool emit_query = false, emit_train = false, emit_judge = false;
-    f3 added;
-    float rec[16];
+        // Radiance cache: the path's open training / judge records, the radiance
+        // this vertex adds (for their targets) and a query or record here.
+        const bool cache = f.cache_mode != 0u;
+        const f3 thr_in = throughput;
+        uint4 open = cache ? b.cache.open[p] : make_uint4(0u, 0u, 0u, 0u);
+        bool emit_query = false, emit_train = false, emit_judge = false;
+        f3 added;
+        float rec[16];
 
-    bool emit_shadow = false, emit_ray = false;
-    Mat m;
-    Surface sf;
-    f3 shadow_org, shadow_dir, direct, emission_after;
-    uint32_t light_instance = 0, light_hash = 0;
-    bool has_emission_after = false;
-    f3 next_dir, next_org;
+        bool emit_shadow = false, emit_ray = false;
+        Mat m;
+        Surface sf;
+        f3 shadow_org, shadow_dir, direct, emission_after;
+        uint32_t light_instance = 0, light_hash = 0;
+        bool has_emission_after = false;
+        f3 next_dir, next_org;
 
-    if (instance == 0xffffffffu) {
-        // Miss: constant white sky (integrate.glsl:272-276).
-        added = f3(1.0f) * throughput;
-        accumulated = accumulated + added;
-    } else {
-        if constexpr (TEXTURED) {
-            TextureHit tex{true, bounce == 0,
-                           PrimaryRay{f.inverse_view, f.inverse_proj, f3(qo.x, qo.y, qo.z), float(f.width),
-                                      float(f.height), qd.w, thr4.w},
-                           lod};
-            sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, m, tex);
-            lod = tex.lod;
+        if (instance == 0xffffffffu) {
+            // Miss: constant white sky (integrate.glsl:272-276).
+            added = f3(1.0f) * throughput;
+            accumulated = accumulated + added;
         } else {
-            sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, m);
-        }
-        if (cache && bounce >= 1 && m.specular_roughness > 0.3f) {
-            if (!(open.w & tt::CACHE_DECIDED)) {
-                // The cache vertex: a query ends the path here; a training or
-                // judge path records the vertex and runs on.
-                uint32_t h = pcg32_output(splitmix64(pcg32_key(p, f.frame_index) ^ 0x6a09e667f3bcc909ull));
-                emit_train = h < f.train_threshold;
-                emit_judge = !emit_train && h < f.judge_threshold;
-                emit_query = !emit_train && !emit_judge;
-            } else if (!(open.w & CACHE_JUDGE_PATH) && (open.w & 3u) < f.cache_records) {
-                // A later rough vertex of a training path: one more sample.
-                emit_train = true;
+            if constexpr (TEXTURED) {
+                TextureHit tex{true, bounce == 0,
+                               PrimaryRay{f.inverse_view, f.inverse_proj, f3(qo.x, qo.y, qo.z), float(f.width),
+                                          float(f.height), qd.w, thr4.w},
+                               lod};
+                sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, m, tex);
+                lod = tex.lod;
+            } else {
+                sf = surface_at_hit(s, instance, __float_as_uint(hit.w), hit.x, hit.y, ray_dir, m);
             }
-            if (emit_train || emit_judge || emit_query) {
-                f3 v[4] = {sf.position, -ray_dir, sf.basis.normal, m.base_color};
-                for (int k = 0; k < 4; k++) {
-                    rec[3 * k + 0] = v[k].x;
-                    rec[3 * k + 1] = v[k].y;
-                    rec[3 * k + 2] = v[k].z;
+            if (cache && bounce >= 1 && m.specular_roughness > 0.3f) {
+                if (!(open.w & tt::CACHE_DECIDED)) {
+                    // The cache vertex: a query ends the path here; a training or
+                    // judge path records the vertex and runs on.
+                    uint32_t h = pcg32_output(splitmix64(pcg32_key(p, f.frame_index) ^ 0x6a09e667f3bcc909ull));
+                    emit_train = h < f.train_threshold;
+                    emit_judge = !emit_train && h < f.judge_threshold;
+                    emit_query = !emit_train && !emit_judge;
+                } else if (!(open.w & CACHE_JUDGE_PATH) && (open.w & 3u) < f.cache_records) {
+                    // A later rough vertex of a training path: one more sample.
+                    emit_train = true;
                 }
-                rec[12] = m.specular_roughness;
-                rec[13] = m.base_metalness;
-                rec[14] = float(bounce);
-                rec[15] = 0.0f;
+                if (emit_train || emit_judge || emit_query) {
+                    f3 v[4] = {sf.position, -ray_dir, sf.basis.normal, m.base_color};
+                    for (int k = 0; k < 4; k++) {
+                        rec[3 * k + 0] = v[k].x;
+                        rec[3 * k + 1] = v[k].y;
+                        rec[3 * k + 2] = v[k].z;
+                    }
+                    rec[12] = m.specular_roughness;
+                    rec[13] = m.base_metalness;
+                    rec[14] = float(bounce);
+                    rec[15] = 0.0f;
+                }
             }
         }
-    }
-    if (instance != 0xffffffffu && !emit_query) {
-        f3 wi = sf.wi;
-        Lobes lobes;
-        if (restir) {
-            uint2 key = b.restir_key ? b.restir_key[p] : make_uint2(p, f.frame_index);
-            Generator lg = restir_lobe_generator(key.x, key.y);
-            lobes = bsdf_prepare(m, wi, lg);
-        } else {
-            lobes = bsdf_prepare(m, wi, g);
-        }
+        if (instance != 0xffffffffu && !emit_query) {
+            f3 wi = sf.wi;
+            Lobes lobes;
+            if (restir) {
+                uint2 key = b.restir_key ? b.restir_key[p] : make_uint2(p, f.frame_index);
+                Generator lg = restir_lobe_generator(key.x, key.y);
+                lobes = bsdf_prepare(m, wi, lg);
+            } else {
+                lobes = bsdf_prepare(m, wi, g);
+            }
 
-        bool gate = s.emissive_count > 0 && m.specular_roughness > 0.15f;
-        bool nee = (nee_mode || restir) && gate;
-        f3 emissive = m.emission_color * m.emission_luminance;
-        bool sampled = false, known_visible = false;
-        LightPoint lp;
-        f3 contribution;
-        if (nee && nee_mode) {
-            // sample_random_light (scene.glsl:114-136) and the unshadowed
-            // part of direct_light_contribution (integrate.glsl:156-193).
-            uint32_t li = random_uint(g, s.emissive_count);
-            f2 r2 = random_vec2(g);
-            if constexpr (TEXTURED) lp = light_point_textured(s, li, sample_triangle(r2));
-            else lp = light_point(s, li, sample_triangle(r2));
-            float light_probability = 1.0f / float(s.emissive_count);
-            f3 to_light = lp.position - sf.position;
-            f3 wo = transform_from_basis(sf.basis, to_light);
-            float distance_squared = length_squared(to_light);
-            float distance = sqrt_(distance_squared);
-            f3 dir = to_light / distance;
-            float density = (1.0f / lp.area) * distance_squared / saturate(dot(-dir, lp.normal)) * light_probability;
-            float bsdf_density;
-            f3 bsdf = bsdf_evaluate(m, lobes, wi, wo, bsdf_density);
-            contribution = bsdf * lp.emission * abs_(cos_theta(wo)) / fmax_(density, PDF_EPSILON);
-            shadow_dir = dir;
-            sampled = true;
-        } else if (nee && restir) {
-            // ReSTIR DI: the reservoir's sample y and contribution weight W
-            // estimate the direct light as f(y) Le(y) G(y) W.
-            Reservoir r = unpack_reservoir(b.restir_final[p]);
-            if (r.light != 0xffffffffu && r.W > 0.0f) {
-                if constexpr (TEXTURED) lp = light_point_textured(s, r.light, reservoir_bary(r.bary));
-                else lp = light_point(s, r.light, reservoir_bary(r.bary));
+            bool gate = s.emissive_count > 0 && m.specular_roughness > 0.15f;
+            bool nee = (nee_mode || restir) && gate;
+            f3 emissive = m.emission_color * m.emission_luminance;
+            bool sampled = false, known_visible = false;
+            LightPoint lp;
+            f3 contribution;
+            if (nee && nee_mode) {
+                // sample_random_light (scene.glsl:114-136) and the unshadowed
+                // part of direct_light_contribution (integrate.glsl:156-193).
+                uint32_t li = random_uint(g, s.emissive_count);
+                f2 r2 = random_vec2(g);
+                if constexpr (TEXTURED) lp = light_point_textured(s, li, sample_triangle(r2));
+                else lp = light_point(s, li, sample_triangle(r2));
+                float light_probability = 1.0f / float(s.emissive_count);
                 f3 to_light = lp.position - sf.position;
                 f3 wo = transform_from_basis(sf.basis, to_light);
                 float distance_squared = length_squared(to_light);
                 float distance = sqrt_(distance_squared);
                 f3 dir = to_light / distance;
-                float cos_light = saturate(dot(-dir, lp.normal));
+                float density = (1.0f / lp.area) * distance_squared / saturate(dot(-dir, lp.normal)) * light_probability;
                 float bsdf_density;
                 f3 bsdf = bsdf_evaluate(m, lobes, wi, wo, bsdf_density);
-                float scale = ((abs_(cos_theta(wo)) * cos_light) / distance_squared) * r.W;
-                contribution = (bsdf * lp.emission) * scale;
+                contribution = bsdf * lp.emission * abs_(cos_theta(wo)) / fmax_(density, PDF_EPSILON);
                 shadow_dir = dir;
                 sampled = true;
-                // The pixel's own candidate that passed the visibility pass
-                // is the same ray from the same point: its answer stands.
-                uint4 own = b.restir_initial[p];
-                known_visible = own.x == r.light && own.y == r.bary && (own.z & 0xffffu) != 0u;
-            }
-        }
-        if (sampled && known_visible) {
-            // As the shadow kernel would: direct light, then the deferred
-            // emission of the primary vertex.
-            direct = throughput * contribution;
-            accumulated = accumulated + direct;
-            accumulated = accumulated + emissive * throughput;
-        } else if (sampled) {
-            direct = throughput * contribution;
-            if (direct.x == 0.0f && direct.y == 0.0f && direct.z == 0.0f) {
-                // Visibility only scales the contribution, so a zero one (the
-                // light's back faces the point, or the direction lies below
-                // the surface) needs no shadow ray: the result is the
-                // occluded one, and at the primary vertex that adds the
-                // deferred emission, as the shadow kernel would.
-                if (bounce == 0) accumulated = accumulated + emissive * throughput;
-            } else {
-                shadow_org = sf.position;
-                light_instance = lp.instance;
-                light_hash = lp.hash;
-                emit_shadow = true;
-                if (bounce == 0) {
-                    emission_after = emissive * throughput;
-                    has_emission_after = true;
+            } else if (nee && restir) {
+                // ReSTIR DI: the reservoir's sample y and contribution weight W
+                // estimate the direct light as f(y) Le(y) G(y) W.
+                Reservoir r = unpack_reservoir(b.restir_final[p]);
+                if (r.light != 0xffffffffu && r.W > 0.0f) {
+                    if constexpr (TEXTURED) lp = light_point_textured(s, r.light, reservoir_bary(r.bary));
+                    else lp = light_point(s, r.light, reservoir_bary(r.bary));
+                    f3 to_light = lp.position - sf.position;
+                    f3 wo = transform_from_basis(sf.basis, to_light);
+                    float distance_squared = length_squared(to_light);
+                    float distance = sqrt_(distance_squared);
+                    f3 dir = to_light / distance;
+                    float cos_light = saturate(dot(-dir, lp.normal));
+                    float bsdf_density;
+                    f3 bsdf = bsdf_evaluate(m, lobes, wi, wo, bsdf_density);
+                    float scale = ((abs_(cos_theta(wo)) * cos_light) / distance_squared) * r.W;
+                    contribution = (bsdf * lp.emission) * scale;
+                    shadow_dir = dir;
+                    sampled = true;
+                    // The pixel's own candidate that passed the visibility pass
+                    // is the same ray from the same point: its answer stands.
+                    uint4 own = b.restir_initial[p];
+                    known_visible = own.x == r.light && own.y == r.bary && (own.z & 0xffffu) != 0u;
                 }
             }
-        } else if (bounce == 0 || !nee) {
-            added = emissive * throughput;
-            accumulated = accumulated + added;
+            if (sampled && known_visible) {
+                // As the shadow kernel would: direct light, then the deferred
+                // emission of the primary vertex.
+                direct = throughput * contribution;
+                accumulated = accumulated + direct;
+                accumulated = accumulated + emissive * throughput;
+            } else if (sampled) {
+                direct = throughput * contribution;
+                if (direct.x == 0.0f && direct.y == 0.0f && direct.z == 0.0f) {
+                    // Visibility only scales the contribution, so a zero one (the
+                    // light's back faces the point, or the direction lies below
+                    // the surface) needs no shadow ray: the result is the
+                    // occluded one, and at the primary vertex that adds the
+                    // deferred emission, as the shadow kernel would.
+                    if (bounce == 0) accumulated = accumulated + emissive * throughput;
+                } else {
+                    shadow_org = sf.position;
+                    light_instance = lp.instance;
+                    light_hash = lp.hash;
+                    emit_shadow = true;
+                    if (bounce == 0) {
+                        emission_after = emissive * throughput;
+                        has_emission_after = true;
+                    }
+                }
+            } else if (bounce == 0 || !nee) {
+                added = emissive * throughput;
+                accumulated = accumulated + added;
+            }
+            if (!last) {
+                f3 wo, value;
+                float density;
+                if (bsdf_sample(m, lobes, wi, g, wo, density, value)) {
+                    next_dir = transform_to_basis(sf.basis, wo);
+                    next_org = sf.position;
+                    throughput = throughput * (value / fmax_(density, PDF_EPSILON) * abs_(cos_theta(wo)));
+                    emit_ray = true;
+                }
+            }
         }
-        if (!last) {
-            f3 wo, value;
-            float density;
-            if (bsdf_sample(m, lobes, wi, g, wo, density, value)) {
-                next_dir = transform_to_basis(sf.basis, wo);
-                next_org = sf.position;
-                throughput = throughput * (value / fmax_(density, PDF_EPSILON) * abs_(cos_theta(wo)));
-                emit_ray = true;
+
+        b.radiance[p] = make_float4(accumulated.x, accumulated.y, accumulated.z, 0.0f);
+        b.throughput[p] = make_float4(throughput.x, throughput.y, throughput.z, lod);
+        b.rng[p] = g.state;
+
+        cont_out = emit_ray;
+        cont_org = next_org;
+        cont_dir = next_dir;
+        cont_path = p;
+        uint32_t sslot = wave_append(b.counters + C_SHADOW + bounce, emit_shadow);
+        if (emit_shadow) {
+            b.shadow[0][sslot] = make_float4(shadow_org.x, shadow_org.y, shadow_org.z, __uint_as_float(p));
+            b.shadow[1][sslot] = make_float4(shadow_dir.x, shadow_dir.y, shadow_dir.z, __uint_as_float(light_instance));
+            b.shadow[2][sslot] = make_float4(direct.x, direct.y, direct.z,
+                                             __uint_as_float(light_hash | (has_emission_after ? 0x10000u : 0u)));
+            b.shadow[3][sslot] = make_float4(emission_after.x, emission_after.y, emission_after.z, 0.0f);
+        }
+        if (cache) {
+            const tt::CacheBuffers& cb = b.cache;
+            uint32_t qs = wave_append(b.counters + tt::C_CACHE_QUERY, emit_query);
+            if (emit_query) {
+                for (int k = 0; k < 4; k++)
+                    cb.query_rec[4 * size_t(qs) + k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
+                cb.query_scale[qs] = make_float4(thr_in.x, thr_in.y, thr_in.z, __uint_as_float(p));
+            }
+            // What this vertex adds reaches the records opened before it.
+            add_to_records(cb, open, added);
+            uint32_t ts = wave_append(b.counters + tt::C_CACHE_TRAIN, emit_train);
+            uint32_t js = wave_append(b.counters + tt::C_CACHE_JUDGE, emit_judge);
+            uint32_t slot = tt::CACHE_NO_RECORD;
+            if (emit_train && ts < cb.train_capacity) slot = ts;
+            if (emit_judge && js < cb.judge_capacity) slot = cb.judge_base + js;
+            if (slot != tt::CACHE_NO_RECORD) {
+                for (int k = 0; k < 4; k++)
+                    cb.train_rec[4 * size_t(slot) + k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
+                cb.train_thr[slot] = make_float4(thr_in.x, thr_in.y, thr_in.z, __uint_as_float(p));
+                f3 t = cache_ratio(added, thr_in);
+                cb.train_target[slot] = make_float4(t.x, t.y, t.z, 0.0f);
+                uint32_t n = open.w & 3u;
+                if (n == 0) open.x = slot;
+                else if (n == 1) open.y = slot;
+                else open.z = slot;
+                open.w += 1u;
+            }
+            if (emit_train || emit_judge) {
+                open.w |= tt::CACHE_DECIDED | (emit_judge ? CACHE_JUDGE_PATH : 0u);
+                cb.open[p] = open;
             }
         }
     }
 
-    b.radiance[p] = make_float4(accumulated.x, accumulated.y, accumulated.z, 0.0f);
-    b.throughput[p] = make_float4(throughput.x, throughput.y, throughput.z, lod);
-    b.rng[p] = g.state;
-
-    uint32_t slot = wave_append(b.counters + C_EXT + bounce + 1, emit_ray);
-    if (emit_ray) {
-        float4* no = b.queue[(bounce + 1) & 1][0];
-        float4* nd = b.queue[(bounce + 1) & 1][1];
-        no[slot] = make_float4(next_org.x, next_org.y, next_org.z, __uint_as_float(p));
-        nd[slot] = make_float4(next_dir.x, next_dir.y, next_dir.z, 0.0f);
+    // Continuation rays binned by direction octant across the workgroup: one
+    // atomic reserves the workgroup's rays, which take consecutive slots
+    // octant by octant, so extend's 64-ray batches mix fewer directions.
+    // Slot order only decides which lanes trace which rays.
+    __shared__ uint32_t octant_count[8];
+    __shared__ uint32_t octant_base[8];
+    if (threadIdx.x < 8) octant_count[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t octant = (cont_dir.x < 0.0f ? 1u : 0u) | (cont_dir.y < 0.0f ? 2u : 0u) | (cont_dir.z < 0.0f ? 4u : 0u);
+    uint32_t rank = 0;
+    if (cont_out) rank = atomicAdd(&octant_count[octant], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (int k = 0; k < 8; k++) {
+            octant_base[k] = total;
+            total += octant_count[k];
+        }
+        const uint32_t first = total ? atomicAdd(b.counters + C_EXT + bounce + 1, total) : 0u;
+        for (int k = 0; k < 8; k++) octant_base[k] += first;
     }
-    uint32_t sslot = wave_append(b.counters + C_SHADOW + bounce, emit_shadow);
-    if (emit_shadow) {
-        b.shadow[0][sslot] = make_float4(shadow_org.x, shadow_org.y, shadow_org.z, __uint_as_float(p));
-        b.shadow[1][sslot] = make_float4(shadow_dir.x, shadow_dir.y, shadow_dir.z, __uint_as_float(light_instance));
-        b.shadow[2][sslot] = make_float4(direct.x, direct.y, direct.z,
-                                         __uint_as_float(light_hash | (has_emission_after ? 0x10000u : 0u)));
-        b.shadow[3][sslot] = make_float4(emission_after.x, emission_after.y, emission_after.z, 0.0f);
-    }
-    if (cache) {
-        const tt::CacheBuffers& cb = b.cache;
-        uint32_t qs = wave_append(b.counters + tt::C_CACHE_QUERY, emit_query);
-        if (emit_query) {
-            for (int k = 0; k < 4; k++)
-                cb.query_rec[4 * size_t(qs) + k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
-            cb.query_scale[qs] = make_float4(thr_in.x, thr_in.y, thr_in.z, __uint_as_float(p));
-        }
-        // What this vertex adds reaches the records opened before it.
-        add_to_records(cb, open, added);
-        uint32_t ts = wave_append(b.counters + tt::C_CACHE_TRAIN, emit_train);
-        uint32_t js = wave_append(b.counters + tt::C_CACHE_JUDGE, emit_judge);
-        uint32_t slot = tt::CACHE_NO_RECORD;
-        if (emit_train && ts < cb.train_capacity) slot = ts;
-        if (emit_judge && js < cb.judge_capacity) slot = cb.judge_base + js;
-        if (slot != tt::CACHE_NO_RECORD) {
-            for (int k = 0; k < 4; k++)
-                cb.train_rec[4 * size_t(slot) + k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
-            cb.train_thr[slot] = make_float4(thr_in.x, thr_in.y, thr_in.z, __uint_as_float(p));
-            f3 t = cache_ratio(added, thr_in);
-            cb.train_target[slot] = make_float4(t.x, t.y, t.z, 0.0f);
-            uint32_t n = open.w & 3u;
-            if (n == 0) open.x = slot;
-            else if (n == 1) open.y = slot;
-            else open.z = slot;
-            open.w += 1u;
-        }
-        if (emit_train || emit_judge) {
-            open.w |= tt::CACHE_DECIDED | (emit_judge ? CACHE_JUDGE_PATH : 0u);
-            cb.open[p] = open;
-        }
+    __syncthreads();
+    if (cont_out) {
+        const uint32_t slot = octant_base[octant] + rank;
+        b.queue[(bounce + 1) & 1][0][slot] = make_float4(cont_org.x, cont_org.y, cont_org.z, __uint_as_float(cont_path));
+        b.queue[(bounce + 1) & 1][1][slot] = make_float4(cont_dir.x, cont_dir.y, cont_dir.z, 0.0f);
     }
 }
 
