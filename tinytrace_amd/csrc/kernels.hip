@@ -165,10 +165,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 
     const uint32_t count = b.counters[C_EXT + bounce];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x * blockDim.x >= count) return;  // the whole workgroup
-    // The continuation ray, appended after the workgroup has binned them.
+    // The continuation ray and the shadow ray, appended after the workgroup
+    // has binned them.
     bool cont_out = false;
     f3 cont_org(0.0f), cont_dir(0.0f);
     uint32_t cont_path = 0;
+    bool shw_out = false;
+    float4 shw0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), shw1 = shw0, shw2 = shw0, shw3 = shw0;
     if (i < count) {
         const bool last = bounce + 1 >= f.bounce_count;
         const bool restir = f.light_sampling == TT_LIGHT_SAMPLING_RESTIR && bounce == 0;
@@ -358,13 +361,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 
         cont_org = next_org;
         cont_dir = next_dir;
         cont_path = p;
-        uint32_t sslot = wave_append(b.counters + C_SHADOW + bounce, emit_shadow);
         if (emit_shadow) {
-            b.shadow[0][sslot] = make_float4(shadow_org.x, shadow_org.y, shadow_org.z, __uint_as_float(p));
-            b.shadow[1][sslot] = make_float4(shadow_dir.x, shadow_dir.y, shadow_dir.z, __uint_as_float(light_instance));
-            b.shadow[2][sslot] = make_float4(direct.x, direct.y, direct.z,
-                                             __uint_as_float(light_hash | (has_emission_after ? 0x10000u : 0u)));
-            b.shadow[3][sslot] = make_float4(emission_after.x, emission_after.y, emission_after.z, 0.0f);
+            shw_out = true;
+            shw0 = make_float4(shadow_org.x, shadow_org.y, shadow_org.z, __uint_as_float(p));
+            shw1 = make_float4(shadow_dir.x, shadow_dir.y, shadow_dir.z, __uint_as_float(light_instance));
+            shw2 = make_float4(direct.x, direct.y, direct.z,
+                               __uint_as_float(light_hash | (has_emission_after ? 0x10000u : 0u)));
+            shw3 = make_float4(emission_after.x, emission_after.y, emission_after.z, 0.0f);
         }
         if (cache) {
             const tt::CacheBuffers& cb = b.cache;
@@ -400,32 +403,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 
         }
     }
 
-    // Continuation rays binned by direction octant across the workgroup: one
-    // atomic reserves the workgroup's rays, which take consecutive slots
-    // octant by octant, so extend's 64-ray batches mix fewer directions.
-    // Slot order only decides which lanes trace which rays.
-    __shared__ uint32_t octant_count[8];
-    __shared__ uint32_t octant_base[8];
-    if (threadIdx.x < 8) octant_count[threadIdx.x] = 0u;
+    // Continuation and shadow rays binned by direction octant across the
+    // workgroup: one atomic per queue reserves the workgroup's rays, which
+    // take consecutive slots octant by octant, so the traversal kernels'
+    // 64-ray batches mix fewer directions.  Slot order only decides which
+    // lanes trace which rays.
+    __shared__ uint32_t octant_count[16];  // continuation rays 0-7, shadow rays 8-15
+    __shared__ uint32_t octant_base[16];
+    if (threadIdx.x < 16) octant_count[threadIdx.x] = 0u;
     __syncthreads();
-    const uint32_t octant = (cont_dir.x < 0.0f ? 1u : 0u) | (cont_dir.y < 0.0f ? 2u : 0u) | (cont_dir.z < 0.0f ? 4u : 0u);
-    uint32_t rank = 0;
-    if (cont_out) rank = atomicAdd(&octant_count[octant], 1u);
+    const uint32_t c_oct = (cont_dir.x < 0.0f ? 1u : 0u) | (cont_dir.y < 0.0f ? 2u : 0u) | (cont_dir.z < 0.0f ? 4u : 0u);
+    const uint32_t s_oct = 8u | (shw1.x < 0.0f ? 1u : 0u) | (shw1.y < 0.0f ? 2u : 0u) | (shw1.z < 0.0f ? 4u : 0u);
+    uint32_t c_rank = 0, s_rank = 0;
+    if (cont_out) c_rank = atomicAdd(&octant_count[c_oct], 1u);
+    if (shw_out) s_rank = atomicAdd(&octant_count[s_oct], 1u);
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 2) {
+        // Thread 0 the continuation queue, thread 1 the shadow queue.
+        const uint32_t q0 = 8u * threadIdx.x;
         uint32_t total = 0;
-        for (int k = 0; k < 8; k++) {
-            octant_base[k] = total;
-            total += octant_count[k];
+        for (uint32_t k = 0; k < 8; k++) {
+            octant_base[q0 + k] = total;
+            total += octant_count[q0 + k];
         }
-        const uint32_t first = total ? atomicAdd(b.counters + C_EXT + bounce + 1, total) : 0u;
-        for (int k = 0; k < 8; k++) octant_base[k] += first;
+        uint32_t* counter = threadIdx.x == 0 ? b.counters + C_EXT + bounce + 1 : b.counters + C_SHADOW + bounce;
+        const uint32_t first = total ? atomicAdd(counter, total) : 0u;
+        for (uint32_t k = 0; k < 8; k++) octant_base[q0 + k] += first;
     }
     __syncthreads();
     if (cont_out) {
-        const uint32_t slot = octant_base[octant] + rank;
+        const uint32_t slot = octant_base[c_oct] + c_rank;
         b.queue[(bounce + 1) & 1][0][slot] = make_float4(cont_org.x, cont_org.y, cont_org.z, __uint_as_float(cont_path));
         b.queue[(bounce + 1) & 1][1][slot] = make_float4(cont_dir.x, cont_dir.y, cont_dir.z, 0.0f);
+    }
+    if (shw_out) {
+        const uint32_t slot = octant_base[s_oct] + s_rank;
+        b.shadow[0][slot] = shw0;
+        b.shadow[1][slot] = shw1;
+        b.shadow[2][slot] = shw2;
+        b.shadow[3][slot] = shw3;
     }
 }
 
