@@ -203,7 +203,11 @@ def main() -> int:
                      "rays_per_launch": round(ext_rays / max(ext_launches, 1), 1),
                      "isolated": {"avg_launch_ms": round(iso_ms, 4), "achieved": round(iso_gbs, 2),
                                   "frac": round(iso_gbs / HBM_PEAK_GBS, 5)},
-                     "rocprof": rocprof},
+                     "rocprof": rocprof,
+                     "note": "achieved = extension rays x bytes_per_ray / extend time; bytes_per_ray = 32 B ray + "
+                             "16 B hit + the built BVH's SAH-expected node and triangle bytes, so a tighter tree "
+                             "lowers achieved and frac for the same launch time; traffic is the measured HBM bytes "
+                             "of one launch"},
     }
 
     if args.quality_frames > 0:
