@@ -5,6 +5,8 @@
 #include <atomic>
 #include <cstring>
 #include <cmath>
+#include <exception>
+#include <mutex>
 #include <thread>
 
 // SAH cost of one traversal step relative to one triangle test, and the
@@ -300,6 +302,41 @@ void build_blas(const tt_scene_desc& d, uint32_t mesh, Blas& out) {
         out.stats);
 }
 
+// Runs fn(i) for every i in [0, count) on up to max_threads host threads.
+// An exception thrown by fn stops the work not yet handed out and is
+// rethrown here once every thread has joined (one escaping a thread's
+// function would end the process).
+template <typename Fn>
+void parallel_for(uint32_t count, unsigned max_threads, Fn&& fn) {
+    std::atomic<uint32_t> next{0};
+    std::exception_ptr error;
+    std::mutex error_mutex;
+    auto worker = [&] {
+        for (;;) {
+            const uint32_t i = next.fetch_add(1);
+            if (i >= count) break;
+            try {
+                fn(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> lock(error_mutex);
+                if (!error) error = std::current_exception();
+                next.store(count);
+            }
+        }
+    };
+    const unsigned nt = std::max(1u, std::min(max_threads, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    try {
+        pool.reserve(nt);
+        for (unsigned t = 0; t < nt; t++) pool.emplace_back(worker);
+    } catch (...) {
+        for (auto& t : pool) t.join();
+        throw;
+    }
+    for (auto& t : pool) t.join();
+    if (error) std::rethrow_exception(error);
+}
+
 // Object-space boxes that together bound mesh m: its triangles binned by
 // centroid into a PART_GRID^3 grid (8^3) over the mesh's box, one box per
 // non-empty cell around the vertices of the cell's triangles.  Every triangle
@@ -378,18 +415,9 @@ bool build_tlas(const tt_scene_desc& d, const std::vector<BlasSummary>& blas, ui
         std::vector<uint8_t> used(d.mesh_count, 0);
         for (uint32_t i = 0; i < d.instance_count; i++)
             if (d.instances[i].mesh_index < d.mesh_count) used[d.instances[i].mesh_index] = 1;
-        std::atomic<uint32_t> next{0};
-        unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-        std::vector<std::thread> pool;
-        for (unsigned t = 0; t < nt; t++)
-            pool.emplace_back([&] {
-                for (;;) {
-                    uint32_t m = next.fetch_add(1);
-                    if (m >= d.mesh_count) break;
-                    if (used[m] && !blas[m].empty) parts[m] = part_boxes(d, m);
-                }
-            });
-        for (auto& t : pool) t.join();
+        parallel_for(d.mesh_count, 16u, [&](uint32_t m) {
+            if (used[m] && !blas[m].empty) parts[m] = part_boxes(d, m);
+        });
     }
     std::vector<Prim> prims;
     std::vector<Box> world(d.instance_count);
@@ -519,20 +547,7 @@ bool build_bvh(const tt_scene_desc& d, BvhBuildResult& out) {
         return false;
     }
     std::vector<Blas> blas(d.mesh_count);
-    {
-        std::atomic<uint32_t> next{0};
-        unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
-        std::vector<std::thread> pool;
-        for (unsigned t = 0; t < nt; t++)
-            pool.emplace_back([&] {
-                for (;;) {
-                    uint32_t m = next.fetch_add(1);
-                    if (m >= d.mesh_count) break;
-                    build_blas(d, m, blas[m]);
-                }
-            });
-        for (auto& t : pool) t.join();
-    }
+    parallel_for(d.mesh_count, 32u, [&](uint32_t m) { build_blas(d, m, blas[m]); });
     // Concatenate every BLAS (references rebased), then the TLAS.
     std::vector<BlasSummary> summary(d.mesh_count);
     for (uint32_t m = 0; m < d.mesh_count; m++) {

@@ -4,11 +4,14 @@
 // (src/integrate/integrate.glsl:323-347).  Here one render is a sequence of
 // launches on one stream, per bounce b:
 //   extend  : closest hit of every ray in queue[b % 2] (persistent waves
-//             fetch 64 rays at a time from an atomic cursor)
+//             fetch 64 rays at a time from partitioned atomic cursors);
+//             primary rays (b = 0) of a whole frame sit in 8 x 8 pixel-tile
+//             order (tt::primary_slot)
 //   shade   : one lane per hit: next_path_segment (integrate.glsl:270-321)
 //             up to the two ray queries, appending the continuation ray to
-//             queue[(b+1) % 2] and the shadow ray to the shadow queue with
-//             wave-ballot compaction (one atomic per wave)
+//             queue[(b+1) % 2] and the shadow ray to the shadow queue, each
+//             workgroup's rays binned by direction octant (one atomic per
+//             workgroup and queue)
 //   shadow  : closest hit of every shadow ray, the reference's visibility
 //             rule (integrate.glsl:175-185) and the radiance update
 // framed by generate (camera rays, integrate.glsl:323-333) and accumulate
