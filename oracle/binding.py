@@ -301,6 +301,15 @@ class BsdfHooks:
                                       wi.ctypes.data, wo.ctypes.data, out.ctypes.data)
         return out
 
+    def lobe_sample(self, material, lobe, wi, rng_state):
+        """One lobe sample from generator state `rng_state`: (ok, wo (3,),
+        bsdf rgb and density (4,))."""
+        wi = self._v(wi)
+        wo, out = np.zeros(3), np.zeros(4)
+        ok = self.L.oracle_h_lobe_sample(self.p, C.addressof(material), LOBES.get(lobe, lobe), wi.ctypes.data,
+                                         int(rng_state), wo.ctypes.data, out.ctypes.data)
+        return bool(ok), wo, out
+
     def lobe_albedo(self, material, lobe, wi, rng_states):
         states = np.ascontiguousarray(rng_states, dtype=np.uint32)
         out = np.zeros((states.size, 3))

@@ -21,12 +21,13 @@
 // Pins (tests/test_bsdf_cpu.py, tests/test_bsdf_pins_cpu.py through
 // bsdf_hooks.cpp): closed forms, quadrature, published special cases and
 // the reference's fitted constants, each shown to fail on a planted slip.
-// Parity unpinned: specular_btdf_evaluate's density and Jacobian (the
-// reference's own, with normal-frame cosines where Walter 2007 has the
-// half-vector dots; its sample path is pinned through the smooth limit), and
-// fuzz_brdf_sample's density, which the reference evaluates at the LTC-space
-// direction (checked as that quirk only).  Both lobes are inactive in every
-// configuration's materials.
+// specular_btdf_evaluate's Jacobian is the reference's own, with normal-frame
+// cosines where Walter 2007 has half-vector dots: it is pinned where the two
+// agree (wo = wi refracted through the macro surface, half vector = normal),
+// against a finite-difference Jacobian of the half-vector map.
+// fuzz_brdf_sample reports density and value at the LTC-space direction; it
+// is pinned through its exact relation to fuzz_brdf_evaluate at the sampled
+// wo and through the distribution of the sampled wo.
 #pragma once
 
 #include "oracle_math.hpp"

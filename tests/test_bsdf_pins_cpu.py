@@ -14,7 +14,13 @@ cannot see (both sides share the Fresnel terms and lobe weights):
   the average albedo A + (2/3 - 64/(45 pi)) B, a white furnace (albedo 1 at
   rho = 1) and the geometric-series average albedo rho E / (1 - rho (1 - E));
 * the fuzz LTC (fuzz_brdf.glsl:18-42): its density integrates to 1 and
-  the lobe's directional albedo is fuzz_albedo;
+  the lobe's directional albedo is fuzz_albedo; its sample (fuzz_brdf.glsl:
+  55-69) draws wo from that density and reports density and value at the
+  LTC-space direction, evaluate's times |M^-1 wo|^3 and |M^-1 wo|^4;
+* specular_btdf_evaluate (specular_btdf.glsl:8-54) at wo = wi refracted
+  through the macro surface, where its Jacobian is Walter 2007's
+  d omega_h / d omega_i (finite differences) and density and value reduce to
+  textbook GGX, Smith and Fresnel factors, on both sides of the surface;
 * the microfacet lobes at roughness 0: their albedo estimates reduce to
   the Fresnel factor (F82-tint, dielectric reflectance, transmittance);
 * lobe_weights / lobe_probabilities (bsdf.glsl:33-113): OpenPBR's layering
@@ -327,9 +333,107 @@ def check_albedo_estimators(h):
         assert abs(est.mean() - want) < 4 * err + 1e-4, (name, est.mean(), want, err)
 
 
+def ggx_lambda(alpha, v):
+    """Smith Lambda of the isotropic GGX distribution (Heitz 2014, Eq. 72)."""
+    t2 = (v[0] ** 2 + v[1] ** 2) / v[2] ** 2
+    return 0.5 * (math.sqrt(1.0 + alpha * alpha * t2) - 1.0)
+
+
+def half_vector_jacobian(wi, wo, eta_i, eta_o, eps=1e-5):
+    """d omega_h / d omega_i of the transmission half vector
+    h = normalize(-(eta_i wi + eta_o wo)) at fixed wo, by central differences
+    of h over wi's spherical angles: |dh/dtheta x dh/dphi| / sin(theta)."""
+    theta, phi = math.acos(wi[2]), math.atan2(wi[1], wi[0])
+
+    def h(t, p):
+        w = np.array([math.sin(t) * math.cos(p), math.sin(t) * math.sin(p), math.cos(t)])
+        v = -(eta_i * w + eta_o * np.asarray(wo))
+        return v / np.linalg.norm(v)
+
+    dt = (h(theta + eps, phi) - h(theta - eps, phi)) / (2.0 * eps)
+    dp = (h(theta, phi + eps) - h(theta, phi - eps)) / (2.0 * eps)
+    return float(np.linalg.norm(np.cross(dt, dp))) / math.sin(theta)
+
+
+def check_btdf_refraction(h):
+    """specular_btdf_evaluate where wo is wi refracted through the macro
+    surface (Snell's law, opposite azimuth), so the transmission half vector
+    is the normal n.  There the reference's Jacobian
+    ior^2 |cos wi| / (cos wo + ior cos wi)^2 (specular_btdf.glsl:12-13) is
+    Walter et al. 2007's d omega_h / d omega_i, measured here by finite
+    differences of the half-vector map, and:
+      wi outside: density = G1(wi) D(n) J, the GGX visible-normal density of
+        n (Heitz 2018, Eq. 3) times J;
+      wi inside: density = 0 (the reference's max(0, wi.h) with h turned to
+        +z, specular_btdf.glsl:16,45-46);
+      both: value = (1 - F) G2 D(n) J / |cos wo| (specular_btdf.glsl:21-23 at
+        h = n), F the dielectric reflectance at wi's side, D, G1, G2 the
+        isotropic GGX and height-correlated Smith forms."""
+    tol = 1e-7 if h.p else 1e-4
+    eta = 1.5
+    for r in (0.25, 0.5, 0.8):
+        r = float(np.float32(r))  # MaterialConstants holds float
+        alpha = r * r
+        d_n = 1.0 / (math.pi * alpha * alpha)
+        m = material(transmission_weight=1.0, specular_roughness=r, specular_ior=eta)
+        for inside, theta in ((False, 0.2), (False, 0.7), (False, 1.2), (True, 0.15), (True, 0.5), (True, 0.7)):
+            eta_i, eta_o = (eta, 1.0) if inside else (1.0, eta)
+            phi = 0.4 + theta
+            s_o = eta_i / eta_o * math.sin(theta)
+            c_i, c_o = math.cos(theta), math.sqrt(1.0 - s_o * s_o)
+            sign = -1.0 if inside else 1.0
+            wi = np.array([math.sin(theta) * math.cos(phi), math.sin(theta) * math.sin(phi), sign * c_i])
+            wo = np.array([-s_o * math.cos(phi), -s_o * math.sin(phi), -sign * c_o])
+            j = half_vector_jacobian(wi, wo, eta_i, eta_o)
+            f = h.lobe_evaluate(m, "specular_btdf", wi, wo)[0]
+            g1 = 1.0 / (1.0 + ggx_lambda(alpha, wi))
+            g2 = 1.0 / (1.0 + ggx_lambda(alpha, wi) + ggx_lambda(alpha, wo))
+            fresnel = float(fresnel_pbrt(c_i, eta_o / eta_i))
+            want_density = 0.0 if inside else g1 * d_n * j
+            assert f[3] == pytest.approx(want_density, rel=tol, abs=1e-12), (r, inside, theta)
+            np.testing.assert_allclose(f[:3], (1.0 - fresnel) * g2 * d_n * j / c_o, rtol=tol, err_msg=str((r, inside, theta)))
+
+
+def check_fuzz_sample(h):
+    """fuzz_brdf_sample (fuzz_brdf.glsl:55-69) draws d cosine-distributed in
+    the LTC's original space and returns wo = normalize(M d) in wi's LTC
+    frame, M the inverse of M^-1 = [[a, 0, b], [0, a, 0], [0, 0, 1]] (a, b
+    from ltc_inverse_coeffs).  So (1) the sampled wo follow the LTC density
+    that evaluate returns: sample means of x, z, x^2, z^2, x z equal its
+    quadrature; and (2) the reference reports density and value at the unit
+    d itself, not at M^-1 wo = d / |M d|, so against evaluate at the same wo
+    they carry the LTC Jacobian (a / |M^-1 wo|^2)^2 at |M^-1 wo| = 1: the
+    sample's density is evaluate's times |M^-1 wo|^3 and its value
+    evaluate's times |M^-1 wo|^4."""
+    dirs, wts = hemisphere()
+    states = np.random.default_rng(11).integers(1, 2 ** 32 - 1, 16000, dtype=np.uint64)
+    tol = 1e-9 if h.p else 2e-5
+    for r, mu in ((0.3, 0.5), (0.7, 0.85), (1.0, 0.3)):
+        r = float(np.float32(r))  # MaterialConstants holds float
+        m = material(fuzz_weight=1.0, fuzz_roughness=r, fuzz_color=(0.5, 0.7, 0.9))
+        wi = direction(mu)  # azimuth 0: the LTC frame is the local frame
+        a, b = h.ltc_inverse_coeffs(mu, r)
+        samples = []
+        for s in states:
+            ok, wo, f = h.lobe_sample(m, "fuzz", wi, s)
+            assert ok
+            samples.append(wo)
+            e = h.lobe_evaluate(m, "fuzz", wi, wo)[0]
+            k = float(np.linalg.norm([a * wo[0] + b * wo[2], a * wo[1], wo[2]]))
+            assert f[3] == pytest.approx(e[3] * k ** 3, rel=tol), (r, mu, wo)
+            np.testing.assert_allclose(f[:3], e[:3] * k ** 4, rtol=tol)
+        samples = np.array(samples)
+        density = h.lobe_evaluate(m, "fuzz", wi, dirs)[:, 3]
+        for g in (lambda v: v[:, 0], lambda v: v[:, 2], lambda v: v[:, 0] ** 2, lambda v: v[:, 2] ** 2,
+                  lambda v: v[:, 0] * v[:, 2]):
+            want = float(np.sum(wts * density * g(dirs)))
+            est = g(samples)
+            assert abs(est.mean() - want) < 4.5 * est.std() / math.sqrt(est.size) + 1e-3, (r, mu, est.mean(), want)
+
+
 PINS = [check_fresnel_metal, check_specular_ior, check_specular_ior_ratio, check_specular_fresnel_with_coat,
         check_average_dielectric_fresnel, check_oren_nayar, check_energy_compensated_oren_nayar, check_fuzz_ltc,
-        check_smooth_limits, check_lobe_weights, check_albedo_estimators]
+        check_smooth_limits, check_lobe_weights, check_albedo_estimators, check_btdf_refraction, check_fuzz_sample]
 
 
 @pytest.mark.parametrize("precision", [1, 0], ids=["double", "float"])
@@ -395,6 +499,13 @@ SLIPS = [
      "(R(1) - average_dielectric_fresnel(m.coat_ior)) / m.coat_ior"),
     (check_lobe_weights, "l.weights[DIFFUSE] = opaque * (R(1) - l.albedos[SPECULAR_BRDF]);",
      "l.weights[DIFFUSE] = opaque;"),
+    (check_btdf_refraction, "R jacobian = pow2(ior) * rabs(cos_theta(wi)) /", "R jacobian = ior * rabs(cos_theta(wi)) /"),
+    (check_btdf_refraction, "rmax(pow2(cos_theta(wo) + ior * cos_theta(wi)), K<R>::DENOM_TOLERANCE);",
+     "rmax(pow2(cos_theta(wi) + ior * cos_theta(wo)), K<R>::DENOM_TOLERANCE);"),
+    (check_btdf_refraction, "V3<R> mn = -wor - ior * wir;", "V3<R> mn = -wor - wir / ior;"),
+    (check_fuzz_sample, "direction.x / ic.x - direction.z * ic.y / ic.x", "direction.x / ic.x + direction.z * ic.y / ic.x"),
+    (check_fuzz_sample, "    ok = true;\n    return fuzz_density_and_brdf(m, direction,",
+     "    ok = true;\n    return fuzz_density_and_brdf(m, wn,"),
 ]
 
 
