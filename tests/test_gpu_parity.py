@@ -295,3 +295,22 @@ def test_render_is_deterministic():
     b = renderer.render_to_texture()
     assert np.array_equal(a, b)
     assert k.frame_index == 0
+
+
+@pytest.mark.parametrize("light_sampling", [LightSampling.NEXT_EVENT_ESTIMATION, LightSampling.RESTIR])
+def test_render_odd_size_matches_oracle(light_sampling):
+    """37 x 29 pixels: the primary rays' 8-row bands end in a 5-row band and
+    the pixel count is no multiple of a wave, so the tile order's last band
+    and the passes' partial workgroups are both exercised (3 samples a pixel
+    with next-event estimation; ReSTIR with both reuses, frame by frame)."""
+    scene, camera = scenes.cornell_box()
+    if light_sampling == LightSampling.RESTIR:
+        config = Config(bounce_count=3, sample_count=1, light_sampling=light_sampling)
+        settings = RenderSettings(rng=_abi.TT_RNG_PCG, seed=5, restir_candidates=8, restir_temporal=True,
+                                  restir_spatial=True)
+        for image, reference in restir_pair(scene, camera, config, 37, 29, 3, settings):
+            assert_radiance_close(image, reference)
+    else:
+        config = Config(bounce_count=4, sample_count=3, light_sampling=light_sampling)
+        image, reference = render_pair(scene, camera, config, 37, 29, 2, RenderSettings(rng=_abi.TT_RNG_PCG, seed=5))
+        assert_radiance_close(image, reference)
