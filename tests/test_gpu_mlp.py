@@ -186,6 +186,33 @@ def test_trained_cache_beats_untrained_cache():
     assert rmse(trained, ref) < 0.7 * rmse(untrained, ref), (rmse(trained, ref), rmse(untrained, ref))
 
 
+def test_cached_frames_repeat_within_a_drift_bound():
+    """Two runs of the same cached frames with the same seeds.  Paths are
+    deterministic, but training records are appended in the order shading
+    waves finish, so the records' chunks, the gradient partials and with
+    them the weights can differ in their last bits from run to run.  The
+    two images must differ by far less than either differs from the FP64
+    reference."""
+    ref = load("fp64_cornell_512")
+
+    def run():
+        renderer = _cornell(6)
+        cache = RadianceCache(renderer.context, seed=2)
+        for _ in range(16):
+            render_cached(renderer, cache, stats=False)
+        renderer.reset_accumulation()
+        for _ in range(8):
+            render_cached(renderer, cache, stats=False)
+        return renderer.target.download().reshape(-1, 4)[:, :3]
+
+    a, b = run(), run()
+    assert np.all(np.isfinite(a)) and np.all(np.isfinite(b))
+    drift = float(np.sqrt(np.mean((a - b) ** 2)))
+    error = rmse(a[ref["pixels"]], ref)
+    print(f"run-to-run RMSE {drift:.3g}, identical pixels {np.mean(a == b):.4f}, RMSE vs FP64 {error:.4g}")
+    assert drift < 0.05 * error, (drift, error)
+
+
 def test_cached_render_close_to_reference():
     ref = load("fp64_cornell_512")
     scene, camera = scenes.cornell_box()
