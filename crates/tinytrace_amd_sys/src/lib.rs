@@ -380,6 +380,7 @@ pub struct tt_cache_settings {
     pub train_steps: u32,
     pub judge_fraction: f32,
     pub records_per_path: u32,
+    pub ordered_training: u32,
 }
 
 #[repr(C)]

@@ -186,6 +186,17 @@ def test_trained_cache_beats_untrained_cache():
     assert rmse(trained, ref) < 0.7 * rmse(untrained, ref), (rmse(trained, ref), rmse(untrained, ref))
 
 
+def _cached_run(ordered):
+    renderer = _cornell(6)
+    cache = RadianceCache(renderer.context, seed=2)
+    for _ in range(16):
+        render_cached(renderer, cache, stats=False, ordered_training=ordered)
+    renderer.reset_accumulation()
+    for _ in range(8):
+        render_cached(renderer, cache, stats=False, ordered_training=ordered)
+    return renderer.target.download().reshape(-1, 4)[:, :3]
+
+
 def test_cached_frames_repeat_within_a_drift_bound():
     """Two runs of the same cached frames with the same seeds.  Paths are
     deterministic, but training records are appended in the order shading
@@ -194,23 +205,26 @@ def test_cached_frames_repeat_within_a_drift_bound():
     two images must differ by far less than either differs from the FP64
     reference."""
     ref = load("fp64_cornell_512")
-
-    def run():
-        renderer = _cornell(6)
-        cache = RadianceCache(renderer.context, seed=2)
-        for _ in range(16):
-            render_cached(renderer, cache, stats=False)
-        renderer.reset_accumulation()
-        for _ in range(8):
-            render_cached(renderer, cache, stats=False)
-        return renderer.target.download().reshape(-1, 4)[:, :3]
-
-    a, b = run(), run()
+    a, b = _cached_run(False), _cached_run(False)
     assert np.all(np.isfinite(a)) and np.all(np.isfinite(b))
     drift = float(np.sqrt(np.mean((a - b) ** 2)))
     error = rmse(a[ref["pixels"]], ref)
     print(f"run-to-run RMSE {drift:.3g}, identical pixels {np.mean(a == b):.4f}, RMSE vs FP64 {error:.4g}")
     assert drift < 0.05 * error, (drift, error)
+
+
+def test_ordered_training_repeats_bit_for_bit():
+    """With ordered_training the records are scattered by (path, record
+    index), counted by a scan and gathered in that order before the Adam
+    steps, so two runs give the same image bit for bit, and it stays as
+    close to the FP64 reference as the default order's."""
+    ref = load("fp64_cornell_512")
+    a, b = _cached_run(True), _cached_run(True)
+    assert np.all(np.isfinite(a))
+    drift = float(np.sqrt(np.mean((a - b) ** 2)))
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (drift, float(np.mean(a == b)))
+    error, default_error = rmse(a[ref["pixels"]], ref), rmse(_cached_run(False)[ref["pixels"]], ref)
+    assert error < 1.1 * default_error, (error, default_error)
 
 
 def test_cached_render_close_to_reference():

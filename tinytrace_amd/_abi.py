@@ -171,7 +171,7 @@ class MlpDesc(C.Structure):
 
 class CacheSettings(C.Structure):
     _fields_ = [("train_samples", C.c_uint32), ("train_steps", C.c_uint32), ("judge_fraction", C.c_float),
-                ("records_per_path", C.c_uint32)]
+                ("records_per_path", C.c_uint32), ("ordered_training", C.c_uint32)]
 
 
 class CacheStats(C.Structure):

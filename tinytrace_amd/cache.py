@@ -83,15 +83,17 @@ class RadianceCache:
 
 def render_cached(renderer: Renderer, cache: RadianceCache, train_samples: int = 65536, train_steps: int = 4,
                   judge_fraction: float = 0.01, records_per_path: int = 1, settings: Optional[RenderSettings] = None,
-                  stats: bool = True) -> Optional[dict]:
+                  stats: bool = True, ordered_training: bool = False) -> Optional[dict]:
     """One frame of `renderer` with the radiance cache (accumulates like
     render_to_target).  With `stats` the call waits for the frame and its
     training and returns the frame's cache statistics; without, it returns
     None as soon as the frame is enqueued (frames then overlap like plain
-    ones).  judge_fraction < 0: no judge paths."""
+    ones).  judge_fraction < 0: no judge paths.  ordered_training: train on
+    the samples in path order, so repeated runs give the same images bit for
+    bit."""
     k = renderer.constants()
     st = settings or renderer.settings
-    cs = _abi.CacheSettings(train_samples, train_steps, judge_fraction, records_per_path)
+    cs = _abi.CacheSettings(train_samples, train_steps, judge_fraction, records_per_path, int(ordered_training))
     out = _abi.CacheStats() if stats else None
     check(lib().tt_render_cached(renderer.context.handle, renderer.scene.handle, C.byref(k), C.byref(st._desc()),
                                  cache.handle, C.byref(cs), renderer.target.handle,

@@ -152,7 +152,7 @@ TT_DEV void add_to_records(const tt::CacheBuffers& cb, uint4 open, f3 a) {
         float4 T = cb.train_thr[slot];
         float4 t = cb.train_target[slot];
         f3 d = cache_ratio(a, f3(T.x, T.y, T.z));
-        cb.train_target[slot] = make_float4(t.x + d.x, t.y + d.y, t.z + d.z, 0.0f);
+        cb.train_target[slot] = make_float4(t.x + d.x, t.y + d.y, t.z + d.z, t.w);  // w: the record's order key
     }
 }
 
@@ -392,8 +392,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEXTURED ? 
                     cb.train_rec[4 * size_t(slot) + k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
                 cb.train_thr[slot] = make_float4(thr_in.x, thr_in.y, thr_in.z, __uint_as_float(p));
                 f3 t = cache_ratio(added, thr_in);
-                cb.train_target[slot] = make_float4(t.x, t.y, t.z, 0.0f);
                 uint32_t n = open.w & 3u;
+                // w: path x records a path takes + the path's record index,
+                // the key training orders records by (slots follow the order
+                // waves finish in).
+                cb.train_target[slot] = make_float4(t.x, t.y, t.z, __uint_as_float(p * f.cache_records + n));
                 if (n == 0) open.x = slot;
                 else if (n == 1) open.y = slot;
                 else open.z = slot;

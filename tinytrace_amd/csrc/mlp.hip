@@ -24,6 +24,8 @@
 // Training runs on the context's API stream, after an event of the render
 // whose records it reads; its `trained` event orders later inference.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -566,6 +568,50 @@ __global__ void mlp_pack_kernel(const float* master, uint16_t* wf, uint16_t* wb,
     }
 }
 
+// Training records in a fixed order.  Slots are taken in the order shading
+// waves finish, so which records share a chunk, and with it the rounding of
+// the gradient sums, would vary from run to run.  Each target's w holds a key
+// (path x records a path takes + the path's record index; unique, below the
+// span): records are scattered to an index by key, the occupied entries are
+// counted by a scan, and gathered in key order, which makes training, and so
+// the cached images, repeat bit for bit.  The gather leaves the index empty
+// again for the next call.
+constexpr uint32_t ORDER_EMPTY = 0xffffffffu;
+
+__global__ __launch_bounds__(256) void mlp_order_fill_kernel(uint32_t* index, uint32_t span) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < span) index[j] = ORDER_EMPTY;
+}
+
+__global__ __launch_bounds__(256) void mlp_order_scatter_kernel(const float4* targets, const uint32_t* count_dev,
+                                                                uint32_t cap, uint32_t span, uint32_t* index) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= min(*count_dev, cap)) return;
+    const uint32_t key = __float_as_uint(targets[i].w);
+    if (key < span) index[key] = i;
+}
+
+struct Occupied {
+    __device__ uint32_t operator()(uint32_t v) const { return v != ORDER_EMPTY ? 1u : 0u; }
+};
+
+// Record index[j] goes to position pos[j]; the last thread writes the number
+// of records gathered, which training then reads as its count.
+__global__ __launch_bounds__(256) void mlp_order_gather_kernel(const float4* records, const float4* targets,
+                                                               uint32_t* index, const uint32_t* pos, uint32_t span,
+                                                               float4* rec_out, float4* tgt_out, uint32_t* count_out) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= span) return;
+    const uint32_t i = index[j];
+    if (j == span - 1u) *count_out = pos[j] + (i != ORDER_EMPTY ? 1u : 0u);
+    if (i == ORDER_EMPTY) return;
+    const uint32_t o = pos[j];
+#pragma unroll
+    for (int k = 0; k < 4; k++) rec_out[4 * size_t(o) + k] = records[4 * size_t(i) + k];
+    tgt_out[o] = targets[i];
+    index[j] = ORDER_EMPTY;
+}
+
 uint32_t blocks_of(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
 }  // namespace
@@ -595,6 +641,19 @@ struct tt_mlp {
     uint16_t* planes = nullptr;
     float* partial = nullptr;
     float* loss = nullptr;
+    // Record ordering for `order_capacity` records over keys below
+    // `order_span`, grown on demand: the index by key (empty between calls),
+    // the scanned positions, the ordered records and targets, their count,
+    // and the scan's temporary storage.
+    uint32_t order_capacity = 0, order_span = 0;
+    void* order_memory = nullptr;
+    uint32_t* order_index = nullptr;
+    uint32_t* order_pos = nullptr;
+    float4* order_rec = nullptr;
+    float4* order_tgt = nullptr;
+    uint32_t* order_count = nullptr;
+    void* order_temp = nullptr;
+    size_t order_temp_bytes = 0;
 };
 
 namespace tt {
@@ -629,6 +688,44 @@ static tt_status ensure_train(tt_mlp* m, uint32_t chunks) {
     m->partial = reinterpret_cast<float*>(p + planes);
     m->loss = reinterpret_cast<float*>(p + planes + partial);
     m->chunk_capacity = chunks;
+    return TT_OK;
+}
+
+static tt_status ensure_order(tt_mlp* m, uint32_t cap, uint32_t span) {
+    tt_context* c = m->context;
+    if (cap <= m->order_capacity && span <= m->order_span) return TT_OK;
+    hipStream_t s = train_stream(m);
+    if (m->order_memory) {
+        TT_HIP(c, hipStreamSynchronize(s));
+        TT_HIP(c, hipFree(m->order_memory));
+        m->order_memory = nullptr;
+        m->order_capacity = m->order_span = 0;
+    }
+    cap = std::max(cap, m->order_capacity);
+    span = std::max(span, m->order_span);
+    size_t temp = 0;
+    {
+        uint32_t* u = nullptr;
+        hipError_t e = rocprim::exclusive_scan(nullptr, temp, rocprim::make_transform_iterator(u, Occupied()), u, 0u,
+                                               size_t(span), rocprim::plus<uint32_t>(), s);
+        if (e != hipSuccess) return tt::hip_fail(c, e, "record order scan size");
+    }
+    auto align = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t index = align(size_t(span) * sizeof(uint32_t));
+    const size_t rec = align(4 * size_t(cap) * sizeof(float4)), tgt = align(size_t(cap) * sizeof(float4));
+    TT_HIP(c, hipMalloc(&m->order_memory, 2 * index + rec + tgt + 256 + align(temp)));
+    char* p = static_cast<char*>(m->order_memory);
+    m->order_index = reinterpret_cast<uint32_t*>(p);
+    m->order_pos = reinterpret_cast<uint32_t*>(p + index);
+    m->order_rec = reinterpret_cast<float4*>(p + 2 * index);
+    m->order_tgt = reinterpret_cast<float4*>(p + 2 * index + rec);
+    m->order_count = reinterpret_cast<uint32_t*>(p + 2 * index + rec + tgt);
+    m->order_temp = p + 2 * index + rec + tgt + 256;
+    m->order_temp_bytes = temp;
+    mlp_order_fill_kernel<<<blocks_of(span, 256), 256, 0, s>>>(m->order_index, span);
+    TT_HIP(c, hipGetLastError());
+    m->order_capacity = cap;
+    m->order_span = span;
     return TT_OK;
 }
 
@@ -674,14 +771,19 @@ tt_status mlp_judge_device(tt_mlp* m, hipStream_t stream, const float4* records,
 
 tt_status mlp_train_device(tt_mlp* m, hipStream_t after, const float4* records, const float4* targets,
                            const uint32_t* count_dev, uint32_t capacity, uint32_t steps, bool apply, float* grad_out,
-                           bool timed) {
+                           bool timed, uint32_t order_span) {
     tt_context* c = m->context;
     if (capacity == 0 || steps == 0) return TT_OK;
     if (steps > MAX_STEPS) return fail(c, TT_INVALID_ARGUMENT, "more than 64 training steps");
+    if (order_span && !count_dev) return fail(c, TT_INVALID_ARGUMENT, "ordered training needs the record count");
     // The largest slice has ceil(capacity / steps) records.
     const uint32_t chunks = blocks_of(blocks_of(capacity, steps), MLP_CHUNK);
     tt_status st = ensure_train(m, chunks);
     if (st != TT_OK) return st;
+    if (order_span) {
+        st = ensure_order(m, capacity, order_span);
+        if (st != TT_OK) return st;
+    }
     hipStream_t s = train_stream(m);
     if (after != s) {
         TT_HIP(c, hipEventRecord(m->ready, after));
@@ -691,6 +793,22 @@ tt_status mlp_train_device(tt_mlp* m, hipStream_t after, const float4* records, 
         if (!m->train_time.begin) TT_HIP(c, hipEventCreate(&m->train_time.begin));
         if (!m->train_time.end) TT_HIP(c, hipEventCreate(&m->train_time.end));
         TT_HIP(c, hipEventRecord(m->train_time.begin, s));
+    }
+    if (order_span) {
+        mlp_order_scatter_kernel<<<blocks_of(capacity, 256), 256, 0, s>>>(targets, count_dev, capacity, order_span,
+                                                                          m->order_index);
+        TT_HIP(c, hipGetLastError());
+        size_t temp = m->order_temp_bytes;
+        TT_HIP(c, rocprim::exclusive_scan(m->order_temp, temp, rocprim::make_transform_iterator(m->order_index,
+                                                                                                Occupied()),
+                                          m->order_pos, 0u, size_t(order_span), rocprim::plus<uint32_t>(), s));
+        mlp_order_gather_kernel<<<blocks_of(order_span, 256), 256, 0, s>>>(records, targets, m->order_index,
+                                                                           m->order_pos, order_span, m->order_rec,
+                                                                           m->order_tgt, m->order_count);
+        TT_HIP(c, hipGetLastError());
+        records = m->order_rec;
+        targets = m->order_tgt;
+        count_dev = m->order_count;
     }
     for (uint32_t i = 0; i < steps; i++) {
         mlp_train_kernel<<<chunks, TRAIN_BLOCK, 0, s>>>(m->wf, m->wb, m->wb5, m->bounds, records, targets, count_dev,
@@ -822,6 +940,7 @@ void tt_mlp_destroy(tt_mlp* m) try {
     if (m->master) (void)hipFree(m->master);
     if (m->wf) (void)hipFree(m->wf);
     if (m->train_memory) (void)hipFree(m->train_memory);
+    if (m->order_memory) (void)hipFree(m->order_memory);
     for (hipEvent_t e : {m->ready, m->trained, m->train_time.begin, m->train_time.end})
         if (e) (void)hipEventDestroy(e);
     delete m;

@@ -219,6 +219,7 @@ tt_status cache_plan(const tt_cache_settings* cs, uint32_t paths, CachePlan& out
         return TT_INVALID_ARGUMENT;
     }
     out.records = cs && cs->records_per_path ? cs->records_per_path : 1u;
+    out.ordered = cs && cs->ordered_training != 0u;
     if (out.records > CACHE_OPEN_RECORDS) {
         *why = "records_per_path > 3";
         return TT_INVALID_ARGUMENT;
@@ -738,7 +739,7 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
 
     tt::FrameArgs f = tt::frame_args(*k, t->width, t->height, st.rng, st.seed);
     f.primary_tiles = 1;
-    uint32_t train_steps = 0, train_capacity = 0, judge_capacity = 0;
+    uint32_t train_steps = 0, train_capacity = 0, judge_capacity = 0, train_order_span = 0;
     if (cache) {
         if (tt::mlp_context(cache) != c) return fail(c, TT_STALE_HANDLE, "cache from another context");
         tt::CachePlan plan;
@@ -747,6 +748,7 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
         train_steps = plan.train_steps;
         train_capacity = plan.train_capacity;
         judge_capacity = plan.judge_capacity;
+        train_order_span = plan.ordered ? paths * plan.records : 0u;
         tt::apply_cache_plan(plan, f);
         tt_status cst = tt::ensure_cache(c, lane, paths, train_capacity, judge_capacity, b.cache);
         if (cst != TT_OK) return cst;
@@ -911,7 +913,7 @@ tt_status render(tt_context* c, const tt_scene* s, const tt_constants* k, const 
         // them, so the lane's end waits for it.
         es = tt::mlp_train_device(cache, stream, b.cache.train_rec, b.cache.train_target,
                                   L.counters + tt::C_CACHE_TRAIN, train_capacity, train_steps, true, nullptr,
-                                  cache_stats != nullptr);
+                                  cache_stats != nullptr, train_order_span);
         if (es != TT_OK) return es;
         es = tt::mlp_wait_trained(cache, stream);
         if (es != TT_OK) return es;

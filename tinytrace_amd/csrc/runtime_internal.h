@@ -175,6 +175,7 @@ void context_release(tt_context* c);
 // hash, record capacities with a margin, Adam steps.
 struct CachePlan {
     uint32_t records;  // training records a training path takes
+    bool ordered;      // training takes the records in path order
     uint32_t train_steps;
     uint32_t train_capacity, judge_capacity;
     uint32_t train_threshold, judge_threshold;
@@ -222,10 +223,13 @@ tt_status mlp_judge_device(tt_mlp* m, hipStream_t stream, const float4* records,
 // records (count_dev null: capacity records), on the cache's stream after the
 // work enqueued on `after` so far.  apply false: one gradient, no update,
 // into grad_out (device, MLP_PARAMS floats).  timed: HIP events around the
-// steps (mlp_train_ms).
+// steps (mlp_train_ms).  order_span > 0: the records are first put in the
+// order of the key in each target's w (the shade kernel's path x records a
+// path takes + record index, below order_span), so the result does not
+// depend on the order their slots were taken in.
 tt_status mlp_train_device(tt_mlp* m, hipStream_t after, const float4* records, const float4* targets,
                            const uint32_t* count_dev, uint32_t capacity, uint32_t steps, bool apply, float* grad_out,
-                           bool timed = false);
+                           bool timed = false, uint32_t order_span = 0);
 // Milliseconds of the last timed training (synchronises).
 tt_status mlp_train_ms(tt_mlp* m, double* out);
 // Mean loss of the first step of the last training, whose record count was

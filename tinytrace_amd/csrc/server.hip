@@ -562,7 +562,8 @@ tt_status enqueue_batch(tt_server* sv, bool& formed) {
         // Train on the batch's records; the lane's queues are free again once
         // training has read them, so the lane's end waits for it.
         st = tt::mlp_train_device(cache, stream, b.cache.train_rec, b.cache.train_target, L.counters + tt::C_CACHE_TRAIN,
-                                  plan.train_capacity, plan.train_steps, true, nullptr);
+                                  plan.train_capacity, plan.train_steps, true, nullptr, false,
+                                  plan.ordered ? paths * plan.records : 0u);
         if (st != TT_OK) return st;
         st = tt::mlp_wait_trained(cache, stream);
         if (st != TT_OK) return st;

@@ -50,12 +50,13 @@ class Server:
         return slot.value
 
     def set_cache(self, scene: int, cache, train_samples: int = 0, train_steps: int = 0, judge_fraction: float = 0.0,
-                  records_per_path: int = 0) -> None:
+                  records_per_path: int = 0, ordered_training: bool = False) -> None:
         """tt_server_set_cache: requests on scene slot `scene` are served
         with the radiance cache `cache` (a cache.RadianceCache; None: plainly
         again), which every batch of them trains (settings per batch as
         render_cached's per frame; 0: the defaults)."""
-        settings = _abi.CacheSettings(train_samples, train_steps, judge_fraction, records_per_path)
+        settings = _abi.CacheSettings(train_samples, train_steps, judge_fraction, records_per_path,
+                                      int(ordered_training))
         check(lib().tt_server_set_cache(self.handle, scene, None if cache is None else cache.handle,
                                         C.byref(settings)), self.context.handle, "tt_server_set_cache")
         if cache is None:
