@@ -184,10 +184,12 @@ def _trained_cornell_cache(renderer, camera, frames=12):
     return cache
 
 
-def test_served_with_cache_queries_trains_and_stays_close_to_plain():
+@pytest.mark.parametrize("ordered", [False, True], ids=["slot_order", "ordered_training"])
+def test_served_with_cache_queries_trains_and_stays_close_to_plain(ordered):
     """Requests on a scene slot with a radiance cache end their paths at cache
-    queries, train the cache batch by batch, report judge statistics, and
-    keep the image's mean radiance near the plainly served one's."""
+    queries, train the cache batch by batch (records in slot order, or
+    gathered in path order first), report judge statistics, and keep the
+    image's mean radiance near the plainly served one's."""
     scene, camera = scenes.cornell_box()
     renderer = Renderer((256, 256), settings=RenderSettings(rng=_abi.TT_RNG_PCG, seed=1))
     renderer.set_scene(scene)
@@ -202,7 +204,7 @@ def test_served_with_cache_queries_trains_and_stays_close_to_plain():
         server = Server(renderer.context, pool_paths=1 << 17)
         slot = server.add_scene(renderer.scene)
         if with_cache:
-            server.set_cache(slot, cache, train_samples=8192, train_steps=2)
+            server.set_cache(slot, cache, train_samples=8192, train_steps=2, ordered_training=ordered)
         ids = [server.submit(slot, cams[i], config, w, h, p, seed=5 + i) for i, (w, h, p) in enumerate(sizes)]
         server.run_until_done(ids)
         images = [server.take_result(i) for i in ids]
