@@ -338,7 +338,7 @@ void parallel_for(uint32_t count, unsigned max_threads, Fn&& fn) {
 }
 
 // Object-space boxes that together bound mesh m: its triangles binned by
-// centroid into a PART_GRID^3 grid (8^3) over the mesh's box, one box per
+// centroid into a PART_GRID^3 grid (16^3) over the mesh's box, one box per
 // non-empty cell around the vertices of the cell's triangles.  Every triangle
 // lies inside its own cell's box, so the union of the boxes' world images
 // bounds the instance; for a rotated mesh that union is much tighter than the
@@ -346,7 +346,7 @@ void parallel_for(uint32_t count, unsigned max_threads, Fn&& fn) {
 // is up to sqrt(3) times wider on each axis).  Vertices are dequantized as the
 // BLAS does (v / 32767 in float).
 #ifndef TT_PART_GRID
-#define TT_PART_GRID 8
+#define TT_PART_GRID 16
 #endif
 constexpr uint32_t PART_GRID = TT_PART_GRID;
 
